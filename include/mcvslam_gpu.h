@@ -1,0 +1,167 @@
+/*
+ * mcvslam_gpu.h — C ABI of the MI355X (gfx950) hot path of MCVSLAM.
+ *
+ * Every entry point returns MCV_OK (0) or a negative MCV_ERR_* code; the
+ * message of the last failure on a context is mcv_last_error(ctx).  No C++
+ * exception crosses this boundary.  Buffers are owned by the caller; host
+ * pointers are ordinary memory, "_dev" entry points take device pointers
+ * (HBM) and enqueue on the context's stream without synchronizing.
+ * Contexts are independent: one per thread replaces the reference's one
+ * ORB extractor per camera thread (src/Frame.cpp:24-26, :120-125).
+ */
+#ifndef MCVSLAM_GPU_H
+#define MCVSLAM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCV_OK 0
+#define MCV_ERR_INVALID (-1)   /* bad argument */
+#define MCV_ERR_HIP (-2)       /* HIP runtime failure */
+#define MCV_ERR_CAPACITY (-3)  /* output too small; required size reported */
+#define MCV_ERR_EMPTY (-4)     /* empty image (ORBextractor::operator() returns -1) */
+#define MCV_ERR_STATE (-5)     /* not configured, or device work overflowed */
+
+#define MCV_FLAG_GRAPH 1       /* replay the launch sequence as a captured hipGraph */
+
+typedef struct mcv_ctx mcv_ctx;
+typedef struct mcv_rig mcv_rig;
+typedef struct mcv_ba mcv_ba;
+
+/* cv::KeyPoint without class_id: pt.x, pt.y, size, angle (deg), response, octave */
+typedef struct {
+    float x, y, size, angle, response;
+    int32_t octave;
+} mcv_keypoint;
+
+/* ---- context --------------------------------------------------------- */
+int mcv_create(mcv_ctx** ctx, int device);
+void mcv_destroy(mcv_ctx* ctx);
+const char* mcv_last_error(const mcv_ctx* ctx);
+/* hipStream_t to enqueue on (NULL: the context's own stream) */
+int mcv_set_stream(mcv_ctx* ctx, void* hip_stream);
+int mcv_synchronize(mcv_ctx* ctx);
+/* Synchronous device -> host copy of `bytes` on the context's stream. */
+int mcv_copy_to_host(mcv_ctx* ctx, void* dst, const void* d_src, uint64_t bytes);
+
+/* ---- ORB extraction ---------------------------------------------------
+ * Replaces ORB_SLAM3::ORBextractor::init (modules/local_feature/ORB/
+ * orb3_extractor/ORBextractor.cc:407-457) as called by ORB::ORB / ORB::Parse
+ * (modules/local_feature/ORB/ORBExtractor.cpp:10-23). */
+int mcv_orb_configure(mcv_ctx* ctx, int nfeatures, float scale_factor, int nlevels, int ini_th_fast,
+                      int min_th_fast);
+/* mvScaleFactor, mvInvScaleFactor, mvLevelSigma2, mvInvLevelSigma2,
+ * mnFeaturesPerLevel (ORBextractor.h:96-99); each array has nlevels entries;
+ * any pointer may be NULL */
+int mcv_orb_tables(const mcv_ctx* ctx, float* scale, float* inv_scale, float* sigma2, float* inv_sigma2,
+                   int32_t* n_per_level);
+/* Upper bound of the keypoints one image of this size can produce. */
+int mcv_orb_capacity(const mcv_ctx* ctx, int width, int height, int n_seeds, int32_t* capacity);
+/* Replaces ORBextractor::operator()(image, mask, keypoints, descriptors, lap)
+ * (ORBextractor.h:57-58, ORBextractor.cc:831-899) behind BaseExtractor::Extract
+ * (modules/local_feature/BaseExtractor/BaseExtractor.hpp:18, ORBExtractor.cpp:27-38).
+ * image: width x height u8 with row stride `stride` (host).  seeds: keypoints
+ * appended to their octave after the octree (the reference's input
+ * `_keypoints`), may be NULL.  Output: `count` keypoints and count x 32
+ * descriptor bytes, in the reference's order.  Returns MCV_ERR_EMPTY for an
+ * empty image, MCV_ERR_CAPACITY (count = required) when capacity is short. */
+int mcv_orb_extract(mcv_ctx* ctx, const uint8_t* image, int width, int height, int stride, const mcv_keypoint* seeds,
+                    int n_seeds, mcv_keypoint* kps, uint8_t* desc, int capacity, int32_t* count);
+/* mvImagePyramid[level] of the last mcv_orb_extract (ORBextractor.h:72, read
+ * by Frame::ComputeStereoMatch, src/Frame.cpp:239-261). */
+int mcv_orb_pyramid(mcv_ctx* ctx, int level, uint8_t* out, int capacity, int32_t* width, int32_t* height);
+/* Batched extraction of device-resident images in one launch sequence.
+ * d_images: host array of n device pointers; widths/heights/strides: host
+ * arrays.  Outputs (device): d_kps[n * capacity], d_desc[n * capacity * 32],
+ * d_counts[n].  capacity must be >= mcv_orb_capacity of every image. */
+int mcv_orb_extract_batch_dev(mcv_ctx* ctx, int n, const uint8_t* const* d_images, const int32_t* widths,
+                              const int32_t* heights, const int32_t* strides, mcv_keypoint* d_kps, uint8_t* d_desc,
+                              int capacity, int32_t* d_counts, int flags);
+
+/* ---- Hamming matching --------------------------------------------------
+ * Replaces Matcher::KnnMatch(const vector<Mat>&, const vector<Mat>&, int k,
+ * uint n_threads) (include/Matcher.hpp:84-86, src/Matcher.cpp:245-302):
+ * per query row (trainIdx0, distance0, trainIdx1, distance1), sentinel
+ * (0, 999) when fewer than two trains exist.  q: nq x 32, t: nt x 32 (host). */
+int mcv_match_knn2(mcv_ctx* ctx, const uint8_t* q, int nq, const uint8_t* t, int nt, int32_t* out);
+int mcv_match_knn2_dev(mcv_ctx* ctx, const uint8_t* d_q, int nq, const uint8_t* d_t, int nt, int32_t* d_out);
+/* Replaces MatchResKnn::FilterRatio(ratio).FilterThreshold(th)
+ * (src/Matcher.cpp:100-111, :23-35): rows (queryIdx, trainIdx, distance) in
+ * the reference's order (the threshold pass reorders).  knn: nq x 4 (host). */
+int mcv_match_filter(mcv_ctx* ctx, const int32_t* knn, int nq, float ratio, int th, int32_t* out, int32_t* count);
+
+/* ---- rig frames: ORB on L, R, W + L-R, W-L, W-R matching ----------------
+ * Frame::Frame extraction (src/Frame.cpp:118-125) followed by brute-force
+ * KnnMatch -> FilterRatio -> FilterThreshold for the three camera pairs.
+ * A rig object owns the plan, device outputs and the captured graph. */
+int mcv_rig_create(mcv_ctx* ctx, int n_frames, const int32_t* widths3, const int32_t* heights3, float ratio, int th,
+                   mcv_rig** rig);
+void mcv_rig_destroy(mcv_rig* rig);
+/* d_images: host array of 3 * n_frames device pointers (frame-major L, R, W),
+ * each image dense (stride = width).  flags: MCV_FLAG_GRAPH replays the whole
+ * batch as one hipGraph; without it every image and every pair is launched on
+ * its own ("plain" path). */
+int mcv_rig_run(mcv_rig* rig, const uint8_t* const* d_images, int flags);
+/* Device outputs: keypoints [3n][capacity], descriptors [3n][capacity][32],
+ * counts [3n], matches [3n pairs][capacity][3] (query, train, distance),
+ * match counts [3n]; knn rows [3n pairs][capacity][4]. */
+int mcv_rig_outputs(mcv_rig* rig, mcv_keypoint** d_kps, uint8_t** d_desc, int32_t** d_counts, int32_t** d_matches,
+                    int32_t** d_match_counts, int32_t** d_knn, int32_t* capacity);
+
+/* ---- projection-guided window search -----------------------------------
+ * Object::AssignFeaturesToGrid + ProjectBunchMapPoints + GetFeaturesInArea +
+ * KnnMatch/FilterRatio/FilterThreshold for every map point in order
+ * (src/Object.cpp:182-236, :249-308).  Host arrays.  Per map point the
+ * output row is (kp0, d0, kp1, d1, n_candidates, match) with keypoint
+ * indices (-1 for the 999 sentinel) and match = accepted keypoint or -1.
+ * R, t: world-to-camera (row-major 3x3 and 3, f32); cam: fx, fy, cx, cy;
+ * scales: per-level scale factors (nlevels entries). */
+int mcv_window_search(mcv_ctx* ctx, const float* points, const uint8_t* pdesc, const int32_t* plevel, int n_points,
+                      const float* R, const float* t, const float* cam, const float* scales, int nlevels, float radius,
+                      float ratio, int th, const float* kxy, const int32_t* koct, const uint8_t* kdesc, int n_kps,
+                      int width, int height, int32_t* out);
+/* Device variant over several cameras at once: arrays are device pointers,
+ * cameras index blocks of keypoints; out is [n_cams][n_points][6]. */
+int mcv_window_search_dev(mcv_ctx* ctx, int n_cams, const float* d_points, const uint8_t* d_pdesc,
+                          const int32_t* d_plevel, int n_points, const float* Rt /* host: n_cams x 12 */,
+                          const float* cams /* host: n_cams x 4 */, const int32_t* sizes /* host: n_cams x 2 */,
+                          const float* scales, int nlevels, float radius, float ratio, int th,
+                          const float* const* d_kxy, const int32_t* const* d_koct, const uint8_t* const* d_kdesc,
+                          const int32_t* n_kps /* host */, int32_t* d_out);
+
+/* ---- bundle-adjustment linearization -----------------------------------
+ * The batched seam of BlockSolver::buildSystem / SparseOptimizer::
+ * computeActiveErrors (ThirdParty/g2o/g2o/core/block_solver.hpp:463-520,
+ * core/sparse_optimizer.cpp:63-90) for EdgeSE3XYZ2Camera edges
+ * (include/OptimizeTypes.hpp:70-117) added by BAoptimizer::addEdge /
+ * addEdgeToBody (src/BAoptimizer.cpp:45-65, :90-108). */
+int mcv_ba_create(mcv_ctx* ctx, int n_obs, int n_points, int n_frames, int n_cams, mcv_ba** ba);
+void mcv_ba_destroy(mcv_ba* ba);
+/* Observations (host): uv (M x 2 f32), point, frame, camera indices (M),
+ * inverse level sigma^2 (M f32: mvInvLevelSigma2[level]). */
+int mcv_ba_set_observations(mcv_ba* ba, const float* uv, const int32_t* point, const int32_t* frame,
+                            const int32_t* cam, const float* inv_sigma2);
+/* State (host, FP64 as g2o keeps it): points P x 3; frames F x 7 body-from-
+ * world (qx qy qz qw tx ty tz); cams C x 12 (fx fy cx cy, camera-from-body
+ * qx qy qz qw tx ty tz, Huber delta). */
+int mcv_ba_set_state(mcv_ba* ba, const double* points, const double* frames, const double* cams);
+/* Linearize every edge into 22 device planes of M floats: e_u, e_v, chi2,
+ * Huber weight, J_pose (2x6 row-major), J_point (2x3 row-major).
+ * residual_only = 1 computes the first four planes only. */
+int mcv_ba_linearize(mcv_ba* ba, int residual_only);
+/* Device pointer of the planes (plane k at d_planes + k * stride). */
+int mcv_ba_planes(mcv_ba* ba, float** d_planes, int64_t* stride);
+/* Copy planes [first, first + count) to host (count x M floats). */
+int mcv_ba_read_planes(mcv_ba* ba, int first, int count, float* host);
+/* Adjoint of the residual: grad_pose[F x 6] = sum J_pose^T g and
+ * grad_point[P x 3] = sum J_point^T g over the last linearization, with
+ * d_g an M x 2 device array; outputs are device arrays. */
+int mcv_ba_adjoint_dev(mcv_ba* ba, const float* d_g, float* d_grad_pose, float* d_grad_point);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCVSLAM_GPU_H */

@@ -1,0 +1,359 @@
+"""Python mirror of the reference's hot-path interfaces over the C ABI.
+
+    ORB                  MCVSLAM::ORB / ORB_SLAM3::ORBextractor
+                         (modules/local_feature/ORB/ORBExtractor.cpp:10-38,
+                          orb3_extractor/ORBextractor.h:46-100)
+    Matcher.KnnMatch     Matcher::KnnMatch (include/Matcher.hpp:84-86)
+    MatchResKnn.FilterRatio(r).FilterThreshold(th)
+                         src/Matcher.cpp:100-111, :23-35
+    window_search        Object::ProjectBunchMapPoints + GetFeaturesInArea
+                         (src/Object.cpp:208-236, :263-308)
+    RigBatch             Frame::Frame extraction + camera-pair matching
+    BALinearizer         batched EdgeSE3XYZ2Camera linearization
+                         (include/OptimizeTypes.hpp:70-117)
+
+Everything runs in the HIP library; nothing here computes a result itself.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+from ._native import KeyPoint, MCVError, check, lib
+
+KP_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("size", "<f4"), ("angle", "<f4"), ("response", "<f4"),
+                     ("octave", "<i4")])
+
+
+def _p(a):
+    return ctypes.c_void_p(0) if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """One device context (stream, plans, buffers); one per thread."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        code = lib().mcv_create(ctypes.byref(h), device)
+        if code != 0:
+            raise MCVError(code, f"mcv_create(device={device}) failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mcv_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_handle: int | None):
+        check(self.h, lib().mcv_set_stream(self.h, ctypes.c_void_p(stream_handle or 0)))
+
+    def use_torch_stream(self):
+        import torch
+        self.set_stream(torch.cuda.current_stream().cuda_stream)
+
+    def synchronize(self):
+        check(self.h, lib().mcv_synchronize(self.h))
+
+    def to_host(self, d_ptr: int, dtype, shape) -> np.ndarray:
+        out = np.zeros(shape, dtype)
+        check(self.h, lib().mcv_copy_to_host(self.h, _p(out), ctypes.c_void_p(d_ptr), out.nbytes))
+        return out
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class ORB:
+    """ORB extractor with the reference's parameters and tables."""
+
+    def __init__(self, nfeatures: int = 2000, scale_factor: float = 1.2, nlevels: int = 8, ini_th_fast: int = 20,
+                 min_th_fast: int = 7, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.nfeatures, self.scaleFactor, self.nlevels = nfeatures, float(scale_factor), nlevels
+        self.iniThFAST, self.minThFAST = ini_th_fast, min_th_fast
+        check(self.ctx.h, lib().mcv_orb_configure(self.ctx.h, nfeatures, self.scaleFactor, nlevels, ini_th_fast,
+                                                  min_th_fast))
+        t = [np.zeros(nlevels, np.float32) for _ in range(4)]
+        npl = np.zeros(nlevels, np.int32)
+        check(self.ctx.h, lib().mcv_orb_tables(self.ctx.h, *[_p(x) for x in t], _p(npl)))
+        self.mvScaleFactor, self.mvInvScaleFactor, self.mvLevelSigma2, self.mvInvLevelSigma2 = t
+        self.mnFeaturesPerLevel = npl
+
+    @classmethod
+    def from_yaml(cls, path: str, ctx: Context | None = None) -> "ORB":
+        """ORB::Parse (ORBExtractor.cpp:10-18): nkeypoints, scale_factor,
+        nlevels, ORBextractor.iniThFAST, ORBextractor.minThFAST."""
+        import yaml
+        with open(path) as f:
+            cfg = yaml.safe_load(f)
+        return cls(int(cfg["nkeypoints"]), float(cfg["scale_factor"]), int(cfg["nlevels"]),
+                   int(cfg["ORBextractor.iniThFAST"]), int(cfg["ORBextractor.minThFAST"]), ctx)
+
+    def capacity(self, width: int, height: int, n_seeds: int = 0) -> int:
+        c = ctypes.c_int32()
+        check(self.ctx.h, lib().mcv_orb_capacity(self.ctx.h, width, height, n_seeds, ctypes.byref(c)))
+        return c.value
+
+    def extract(self, image: np.ndarray, seeds: np.ndarray | None = None):
+        """ORB::Extract: returns (keypoints as KP_DTYPE records, N x 32 u8)."""
+        img = np.ascontiguousarray(image, dtype=np.uint8)
+        if img.ndim != 2 or img.size == 0:
+            raise MCVError(_native.MCV_ERR_EMPTY, "empty image")
+        h, w = img.shape
+        sd = None
+        ns = 0
+        if seeds is not None and len(seeds):
+            sd = np.ascontiguousarray(seeds, dtype=KP_DTYPE)
+            ns = len(sd)
+        cap = self.capacity(w, h, ns)
+        kps = np.zeros(cap, KP_DTYPE)
+        desc = np.zeros((cap, 32), np.uint8)
+        cnt = ctypes.c_int32()
+        check(self.ctx.h, lib().mcv_orb_extract(self.ctx.h, _p(img), w, h, w, _p(sd), ns, _p(kps), _p(desc), cap,
+                                                ctypes.byref(cnt)))
+        n = cnt.value
+        return kps[:n].copy(), desc[:n].copy()
+
+    __call__ = extract
+
+    def pyramid(self, level: int) -> np.ndarray:
+        """mvImagePyramid[level] of the last extract()."""
+        w, h = ctypes.c_int32(), ctypes.c_int32()
+        check(self.ctx.h, lib().mcv_orb_pyramid(self.ctx.h, level, None, 0, ctypes.byref(w), ctypes.byref(h)))
+        out = np.zeros((h.value, w.value), np.uint8)
+        check(self.ctx.h, lib().mcv_orb_pyramid(self.ctx.h, level, _p(out), out.size, ctypes.byref(w), ctypes.byref(h)))
+        return out
+
+    def extract_batch_dev(self, images, graph: bool = False):
+        """Batched extraction of torch.cuda uint8 images (H x W, contiguous).
+        Returns device tensors (keypoints n x cap x 6 as float32 view of
+        KP records, descriptors n x cap x 32, counts n)."""
+        import torch
+        n = len(images)
+        ws = np.array([im.shape[1] for im in images], np.int32)
+        hs = np.array([im.shape[0] for im in images], np.int32)
+        cap = max(self.capacity(int(w), int(h)) for w, h in zip(ws, hs))
+        dev = images[0].device
+        kps = torch.zeros((n, cap, 6), dtype=torch.float32, device=dev)
+        desc = torch.zeros((n, cap, 32), dtype=torch.uint8, device=dev)
+        counts = torch.zeros(n, dtype=torch.int32, device=dev)
+        ptrs = (ctypes.c_void_p * n)(*[im.data_ptr() for im in images])
+        self.ctx.use_torch_stream()
+        check(self.ctx.h, lib().mcv_orb_extract_batch_dev(
+            self.ctx.h, n, ptrs, _p(ws), _p(hs), _p(ws), ctypes.c_void_p(kps.data_ptr()),
+            ctypes.c_void_p(desc.data_ptr()), cap, ctypes.c_void_p(counts.data_ptr()),
+            _native.MCV_FLAG_GRAPH if graph else 0))
+        return kps, desc, counts
+
+
+def kp_records(kps6) -> np.ndarray:
+    """Reinterpret rows of 6 float32 (device output) as KP_DTYPE records."""
+    a = np.ascontiguousarray(np.asarray(kps6, dtype=np.float32))
+    return a.view(KP_DTYPE).reshape(a.shape[:-1])
+
+
+class MatchRes(np.ndarray):
+    """Rows of (queryIdx, trainIdx, distance)."""
+
+
+class MatchResKnn:
+    """Rows of (trainIdx0, distance0, trainIdx1, distance1) per query."""
+
+    def __init__(self, rows: np.ndarray, ctx: Context):
+        self.rows = rows
+        self.ctx = ctx
+        self._ratio = None
+
+    def __len__(self):
+        return len(self.rows)
+
+    def FilterRatio(self, ratio: float = 0.6) -> "_RatioFiltered":
+        return _RatioFiltered(self, float(ratio))
+
+
+class _RatioFiltered:
+    """FilterRatio result; materialised by FilterThreshold (or .matches())."""
+
+    def __init__(self, knn: MatchResKnn, ratio: float):
+        self.knn, self.ratio = knn, ratio
+
+    def FilterThreshold(self, th: int = 46) -> np.ndarray:
+        return Matcher.filter(self.knn.rows, self.ratio, int(th), self.knn.ctx)
+
+    def matches(self) -> np.ndarray:
+        # no threshold: every distance passes (<= 256), order unchanged
+        return Matcher.filter(self.knn.rows, self.ratio, 1 << 30, self.knn.ctx)
+
+
+class Matcher:
+    @staticmethod
+    def KnnMatch(desp1, desp2, k: int = 2, n_threads: int = 0, ctx: Context | None = None) -> MatchResKnn:
+        ctx = ctx or default_context()
+        q = np.ascontiguousarray(desp1, dtype=np.uint8).reshape(-1, 32)
+        t = np.ascontiguousarray(desp2, dtype=np.uint8).reshape(-1, 32)
+        out = np.zeros((len(q), 4), np.int32)
+        check(ctx.h, lib().mcv_match_knn2(ctx.h, _p(q), len(q), _p(t), len(t), _p(out)))
+        return MatchResKnn(out, ctx)
+
+    @staticmethod
+    def filter(knn_rows: np.ndarray, ratio: float, th: int, ctx: Context | None = None) -> np.ndarray:
+        ctx = ctx or default_context()
+        k = np.ascontiguousarray(knn_rows, dtype=np.int32).reshape(-1, 4)
+        out = np.zeros((max(len(k), 1), 3), np.int32)
+        cnt = ctypes.c_int32()
+        check(ctx.h, lib().mcv_match_filter(ctx.h, _p(k), len(k), ctypes.c_float(ratio), th, _p(out),
+                                            ctypes.byref(cnt)))
+        return out[:cnt.value].copy()
+
+
+def window_search(points, pdesc, plevel, R, t, cam, scales, radius, ratio, th, kxy, koct, kdesc, width, height,
+                  ctx: Context | None = None) -> np.ndarray:
+    """Per map point (kp0, d0, kp1, d1, n_candidates, match)."""
+    ctx = ctx or default_context()
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    pd = np.ascontiguousarray(pdesc, np.uint8).reshape(-1, 32)
+    pl = np.ascontiguousarray(plevel, np.int32)
+    Rm = np.ascontiguousarray(R, np.float32).reshape(9)
+    tv = np.ascontiguousarray(t, np.float32).reshape(3)
+    cm = np.ascontiguousarray(cam, np.float32).reshape(4)
+    sc = np.ascontiguousarray(scales, np.float32)
+    kx = np.ascontiguousarray(kxy, np.float32).reshape(-1, 2)
+    ko = np.ascontiguousarray(koct, np.int32)
+    kd = np.ascontiguousarray(kdesc, np.uint8).reshape(-1, 32)
+    out = np.zeros((len(pts), 6), np.int32)
+    check(ctx.h, lib().mcv_window_search(ctx.h, _p(pts), _p(pd), _p(pl), len(pts), _p(Rm), _p(tv), _p(cm), _p(sc),
+                                         len(sc), ctypes.c_float(radius), ctypes.c_float(ratio), th, _p(kx), _p(ko),
+                                         _p(kd), len(kx), width, height, _p(out)))
+    return out
+
+
+class RigBatch:
+    """n rig frames (L, R, W) extracted and matched (L-R, W-L, W-R)."""
+
+    def __init__(self, orb: ORB, n_frames: int, sizes=((1280, 720), (1280, 720), (640, 480)), ratio: float = 0.7,
+                 th: int = 50):
+        self.orb = orb
+        self.ctx = orb.ctx
+        self.n_frames = n_frames
+        w = np.array([s[0] for s in sizes], np.int32)
+        h = np.array([s[1] for s in sizes], np.int32)
+        self.sizes = sizes
+        hd = ctypes.c_void_p()
+        check(self.ctx.h, lib().mcv_rig_create(self.ctx.h, n_frames, _p(w), _p(h), ctypes.c_float(ratio), th,
+                                               ctypes.byref(hd)))
+        self.h = hd
+        ptrs = [ctypes.c_void_p() for _ in range(6)]
+        cap = ctypes.c_int32()
+        check(self.ctx.h, lib().mcv_rig_outputs(self.h, *[ctypes.byref(p) for p in ptrs], ctypes.byref(cap)))
+        self.d_kps, self.d_desc, self.d_counts, self.d_matches, self.d_mcounts, self.d_knn = [p.value for p in ptrs]
+        self.cap = cap.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mcv_rig_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, image_ptrs, graph: bool = True):
+        n = 3 * self.n_frames
+        if len(image_ptrs) != n:
+            raise ValueError(f"expected {n} image pointers")
+        arr = (ctypes.c_void_p * n)(*image_ptrs)
+        check(self.ctx.h, lib().mcv_rig_run(self.h, arr, _native.MCV_FLAG_GRAPH if graph else 0))
+
+    def read(self):
+        """Host copies: (kps records [3n][cap], desc [3n][cap][32], counts [3n],
+        matches [3n][cap][3], match counts [3n], knn [3n][cap][4])."""
+        n, c = 3 * self.n_frames, self.cap
+        kps = self.ctx.to_host(self.d_kps, np.float32, (n, c, 6))
+        desc = self.ctx.to_host(self.d_desc, np.uint8, (n, c, 32))
+        counts = self.ctx.to_host(self.d_counts, np.int32, (n,))
+        matches = self.ctx.to_host(self.d_matches, np.int32, (n, c, 3))
+        mcounts = self.ctx.to_host(self.d_mcounts, np.int32, (n,))
+        knn = self.ctx.to_host(self.d_knn, np.int32, (n, c, 4))
+        return kp_records(kps), desc, counts, matches, mcounts, knn
+
+
+class BALinearizer:
+    """Batched linearization of EdgeSE3XYZ2Camera edges (see mcvslam_gpu.h)."""
+
+    N_PLANES = 22
+
+    def __init__(self, n_obs: int, n_points: int, n_frames: int, n_cams: int, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.M, self.P, self.F, self.C = n_obs, n_points, n_frames, n_cams
+        h = ctypes.c_void_p()
+        check(self.ctx.h, lib().mcv_ba_create(self.ctx.h, n_obs, n_points, n_frames, n_cams, ctypes.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_problem(cls, prob, ctx: Context | None = None) -> "BALinearizer":
+        b = cls(len(prob.obs_pt), len(prob.points), len(prob.frames), len(prob.cams), ctx)
+        b.set_observations(prob.obs_uv, prob.obs_pt, prob.obs_frame, prob.obs_cam, prob.obs_isig)
+        b.set_state(prob.points, prob.frames, prob.cams)
+        return b
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mcv_ba_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_observations(self, uv, pt, frame, cam, isig):
+        a = [np.ascontiguousarray(uv, np.float32), np.ascontiguousarray(pt, np.int32),
+             np.ascontiguousarray(frame, np.int32), np.ascontiguousarray(cam, np.int32),
+             np.ascontiguousarray(isig, np.float32)]
+        check(self.ctx.h, lib().mcv_ba_set_observations(self.h, *[_p(x) for x in a]))
+
+    def set_state(self, points, frames, cams):
+        a = [np.ascontiguousarray(points, np.float64), np.ascontiguousarray(frames, np.float64),
+             np.ascontiguousarray(cams, np.float64)]
+        check(self.ctx.h, lib().mcv_ba_set_state(self.h, *[_p(x) for x in a]))
+
+    def linearize(self, residual_only: bool = False):
+        check(self.ctx.h, lib().mcv_ba_linearize(self.h, 1 if residual_only else 0))
+
+    def planes_ptr(self):
+        p, s = ctypes.c_void_p(), ctypes.c_int64()
+        check(self.ctx.h, lib().mcv_ba_planes(self.h, ctypes.byref(p), ctypes.byref(s)))
+        return p.value, s.value
+
+    def read(self, first: int = 0, count: int = N_PLANES) -> np.ndarray:
+        out = np.zeros((count, self.M), np.float32)
+        check(self.ctx.h, lib().mcv_ba_read_planes(self.h, first, count, _p(out)))
+        return out
+
+    def results(self):
+        """(e M x 2, chi2 M, w M, J_pose M x 2 x 6, J_point M x 2 x 3)"""
+        pl = self.read()
+        e = np.stack([pl[0], pl[1]], 1)
+        return e, pl[2], pl[3], pl[4:16].T.reshape(-1, 2, 6), pl[16:22].T.reshape(-1, 2, 3)
+
+    def adjoint_dev(self, g_ptr: int, gpose_ptr: int, gpoint_ptr: int):
+        check(self.ctx.h, lib().mcv_ba_adjoint_dev(self.h, ctypes.c_void_p(g_ptr), ctypes.c_void_p(gpose_ptr),
+                                                   ctypes.c_void_p(gpoint_ptr)))

@@ -1,0 +1,1347 @@
+// Host side of the C ABI declared in include/mcvslam_gpu.h: contexts, the
+// ORB batch planner, the libm sin/cos calibration table, graph capture, and
+// the launchers of every kernel.  Kernels live in the included .hip files.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <fstream>
+#include <gnu/libc-version.h>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include "../../include/mcvslam_gpu.h"
+#include "mcv_ba.hip"
+#include "mcv_match.hip"
+#include "mcv_orb.hip"
+#include "mcv_window.hip"
+
+using namespace mcv;
+
+namespace mcv {
+// clears counters and flags; used instead of hipMemsetAsync so that the
+// sequence stays safe to capture into a graph
+__global__ void k_zero_words(uint32_t* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+}  // namespace mcv
+
+namespace {
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+#define HIP_OK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) throw Fail{MCV_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+#define REQUIRE(cond, code, msg)                   \
+    do {                                           \
+        if (!(cond)) throw Fail{(code), (msg)};    \
+    } while (0)
+
+// Grow-only device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t n) {
+        if (n <= bytes) return;
+        if (p) HIP_OK(hipFree(p));
+        p = nullptr;
+        bytes = 0;
+        if (n == 0) return;
+        HIP_OK(hipMalloc(&p, n));
+        bytes = n;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+template <typename T>
+void upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+    b.ensure(std::max<size_t>(v.size() * sizeof(T), 16));
+    if (!v.empty()) HIP_OK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+// ---------------------------------------------------------------------------
+// cosf / sinf calibration.  The descriptor rotation uses the host libm's
+// single-precision cosf/sinf (ORBextractor.cc:102-103).  The device computes
+// sincos_base(); this table lists every float input in [0, 6.3] where that
+// differs from the host libm, with the libm value.  Built once per process
+// with all host threads and cached on disk keyed by the libm fingerprint.
+struct SinCosHost {
+    std::vector<uint32_t> ck, sk, cidx, sidx;
+    std::vector<float> cv, sv;
+    uint32_t max_bits = 0;
+};
+
+static uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+    const uint8_t* b = static_cast<const uint8_t*>(p);
+    for (size_t i = 0; i < n; i++) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+static uint64_t libm_fingerprint() {
+    uint64_t h = 1469598103934665603ull;
+    const char* v = gnu_get_libc_version();
+    h = fnv1a(h, v, strlen(v));
+    for (uint32_t u = 0; u < 0x40C9999Au; u += 0x3FFF) {
+        float x;
+        memcpy(&x, &u, 4);
+        const float c = cosf(x), s = sinf(x);
+        h = fnv1a(h, &c, 4);
+        h = fnv1a(h, &s, 4);
+    }
+    return h;
+}
+
+static void build_index(const std::vector<uint32_t>& keys, std::vector<uint32_t>& idx) {
+    idx.assign(65537, 0);
+    size_t p = 0;
+    for (uint32_t h = 0; h <= 65536; h++) {
+        while (p < keys.size() && (keys[p] >> 16) < h) p++;
+        idx[h] = (uint32_t)p;
+    }
+}
+
+static std::string cache_path(uint64_t fp) {
+    const char* dir = getenv("MCV_CACHE_DIR");
+    std::string d;
+    if (dir && *dir) {
+        d = dir;
+    } else {
+        const char* home = getenv("HOME");
+        d = std::string(home && *home ? home : "/tmp") + "/.cache/mcvslam";
+    }
+    char name[64];
+    snprintf(name, sizeof name, "/sincos_v1_%016llx.bin", (unsigned long long)fp);
+    return d + name;
+}
+
+static bool load_cache(const std::string& path, SinCosHost& t) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    uint64_t nc = 0, ns = 0;
+    f.read(reinterpret_cast<char*>(&t.max_bits), 4);
+    f.read(reinterpret_cast<char*>(&nc), 8);
+    f.read(reinterpret_cast<char*>(&ns), 8);
+    if (!f || nc > (1u << 26) || ns > (1u << 26)) return false;
+    t.ck.resize(nc);
+    t.cv.resize(nc);
+    t.sk.resize(ns);
+    t.sv.resize(ns);
+    f.read(reinterpret_cast<char*>(t.ck.data()), nc * 4);
+    f.read(reinterpret_cast<char*>(t.cv.data()), nc * 4);
+    f.read(reinterpret_cast<char*>(t.sk.data()), ns * 4);
+    f.read(reinterpret_cast<char*>(t.sv.data()), ns * 4);
+    return (bool)f;
+}
+
+static void save_cache(const std::string& path, const SinCosHost& t) {
+    std::string dir = path.substr(0, path.rfind('/'));
+    std::string cmd_dir = dir;
+    // create the directory chain without a shell
+    for (size_t i = 1; i <= cmd_dir.size(); i++)
+        if (i == cmd_dir.size() || cmd_dir[i] == '/') {
+            std::string sub = cmd_dir.substr(0, i);
+            (void)mkdir(sub.c_str(), 0755);
+        }
+    std::string tmp = path + ".tmp" + std::to_string(getpid());
+    std::ofstream f(tmp, std::ios::binary);
+    if (!f) return;
+    uint64_t nc = t.ck.size(), ns = t.sk.size();
+    f.write(reinterpret_cast<const char*>(&t.max_bits), 4);
+    f.write(reinterpret_cast<const char*>(&nc), 8);
+    f.write(reinterpret_cast<const char*>(&ns), 8);
+    f.write(reinterpret_cast<const char*>(t.ck.data()), nc * 4);
+    f.write(reinterpret_cast<const char*>(t.cv.data()), nc * 4);
+    f.write(reinterpret_cast<const char*>(t.sk.data()), ns * 4);
+    f.write(reinterpret_cast<const char*>(t.sv.data()), ns * 4);
+    f.close();
+    if (f) (void)rename(tmp.c_str(), path.c_str());
+}
+
+static const SinCosHost& sincos_host() {
+    static std::once_flag once;
+    static SinCosHost t;
+    std::call_once(once, [] {
+        const float xmax = 6.3f;
+        memcpy(&t.max_bits, &xmax, 4);
+        const uint64_t fp = libm_fingerprint();
+        const std::string path = cache_path(fp);
+        SinCosHost c;
+        if (load_cache(path, c) && c.max_bits == t.max_bits) {
+            t.ck.swap(c.ck); t.cv.swap(c.cv); t.sk.swap(c.sk); t.sv.swap(c.sv);
+        } else {
+            unsigned nt = std::thread::hardware_concurrency();
+            if (const char* e = getenv("MCV_SINCOS_THREADS")) nt = (unsigned)atoi(e);
+            nt = std::max(1u, std::min(nt ? nt : 1u, 16u));
+            const uint32_t nchunk = 256;
+            std::vector<std::vector<uint32_t>> ck(nchunk), sk(nchunk);
+            std::vector<std::vector<float>> cvv(nchunk), svv(nchunk);
+            std::atomic<uint32_t> next{0};
+            const uint64_t total = (uint64_t)t.max_bits + 1;
+            auto work = [&] {
+                for (;;) {
+                    const uint32_t ci = next.fetch_add(1);
+                    if (ci >= nchunk) return;
+                    const uint64_t a = total * ci / nchunk, b = total * (ci + 1) / nchunk;
+                    for (uint64_t u = a; u < b; u++) {
+                        const uint32_t bits = (uint32_t)u;
+                        float x;
+                        memcpy(&x, &bits, 4);
+                        float bc, bs;
+                        sincos_base(x, &bc, &bs);
+                        const float lc = cosf(x), ls = sinf(x);
+                        if (memcmp(&bc, &lc, 4) != 0) { ck[ci].push_back(bits); cvv[ci].push_back(lc); }
+                        if (memcmp(&bs, &ls, 4) != 0) { sk[ci].push_back(bits); svv[ci].push_back(ls); }
+                    }
+                }
+            };
+            std::vector<std::thread> th;
+            for (unsigned i = 0; i < nt; i++) th.emplace_back(work);
+            for (auto& x : th) x.join();
+            for (uint32_t i = 0; i < nchunk; i++) {
+                t.ck.insert(t.ck.end(), ck[i].begin(), ck[i].end());
+                t.cv.insert(t.cv.end(), cvv[i].begin(), cvv[i].end());
+                t.sk.insert(t.sk.end(), sk[i].begin(), sk[i].end());
+                t.sv.insert(t.sv.end(), svv[i].begin(), svv[i].end());
+            }
+            save_cache(path, t);
+        }
+        build_index(t.ck, t.cidx);
+        build_index(t.sk, t.sidx);
+    });
+    return t;
+}
+
+struct SinCosDev {
+    DevBuf ck, cv, cidx, sk, sv, sidx;
+    bool ready = false;
+    SinCosTables view() const {
+        SinCosTables v;
+        v.cos_key = ck.as<uint32_t>();
+        v.cos_val = cv.as<float>();
+        v.cos_idx = cidx.as<uint32_t>();
+        v.sin_key = sk.as<uint32_t>();
+        v.sin_val = sv.as<float>();
+        v.sin_idx = sidx.as<uint32_t>();
+        v.max_bits = sincos_host().max_bits;
+        return v;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// ORBextractor::init tables (ORBextractor.cc:407-457).
+struct OrbConfig {
+    int nfeatures = 0, nlevels = 0, ini = 0, mn = 0;
+    float scale_factor = 1.f;
+    std::vector<float> scale, inv_scale, sigma2, inv_sigma2;
+    std::vector<int> npl;
+    int umax[16];
+    int gauss[7];
+};
+
+static void gauss_taps(int* taps) {
+    // GaussianBlur(Size(7,7), 2, 2) on CV_8U: normalised Gaussian, error-
+    // diffused to Q8, centre tap takes the remainder (ORBextractor.cc:875)
+    const int n = 7;
+    const double sigma = 2.0;
+    const double scale2X = -0.125 / (sigma * sigma);
+    const int n2 = (n - 1) / 2;
+    double values[4], sum = 0;
+    for (int i = 0, x = 1 - n; i < n2; i++, x += 2) {
+        values[i] = std::exp((double)(x * x) * scale2X);
+        sum += values[i];
+    }
+    sum = sum * 2 + 1.0;
+    const double mul1 = 1.0 / sum;
+    double err = 0;
+    int s = 0;
+    for (int i = 0; i < n2; i++) {
+        const double adj = values[i] * mul1 * 256.0 + err;
+        const int v0 = (int)lrint(adj);
+        err = adj - v0;
+        taps[i] = taps[n - 1 - i] = v0;
+        s += v0;
+    }
+    taps[n2] = 256 - 2 * s;
+}
+
+static void orb_init(OrbConfig& c, int nf, float sf, int nl, int ini, int mn) {
+    c.nfeatures = nf; c.scale_factor = sf; c.nlevels = nl; c.ini = ini; c.mn = mn;
+    c.scale.assign(nl, 0.f); c.sigma2.assign(nl, 0.f); c.inv_scale.assign(nl, 0.f); c.inv_sigma2.assign(nl, 0.f);
+    c.scale[0] = 1.0f; c.sigma2[0] = 1.0f;
+    for (int i = 1; i < nl; i++) {
+        c.scale[i] = c.scale[i - 1] * sf;
+        c.sigma2[i] = c.scale[i] * c.scale[i];
+    }
+    for (int i = 0; i < nl; i++) {
+        c.inv_scale[i] = 1.0f / c.scale[i];
+        c.inv_sigma2[i] = 1.0f / c.sigma2[i];
+    }
+    c.npl.assign(nl, 0);
+    const float factor = 1.0f / sf;
+    float nd = nf * (1 - factor) / (1 - (float)std::pow((double)factor, (double)nl));
+    int sumF = 0;
+    for (int l = 0; l < nl - 1; l++) {
+        c.npl[l] = (int)lrintf(nd);
+        sumF += c.npl[l];
+        nd *= factor;
+    }
+    c.npl[nl - 1] = std::max(nf - sumF, 0);
+    int v, v0, vmax = (int)std::floor(15 * std::sqrt(2.f) / 2 + 1);
+    int vmin = (int)std::ceil(15 * std::sqrt(2.f) / 2);
+    const double hp2 = 15 * 15;
+    for (v = 0; v <= vmax; ++v) c.umax[v] = (int)lrint(std::sqrt(hp2 - v * v));
+    for (v = 15, v0 = 0; v >= vmin; --v) {
+        while (c.umax[v0] == c.umax[v0 + 1]) ++v0;
+        c.umax[v] = v0;
+        ++v0;
+    }
+    gauss_taps(c.gauss);
+}
+
+// resize coefficients for one axis (cv::resize INTER_LINEAR, fixed point)
+static void resize_axis(int src, int dst, std::vector<int2>& out) {
+    const double inv = (double)dst / src;
+    const double scale = 1. / inv;
+    for (int d = 0; d < dst; d++) {
+        float f = (float)((d + 0.5) * scale - 0.5);
+        int s = (int)std::floor(f);
+        f -= s;
+        if (s < 0) { f = 0; s = 0; }
+        if (s + 1 >= src && s >= src - 1) { f = 0; s = src - 1; }
+        const int a0 = std::min(32767, std::max(-32768, (int)lrintf((1.f - f) * 2048)));
+        const int a1 = std::min(32767, std::max(-32768, (int)lrintf(f * 2048)));
+        out.push_back(make_int2(s, (a0 & 0xFFFF) | (a1 << 16)));
+    }
+}
+
+// keypoint slots the octree of one level may fill: DistributeOctTree stops
+// below N + 3 nodes, or emits all roots when they alone reach N
+static int level_out_cap(int nfeat, int n_ini) { return std::max(nfeat + 3, n_ini + 1); }
+
+// octree root count of a level (ORBextractor.cc:527), 1 when it has no cells
+static int level_roots(int w, int h) {
+    const float width = (float)(w - 32), height = (float)(h - 32);
+    const int nCols = (int)(width / 35.f), nRows = (int)(height / 35.f);
+    if (!(nCols > 0 && nRows > 0 && width > 0 && height > 0)) return 1;
+    return (int)std::round(width / height);
+}
+
+struct ImageSpec {
+    int w, h;
+    std::vector<std::vector<mcv_keypoint>> seeds;  // per level, in input order
+};
+
+// A batch plan: geometry and work tables for a list of images.
+struct OrbPlan {
+    int n_images = 0, nlevels = 0, cap = 0, n_slots = 0;
+    std::vector<LevelDesc> lv;
+    std::vector<CellDesc> cells;
+    std::vector<Tile> tiles;
+    std::vector<int> tile_begin;  // per level index l: [tile_begin[l], tile_begin[l+1])
+    std::vector<int2> coef;
+    std::vector<int> slot_lvl;
+    std::vector<float> seeds;
+    long long pyr_bytes = 0, pool_cap = 0;
+    int n_cand = 0, max_slots_img = 0;
+    DevBuf d_lv, d_cells, d_tiles, d_coef, d_slot, d_seeds, d_pyr, d_blur, d_cell_cand, d_cell_n, d_lvl_cand, d_oct,
+        d_pool, d_used, d_nodes, d_src, d_stride;
+    hipGraphExec_t graph = nullptr;
+    const void* graph_key = nullptr;
+
+    ~OrbPlan() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+    }
+
+    void build(const OrbConfig& c, const std::vector<ImageSpec>& imgs, long long pool_words) {
+        n_images = (int)imgs.size();
+        nlevels = c.nlevels;
+        lv.clear(); cells.clear(); tiles.clear(); coef.clear(); slot_lvl.clear(); seeds.clear();
+        pyr_bytes = 0;
+        n_cand = 0;
+        int slot = 0;
+        std::vector<std::vector<int>> lvl_of(n_images, std::vector<int>(nlevels));
+        for (int i = 0; i < n_images; i++) {
+            const ImageSpec& im = imgs[i];
+            REQUIRE(im.w > 0 && im.h > 0, MCV_ERR_EMPTY, "empty image");
+            REQUIRE(im.w <= 4096 && im.h <= 4096, MCV_ERR_INVALID, "image larger than 4096 px");
+            int first = (int)lv.size();
+            int img_slots = 0;
+            for (int l = 0; l < nlevels; l++) {
+                LevelDesc d{};
+                d.img = i;
+                d.level = l;
+                d.w = l == 0 ? im.w : (int)lrintf((float)im.w * c.inv_scale[l]);
+                d.h = l == 0 ? im.h : (int)lrintf((float)im.h * c.inv_scale[l]);
+                REQUIRE(d.w >= 8 && d.h >= 8, MCV_ERR_INVALID, "pyramid level smaller than 8 px");
+                d.src = l == 0 ? -1 : (int)lv.size() - 1;
+                d.nfeat = c.npl[l];
+                d.pix = pyr_bytes;
+                pyr_bytes += ((long long)d.w * d.h + 255) / 256 * 256;
+                d.coef_x = d.coef_y = 0;
+                if (l > 0) {
+                    const LevelDesc& s = lv[d.src];
+                    d.coef_x = (int)coef.size();
+                    resize_axis(s.w, d.w, coef);
+                    d.coef_y = (int)coef.size();
+                    resize_axis(s.h, d.h, coef);
+                }
+                // ComputeKeyPointsOctTree cell grid (ORBextractor.cc:585-633)
+                d.minBX = 16; d.minBY = 16;
+                d.maxBX = d.w - 16; d.maxBY = d.h - 16;
+                const float width = (float)(d.maxBX - d.minBX), height = (float)(d.maxBY - d.minBY);
+                const int nCols = (int)(width / 35.f), nRows = (int)(height / 35.f);
+                d.cell_first = (int)cells.size();
+                d.cand_base = n_cand;
+                if (nCols > 0 && nRows > 0 && width > 0 && height > 0) {
+                    const int wCell = (int)std::ceil(width / nCols), hCell = (int)std::ceil(height / nRows);
+                    for (int r = 0; r < nRows; r++) {
+                        const float iniY = (float)(d.minBY + r * hCell);
+                        float maxY = iniY + hCell + 6;
+                        if (iniY >= d.maxBY - 3) continue;
+                        if (maxY > d.maxBY) maxY = (float)d.maxBY;
+                        for (int j = 0; j < nCols; j++) {
+                            const float iniX = (float)(d.minBX + j * wCell);
+                            float maxX = iniX + wCell + 6;
+                            if (iniX >= d.maxBX - 6) continue;
+                            if (maxX > d.maxBX) maxX = (float)d.maxBX;
+                            CellDesc cd{};
+                            cd.lvl = (int)lv.size();
+                            cd.x0 = (int)iniX; cd.y0 = (int)iniY; cd.x1 = (int)maxX; cd.y1 = (int)maxY;
+                            cd.offx = j * wCell; cd.offy = r * hCell;
+                            const int W = cd.x1 - cd.x0, H = cd.y1 - cd.y0;
+                            REQUIRE(W <= kMaxWin && H <= kMaxWin, MCV_ERR_INVALID, "FAST cell window too large");
+                            cd.cap_base = n_cand;
+                            const int cap = (W >= 7 && H >= 7) ? ((W - 6 + 1) / 2) * ((H - 6 + 1) / 2) : 0;
+                            n_cand += cap;
+                            cells.push_back(cd);
+                        }
+                    }
+                    d.n_ini = (int)std::round(width / (float)(d.maxBY - d.minBY));
+                    REQUIRE(d.n_ini >= 1 && d.n_ini <= 15, MCV_ERR_INVALID, "image aspect ratio outside the octree root range");
+                    d.hX = width / (float)d.n_ini;
+                } else {
+                    d.n_ini = 1;
+                    d.hX = 1.f;
+                }
+                d.cell_count = (int)cells.size() - d.cell_first;
+                d.cand_cap = n_cand - d.cand_base;
+                REQUIRE(d.nfeat <= 4000, MCV_ERR_INVALID, "more than 4000 features on one level");
+                d.out_cap = level_out_cap(d.nfeat, d.n_ini);
+                d.seed_first = (int)(seeds.size() / 6);
+                d.seed_count = (int)im.seeds[l].size();
+                for (const mcv_keypoint& k : im.seeds[l]) {
+                    seeds.insert(seeds.end(), {k.x, k.y, k.size, k.angle, k.response, (float)k.octave});
+                }
+                d.kp_slot = slot;
+                for (int j = 0; j < d.out_cap + d.seed_count; j++) {
+                    slot_lvl.push_back((int)lv.size());
+                    slot_lvl.push_back(j);
+                }
+                slot += d.out_cap + d.seed_count;
+                img_slots += d.out_cap + d.seed_count;
+                d.scale = c.scale[l];
+                d.patch = (float)(int)(31 * c.scale[l]);
+                d.first_level = first;
+                d.nlevels = nlevels;
+                lvl_of[i][l] = (int)lv.size();
+                lv.push_back(d);
+            }
+            max_slots_img = std::max(max_slots_img, img_slots);
+        }
+        n_slots = slot;
+        cap = max_slots_img;
+        // tiles grouped by level index, then image
+        tile_begin.assign(nlevels + 1, 0);
+        for (int l = 0; l < nlevels; l++) {
+            tile_begin[l] = (int)tiles.size();
+            for (int i = 0; i < n_images; i++) {
+                const LevelDesc& d = lv[lvl_of[i][l]];
+                for (int y = 0; y < d.h; y += TH)
+                    for (int x = 0; x < d.w; x += TW) tiles.push_back(Tile{lvl_of[i][l], x, y});
+            }
+        }
+        tile_begin[nlevels] = (int)tiles.size();
+        pool_cap = pool_words;
+    }
+
+    void upload_all(hipStream_t s) {
+        upload(d_lv, lv, s);
+        upload(d_cells, cells, s);
+        upload(d_tiles, tiles, s);
+        upload(d_coef, coef, s);
+        upload(d_slot, slot_lvl, s);
+        upload(d_seeds, seeds, s);
+        d_pyr.ensure(pyr_bytes + 256);
+        d_blur.ensure(pyr_bytes + 256);
+        d_cell_cand.ensure((size_t)std::max(n_cand, 1) * 4);
+        d_cell_n.ensure((size_t)std::max<size_t>(cells.size(), 1) * 4);
+        d_lvl_cand.ensure((size_t)std::max(n_cand, 1) * 4);
+        d_oct.ensure(lv.size() * sizeof(OctLevel));
+        d_pool.ensure((size_t)pool_cap * 4);
+        d_used.ensure(64);
+        d_nodes.ensure((size_t)std::max(n_slots, 1) * 4);
+        d_src.ensure((size_t)n_images * sizeof(void*));
+        d_stride.ensure((size_t)n_images * sizeof(int));
+    }
+
+    OrbGlobals globals(const OrbConfig& c) const {
+        OrbGlobals g{};
+        g.lv = d_lv.as<LevelDesc>();
+        g.cells = d_cells.as<CellDesc>();
+        g.pyr = d_pyr.as<uint8_t>();
+        g.blur = d_blur.as<uint8_t>();
+        g.cell_cand = d_cell_cand.as<uint32_t>();
+        g.cell_n = d_cell_n.as<int>();
+        g.lvl_cand = d_lvl_cand.as<uint32_t>();
+        g.oct = d_oct.as<OctLevel>();
+        g.pool = d_pool.as<uint32_t>();
+        g.pool_used = d_used.as<unsigned long long>();
+        g.pool_cap = pool_cap;
+        g.nodes_out = d_nodes.as<uint32_t>();
+        g.seeds = d_seeds.as<float>();
+        for (int i = 0; i < 16; i++) g.umax[i] = c.umax[i];
+        g.ini_th = std::min(std::max(c.ini, 0), 255);
+        g.min_th = std::min(std::max(c.mn, 0), 255);
+        g.err = reinterpret_cast<int*>(d_used.as<char>() + 8);
+        return g;
+    }
+
+    // The whole extraction for the batch, enqueued on s.
+    void enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s, mcv_keypoint* d_kps, uint8_t* d_desc,
+                 int out_cap, int* d_counts) {
+        const OrbGlobals g = globals(c);
+        hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(d_used.p), 4);
+        const Tile* t = d_tiles.as<Tile>();
+        int n0 = tile_begin[1] - tile_begin[0];
+        if (n0 > 0)
+            hipLaunchKernelGGL(k_copy_level0, dim3(n0), dim3(256), 0, s, t + tile_begin[0], g.lv,
+                               d_src.as<const uint8_t*>(), d_stride.as<int>(), g.pyr);
+        for (int l = 1; l < nlevels; l++) {
+            const int n = tile_begin[l + 1] - tile_begin[l];
+            if (n > 0) hipLaunchKernelGGL(k_resize, dim3(n), dim3(256), 0, s, t + tile_begin[l], g.lv, d_coef.as<int2>(), g.pyr);
+        }
+        BlurTaps taps;
+        for (int k = 0; k < 7; k++) taps.t[k] = c.gauss[k];
+        if (!tiles.empty()) hipLaunchKernelGGL(k_blur, dim3(tiles.size()), dim3(256), 0, s, t, g.lv, g.pyr, g.blur, taps);
+        if (!cells.empty()) hipLaunchKernelGGL(k_fast_cells, dim3(cells.size()), dim3(256), 0, s, g);
+        hipLaunchKernelGGL(k_octree_build, dim3(lv.size()), dim3(kOctThreads), 0, s, g);
+        hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), 0, s, g);
+        hipLaunchKernelGGL(k_keypoints, dim3((n_slots + 3) / 4), dim3(256), 0, s, g, d_slot.as<int>(), n_slots, tb,
+                           reinterpret_cast<float*>(d_kps), d_desc, out_cap, d_counts);
+        HIP_OK(hipGetLastError());
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct mcv_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    OrbConfig orb;
+    bool orb_ready = false;
+    SinCosDev sc;
+    std::unique_ptr<OrbPlan> single;  // plan of the last single-image extract
+    int single_key_w = -1, single_key_h = -1;
+    std::vector<int> single_key_seeds;
+    DevBuf in_img, out_kps, out_desc, out_cnt;
+    DevBuf m_q, m_t, m_out, m_pairs, f_knn, f_out, f_cnt, f_scr, f_jobs;
+    DevBuf w_pts, w_pdesc, w_plev, w_kxy, w_kdesc, w_cs, w_ci, w_out, w_view, w_cams;
+    std::map<std::string, std::unique_ptr<OrbPlan>> plans;
+
+    void set_device() { HIP_OK(hipSetDevice(device)); }
+
+    const SinCosTables tables() {
+        if (!sc.ready) {
+            const SinCosHost& h = sincos_host();
+            upload(sc.ck, h.ck, stream);
+            upload(sc.cv, h.cv, stream);
+            upload(sc.cidx, h.cidx, stream);
+            upload(sc.sk, h.sk, stream);
+            upload(sc.sv, h.sv, stream);
+            upload(sc.sidx, h.sidx, stream);
+            HIP_OK(hipStreamSynchronize(stream));
+            sc.ready = true;
+        }
+        return sc.view();
+    }
+};
+
+struct mcv_rig {
+    mcv_ctx* ctx;
+    int n_frames;
+    int w[3], h[3];
+    float ratio;
+    int th;
+    std::unique_ptr<OrbPlan> batch;                 // all 3n images
+    std::vector<std::unique_ptr<OrbPlan>> single;   // per camera, one image
+    int cap = 0;
+    DevBuf kps, desc, counts, knn, matches, mcounts, pairs, jobs, scratch;
+    std::vector<MatchPair> hp;
+    std::vector<FilterJob> hj;
+    hipGraphExec_t graph = nullptr;
+    std::vector<const uint8_t*> graph_inputs;
+    ~mcv_rig() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+    }
+};
+
+struct mcv_ba {
+    mcv_ctx* ctx;
+    int M, P, F, C;
+    DevBuf uv, pt, pair, isig, phi, plo, pairs, planes, fptr, fobs, pptr;
+    std::vector<int> h_pair;
+    bool have_obs = false, have_state = false, linearized = false;
+};
+
+static int fail(mcv_ctx* ctx, const Fail& f) {
+    if (ctx) ctx->err = f.msg;
+    return f.code;
+}
+
+#define API_BEGIN(ctxp) \
+    mcv_ctx* api_ctx_ = (ctxp); \
+    try {
+#define API_END                                                   \
+    }                                                             \
+    catch (const Fail& f) { return fail(api_ctx_, f); }           \
+    catch (const std::exception& e) { return fail(api_ctx_, Fail{MCV_ERR_STATE, e.what()}); } \
+    return MCV_OK;
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int mcv_create(mcv_ctx** out, int device) {
+    if (!out) return MCV_ERR_INVALID;
+    *out = nullptr;
+    auto* c = new mcv_ctx();
+    try {
+        int n = 0;
+        HIP_OK(hipGetDeviceCount(&n));
+        REQUIRE(device >= 0 && device < n, MCV_ERR_INVALID, "no such device");
+        c->device = device;
+        c->set_device();
+        HIP_OK(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+        c->stream = c->own;
+    } catch (const Fail& f) {
+        int code = f.code;
+        delete c;
+        return code;
+    }
+    *out = c;
+    return MCV_OK;
+}
+
+void mcv_destroy(mcv_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->own) (void)hipStreamSynchronize(c->own);
+    c->plans.clear();
+    c->single.reset();
+    if (c->own) (void)hipStreamDestroy(c->own);
+    delete c;
+}
+
+const char* mcv_last_error(const mcv_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mcv_set_stream(mcv_ctx* c, void* s) {
+    API_BEGIN(c)
+    REQUIRE(c, MCV_ERR_INVALID, "null context");
+    c->stream = s ? static_cast<hipStream_t>(s) : c->own;
+    API_END
+}
+
+int mcv_synchronize(mcv_ctx* c) {
+    API_BEGIN(c)
+    REQUIRE(c, MCV_ERR_INVALID, "null context");
+    c->set_device();
+    HIP_OK(hipStreamSynchronize(c->stream));
+    API_END
+}
+
+int mcv_copy_to_host(mcv_ctx* c, void* dst, const void* src, uint64_t bytes) {
+    API_BEGIN(c)
+    REQUIRE(c && (bytes == 0 || (dst && src)), MCV_ERR_INVALID, "null copy buffer");
+    c->set_device();
+    if (bytes) HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    API_END
+}
+
+int mcv_orb_configure(mcv_ctx* c, int nf, float sf, int nl, int ini, int mn) {
+    API_BEGIN(c)
+    REQUIRE(c, MCV_ERR_INVALID, "null context");
+    REQUIRE(nf > 0 && nl >= 1 && nl <= kMaxLevels && sf > 1.f, MCV_ERR_INVALID, "invalid ORB parameters");
+    orb_init(c->orb, nf, sf, nl, ini, mn);
+    c->orb_ready = true;
+    c->single.reset();
+    c->plans.clear();
+    API_END
+}
+
+int mcv_orb_tables(const mcv_ctx* c, float* scale, float* inv_scale, float* sigma2, float* inv_sigma2, int32_t* npl) {
+    if (!c || !c->orb_ready) return MCV_ERR_STATE;
+    for (int i = 0; i < c->orb.nlevels; i++) {
+        if (scale) scale[i] = c->orb.scale[i];
+        if (inv_scale) inv_scale[i] = c->orb.inv_scale[i];
+        if (sigma2) sigma2[i] = c->orb.sigma2[i];
+        if (inv_sigma2) inv_sigma2[i] = c->orb.inv_sigma2[i];
+        if (npl) npl[i] = c->orb.npl[i];
+    }
+    return MCV_OK;
+}
+
+int mcv_orb_capacity(const mcv_ctx* c, int w, int h, int n_seeds, int32_t* cap) {
+    if (!c || !c->orb_ready || !cap) return MCV_ERR_STATE;
+    int total = n_seeds;
+    for (int l = 0; l < c->orb.nlevels; l++) {
+        const int lw = l == 0 ? w : (int)lrintf((float)w * c->orb.inv_scale[l]);
+        const int lh = l == 0 ? h : (int)lrintf((float)h * c->orb.inv_scale[l]);
+        total += level_out_cap(c->orb.npl[l], level_roots(lw, lh));
+    }
+    *cap = total;
+    return MCV_OK;
+}
+
+static long long pool_words_for(int n_images) { return 4ll * 1024 * 1024 + (long long)n_images * 1024 * 1024; }
+
+int mcv_orb_extract(mcv_ctx* c, const uint8_t* image, int w, int h, int stride, const mcv_keypoint* seeds, int n_seeds,
+                    mcv_keypoint* kps, uint8_t* desc, int capacity, int32_t* count) {
+    API_BEGIN(c)
+    REQUIRE(c && c->orb_ready, MCV_ERR_STATE, "ORB not configured");
+    REQUIRE(count, MCV_ERR_INVALID, "null count");
+    *count = 0;
+    if (!image || w <= 0 || h <= 0) throw Fail{MCV_ERR_EMPTY, "empty image"};
+    REQUIRE(stride >= w, MCV_ERR_INVALID, "stride < width");
+    REQUIRE(n_seeds >= 0 && (n_seeds == 0 || seeds), MCV_ERR_INVALID, "bad seeds");
+    c->set_device();
+    ImageSpec spec{w, h, std::vector<std::vector<mcv_keypoint>>(c->orb.nlevels)};
+    std::vector<int> seed_key(c->orb.nlevels, 0);
+    for (int i = 0; i < n_seeds; i++) {
+        REQUIRE(seeds[i].octave >= 0 && seeds[i].octave < c->orb.nlevels, MCV_ERR_INVALID, "seed octave out of range");
+        spec.seeds[seeds[i].octave].push_back(seeds[i]);
+        seed_key[seeds[i].octave]++;
+    }
+    if (!c->single || c->single_key_w != w || c->single_key_h != h || c->single_key_seeds != seed_key || n_seeds > 0) {
+        c->single.reset(new OrbPlan());
+        c->single->build(c->orb, {spec}, pool_words_for(1));
+        c->single->upload_all(c->stream);
+        c->single_key_w = w;
+        c->single_key_h = h;
+        c->single_key_seeds = seed_key;
+    }
+    OrbPlan& P = *c->single;
+    const SinCosTables tb = c->tables();
+    c->in_img.ensure((size_t)w * h);
+    HIP_OK(hipMemcpy2DAsync(c->in_img.p, w, image, stride, w, h, hipMemcpyHostToDevice, c->stream));
+    const uint8_t* src = c->in_img.as<uint8_t>();
+    const int st = w;
+    HIP_OK(hipMemcpyAsync(P.d_src.p, &src, sizeof(src), hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(P.d_stride.p, &st, sizeof(st), hipMemcpyHostToDevice, c->stream));
+    c->out_kps.ensure((size_t)P.cap * sizeof(mcv_keypoint));
+    c->out_desc.ensure((size_t)P.cap * 32);
+    c->out_cnt.ensure(16);
+    P.enqueue(c->orb, tb, c->stream, c->out_kps.as<mcv_keypoint>(), c->out_desc.as<uint8_t>(), P.cap, c->out_cnt.as<int>());
+    int n = 0;
+    int err[2] = {0, 0};
+    HIP_OK(hipMemcpyAsync(&n, c->out_cnt.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(err, P.d_used.as<char>() + 8, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    REQUIRE(err[0] == 0, MCV_ERR_STATE,
+            err[0] & 2 ? "a pyramid level has more FAST candidates than the octree sort holds"
+                       : "octree node pool exhausted");
+    *count = n;
+    if (n > capacity) throw Fail{MCV_ERR_CAPACITY, "keypoint capacity too small"};
+    if (n > 0) {
+        if (kps) HIP_OK(hipMemcpy(kps, c->out_kps.p, (size_t)n * sizeof(mcv_keypoint), hipMemcpyDeviceToHost));
+        if (desc) HIP_OK(hipMemcpy(desc, c->out_desc.p, (size_t)n * 32, hipMemcpyDeviceToHost));
+    }
+    API_END
+}
+
+int mcv_orb_pyramid(mcv_ctx* c, int level, uint8_t* out, int capacity, int32_t* w, int32_t* h) {
+    API_BEGIN(c)
+    REQUIRE(c && c->single, MCV_ERR_STATE, "no extraction yet");
+    REQUIRE(level >= 0 && level < c->single->nlevels, MCV_ERR_INVALID, "level out of range");
+    const LevelDesc& d = c->single->lv[level];
+    if (w) *w = d.w;
+    if (h) *h = d.h;
+    if (out) {
+        REQUIRE(capacity >= d.w * d.h, MCV_ERR_CAPACITY, "pyramid buffer too small");
+        c->set_device();
+        HIP_OK(hipMemcpyAsync(out, c->single->d_pyr.as<uint8_t>() + d.pix, (size_t)d.w * d.h, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    }
+    API_END
+}
+
+int mcv_orb_extract_batch_dev(mcv_ctx* c, int n, const uint8_t* const* d_images, const int32_t* widths,
+                              const int32_t* heights, const int32_t* strides, mcv_keypoint* d_kps, uint8_t* d_desc,
+                              int capacity, int32_t* d_counts, int flags) {
+    API_BEGIN(c)
+    REQUIRE(c && c->orb_ready, MCV_ERR_STATE, "ORB not configured");
+    REQUIRE(n > 0 && d_images && widths && heights, MCV_ERR_INVALID, "bad batch");
+    c->set_device();
+    std::string key = std::to_string(n);
+    std::vector<ImageSpec> specs;
+    for (int i = 0; i < n; i++) {
+        specs.push_back(ImageSpec{widths[i], heights[i], std::vector<std::vector<mcv_keypoint>>(c->orb.nlevels)});
+        key += "," + std::to_string(widths[i]) + "x" + std::to_string(heights[i]);
+    }
+    auto& slot = c->plans[key];
+    if (!slot) {
+        slot.reset(new OrbPlan());
+        slot->build(c->orb, specs, pool_words_for(n));
+        slot->upload_all(c->stream);
+    }
+    OrbPlan& P = *slot;
+    REQUIRE(capacity >= P.cap, MCV_ERR_CAPACITY, "capacity below mcv_orb_capacity");
+    const SinCosTables tb = c->tables();
+    std::vector<int> st(n);
+    for (int i = 0; i < n; i++) st[i] = strides ? strides[i] : widths[i];
+    HIP_OK(hipMemcpyAsync(P.d_src.p, d_images, sizeof(void*) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(P.d_stride.p, st.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    if (flags & MCV_FLAG_GRAPH) {
+        const void* gk = d_kps;
+        if (!P.graph || P.graph_key != gk) {
+            if (P.graph) HIP_OK(hipGraphExecDestroy(P.graph));
+            P.graph = nullptr;
+            hipGraph_t gr;
+            HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            P.enqueue(c->orb, tb, c->stream, d_kps, d_desc, capacity, d_counts);
+            HIP_OK(hipStreamEndCapture(c->stream, &gr));
+            HIP_OK(hipGraphInstantiate(&P.graph, gr, nullptr, nullptr, 0));
+            HIP_OK(hipGraphDestroy(gr));
+            P.graph_key = gk;
+        }
+        HIP_OK(hipGraphLaunch(P.graph, c->stream));
+    } else {
+        P.enqueue(c->orb, tb, c->stream, d_kps, d_desc, capacity, d_counts);
+    }
+    API_END
+}
+
+// ---- matching ---------------------------------------------------------------
+static void launch_knn(hipStream_t s, DevBuf& pbuf, const std::vector<MatchPair>& pairs, int max_nq) {
+    if (pairs.empty() || max_nq <= 0) return;
+    upload(pbuf, pairs, s);
+    hipLaunchKernelGGL(k_knn2, dim3((max_nq + 63) / 64, pairs.size()), dim3(256), 0, s, pbuf.as<MatchPair>());
+    HIP_OK(hipGetLastError());
+}
+
+int mcv_match_knn2_dev(mcv_ctx* c, const uint8_t* d_q, int nq, const uint8_t* d_t, int nt, int32_t* d_out) {
+    API_BEGIN(c)
+    REQUIRE(c && nq >= 0 && nt >= 0 && nt < (1 << 22), MCV_ERR_INVALID, "bad sizes");
+    c->set_device();
+    MatchPair p{d_q, d_t, nullptr, nullptr, nq, nt, d_out};
+    launch_knn(c->stream, c->m_pairs, {p}, nq);
+    API_END
+}
+
+int mcv_match_knn2(mcv_ctx* c, const uint8_t* q, int nq, const uint8_t* t, int nt, int32_t* out) {
+    API_BEGIN(c)
+    REQUIRE(c && nq >= 0 && nt >= 0 && nt < (1 << 22), MCV_ERR_INVALID, "bad sizes");
+    REQUIRE(nq == 0 || (q && out), MCV_ERR_INVALID, "null buffer");
+    if (nq == 0) return MCV_OK;
+    c->set_device();
+    c->m_q.ensure((size_t)nq * 32);
+    c->m_t.ensure((size_t)std::max(nt, 1) * 32);
+    c->m_out.ensure((size_t)nq * 16);
+    HIP_OK(hipMemcpyAsync(c->m_q.p, q, (size_t)nq * 32, hipMemcpyHostToDevice, c->stream));
+    if (nt) HIP_OK(hipMemcpyAsync(c->m_t.p, t, (size_t)nt * 32, hipMemcpyHostToDevice, c->stream));
+    MatchPair p{c->m_q.as<uint8_t>(), c->m_t.as<uint8_t>(), nullptr, nullptr, nq, nt, c->m_out.as<int32_t>()};
+    launch_knn(c->stream, c->m_pairs, {p}, nq);
+    HIP_OK(hipMemcpyAsync(out, c->m_out.p, (size_t)nq * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    API_END
+}
+
+int mcv_match_filter(mcv_ctx* c, const int32_t* knn, int nq, float ratio, int th, int32_t* out, int32_t* count) {
+    API_BEGIN(c)
+    REQUIRE(c && count && nq >= 0, MCV_ERR_INVALID, "bad arguments");
+    *count = 0;
+    if (nq == 0) return MCV_OK;
+    REQUIRE(knn && out, MCV_ERR_INVALID, "null buffer");
+    c->set_device();
+    c->f_knn.ensure((size_t)nq * 16);
+    c->f_out.ensure((size_t)nq * 12);
+    c->f_cnt.ensure(16);
+    c->f_scr.ensure((size_t)nq * 8);
+    HIP_OK(hipMemcpyAsync(c->f_knn.p, knn, (size_t)nq * 16, hipMemcpyHostToDevice, c->stream));
+    FilterJob j{c->f_knn.as<int32_t>(), nullptr, nq, ratio, th, c->f_out.as<int32_t>(), c->f_cnt.as<int>(),
+                c->f_scr.as<int>()};
+    upload(c->f_jobs, std::vector<FilterJob>{j}, c->stream);
+    hipLaunchKernelGGL(k_filter, dim3(1), dim3(1024), 0, c->stream, c->f_jobs.as<FilterJob>());
+    HIP_OK(hipGetLastError());
+    int n = 0;
+    HIP_OK(hipMemcpyAsync(&n, c->f_cnt.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    if (n > 0) HIP_OK(hipMemcpy(out, c->f_out.p, (size_t)n * 12, hipMemcpyDeviceToHost));
+    *count = n;
+    API_END
+}
+
+// ---- rig frames -------------------------------------------------------------
+int mcv_rig_create(mcv_ctx* c, int n_frames, const int32_t* w3, const int32_t* h3, float ratio, int th, mcv_rig** out) {
+    API_BEGIN(c)
+    REQUIRE(c && c->orb_ready, MCV_ERR_STATE, "ORB not configured");
+    REQUIRE(out && n_frames > 0 && w3 && h3, MCV_ERR_INVALID, "bad rig arguments");
+    c->set_device();
+    std::unique_ptr<mcv_rig> r(new mcv_rig());
+    r->ctx = c;
+    r->n_frames = n_frames;
+    for (int k = 0; k < 3; k++) { r->w[k] = w3[k]; r->h[k] = h3[k]; }
+    r->ratio = ratio;
+    r->th = th;
+    std::vector<ImageSpec> specs;
+    for (int f = 0; f < n_frames; f++)
+        for (int k = 0; k < 3; k++)
+            specs.push_back(ImageSpec{w3[k], h3[k], std::vector<std::vector<mcv_keypoint>>(c->orb.nlevels)});
+    r->batch.reset(new OrbPlan());
+    r->batch->build(c->orb, specs, pool_words_for(3 * n_frames));
+    r->batch->upload_all(c->stream);
+    for (int k = 0; k < 3; k++) {
+        r->single.emplace_back(new OrbPlan());
+        r->single[k]->build(c->orb, {specs[k]}, pool_words_for(1));
+        r->single[k]->upload_all(c->stream);
+    }
+    r->cap = r->batch->cap;
+    const int n = 3 * n_frames;
+    r->kps.ensure((size_t)n * r->cap * sizeof(mcv_keypoint));
+    r->desc.ensure((size_t)n * r->cap * 32);
+    r->counts.ensure((size_t)n * 4);
+    r->knn.ensure((size_t)n * r->cap * 16);
+    r->matches.ensure((size_t)n * r->cap * 12);
+    r->mcounts.ensure((size_t)n * 4);
+    r->scratch.ensure((size_t)n * r->cap * 8);
+    // pairs per frame: L-R (query L), W-L (query W), W-R (query W)
+    const int qa[3] = {0, 2, 2}, ta[3] = {1, 0, 1};
+    for (int f = 0; f < n_frames; f++)
+        for (int k = 0; k < 3; k++) {
+            const int qi = 3 * f + qa[k], ti = 3 * f + ta[k], pi = 3 * f + k;
+            MatchPair p;
+            p.q = r->desc.as<uint8_t>() + (size_t)qi * r->cap * 32;
+            p.t = r->desc.as<uint8_t>() + (size_t)ti * r->cap * 32;
+            p.nq_dev = r->counts.as<int>() + qi;
+            p.nt_dev = r->counts.as<int>() + ti;
+            p.nq = r->cap;
+            p.nt = r->cap;
+            p.out = r->knn.as<int32_t>() + (size_t)pi * r->cap * 4;
+            r->hp.push_back(p);
+            FilterJob j;
+            j.knn = p.out;
+            j.nq_dev = p.nq_dev;
+            j.nq = r->cap;
+            j.ratio = ratio;
+            j.th = th;
+            j.out = r->matches.as<int32_t>() + (size_t)pi * r->cap * 3;
+            j.count = r->mcounts.as<int>() + pi;
+            j.scratch = r->scratch.as<int>() + (size_t)pi * r->cap * 2;
+            r->hj.push_back(j);
+        }
+    upload(r->pairs, r->hp, c->stream);
+    upload(r->jobs, r->hj, c->stream);
+    HIP_OK(hipStreamSynchronize(c->stream));
+    *out = r.release();
+    API_END
+}
+
+void mcv_rig_destroy(mcv_rig* r) { delete r; }
+
+static void rig_enqueue_batched(mcv_rig* r, const SinCosTables& tb, hipStream_t s) {
+    OrbPlan& P = *r->batch;
+    P.enqueue(r->ctx->orb, tb, s, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>());
+    const int np = (int)r->hp.size();
+    hipLaunchKernelGGL(k_knn2, dim3((r->cap + 63) / 64, np), dim3(256), 0, s, r->pairs.as<MatchPair>());
+    hipLaunchKernelGGL(k_filter, dim3(np), dim3(1024), 0, s, r->jobs.as<FilterJob>());
+    HIP_OK(hipGetLastError());
+}
+
+int mcv_rig_run(mcv_rig* r, const uint8_t* const* d_images, int flags) {
+    mcv_ctx* c = r ? r->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(r && d_images, MCV_ERR_INVALID, "bad rig run");
+    c->set_device();
+    const SinCosTables tb = c->tables();
+    const int n = 3 * r->n_frames;
+    hipStream_t s = c->stream;
+    if (flags & MCV_FLAG_GRAPH) {
+        OrbPlan& P = *r->batch;
+        std::vector<int> st(n);
+        for (int i = 0; i < n; i++) st[i] = r->w[i % 3];
+        HIP_OK(hipMemcpyAsync(P.d_src.p, d_images, sizeof(void*) * n, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(P.d_stride.p, st.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+        if (!r->graph) {
+            hipGraph_t gr;
+            HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            rig_enqueue_batched(r, tb, s);
+            HIP_OK(hipStreamEndCapture(s, &gr));
+            HIP_OK(hipGraphInstantiate(&r->graph, gr, nullptr, nullptr, 0));
+            HIP_OK(hipGraphDestroy(gr));
+        }
+        HIP_OK(hipGraphLaunch(r->graph, s));
+    } else {
+        // plain path: one rig frame at a time, every image on its own launch
+        // sequence (one single-image plan per camera), then the three pairs
+        for (int f = 0; f < r->n_frames; f++) {
+            for (int k = 0; k < 3; k++) {
+                const int i = 3 * f + k;
+                OrbPlan& P = *r->single[k];
+                HIP_OK(hipMemcpyAsync(P.d_src.p, &d_images[i], sizeof(void*), hipMemcpyHostToDevice, s));
+                HIP_OK(hipMemcpyAsync(P.d_stride.p, &r->w[k], sizeof(int), hipMemcpyHostToDevice, s));
+                P.enqueue(c->orb, tb, s, r->kps.as<mcv_keypoint>() + (size_t)i * r->cap,
+                          r->desc.as<uint8_t>() + (size_t)i * r->cap * 32, r->cap, r->counts.as<int>() + i);
+            }
+            for (int k = 0; k < 3; k++) {
+                const int p = 3 * f + k;
+                hipLaunchKernelGGL(k_knn2, dim3((r->cap + 63) / 64, 1), dim3(256), 0, s, r->pairs.as<MatchPair>() + p);
+                hipLaunchKernelGGL(k_filter, dim3(1), dim3(1024), 0, s, r->jobs.as<FilterJob>() + p);
+            }
+            // the single-image plans are reused by the next frame
+            HIP_OK(hipStreamSynchronize(s));
+        }
+        HIP_OK(hipGetLastError());
+    }
+    API_END
+}
+
+int mcv_rig_outputs(mcv_rig* r, mcv_keypoint** kps, uint8_t** desc, int32_t** counts, int32_t** matches,
+                    int32_t** mcounts, int32_t** knn, int32_t* cap) {
+    if (!r) return MCV_ERR_INVALID;
+    if (kps) *kps = r->kps.as<mcv_keypoint>();
+    if (desc) *desc = r->desc.as<uint8_t>();
+    if (counts) *counts = r->counts.as<int32_t>();
+    if (matches) *matches = r->matches.as<int32_t>();
+    if (mcounts) *mcounts = r->mcounts.as<int32_t>();
+    if (knn) *knn = r->knn.as<int32_t>();
+    if (cap) *cap = r->cap;
+    return MCV_OK;
+}
+
+// ---- window search ----------------------------------------------------------
+static void window_run(mcv_ctx* c, int n_cams, const float* d_pts, const uint8_t* d_pdesc, const int32_t* d_plev,
+                       int P, const float* Rt, const float* cams, const int32_t* sizes, const float* scales, int nl,
+                       float radius, float ratio, int th, const float* const* d_kxy, const uint8_t* const* d_kdesc,
+                       const int32_t* n_kps, int32_t* d_out) {
+    REQUIRE(nl >= 1 && nl <= 16, MCV_ERR_INVALID, "nlevels out of range");
+    hipStream_t s = c->stream;
+    int total_k = 0;
+    for (int k = 0; k < n_cams; k++) {
+        REQUIRE(n_kps[k] >= 0 && n_kps[k] <= kGridSortCap && n_kps[k] < 65536, MCV_ERR_INVALID, "too many keypoints");
+        total_k += n_kps[k];
+    }
+    c->w_cs.ensure((size_t)n_cams * (kGridCells + 1) * 4);
+    c->w_ci.ensure((size_t)std::max(total_k, 1) * 4);
+    std::vector<GridView> views(n_cams);
+    std::vector<WindowCam> wc(n_cams);
+    int off = 0;
+    for (int k = 0; k < n_cams; k++) {
+        GridView& g = views[k];
+        g.kxy = d_kxy[k];
+        g.kdesc = d_kdesc[k];
+        g.n = n_kps[k];
+        g.width = sizes[2 * k];
+        g.height = sizes[2 * k + 1];
+        g.inv_w = (float)(30.0 / g.width);
+        g.inv_h = (float)(30.0 / g.height);
+        g.cell_start = c->w_cs.as<int>() + (size_t)k * (kGridCells + 1);
+        g.cell_items = c->w_ci.as<int>() + off;
+        off += n_kps[k];
+        WindowCam& w = wc[k];
+        w.grid = g;
+        for (int i = 0; i < 9; i++) w.R[i] = Rt[12 * k + i];
+        for (int i = 0; i < 3; i++) w.t[i] = Rt[12 * k + 9 + i];
+        w.fx = cams[4 * k]; w.fy = cams[4 * k + 1]; w.cx = cams[4 * k + 2]; w.cy = cams[4 * k + 3];
+        w.out = d_out + (size_t)k * P * 6;
+    }
+    upload(c->w_view, views, s);
+    upload(c->w_cams, wc, s);
+    hipLaunchKernelGGL(k_grid_build, dim3(n_cams), dim3(1024), 0, s, c->w_view.as<GridView>());
+    WindowPoints wp{};
+    wp.pts = d_pts;
+    wp.pdesc = d_pdesc;
+    wp.plevel = d_plev;
+    wp.P = P;
+    wp.radius = radius;
+    wp.ratio = ratio;
+    wp.th = th;
+    for (int i = 0; i < nl; i++) wp.scales[i] = scales[i];
+    if (P > 0) hipLaunchKernelGGL(k_window_knn, dim3((P + 3) / 4, n_cams), dim3(256), 0, s, c->w_cams.as<WindowCam>(), wp);
+    HIP_OK(hipGetLastError());
+}
+
+int mcv_window_search_dev(mcv_ctx* c, int n_cams, const float* d_points, const uint8_t* d_pdesc, const int32_t* d_plevel,
+                          int n_points, const float* Rt, const float* cams, const int32_t* sizes, const float* scales,
+                          int nlevels, float radius, float ratio, int th, const float* const* d_kxy,
+                          const int32_t* const* d_koct, const uint8_t* const* d_kdesc, const int32_t* n_kps,
+                          int32_t* d_out) {
+    API_BEGIN(c)
+    REQUIRE(c && n_cams > 0 && n_points >= 0, MCV_ERR_INVALID, "bad window search");
+    (void)d_koct;
+    c->set_device();
+    window_run(c, n_cams, d_points, d_pdesc, d_plevel, n_points, Rt, cams, sizes, scales, nlevels, radius, ratio, th,
+               d_kxy, d_kdesc, n_kps, d_out);
+    API_END
+}
+
+int mcv_window_search(mcv_ctx* c, const float* points, const uint8_t* pdesc, const int32_t* plevel, int P, const float* R,
+                      const float* t, const float* cam, const float* scales, int nl, float radius, float ratio, int th,
+                      const float* kxy, const int32_t* koct, const uint8_t* kdesc, int K, int width, int height,
+                      int32_t* out) {
+    API_BEGIN(c)
+    REQUIRE(c && P >= 0 && K >= 0, MCV_ERR_INVALID, "bad window search");
+    (void)koct;
+    if (P == 0) return MCV_OK;
+    c->set_device();
+    hipStream_t s = c->stream;
+    c->w_pts.ensure((size_t)P * 12);
+    c->w_pdesc.ensure((size_t)P * 32);
+    c->w_plev.ensure((size_t)P * 4);
+    c->w_kxy.ensure((size_t)std::max(K, 1) * 8);
+    c->w_kdesc.ensure((size_t)std::max(K, 1) * 32);
+    c->w_out.ensure((size_t)P * 24);
+    HIP_OK(hipMemcpyAsync(c->w_pts.p, points, (size_t)P * 12, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(c->w_pdesc.p, pdesc, (size_t)P * 32, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(c->w_plev.p, plevel, (size_t)P * 4, hipMemcpyHostToDevice, s));
+    if (K) {
+        HIP_OK(hipMemcpyAsync(c->w_kxy.p, kxy, (size_t)K * 8, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(c->w_kdesc.p, kdesc, (size_t)K * 32, hipMemcpyHostToDevice, s));
+    }
+    float Rt[12];
+    for (int i = 0; i < 9; i++) Rt[i] = R[i];
+    for (int i = 0; i < 3; i++) Rt[9 + i] = t[i];
+    const int32_t sz[2] = {width, height};
+    const float* kx = c->w_kxy.as<float>();
+    const uint8_t* kd = c->w_kdesc.as<uint8_t>();
+    window_run(c, 1, c->w_pts.as<float>(), c->w_pdesc.as<uint8_t>(), c->w_plev.as<int32_t>(), P, Rt, cam, sz, scales, nl,
+               radius, ratio, th, &kx, &kd, &K, c->w_out.as<int32_t>());
+    HIP_OK(hipMemcpyAsync(out, c->w_out.p, (size_t)P * 24, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    API_END
+}
+
+// ---- bundle adjustment --------------------------------------------------------
+int mcv_ba_create(mcv_ctx* c, int M, int P, int F, int C, mcv_ba** out) {
+    API_BEGIN(c)
+    REQUIRE(c && out && M >= 0 && P > 0 && F > 0 && C > 0, MCV_ERR_INVALID, "bad BA sizes");
+    c->set_device();
+    std::unique_ptr<mcv_ba> b(new mcv_ba());
+    b->ctx = c;
+    b->M = M; b->P = P; b->F = F; b->C = C;
+    const size_t m = (size_t)std::max(M, 1);
+    b->uv.ensure(m * 8);
+    b->pt.ensure(m * 4);
+    b->pair.ensure(m * 4);
+    b->isig.ensure(m * 4);
+    b->phi.ensure((size_t)P * 16);
+    b->plo.ensure((size_t)P * 16);
+    b->pairs.ensure((size_t)F * C * sizeof(BAPair));
+    b->planes.ensure(m * 22 * 4);
+    b->pptr.ensure((size_t)(P + 1) * 4);
+    b->fptr.ensure((size_t)(F + 1) * 4);
+    b->fobs.ensure(m * 4);
+    *out = b.release();
+    API_END
+}
+
+void mcv_ba_destroy(mcv_ba* b) { delete b; }
+
+int mcv_ba_set_observations(mcv_ba* b, const float* uv, const int32_t* pt, const int32_t* fr, const int32_t* cam,
+                            const float* isig) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && (b->M == 0 || (uv && pt && fr && cam && isig)), MCV_ERR_INVALID, "null observation arrays");
+    c->set_device();
+    hipStream_t s = c->stream;
+    const int M = b->M;
+    std::vector<uint32_t> pair(M);
+    std::vector<int> pcount(b->P + 1, 0), fcount(b->F + 1, 0);
+    bool sorted = true;
+    for (int i = 0; i < M; i++) {
+        REQUIRE(pt[i] >= 0 && pt[i] < b->P && fr[i] >= 0 && fr[i] < b->F && cam[i] >= 0 && cam[i] < b->C,
+                MCV_ERR_INVALID, "observation index out of range");
+        pair[i] = (uint32_t)(fr[i] * b->C + cam[i]);
+        pcount[pt[i] + 1]++;
+        fcount[fr[i] + 1]++;
+        if (i && pt[i] < pt[i - 1]) sorted = false;
+    }
+    for (int p = 0; p < b->P; p++) pcount[p + 1] += pcount[p];
+    for (int f = 0; f < b->F; f++) fcount[f + 1] += fcount[f];
+    std::vector<int> fobs(std::max(M, 1));
+    {
+        std::vector<int> pos(fcount.begin(), fcount.end() - 1);
+        for (int i = 0; i < M; i++) fobs[pos[fr[i]]++] = i;
+    }
+    if (M) {
+        HIP_OK(hipMemcpyAsync(b->uv.p, uv, (size_t)M * 8, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(b->pt.p, pt, (size_t)M * 4, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(b->pair.p, pair.data(), (size_t)M * 4, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(b->isig.p, isig, (size_t)M * 4, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(b->fobs.p, fobs.data(), (size_t)M * 4, hipMemcpyHostToDevice, s));
+    }
+    HIP_OK(hipMemcpyAsync(b->fptr.p, fcount.data(), (size_t)(b->F + 1) * 4, hipMemcpyHostToDevice, s));
+    if (sorted) HIP_OK(hipMemcpyAsync(b->pptr.p, pcount.data(), (size_t)(b->P + 1) * 4, hipMemcpyHostToDevice, s));
+    HIP_OK(hipStreamSynchronize(s));
+    b->have_obs = sorted;
+    REQUIRE(sorted, MCV_ERR_INVALID, "observations must be ordered by point");
+    API_END
+}
+
+// double -> (hi, lo) float pair
+static inline void split(double v, float& hi, float& lo) {
+    hi = (float)v;
+    lo = (float)(v - (double)hi);
+}
+
+static void quat_to_R(const double* q, double R[9]) {
+    const double x = q[0], y = q[1], z = q[2], w = q[3];
+    const double tx = 2 * x, ty = 2 * y, tz = 2 * z;
+    const double twx = tx * w, twy = ty * w, twz = tz * w;
+    const double txx = tx * x, txy = ty * x, txz = tz * x;
+    const double tyy = ty * y, tyz = tz * y, tzz = tz * z;
+    R[0] = 1 - (tyy + tzz); R[1] = txy - twz; R[2] = txz + twy;
+    R[3] = txy + twz; R[4] = 1 - (txx + tzz); R[5] = tyz - twx;
+    R[6] = txz - twy; R[7] = tyz + twx; R[8] = 1 - (txx + tyy);
+}
+
+int mcv_ba_set_state(mcv_ba* b, const double* points, const double* frames, const double* cams) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && points && frames && cams, MCV_ERR_INVALID, "null state arrays");
+    c->set_device();
+    hipStream_t s = c->stream;
+    std::vector<float4> hi(b->P), lo(b->P);
+    for (int p = 0; p < b->P; p++) {
+        float h[3], l[3];
+        for (int k = 0; k < 3; k++) split(points[3 * p + k], h[k], l[k]);
+        hi[p] = make_float4(h[0], h[1], h[2], 0.f);
+        lo[p] = make_float4(l[0], l[1], l[2], 0.f);
+    }
+    std::vector<BAPair> pr((size_t)b->F * b->C);
+    for (int f = 0; f < b->F; f++) {
+        double Rbw[9];
+        const double* fq = frames + 7 * f;
+        // g2o SE3Quat keeps a normalised quaternion with w >= 0 (se3quat.h:280-285)
+        double q[4] = {fq[0], fq[1], fq[2], fq[3]};
+        double qn = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        for (double& v : q) v /= qn;
+        quat_to_R(q, Rbw);
+        const double* tbw = fq + 4;
+        for (int k = 0; k < b->C; k++) {
+            const double* cm = cams + 12 * k;
+            double Rcb[9];
+            double qc[4] = {cm[4], cm[5], cm[6], cm[7]};
+            double qcn = std::sqrt(qc[0] * qc[0] + qc[1] * qc[1] + qc[2] * qc[2] + qc[3] * qc[3]);
+            for (double& v : qc) v /= qcn;
+            quat_to_R(qc, Rcb);
+            const double* tcb = cm + 8;
+            BAPair& P = pr[(size_t)f * b->C + k];
+            // Tcw = Tcb * Tbw
+            for (int r = 0; r < 3; r++) {
+                for (int cc = 0; cc < 3; cc++) {
+                    double v = 0;
+                    for (int m = 0; m < 3; m++) v += Rcb[3 * r + m] * Rbw[3 * m + cc];
+                    split(v, P.Rh[3 * r + cc], P.Rl[3 * r + cc]);
+                }
+                double tv = tcb[r];
+                for (int m = 0; m < 3; m++) tv += Rcb[3 * r + m] * tbw[m];
+                split(tv, P.th[r], P.tl[r]);
+            }
+            for (int i = 0; i < 9; i++) { P.Rbw[i] = (float)Rbw[i]; P.Rcb[i] = (float)Rcb[i]; }
+            for (int i = 0; i < 3; i++) P.tbw[i] = (float)tbw[i];
+            P.fx = (float)cm[0]; P.fy = (float)cm[1]; P.cx = (float)cm[2]; P.cy = (float)cm[3];
+            P.delta = (float)cm[11];
+            P.pad[0] = P.pad[1] = P.pad[2] = 0.f;
+        }
+    }
+    upload(b->phi, hi, s);
+    upload(b->plo, lo, s);
+    upload(b->pairs, pr, s);
+    HIP_OK(hipStreamSynchronize(s));
+    b->have_state = true;
+    API_END
+}
+
+int mcv_ba_linearize(mcv_ba* b, int residual_only) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && b->have_obs && b->have_state, MCV_ERR_STATE, "observations or state not set");
+    c->set_device();
+    if (b->M > 0) {
+        BAObs ob{b->uv.as<float2>(), b->pt.as<uint32_t>(), b->pair.as<uint32_t>(), b->isig.as<float>(), b->M};
+        BAOut o{b->planes.as<float>(), (long long)b->M};
+        hipLaunchKernelGGL(k_ba_linearize, dim3((b->M + 255) / 256), dim3(256), 0, c->stream, ob, b->phi.as<float4>(),
+                           b->plo.as<float4>(), b->pairs.as<BAPair>(), o, residual_only);
+        HIP_OK(hipGetLastError());
+    }
+    b->linearized = !residual_only;
+    API_END
+}
+
+int mcv_ba_planes(mcv_ba* b, float** d_planes, int64_t* stride) {
+    if (!b) return MCV_ERR_INVALID;
+    if (d_planes) *d_planes = b->planes.as<float>();
+    if (stride) *stride = b->M;
+    return MCV_OK;
+}
+
+int mcv_ba_read_planes(mcv_ba* b, int first, int count, float* host) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && host && first >= 0 && count >= 0 && first + count <= 22, MCV_ERR_INVALID, "bad plane range");
+    c->set_device();
+    if (b->M && count)
+        HIP_OK(hipMemcpyAsync(host, b->planes.as<float>() + (size_t)first * b->M, (size_t)count * b->M * 4,
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    API_END
+}
+
+int mcv_ba_adjoint_dev(mcv_ba* b, const float* d_g, float* d_gpose, float* d_gpoint) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && b->linearized, MCV_ERR_STATE, "no linearization to take the adjoint of");
+    REQUIRE(d_g && d_gpose && d_gpoint, MCV_ERR_INVALID, "null gradient buffer");
+    c->set_device();
+    hipStream_t s = c->stream;
+    hipLaunchKernelGGL(k_ba_point_adj, dim3((b->P + 255) / 256), dim3(256), 0, s, b->planes.as<float>(), (long long)b->M,
+                       reinterpret_cast<const float2*>(d_g), b->pptr.as<int>(), b->P, d_gpoint);
+    hipLaunchKernelGGL(k_ba_pose_adj, dim3(b->F), dim3(256), 0, s, b->planes.as<float>(), (long long)b->M,
+                       reinterpret_cast<const float2*>(d_g), b->fptr.as<int>(), b->fobs.as<int>(), d_gpose);
+    HIP_OK(hipGetLastError());
+    API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,213 @@
+// Bundle-adjustment linearization kernels for gfx950.
+//
+//   k_ba_linearize  EdgeSE3XYZ2Camera::computeError + linearizeOplus
+//                   (include/OptimizeTypes.hpp:80-113), chi2 and the Huber
+//                   weight (ThirdParty/g2o/g2o/core/base_edge.h:94-97,
+//                   core/robust_kernel_impl.cpp:65-78)
+//   k_ba_point_adj  sum_obs J_point^T g per point (CSR by point)
+//   k_ba_pose_adj   sum_obs J_pose^T g per frame (observation list by frame)
+//
+// Arithmetic is FP32.  The residual e = (u - cx) - fx * X / Z subtracts two
+// numbers of a few hundred pixels to get about one pixel, so the camera-frame
+// point and the projection are carried as unevaluated float pairs
+// (hi + lo, error-free transforms with fmaf): the point and pose state come
+// in as such pairs too, which keeps the FP64 state of the optimizer.  The
+// Jacobians, chi2 and weight are plain FP32.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcv {
+
+struct f2 {
+    float hi, lo;
+};
+
+__device__ __forceinline__ f2 two_sum(float a, float b) {
+    const float s = a + b;
+    const float bb = s - a;
+    const float e = (a - (s - bb)) + (b - bb);
+    return {s, e};
+}
+__device__ __forceinline__ f2 fast_two_sum(float a, float b) {
+    const float s = a + b;
+    return {s, b - (s - a)};
+}
+__device__ __forceinline__ f2 two_prod(float a, float b) {
+    const float p = a * b;
+    return {p, __builtin_fmaf(a, b, -p)};
+}
+__device__ __forceinline__ f2 df_add(f2 a, f2 b) {
+    f2 s = two_sum(a.hi, b.hi);
+    const f2 t = two_sum(a.lo, b.lo);
+    s.lo += t.hi;
+    s = fast_two_sum(s.hi, s.lo);
+    s.lo += t.lo;
+    return fast_two_sum(s.hi, s.lo);
+}
+__device__ __forceinline__ f2 df_neg(f2 a) { return {-a.hi, -a.lo}; }
+__device__ __forceinline__ f2 df_mul(f2 a, f2 b) {
+    f2 p = two_prod(a.hi, b.hi);
+    p.lo = __builtin_fmaf(a.hi, b.lo, p.lo);
+    p.lo = __builtin_fmaf(a.lo, b.hi, p.lo);
+    return fast_two_sum(p.hi, p.lo);
+}
+__device__ __forceinline__ f2 df_mul_f(f2 a, float b) {
+    f2 p = two_prod(a.hi, b);
+    p.lo = __builtin_fmaf(a.lo, b, p.lo);
+    return fast_two_sum(p.hi, p.lo);
+}
+__device__ __forceinline__ f2 df_div(f2 a, f2 b) {
+    const float q1 = a.hi / b.hi;
+    f2 r = df_add(a, df_neg(df_mul_f(b, q1)));
+    const float q2 = r.hi / b.hi;
+    r = df_add(r, df_neg(df_mul_f(b, q2)));
+    const float q3 = r.hi / b.hi;
+    const f2 q = fast_two_sum(q1, q2);
+    return df_add(q, f2{q3, 0.f});
+}
+
+// Per (frame, camera) pair.  Tcw = Tcb * Tbw as float pairs for the
+// residual; Tbw and Rcb in float for the Jacobians.
+struct BAPair {
+    float Rh[9], Rl[9];   // R_cw hi / lo, row-major
+    float th[3], tl[3];   // t_cw hi / lo
+    float Rbw[9], tbw[3]; // body from world
+    float Rcb[9];         // camera from body
+    float fx, fy, cx, cy, delta;
+    float pad[3];
+};
+
+struct BAObs {
+    const float2* uv;       // M
+    const uint32_t* pt;     // M point index
+    const uint32_t* pair;   // M (frame, camera) pair index
+    const float* isig;      // M inverse level sigma^2
+    int M;
+};
+
+struct BAOut {
+    float* planes;   // 22 planes of M floats: e_u e_v chi2 w Jpose[12] Jpoint[6]
+    long long stride;
+};
+
+__global__ __launch_bounds__(256) void k_ba_linearize(BAObs ob, const float4* __restrict__ pts_hi,
+                                                      const float4* __restrict__ pts_lo,
+                                                      const BAPair* __restrict__ pairs, BAOut out, int residual_only) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ob.M) return;
+    const float2 uv = ob.uv[i];
+    const uint32_t pi = ob.pt[i];
+    const BAPair& T = pairs[ob.pair[i]];
+    const float isig = ob.isig[i];
+    const float4 ph = pts_hi[pi], pl = pts_lo[pi];
+    const f2 P[3] = {{ph.x, pl.x}, {ph.y, pl.y}, {ph.z, pl.z}};
+    // P_c = R_cw P_w + t_cw (float pairs)
+    f2 C[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        f2 acc = {T.th[r], T.tl[r]};
+#pragma unroll
+        for (int k = 0; k < 3; k++) acc = df_add(acc, df_mul(f2{T.Rh[3 * r + k], T.Rl[3 * r + k]}, P[k]));
+        C[r] = acc;
+    }
+    // e = (obs - c) - f * X / Z
+    const f2 qx = df_div(C[0], C[2]);
+    const f2 qy = df_div(C[1], C[2]);
+    const f2 eu = df_add(two_sum(uv.x, -T.cx), df_neg(df_mul_f(qx, T.fx)));
+    const f2 ev = df_add(two_sum(uv.y, -T.cy), df_neg(df_mul_f(qy, T.fy)));
+    const float e0 = eu.hi + eu.lo, e1 = ev.hi + ev.lo;
+    const float chi2 = e0 * (isig * e0) + e1 * (isig * e1);
+    const float d2 = T.delta * T.delta;
+    const float w = chi2 <= d2 ? 1.f : T.delta / sqrtf(chi2);
+    const long long S = out.stride;
+    float* o = out.planes + i;
+    o[0] = e0;
+    o[S] = e1;
+    o[2 * S] = chi2;
+    o[3 * S] = w;
+    if (residual_only) return;
+    // Jacobians (FP32).  Jpi = [[fx/z, 0, -fx x/z^2], [0, fy/z, -fy y/z^2]]
+    const float x = C[0].hi, y = C[1].hi, z = C[2].hi;
+    const float iz = 1.f / z;
+    const float j00 = T.fx * iz, j02 = -T.fx * x / (z * z);
+    const float j11 = T.fy * iz, j12 = -T.fy * y / (z * z);
+    // J_point = -Jpi R_cw
+    float Jpt[6];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        Jpt[k] = -(j00 * T.Rh[k] + j02 * T.Rh[6 + k]);
+        Jpt[3 + k] = -(j11 * T.Rh[3 + k] + j12 * T.Rh[6 + k]);
+    }
+    // P_b = R_bw P_w + t_bw; M = Jpi R_cb; J_pose = [M [P_b]x | -M]
+    float pb[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) pb[r] = T.Rbw[3 * r] * ph.x + T.Rbw[3 * r + 1] * ph.y + T.Rbw[3 * r + 2] * ph.z + T.tbw[r];
+    float Mr[6];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        Mr[k] = j00 * T.Rcb[k] + j02 * T.Rcb[6 + k];
+        Mr[3 + k] = j11 * T.Rcb[3 + k] + j12 * T.Rcb[6 + k];
+    }
+    float Jp[12];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const float m0 = Mr[3 * r], m1 = Mr[3 * r + 1], m2 = Mr[3 * r + 2];
+        // M * [p]x, [p]x = [[0,-z,y],[z,0,-x],[-y,x,0]]
+        Jp[6 * r + 0] = m1 * pb[2] - m2 * pb[1];
+        Jp[6 * r + 1] = m2 * pb[0] - m0 * pb[2];
+        Jp[6 * r + 2] = m0 * pb[1] - m1 * pb[0];
+        Jp[6 * r + 3] = -m0;
+        Jp[6 * r + 4] = -m1;
+        Jp[6 * r + 5] = -m2;
+    }
+#pragma unroll
+    for (int k = 0; k < 12; k++) o[(4 + k) * S] = Jp[k];
+#pragma unroll
+    for (int k = 0; k < 6; k++) o[(16 + k) * S] = Jpt[k];
+}
+
+// grad_point[p] = sum over the observations of p (CSR, observation order) of
+// J_point^T g.  One thread per point, fixed summation order.
+__global__ __launch_bounds__(256) void k_ba_point_adj(const float* planes, long long S, const float2* g,
+                                                      const int* ptr, int P, float* gout) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    for (int i = ptr[p]; i < ptr[p + 1]; i++) {
+        const float2 gi = g[i];
+        a0 += planes[16 * S + i] * gi.x + planes[19 * S + i] * gi.y;
+        a1 += planes[17 * S + i] * gi.x + planes[20 * S + i] * gi.y;
+        a2 += planes[18 * S + i] * gi.x + planes[21 * S + i] * gi.y;
+    }
+    gout[3 * p] = a0;
+    gout[3 * p + 1] = a1;
+    gout[3 * p + 2] = a2;
+}
+
+// grad_pose[f] = sum over the observations of frame f of J_pose^T g.  One
+// workgroup per frame; each thread sums a strided slice, then a fixed-order
+// tree reduction: deterministic for a given launch shape.
+__global__ __launch_bounds__(256) void k_ba_pose_adj(const float* planes, long long S, const float2* g,
+                                                     const int* fptr, const int* fobs, float* gout) {
+    __shared__ float red[6][256];
+    const int f = blockIdx.x;
+    float a[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = fptr[f] + threadIdx.x; k < fptr[f + 1]; k += 256) {
+        const int i = fobs[k];
+        const float2 gi = g[i];
+#pragma unroll
+        for (int c = 0; c < 6; c++) a[c] += planes[(4 + c) * S + i] * gi.x + planes[(10 + c) * S + i] * gi.y;
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++) red[c][threadIdx.x] = a[c];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int c = 0; c < 6; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) gout[6 * f + threadIdx.x] = red[threadIdx.x][0];
+}
+
+}  // namespace mcv

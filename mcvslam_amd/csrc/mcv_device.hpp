@@ -1,0 +1,147 @@
+// Device helpers shared by the gfx950 kernels.  Functions marked
+// MCV_HD compile for both host and device so that the host can check them
+// against the real libstdc++ / libm they emulate.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MCV_HD __host__ __device__ __forceinline__
+#else
+#define MCV_HD inline
+#endif
+
+namespace mcv {
+
+// ---------------------------------------------------------------------------
+// std::priority_queue<T, vector<T>, Less> emulation on a plain array, with the
+// exact sift order of libstdc++'s push_heap / pop_heap (bits/stl_heap.h:
+// __push_heap, __adjust_heap, __pop_heap).  Heap entries are u64 with the
+// ordering key in the high 32 bits; the low 32 bits are payload that never
+// takes part in a comparison, exactly like the shared_ptr payload of the
+// reference's node queue (ORBextractor.cc:530-531).
+MCV_HD bool heap_less(uint64_t a, uint64_t b) { return (uint32_t)(a >> 32) < (uint32_t)(b >> 32); }
+
+template <typename Ptr>
+MCV_HD void heap_sift_up(Ptr h, int hole, int top, uint64_t value) {
+    int parent = (hole - 1) / 2;
+    while (hole > top && heap_less(h[parent], value)) {
+        h[hole] = h[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    h[hole] = value;
+}
+
+template <typename Ptr>
+MCV_HD void heap_push(Ptr h, int& len, uint64_t value) {
+    h[len] = value;
+    heap_sift_up(h, len, 0, value);
+    len++;
+}
+
+template <typename Ptr>
+MCV_HD void heap_adjust(Ptr h, int hole, int len, uint64_t value) {
+    const int top = hole;
+    int second = hole;
+    while (second < (len - 1) / 2) {
+        second = 2 * (second + 1);
+        if (heap_less(h[second], h[second - 1])) second--;
+        h[hole] = h[second];
+        hole = second;
+    }
+    if ((len & 1) == 0 && second == (len - 2) / 2) {
+        second = 2 * (second + 1);
+        h[hole] = h[second - 1];
+        hole = second - 1;
+    }
+    heap_sift_up(h, hole, top, value);
+}
+
+template <typename Ptr>
+MCV_HD void heap_pop(Ptr h, int& len) {
+    if (len > 1) {
+        const int last = len - 1;
+        uint64_t value = h[last];
+        h[last] = h[0];
+        heap_adjust(h, 0, last, value);
+    }
+    len--;
+}
+
+// ---------------------------------------------------------------------------
+// Base sin/cos of a float angle in [0, 8): Cody-Waite reduction by pi/2 in
+// double, Taylor series to degree 19/20, one rounding to float.  Identical
+// IEEE double arithmetic on host and device (no FMA contraction, no library
+// calls), so the host can list every input where the result differs from
+// the host libm's cosf/sinf; those inputs are patched from a table.
+MCV_HD void sincos_base(float xf, float* c_out, float* s_out) {
+    const double x = (double)xf;
+    const double PIO2_HI = 1.5707963267948966;      // double(pi/2); its low 3 mantissa bits are 0
+    const double PIO2_LO = 6.123233995736766e-17;   // pi/2 - PIO2_HI
+    const double TWO_OVER_PI = 0.6366197723675814;
+    int n = (int)(x * TWO_OVER_PI + 0.5);
+    const double dn = (double)n;
+    double r = x - dn * PIO2_HI;
+    r = r - dn * PIO2_LO;
+    const double r2 = r * r;
+    // sin r = r - r^3 (1/3! - r^2/5! + ...), cos r = 1 - r^2 (1/2! - r^2/4! + ...)
+    double s = 1.0 / 121645100408832000.0;    // 1/19!
+    s = s * (-r2) + 1.0 / 355687428096000.0;  // 1/17!
+    s = s * (-r2) + 1.0 / 1307674368000.0;    // 1/15!
+    s = s * (-r2) + 1.0 / 6227020800.0;       // 1/13!
+    s = s * (-r2) + 1.0 / 39916800.0;         // 1/11!
+    s = s * (-r2) + 1.0 / 362880.0;           // 1/9!
+    s = s * (-r2) + 1.0 / 5040.0;             // 1/7!
+    s = s * (-r2) + 1.0 / 120.0;              // 1/5!
+    s = s * (-r2) + 1.0 / 6.0;                // 1/3!
+    s = r - (r * r2) * s;
+    double c = 1.0 / 2432902008176640000.0;   // 1/20!
+    c = c * (-r2) + 1.0 / 6402373705728000.0; // 1/18!
+    c = c * (-r2) + 1.0 / 20922789888000.0;   // 1/16!
+    c = c * (-r2) + 1.0 / 87178291200.0;      // 1/14!
+    c = c * (-r2) + 1.0 / 479001600.0;        // 1/12!
+    c = c * (-r2) + 1.0 / 3628800.0;          // 1/10!
+    c = c * (-r2) + 1.0 / 40320.0;            // 1/8!
+    c = c * (-r2) + 1.0 / 720.0;              // 1/6!
+    c = c * (-r2) + 1.0 / 24.0;               // 1/4!
+    c = c * (-r2) + 0.5;                      // 1/2!
+    c = 1.0 - r2 * c;
+    double cs, sn;
+    switch (n & 3) {
+        case 0: cs = c; sn = s; break;
+        case 1: cs = -s; sn = c; break;
+        case 2: cs = -c; sn = -s; break;
+        default: cs = s; sn = -c; break;
+    }
+    *c_out = (float)cs;
+    *s_out = (float)sn;
+}
+
+// ---------------------------------------------------------------------------
+// cv::fastAtan2 in float (degrees); the constants are the float products
+// 0.9997878412794807f * (float)(180/pi) etc. evaluated in float.
+MCV_HD float fast_atan2_deg(float y, float x) {
+    const float k = (float)(180.0 / 3.14159265358979323846);
+    const float p1 = 0.9997878412794807f * k;
+    const float p3 = -0.3258083974640975f * k;
+    const float p5 = 0.1555786518463281f * k;
+    const float p7 = -0.04432655554792128f * k;
+    const float eps = (float)2.220446049250313080847e-16;  // (float)DBL_EPSILON
+    float ax = x < 0 ? -x : x, ay = y < 0 ? -y : y;
+    float a, c, c2;
+    if (ax >= ay) {
+        c = ay / (ax + eps);
+        c2 = c * c;
+        a = (((p7 * c2 + p5) * c2 + p3) * c2 + p1) * c;
+    } else {
+        c = ax / (ay + eps);
+        c2 = c * c;
+        a = 90.f - (((p7 * c2 + p5) * c2 + p3) * c2 + p1) * c;
+    }
+    if (x < 0) a = 180.f - a;
+    if (y < 0) a = 360.f - a;
+    return a;
+}
+
+}  // namespace mcv

@@ -1,0 +1,70 @@
+"""BA linearization on the GPU (FP32, compensated residual) against the
+FP64 oracle of EdgeSE3XYZ2Camera (include/OptimizeTypes.hpp:70-117)."""
+import numpy as np
+import pytest
+
+import oracle
+from mcvslam_amd import BALinearizer, synth
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel_rows(a, b, floor=0.0):
+    """per-row normwise relative error ||a - b|| / max(||b||, floor)"""
+    a = a.reshape(len(a), -1).astype(np.float64)
+    b = b.reshape(len(b), -1).astype(np.float64)
+    num = np.linalg.norm(a - b, axis=1)
+    den = np.maximum(np.linalg.norm(b, axis=1), floor)
+    return num / den
+
+
+def check_against_oracle(prob, got):
+    e, chi2, w, Jp, Jx = got
+    re, rchi2, rw, rJp, rJx = oracle.ba_linearize(prob)
+    # residual: normwise relative, floor 1e-9 px only guards e == 0
+    assert rel_rows(e, re, 1e-9).max() < TOL
+    assert (np.abs(chi2 - rchi2) / np.maximum(rchi2, 1e-12)).max() < TOL
+    # Huber boundary (chi2 within 1e-6 of delta^2) may flip the branch
+    d2 = prob.cams[prob.obs_cam, 11] ** 2
+    ok = np.abs(rchi2 - d2) > 1e-6 * d2
+    assert (np.abs(w - rw)[ok] / rw[ok]).max() < TOL
+    for r in range(2):
+        assert rel_rows(Jp[:, r], rJp[:, r]).max() < TOL, f"J_pose row {r}"
+        assert rel_rows(Jx[:, r], rJx[:, r]).max() < TOL, f"J_point row {r}"
+
+
+def test_ba_150k(gpu_ctx):
+    prob = synth.ba_problem(seed=30, n_kf=10, n_points=20000, target_obs=150000)
+    b = BALinearizer.from_problem(prob, ctx=gpu_ctx)
+    b.linearize()
+    check_against_oracle(prob, b.results())
+    assert (b.results()[2] < 1).sum() > 0.03 * len(prob.obs_pt)   # outliers hit the Huber branch
+
+
+def test_ba_perturbed_state(gpu_ctx):
+    """State with full FP64 precision (not float-representable)."""
+    prob = synth.ba_problem(seed=31, n_kf=6, n_points=3000, target_obs=30000, perturb=0.01)
+    prob.frames[:, 4:] += np.random.default_rng(1).normal(0, 1e-3, (len(prob.frames), 3))
+    b = BALinearizer.from_problem(prob, ctx=gpu_ctx)
+    b.linearize()
+    check_against_oracle(prob, b.results())
+
+
+def test_ba_reference_huber_deltas(gpu_ctx):
+    """LEFT edges use sqrt(5.99), WIDE sqrt(7.815) (src/BAoptimizer.cpp:62,105)."""
+    prob = synth.ba_problem(seed=32, n_kf=4, n_points=2000, target_obs=20000)
+    prob.cams[:, 11] = [np.float32(np.sqrt(5.99)), np.float32(np.sqrt(7.815)), np.float32(np.sqrt(7.815))]
+    b = BALinearizer.from_problem(prob, ctx=gpu_ctx)
+    b.linearize()
+    check_against_oracle(prob, b.results())
+
+
+def test_ba_residual_only(gpu_ctx):
+    prob = synth.ba_problem(seed=33, n_kf=3, n_points=1000, target_obs=8000)
+    b = BALinearizer.from_problem(prob, ctx=gpu_ctx)
+    b.linearize(residual_only=True)
+    pl = b.read(0, 4)
+    re, rchi2, rw, _, _ = oracle.ba_linearize(prob)
+    assert rel_rows(np.stack([pl[0], pl[1]], 1), re, 1e-9).max() < TOL

@@ -1,0 +1,76 @@
+"""Brute-force Hamming top-2 and the ratio/threshold filters on the GPU
+against the oracle (src/Matcher.cpp:23-35, :100-111, :245-302)."""
+import numpy as np
+import pytest
+
+import oracle
+from mcvslam_amd import Matcher
+
+pytestmark = pytest.mark.gpu
+
+
+def _desc(rng, n):
+    return rng.integers(0, 256, (n, 32), dtype=np.uint8)
+
+
+@pytest.mark.parametrize("nq,nt", [(1, 0), (5, 1), (7, 2), (64, 65), (300, 1000), (2000, 2000)])
+def test_knn2_random(gpu_ctx, nq, nt):
+    rng = np.random.default_rng(nq * 1000 + nt)
+    q, t = _desc(rng, nq), _desc(rng, nt)
+    got = Matcher.KnnMatch(q, t, ctx=gpu_ctx).rows
+    np.testing.assert_array_equal(got, oracle.knn2(q, t))
+
+
+def test_knn2_ties_keep_scan_order(gpu_ctx):
+    """Duplicated trains: the first occurrence wins (strict '<' in order)."""
+    rng = np.random.default_rng(3)
+    base = _desc(rng, 50)
+    t = np.concatenate([base, base, base[::-1]])
+    q = np.concatenate([base[:20], _desc(rng, 30)])
+    got = Matcher.KnnMatch(q, t, ctx=gpu_ctx).rows
+    want = oracle.knn2(q, t)
+    np.testing.assert_array_equal(got, want)
+    assert (got[:20, 1] == 0).all() and (got[:20, 3] == 0).all()
+
+
+def test_knn2_near_duplicates(gpu_ctx):
+    """Trains one or two bit flips from the query: many equal distances."""
+    rng = np.random.default_rng(4)
+    q = _desc(rng, 256)
+    t = np.repeat(q, 4, axis=0).copy()
+    for i in range(len(t)):
+        b = rng.integers(0, 256, rng.integers(0, 3))
+        for bit in b:
+            t[i, bit >> 3] ^= np.uint8(1 << (bit & 7))
+    rng.shuffle(t)
+    np.testing.assert_array_equal(Matcher.KnnMatch(q, t, ctx=gpu_ctx).rows, oracle.knn2(q, t))
+
+
+@pytest.mark.parametrize("ratio,th", [(0.7, 50), (0.6, 46), (0.7, 34), (1.0, 256), (0.0, 0)])
+def test_filter(gpu_ctx, ratio, th):
+    rng = np.random.default_rng(int(ratio * 100) + th)
+    q, t = _desc(rng, 1500), _desc(rng, 1200)
+    # half the queries get a close train so the filters have work to do
+    for i in range(0, 1500, 2):
+        t[rng.integers(0, 1200)] = q[i] ^ rng.integers(0, 2, 32, dtype=np.uint8)
+    knn = oracle.knn2(q, t)
+    got = Matcher.filter(knn, ratio, th, ctx=gpu_ctx)
+    np.testing.assert_array_equal(got, oracle.filter_ratio_threshold(knn, ratio, th))
+
+
+def test_filter_edge_rows(gpu_ctx):
+    """Sentinels: one candidate (d1 = 999) passes the ratio, none (999/999)
+    and 0/0 do not; the threshold pass reorders by swap-with-last."""
+    knn = np.array([[0, 999, 0, 999], [3, 10, 0, 999], [1, 0, 2, 0], [4, 60, 5, 200], [2, 5, 7, 9],
+                    [6, 12, 8, 40], [9, 51, 1, 300], [5, 1, 3, 90]], np.int32)
+    for ratio, th in [(0.7, 50), (0.6, 46), (2.0, 30)]:
+        got = Matcher.filter(knn, ratio, th, ctx=gpu_ctx)
+        np.testing.assert_array_equal(got, oracle.filter_ratio_threshold(knn, ratio, th))
+
+
+def test_reference_chain(gpu_ctx):
+    rng = np.random.default_rng(9)
+    q, t = _desc(rng, 400), _desc(rng, 400)
+    t[:200] = q[:200]
+    res = Matcher.KnnMatch(q, t, ctx=gpu_ctx).FilterRatio(0.7).FilterThreshold(50)
+    np.testing.assert_array_equal(res, oracle.filter_ratio_threshold(oracle.knn2(q, t), 0.7, 50))

@@ -1,0 +1,40 @@
+"""Rig-frame batch (L, R, W extraction + L-R, W-L, W-R matching): the
+hipGraph batch, the plain per-frame path and the oracle agree bit for bit."""
+import numpy as np
+import pytest
+
+import oracle
+from mcvslam_amd import RigBatch, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def test_rig_two_frames(orb_default):
+    import torch
+    frames = [synth.rig_frame(100 + 10 * f) for f in range(2)]
+    imgs = [torch.from_numpy(im).cuda() for fr in frames for im in fr]
+    rig = RigBatch(orb_default, 2, ratio=0.7, th=50)
+    results = []
+    for graph in (True, False, True):
+        rig.run([im.data_ptr() for im in imgs], graph=graph)
+        results.append(rig.read())
+    for r in results[1:]:
+        for a, b in zip(results[0], r):
+            np.testing.assert_array_equal(a, b)
+    kps, desc, counts, matches, mcounts, knn = results[0]
+    ref = [oracle.orb_extract(im) for fr in frames for im in fr]
+    for i, (rk, rd) in enumerate(ref):
+        n = counts[i]
+        assert n == len(rk)
+        for f in ("x", "y", "size", "angle", "response", "octave"):
+            np.testing.assert_array_equal(kps[i, :n][f], rk[f])
+        np.testing.assert_array_equal(desc[i, :n], rd)
+    for f in range(2):
+        for k, (qa, ta) in enumerate([(0, 1), (2, 0), (2, 1)]):
+            rq, rt = ref[3 * f + qa][1], ref[3 * f + ta][1]
+            want_knn = oracle.knn2(rq, rt)
+            p = 3 * f + k
+            np.testing.assert_array_equal(knn[p, :len(rq)], want_knn)
+            want = oracle.filter_ratio_threshold(want_knn, 0.7, 50)
+            assert mcounts[p] == len(want)
+            np.testing.assert_array_equal(matches[p, :mcounts[p]], want)

@@ -1,0 +1,257 @@
+"""Benchmark of the MI355X hot path (one GPU).
+
+Headline: rig frames per second of batched ORB extraction + matching.  One
+step = one hipGraph launch over a batch of rig frames (L and R 1280x720, W
+640x480, 2000 features per image, 8 levels, scale 1.2, FAST 20/7), then
+brute-force Hamming top-2 + ratio 0.7 + threshold 50 for L-R, W-L and W-R.
+Inputs are resident in HBM before timing starts.
+
+Also reported on the same line: the plain path (one rig frame at a time, no
+graph), BA linearization throughput (50 keyframes x 3 cameras, 200k points,
+~1.5M observations), the roofline of the dominant kernel (HIP events on the
+stream it runs on) and the CPU oracle on this host's cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+FRAMES = 64
+UNIQUE_FRAMES = 16
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_frames(n_unique: int):
+    from mcvslam_amd import synth
+    return [synth.rig_frame(1000 + 7 * f) for f in range(n_unique)]
+
+
+def orb_level_pixels(w, h, inv_scale):
+    px = []
+    for l, s in enumerate(inv_scale):
+        lw = w if l == 0 else int(np.rint(np.float32(w) * s))
+        lh = h if l == 0 else int(np.rint(np.float32(h) * s))
+        px.append(lw * lh)
+    return px
+
+
+def algorithmic_bytes(kernel, orb, frames, counts):
+    """Compulsory HBM bytes of one launch of `kernel` over the batch."""
+    levels = [orb_level_pixels(w, h, orb.mvInvScaleFactor) for (w, h) in ((1280, 720), (1280, 720), (640, 480))]
+    P = sum(sum(l) for l in levels) * frames          # pyramid pixels
+    P0 = sum(l[0] for l in levels) * frames
+    nkp = int(counts.sum())
+    if kernel == "k_blur":
+        return 2 * P
+    if kernel == "k_fast_cells":
+        return P
+    if kernel == "k_resize":
+        return 2 * (P - P0)
+    if kernel == "k_copy_level0":
+        return 2 * P0
+    if kernel == "k_keypoints":
+        return 56 * nkp
+    if kernel == "k_knn2":
+        return frames * 3 * (32 * 2 * 2000 + 16 * 2000)
+    return None
+
+
+def cpu_baseline(frames_np, budget_s=12.0):
+    """Oracle on the host: per rig frame 3 extraction threads (one per camera,
+    as src/Frame.cpp:120-125) then the three brute-force matches."""
+    import oracle
+    done = 0
+    t0 = time.perf_counter()
+    for f in frames_np:
+        res = [None] * 3
+
+        def work(k):
+            res[k] = oracle.orb_extract(f[k])
+        th = [threading.Thread(target=work, args=(k,)) for k in range(3)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        for qa, ta in ((0, 1), (2, 0), (2, 1)):
+            knn = oracle.knn2(res[qa][1], res[ta][1])
+            oracle.filter_ratio_threshold(knn, 0.7, 50)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return done / dt, done, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--frames", type=int, default=FRAMES)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ba", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (every box has one GPU; "
+              "the path would shard by rig frame and by BA observation, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from mcvslam_amd import BALinearizer, Context, ORB, RigBatch, synth
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    ctx = Context(0)
+    ctx.use_torch_stream()
+    orb = ORB(2000, 1.2, 8, 20, 7, ctx=ctx)
+
+    log("generating frames ...")
+    t0 = time.perf_counter()
+    uniq = make_frames(min(UNIQUE_FRAMES, args.frames))
+    log(f"  {len(uniq)} unique rig frames in {time.perf_counter() - t0:.1f}s")
+    n = args.frames
+    L = torch.empty((n, 720, 1280), dtype=torch.uint8, device=dev)
+    R = torch.empty((n, 720, 1280), dtype=torch.uint8, device=dev)
+    W = torch.empty((n, 480, 640), dtype=torch.uint8, device=dev)
+    for f in range(n):
+        l, r, w = uniq[f % len(uniq)]
+        L[f].copy_(torch.from_numpy(l))
+        R[f].copy_(torch.from_numpy(r))
+        W[f].copy_(torch.from_numpy(w))
+    ptrs = []
+    for f in range(n):
+        ptrs += [L[f].data_ptr(), R[f].data_ptr(), W[f].data_ptr()]
+    torch.cuda.synchronize()
+
+    rig = RigBatch(orb, n, ratio=0.7, th=50)
+    log("warmup (graph) ...")
+    for _ in range(args.warmup):
+        rig.run(ptrs, graph=True)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(args.steps):
+        rig.run(ptrs, graph=True)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms_step = ev0.elapsed_time(ev1) / args.steps
+    fps = n / (ms_step / 1e3)
+    log(f"batched: {ms_step:.3f} ms per {n}-frame step -> {fps:.0f} rig frames/s")
+    _, _, counts, _, mcounts, _ = rig.read()
+
+    # plain path: one rig frame at a time, no graph
+    rig.run(ptrs, graph=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_reps = 2
+    for _ in range(plain_reps):
+        rig.run(ptrs, graph=False)
+    torch.cuda.synchronize()
+    plain_fps = n * plain_reps / (time.perf_counter() - t0)
+    log(f"plain: {plain_fps:.0f} rig frames/s (batched/plain = {fps / plain_fps:.1f}x)")
+
+    # per-kernel durations (events after every launch, eager sequence)
+    per = {}
+    reps = 5
+    for _ in range(reps):
+        for name, ms in rig.run_timed(ptrs):
+            per.setdefault(name, []).append(ms)
+    kern = {k: sum(v) / reps for k, v in per.items()}           # ms per step, summed over launches
+    launches = {k: len(v) / reps for k, v in per.items()}
+    dom = max(kern, key=kern.get)
+    log("per-kernel ms/step: " + ", ".join(f"{k}={v:.3f}" for k, v in sorted(kern.items(), key=lambda x: -x[1])))
+
+    result = {
+        "metric": "3-cam rig frames/s (ORB + L-R/W-L/W-R match, bit-exact), batched hipGraph",
+        "value": round(fps, 2),
+        "unit": "rig frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8/int32 (f32 angle, rotation)",
+        "data": f"synthetic: seeded blob/rectangle/noise images, {len(uniq)} unique rig frames tiled to {n}",
+        "config": {"workload": f"{n} rig frames per step: L,R 1280x720 + W 640x480, 2000 ORB features, 8 levels, "
+                               "scale 1.2, FAST 20/7, match ratio 0.7 threshold 50",
+                   "frames_per_step": n, "parallelism": "single GPU"},
+        "plain_frames_per_s": round(plain_fps, 2),
+        "batched_over_plain": round(fps / plain_fps, 2),
+        "kernels_ms_per_step": {k: round(v, 4) for k, v in kern.items()},
+        "keypoints_per_step": int(counts.sum()),
+        "matches_per_step": int(mcounts.sum()),
+    }
+    ab = algorithmic_bytes(dom, orb, n, counts)
+    per_launch_ms = kern[dom] / launches[dom]
+    if ab is not None:
+        per_launch_bytes = ab / launches[dom]
+        ach = per_launch_bytes / (per_launch_ms / 1e3) / 1e9
+        result["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+                              "avg_launch_ms": round(per_launch_ms, 5)}
+    else:
+        result["roofline"] = {"kernel": dom, "bound": "latency", "achieved": None, "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": None, "traffic": None,
+                              "avg_launch_ms": round(per_launch_ms, 5)}
+
+    if not args.no_ba:
+        log("BA problem ...")
+        t0 = time.perf_counter()
+        prob = synth.ba_problem(seed=40, n_kf=50, n_points=200000, target_obs=1500000)
+        ba = BALinearizer.from_problem(prob, ctx=ctx)
+        log(f"  {len(prob.obs_pt)} observations in {time.perf_counter() - t0:.1f}s")
+        for _ in range(3):
+            ba.linearize()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        iters = 50
+        e0.record()
+        for _ in range(iters):
+            ba.linearize()
+        e1.record()
+        torch.cuda.synchronize()
+        ba_ms = e0.elapsed_time(e1) / iters
+        M = len(prob.obs_pt)
+        P = len(prob.points)
+        ba_bytes = 20 * M + 88 * M + 32 * P
+        ach = ba_bytes / (ba_ms / 1e3) / 1e9
+        result["ba"] = {"workload": "50 keyframes x 3 cams, 200k points, EdgeSE3XYZ2Camera linearize",
+                        "observations": M, "obs_per_s": round(M / (ba_ms / 1e3), 1), "ms_per_launch": round(ba_ms, 5),
+                        "roofline": {"kernel": "k_ba_linearize", "bound": "hbm", "achieved": round(ach, 2),
+                                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                                     "bytes_per_obs": round(ba_bytes / M, 2)}}
+        log(f"BA linearize: {ba_ms * 1e3:.1f} us per {M} obs -> {M / ba_ms / 1e3:.3e} obs/s, {ach:.0f} GB/s")
+        ba.close()
+
+    if not args.no_cpu:
+        log("CPU oracle baseline ...")
+        fr = [uniq[i % len(uniq)] for i in range(8)]
+        cfps, done, dt = cpu_baseline(fr)
+        result["cpu_baseline"] = {"value": round(cfps, 3), "unit": "rig frames/s", "cores": 3, "kind": "port",
+                                  "sample": f"{done} rig frames (3 extraction threads + 3 brute-force matches) "
+                                            f"in {dt:.1f}s on {os.cpu_count()} host cpus"}
+        log(f"CPU: {cfps:.2f} rig frames/s")
+    rig.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

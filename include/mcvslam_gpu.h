@@ -105,6 +105,14 @@ void mcv_rig_destroy(mcv_rig* rig);
  * batch as one hipGraph; without it every image and every pair is launched on
  * its own ("plain" path). */
 int mcv_rig_run(mcv_rig* rig, const uint8_t* const* d_images, int flags);
+/* Waits for the last run and reports device-side overflow of the octree
+ * node pool (MCV_ERR_STATE; the pool is grown for the next run). */
+int mcv_rig_check(mcv_rig* rig);
+/* The batched sequence launched eagerly with an event after every launch:
+ * ms[i] = duration of launch i (ms), names[i] = its kernel (static strings),
+ * n = number of launches.  For kernel timing only. */
+int mcv_rig_run_timed(mcv_rig* rig, const uint8_t* const* d_images, float* ms, const char** names, int cap,
+                      int32_t* n);
 /* Device outputs: keypoints [3n][capacity], descriptors [3n][capacity][32],
  * counts [3n], matches [3n pairs][capacity][3] (query, train, distance),
  * match counts [3n]; knn rows [3n pairs][capacity][4]. */
@@ -156,6 +164,8 @@ int mcv_ba_linearize(mcv_ba* ba, int residual_only);
 int mcv_ba_planes(mcv_ba* ba, float** d_planes, int64_t* stride);
 /* Copy planes [first, first + count) to host (count x M floats). */
 int mcv_ba_read_planes(mcv_ba* ba, int first, int count, float* host);
+/* Device-to-device copy of planes [first, first + count) (enqueued). */
+int mcv_ba_copy_planes_dev(mcv_ba* ba, int first, int count, float* d_dst);
 /* Adjoint of the residual: grad_pose[F x 6] = sum J_pose^T g and
  * grad_point[P x 3] = sum J_point^T g over the last linearization, with
  * d_g an M x 2 device array; outputs are device arrays. */

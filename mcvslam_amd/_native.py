@@ -51,6 +51,8 @@ SIGNATURES = {
     "mcv_rig_destroy": (None, [P]),
     "mcv_rig_run": (I, [P, P, I]),
     "mcv_rig_outputs": (I, [P, P, P, P, P, P, P, P]),
+    "mcv_rig_run_timed": (I, [P, P, P, P, I, P]),
+    "mcv_rig_check": (I, [P]),
     "mcv_window_search": (I, [P, P, P, P, I, P, P, P, P, I, F, F, I, P, P, P, I, I, I, P]),
     "mcv_window_search_dev": (I, [P, I, P, P, P, I, P, P, P, P, I, F, F, I, P, P, P, P, P]),
     "mcv_ba_create": (I, [P, I, I, I, I, P]),
@@ -60,6 +62,7 @@ SIGNATURES = {
     "mcv_ba_linearize": (I, [P, I]),
     "mcv_ba_planes": (I, [P, P, P]),
     "mcv_ba_read_planes": (I, [P, I, I, P]),
+    "mcv_ba_copy_planes_dev": (I, [P, I, I, P]),
     "mcv_ba_adjoint_dev": (I, [P, P, P, P]),
 }
 

@@ -281,9 +281,20 @@ class RigBatch:
         arr = (ctypes.c_void_p * n)(*image_ptrs)
         check(self.ctx.h, lib().mcv_rig_run(self.h, arr, _native.MCV_FLAG_GRAPH if graph else 0))
 
+    def run_timed(self, image_ptrs):
+        """Eager run with an event after every launch: [(kernel, ms), ...]."""
+        n = 3 * self.n_frames
+        arr = (ctypes.c_void_p * n)(*image_ptrs)
+        ms = np.zeros(64, np.float32)
+        names = (ctypes.c_char_p * 64)()
+        k = ctypes.c_int32()
+        check(self.ctx.h, lib().mcv_rig_run_timed(self.h, arr, _p(ms), names, 64, ctypes.byref(k)))
+        return [(names[i].decode(), float(ms[i])) for i in range(min(k.value, 64))]
+
     def read(self):
         """Host copies: (kps records [3n][cap], desc [3n][cap][32], counts [3n],
         matches [3n][cap][3], match counts [3n], knn [3n][cap][4])."""
+        check(self.ctx.h, lib().mcv_rig_check(self.h))
         n, c = 3 * self.n_frames, self.cap
         kps = self.ctx.to_host(self.d_kps, np.float32, (n, c, 6))
         desc = self.ctx.to_host(self.d_desc, np.uint8, (n, c, 32))
@@ -353,6 +364,9 @@ class BALinearizer:
         pl = self.read()
         e = np.stack([pl[0], pl[1]], 1)
         return e, pl[2], pl[3], pl[4:16].T.reshape(-1, 2, 6), pl[16:22].T.reshape(-1, 2, 3)
+
+    def copy_planes_dev(self, first: int, count: int, d_ptr: int):
+        check(self.ctx.h, lib().mcv_ba_copy_planes_dev(self.h, first, count, ctypes.c_void_p(d_ptr)))
 
     def adjoint_dev(self, g_ptr: int, gpose_ptr: int, gpoint_ptr: int):
         check(self.ctx.h, lib().mcv_ba_adjoint_dev(self.h, ctypes.c_void_p(g_ptr), ctypes.c_void_p(gpose_ptr),

@@ -535,31 +535,80 @@ struct OrbPlan {
         return g;
     }
 
-    // The whole extraction for the batch, enqueued on s.
+    // The whole extraction for the batch, enqueued on s.  With `rec`, an
+    // event is recorded after every launch (kernel timing, never in a graph).
     void enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s, mcv_keypoint* d_kps, uint8_t* d_desc,
-                 int out_cap, int* d_counts) {
-        const OrbGlobals g = globals(c);
-        hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(d_used.p), 4);
-        const Tile* t = d_tiles.as<Tile>();
-        int n0 = tile_begin[1] - tile_begin[0];
-        if (n0 > 0)
-            hipLaunchKernelGGL(k_copy_level0, dim3(n0), dim3(256), 0, s, t + tile_begin[0], g.lv,
-                               d_src.as<const uint8_t*>(), d_stride.as<int>(), g.pyr);
-        for (int l = 1; l < nlevels; l++) {
-            const int n = tile_begin[l + 1] - tile_begin[l];
-            if (n > 0) hipLaunchKernelGGL(k_resize, dim3(n), dim3(256), 0, s, t + tile_begin[l], g.lv, d_coef.as<int2>(), g.pyr);
+                 int out_cap, int* d_counts, struct KernelTimer* rec = nullptr);
+
+};
+
+// Events after every launch of a sequence: kernel i ran between events i
+// and i + 1 on the same stream.
+struct KernelTimer {
+    std::vector<hipEvent_t> ev;
+    std::vector<const char*> names;
+    hipStream_t s = nullptr;
+    void start(hipStream_t st) {
+        s = st;
+        names.clear();
+        mark(nullptr);
+    }
+    void mark(const char* name) {
+        if (name) names.push_back(name);
+        const size_t k = names.size();
+        if (ev.size() <= k) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            ev.push_back(e);
         }
-        BlurTaps taps;
-        for (int k = 0; k < 7; k++) taps.t[k] = c.gauss[k];
-        if (!tiles.empty()) hipLaunchKernelGGL(k_blur, dim3(tiles.size()), dim3(256), 0, s, t, g.lv, g.pyr, g.blur, taps);
-        if (!cells.empty()) hipLaunchKernelGGL(k_fast_cells, dim3(cells.size()), dim3(256), 0, s, g);
-        hipLaunchKernelGGL(k_octree_build, dim3(lv.size()), dim3(kOctThreads), 0, s, g);
-        hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), 0, s, g);
-        hipLaunchKernelGGL(k_keypoints, dim3((n_slots + 3) / 4), dim3(256), 0, s, g, d_slot.as<int>(), n_slots, tb,
-                           reinterpret_cast<float*>(d_kps), d_desc, out_cap, d_counts);
-        HIP_OK(hipGetLastError());
+        HIP_OK(hipEventRecord(ev[k], s));
+    }
+    ~KernelTimer() {
+        for (auto e : ev) (void)hipEventDestroy(e);
     }
 };
+
+void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s, mcv_keypoint* d_kps, uint8_t* d_desc,
+                      int out_cap, int* d_counts, KernelTimer* rec) {
+    auto mark = [&](const char* n) {
+        if (rec) rec->mark(n);
+    };
+    const OrbGlobals g = globals(c);
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(d_used.p), 4);
+    mark("k_zero_words");
+    const Tile* t = d_tiles.as<Tile>();
+    int n0 = tile_begin[1] - tile_begin[0];
+    if (n0 > 0) {
+        hipLaunchKernelGGL(k_copy_level0, dim3(n0), dim3(256), 0, s, t + tile_begin[0], g.lv,
+                           d_src.as<const uint8_t*>(), d_stride.as<int>(), g.pyr);
+        mark("k_copy_level0");
+    }
+    for (int l = 1; l < nlevels; l++) {
+        const int n = tile_begin[l + 1] - tile_begin[l];
+        if (n > 0) {
+            hipLaunchKernelGGL(k_resize, dim3(n), dim3(256), 0, s, t + tile_begin[l], g.lv, d_coef.as<int2>(), g.pyr);
+            mark("k_resize");
+        }
+    }
+    BlurTaps taps;
+    for (int k = 0; k < 7; k++) taps.t[k] = c.gauss[k];
+    if (!tiles.empty()) {
+        hipLaunchKernelGGL(k_blur, dim3(tiles.size()), dim3(256), 0, s, t, g.lv, g.pyr, g.blur, taps);
+        mark("k_blur");
+    }
+    if (!cells.empty()) {
+        hipLaunchKernelGGL(k_fast_cells, dim3(cells.size()), dim3(256), 0, s, g);
+        mark("k_fast_cells");
+    }
+    hipLaunchKernelGGL(k_octree_build, dim3(lv.size()), dim3(kOctThreads), 0, s, g);
+    mark("k_octree_build");
+    hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), 0, s, g);
+    mark("k_octree_heap");
+    hipLaunchKernelGGL(k_keypoints, dim3((n_slots + 3) / 4), dim3(256), 0, s, g, d_slot.as<int>(), n_slots, tb,
+                       reinterpret_cast<float*>(d_kps), d_desc, out_cap, d_counts);
+    mark("k_keypoints");
+    HIP_OK(hipGetLastError());
+}
 
 }  // namespace
 
@@ -612,6 +661,7 @@ struct mcv_rig {
     std::vector<FilterJob> hj;
     hipGraphExec_t graph = nullptr;
     std::vector<const uint8_t*> graph_inputs;
+    KernelTimer timer;
     ~mcv_rig() {
         if (graph) (void)hipGraphExecDestroy(graph);
     }
@@ -734,7 +784,7 @@ int mcv_orb_capacity(const mcv_ctx* c, int w, int h, int n_seeds, int32_t* cap) 
     return MCV_OK;
 }
 
-static long long pool_words_for(int n_images) { return 4ll * 1024 * 1024 + (long long)n_images * 1024 * 1024; }
+static long long pool_words_for(int n_images) { return 4ll * 1024 * 1024 + (long long)n_images * 2 * 1024 * 1024; }
 
 int mcv_orb_extract(mcv_ctx* c, const uint8_t* image, int w, int h, int stride, const mcv_keypoint* seeds, int n_seeds,
                     mcv_keypoint* kps, uint8_t* desc, int capacity, int32_t* count) {
@@ -772,15 +822,22 @@ int mcv_orb_extract(mcv_ctx* c, const uint8_t* image, int w, int h, int stride, 
     c->out_kps.ensure((size_t)P.cap * sizeof(mcv_keypoint));
     c->out_desc.ensure((size_t)P.cap * 32);
     c->out_cnt.ensure(16);
-    P.enqueue(c->orb, tb, c->stream, c->out_kps.as<mcv_keypoint>(), c->out_desc.as<uint8_t>(), P.cap, c->out_cnt.as<int>());
     int n = 0;
-    int err[2] = {0, 0};
-    HIP_OK(hipMemcpyAsync(&n, c->out_cnt.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipMemcpyAsync(err, P.d_used.as<char>() + 8, 8, hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
-    REQUIRE(err[0] == 0, MCV_ERR_STATE,
-            err[0] & 2 ? "a pyramid level has more FAST candidates than the octree sort holds"
-                       : "octree node pool exhausted");
+    for (int attempt = 0;; attempt++) {
+        P.enqueue(c->orb, tb, c->stream, c->out_kps.as<mcv_keypoint>(), c->out_desc.as<uint8_t>(), P.cap,
+                  c->out_cnt.as<int>());
+        unsigned long long st[2] = {0, 0};  // pool words requested, error bits
+        HIP_OK(hipMemcpyAsync(&n, c->out_cnt.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(st, P.d_used.p, 16, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        const int err = (int)(st[1] & 0xFFFFFFFFu);
+        if (err == 0) break;
+        REQUIRE(!(err & 2), MCV_ERR_STATE, "a pyramid level has more FAST candidates than its cell capacity");
+        // pool exhausted: the counter holds what the run asked for
+        REQUIRE(attempt < 2, MCV_ERR_STATE, "octree node pool exhausted");
+        P.pool_cap = (long long)(st[0] + st[0] / 4 + 1024);
+        P.d_pool.ensure((size_t)P.pool_cap * 4);
+    }
     *count = n;
     if (n > capacity) throw Fail{MCV_ERR_CAPACITY, "keypoint capacity too small"};
     if (n > 0) {
@@ -980,13 +1037,44 @@ int mcv_rig_create(mcv_ctx* c, int n_frames, const int32_t* w3, const int32_t* h
 
 void mcv_rig_destroy(mcv_rig* r) { delete r; }
 
-static void rig_enqueue_batched(mcv_rig* r, const SinCosTables& tb, hipStream_t s) {
+static void rig_enqueue_batched(mcv_rig* r, const SinCosTables& tb, hipStream_t s, KernelTimer* rec = nullptr) {
     OrbPlan& P = *r->batch;
-    P.enqueue(r->ctx->orb, tb, s, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>());
+    P.enqueue(r->ctx->orb, tb, s, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>(), rec);
     const int np = (int)r->hp.size();
     hipLaunchKernelGGL(k_knn2, dim3((r->cap + 63) / 64, np), dim3(256), 0, s, r->pairs.as<MatchPair>());
+    if (rec) rec->mark("k_knn2");
     hipLaunchKernelGGL(k_filter, dim3(np), dim3(1024), 0, s, r->jobs.as<FilterJob>());
+    if (rec) rec->mark("k_filter");
     HIP_OK(hipGetLastError());
+}
+
+// The batched sequence launched eagerly with an event after every kernel;
+// ms[i] is the duration of the i-th launch, names[i] its kernel.
+int mcv_rig_run_timed(mcv_rig* r, const uint8_t* const* d_images, float* ms, const char** names, int cap, int* n) {
+    mcv_ctx* c = r ? r->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(r && d_images && ms && n, MCV_ERR_INVALID, "bad timed run");
+    c->set_device();
+    const SinCosTables tb = c->tables();
+    hipStream_t s = c->stream;
+    OrbPlan& P = *r->batch;
+    const int ni = 3 * r->n_frames;
+    std::vector<int> st(ni);
+    for (int i = 0; i < ni; i++) st[i] = r->w[i % 3];
+    HIP_OK(hipMemcpyAsync(P.d_src.p, d_images, sizeof(void*) * ni, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(P.d_stride.p, st.data(), sizeof(int) * ni, hipMemcpyHostToDevice, s));
+    r->timer.start(s);
+    rig_enqueue_batched(r, tb, s, &r->timer);
+    HIP_OK(hipStreamSynchronize(s));
+    const int k = (int)r->timer.names.size();
+    *n = k;
+    for (int i = 0; i < k && i < cap; i++) {
+        float t = 0;
+        HIP_OK(hipEventElapsedTime(&t, r->timer.ev[i], r->timer.ev[i + 1]));
+        ms[i] = t;
+        if (names) names[i] = r->timer.names[i];
+    }
+    API_END
 }
 
 int mcv_rig_run(mcv_rig* r, const uint8_t* const* d_images, int flags) {
@@ -1033,6 +1121,32 @@ int mcv_rig_run(mcv_rig* r, const uint8_t* const* d_images, int flags) {
             HIP_OK(hipStreamSynchronize(s));
         }
         HIP_OK(hipGetLastError());
+    }
+    API_END
+}
+
+int mcv_rig_check(mcv_rig* r) {
+    mcv_ctx* c = r ? r->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(r, MCV_ERR_INVALID, "null rig");
+    c->set_device();
+    HIP_OK(hipStreamSynchronize(c->stream));
+    for (int k = -1; k < 3; k++) {
+        OrbPlan& P = k < 0 ? *r->batch : *r->single[k];
+        unsigned long long st[2] = {0, 0};
+        HIP_OK(hipMemcpy(st, P.d_used.p, 16, hipMemcpyDeviceToHost));
+        const int err = (int)(st[1] & 0xFFFFFFFFu);
+        if (err & 1) {
+            // grow for the next run; the captured graph holds the old pool
+            P.pool_cap = (long long)(st[0] + st[0] / 4 + 1024);
+            P.d_pool.ensure((size_t)P.pool_cap * 4);
+            if (k < 0 && r->graph) {
+                HIP_OK(hipGraphExecDestroy(r->graph));
+                r->graph = nullptr;
+            }
+            throw Fail{MCV_ERR_STATE, "octree node pool exhausted in the last run (grown for the next one)"};
+        }
+        REQUIRE(!(err & 2), MCV_ERR_STATE, "a pyramid level has more FAST candidates than its cell capacity");
     }
     API_END
 }
@@ -1326,6 +1440,17 @@ int mcv_ba_read_planes(mcv_ba* b, int first, int count, float* host) {
         HIP_OK(hipMemcpyAsync(host, b->planes.as<float>() + (size_t)first * b->M, (size_t)count * b->M * 4,
                               hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
+    API_END
+}
+
+int mcv_ba_copy_planes_dev(mcv_ba* b, int first, int count, float* d_dst) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && d_dst && first >= 0 && count >= 0 && first + count <= 22, MCV_ERR_INVALID, "bad plane range");
+    c->set_device();
+    if (b->M && count)
+        HIP_OK(hipMemcpyAsync(d_dst, b->planes.as<float>() + (size_t)first * b->M, (size_t)count * b->M * 4,
+                              hipMemcpyDeviceToDevice, c->stream));
     API_END
 }
 

@@ -359,8 +359,7 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g) {
         run += tot;
     }
     const int n = run;
-    if (n > L.cand_cap || n > kSortCapLds) {
-        // candidate list larger than the LDS sort: reported, level emits nothing
+    if (n > L.cand_cap) {
         if (threadIdx.x == 0) {
             O->n = n; O->failed = 2; O->n_roots = 0; O->depth = 0; O->total = 0; O->nout = 0;
             atomicOr(g.err, 2);
@@ -372,9 +371,33 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g) {
         return;
     }
     __syncthreads();
-    // 2. keys = path << 24 | candidate index; sort
     int n2 = 1;
     while (n2 < n) n2 <<= 1;
+    // keys and divergence bytes live in LDS; a level with more candidates
+    // than the LDS sort holds uses pool memory instead (same code, flat
+    // pointers; the workgroup's waves share one L1, so barriers order it)
+    unsigned long long* keys = s_keys;
+    uint8_t* dv = s_div;
+    if (n > kSortCapLds) {
+        if (threadIdx.x == 0) {
+            const long long need = 2ll * n2 + (n + 3) / 4 + 2;
+            const unsigned long long off = atomicAdd(g.pool_used, (unsigned long long)need);
+            s_off = ((long long)off + need <= g.pool_cap) ? (long long)off : -1;
+        }
+        __syncthreads();
+        if (s_off < 0) {
+            if (threadIdx.x == 0) {
+                O->n = n; O->failed = 1; O->n_roots = 0; O->depth = 0; O->total = 0; O->nout = 0;
+                atomicOr(g.err, 1);
+            }
+            return;
+        }
+        const long long base = (s_off + 1) & ~1ll;  // 8-byte aligned
+        keys = reinterpret_cast<unsigned long long*>(g.pool + base);
+        dv = reinterpret_cast<uint8_t*>(g.pool + base + 2ll * n2);
+        __syncthreads();
+    }
+    // 2. keys = path << 24 | candidate index; sort
     for (int i = threadIdx.x; i < n2; i += kOctThreads) {
         unsigned long long k = ~0ull;
         if (i < n) {
@@ -382,18 +405,18 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g) {
             const int x = r & 0xFFF, y = (r >> 12) & 0xFFF;
             k = (path_code(x, y, L) << 24) | (unsigned long long)i;
         }
-        s_keys[i] = k;
+        keys[i] = k;
     }
     __syncthreads();
-    bitonic_sort(s_keys, n2);
+    bitonic_sort(keys, n2);
     // 3. divergence depth of each sorted position, histogram, deepest depth
     if (threadIdx.x < kOctreeDepth + 2) s_hist[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_maxdiv = 0;
     __syncthreads();
     int mymax = 0;
     for (int i = threadIdx.x; i < n; i += kOctThreads) {
-        const int d = i == 0 ? 0 : code_div(s_keys[i] >> 24, s_keys[i - 1] >> 24);
-        s_div[i] = (uint8_t)d;
+        const int d = i == 0 ? 0 : code_div(keys[i] >> 24, keys[i - 1] >> 24);
+        dv[i] = (uint8_t)d;
         atomicAdd(&s_hist[d], 1);
         mymax = max(mymax, d);
     }
@@ -428,23 +451,23 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g) {
     uint32_t* start = g.pool + off;
     uint32_t* child = start + T;
     uint32_t* sorted = child + T;
-    for (int i = threadIdx.x; i < n; i += kOctThreads) sorted[i] = (uint32_t)(s_keys[i] & 0xFFFFFF);
+    for (int i = threadIdx.x; i < n; i += kOctThreads) sorted[i] = (uint32_t)(keys[i] & 0xFFFFFF);
     __syncthreads();
     // 4. node tables, depth by depth.  rank[] (reusing the key storage) holds
     //    each position's node index at the previous depth.
-    int* rank = reinterpret_cast<int*>(s_keys);
+    int* rank = reinterpret_cast<int*>(keys);
     int doff = 0, prev_off = 0;
     for (int d = 0; d <= D; d++) {
         int run2 = 0;
         for (int i0 = 0; i0 < n; i0 += kOctThreads) {
             const int i = i0 + threadIdx.x;
-            const int dv = i < n ? (int)s_div[i] : 255;
-            const int f = dv <= d ? 1 : 0;
+            const int dvi = i < n ? (int)dv[i] : 255;
+            const int f = dvi <= d ? 1 : 0;
             int tot;
             const int r = run2 + block_excl_scan256(f, ws, tot);
             if (f) {
                 start[doff + r] = (uint32_t)i;
-                if (d > 0 && dv <= d - 1) child[prev_off + rank[i]] = (uint32_t)(doff + r);
+                if (d > 0 && dvi <= d - 1) child[prev_off + rank[i]] = (uint32_t)(doff + r);
                 rank[i] = r;  // only read by the same thread, at the next depth
             }
             run2 += tot;

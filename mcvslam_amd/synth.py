@@ -237,7 +237,9 @@ def ba_problem(seed: int = 30, n_kf: int = 10, n_points: int = 20000, target_obs
     cams_geo = rig_cameras()
     cams = []
     for c in cams_geo:
-        cams.append([c.fx, c.fy, c.cx, c.cy, 0, 0, 0, 1, -c.baseline_x, 0, 0, delta])
+        # Pinhole keeps its intrinsics as float (modules/camera/BaseCamera.hpp:37-43)
+        f32 = lambda v: float(np.float32(v))
+        cams.append([f32(c.fx), f32(c.fy), f32(c.cx), f32(c.cy), 0, 0, 0, 1, -c.baseline_x, 0, 0, delta])
     cams = np.array(cams, np.float64)
     frames = []
     Rs, ts = [], []

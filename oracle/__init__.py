@@ -30,6 +30,9 @@ _SIG = {
     "orc_fast_atan2": (F, [F, F]),
     "orc_octree": (I, [P, I, I, I, I, I, I, P, I]),
     "orc_orb_extract": (I, [P, I, I, I, F, I, I, I, P, I, P, P, I, P, P, P, P, I]),
+    "orc_ic_angle": (F, [P, I, I, F, F]),
+    "orc_descriptor": (None, [P, I, I, F, F, F, P]),
+    "orc_libm_sincos": (None, [P, I, P, P]),
     "orc_knn2": (None, [P, I, P, I, P]),
     "orc_hamming": (ctypes.c_uint, [P, P]),
     "orc_filter": (I, [P, I, F, I, P]),
@@ -164,6 +167,26 @@ def orb_candidates(img: np.ndarray, nfeatures=2000, scale=1.2, nlevels=8, ini=20
     lib().orc_orb_extract(_p(im), w, h, nfeatures, scale, nlevels, ini, mn, None, 0, _p(kps), _p(desc), cap,
                           _p(info), None, None, _p(cand), ccap)
     return cand[:int(info[:, 2].sum())].copy(), info
+
+
+def ic_angle(img: np.ndarray, x: float, y: float) -> float:
+    im = np.ascontiguousarray(img, np.uint8)
+    return float(lib().orc_ic_angle(_p(im), im.shape[1], im.shape[0], x, y))
+
+
+def descriptor(img: np.ndarray, x: float, y: float, angle: float) -> np.ndarray:
+    im = np.ascontiguousarray(img, np.uint8)
+    out = np.zeros(32, np.uint8)
+    lib().orc_descriptor(_p(im), im.shape[1], im.shape[0], x, y, angle, _p(out))
+    return out
+
+
+def libm_sincos(x: np.ndarray):
+    xs = np.ascontiguousarray(x, np.float32)
+    c = np.zeros_like(xs)
+    s = np.zeros_like(xs)
+    lib().orc_libm_sincos(_p(xs), len(xs), _p(c), _p(s))
+    return c, s
 
 
 def knn2(q: np.ndarray, t: np.ndarray) -> np.ndarray:

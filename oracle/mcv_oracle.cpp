@@ -963,6 +963,31 @@ int orc_orb_extract(const uint8_t* img, int w, int h, int nf, float sf, int nl, 
     return n;
 }
 
+// IC_Angle of one keypoint on an image (umax of ORBextractor::init)
+float orc_ic_angle(const uint8_t* img, int w, int h, float x, float y) {
+    Orb o;
+    o.init(500, 1.2f, 1, 20, 7);
+    Image im = make_image(img, w, h);
+    return o.icAngle(im, x, y);
+}
+
+// computeOrbDescriptor of one keypoint on an (already blurred) image
+void orc_descriptor(const uint8_t* img, int w, int h, float x, float y, float angle, uint8_t* out) {
+    Orb o;
+    o.init(500, 1.2f, 1, 20, 7);
+    Image im = make_image(img, w, h);
+    KP k{x, y, 31.f, angle, 0.f, 0};
+    o.descriptor(k, im, out);
+}
+
+// host libm cosf / sinf over an array (the functions the reference calls)
+void orc_libm_sincos(const float* x, int n, float* c, float* s) {
+    for (int i = 0; i < n; i++) {
+        c[i] = cosf(x[i]);
+        s[i] = sinf(x[i]);
+    }
+}
+
 // Matcher::KnnMatch(vector<Mat>, vector<Mat>) -> rows (i0, d0, i1, d1)
 void orc_knn2(const uint8_t* q, int nq, const uint8_t* t, int nt, int32_t* out) {
     for (int i = 0; i < nq; i++) top2(q + 32 * (size_t)i, t, nt, nullptr, out + 4 * i);

@@ -1,0 +1,231 @@
+"""Pins the oracle's matcher, window search and FP64 BA edge with brute force,
+closed forms, known answers and finite differences (no retyped code)."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from mcvslam_amd import synth
+
+
+# ---------------------------------------------------------------- matcher
+def test_hamming_popcount():
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        a, b = rng.integers(0, 256, 32, dtype=np.uint8), rng.integers(0, 256, 32, dtype=np.uint8)
+        assert oracle.hamming(a, b) == int(np.unpackbits(a ^ b).sum())
+
+
+def knn_brute(q, t):
+    """first two entries of a stable sort by distance; sentinel (0, 999)"""
+    out = np.zeros((len(q), 4), np.int32)
+    for i, r in enumerate(q):
+        d = np.unpackbits(r[None] ^ t, axis=1).sum(1) if len(t) else np.zeros(0, int)
+        order = np.argsort(d, kind="stable")
+        best = [(int(j), int(d[j])) for j in order[:2]] + [(0, 999)] * 2
+        out[i] = [best[0][0], best[0][1], best[1][0], best[1][1]]
+    return out
+
+
+@pytest.mark.parametrize("nq,nt", [(3, 0), (4, 1), (20, 2), (50, 300)])
+def test_knn_matches_stable_sort(nq, nt):
+    rng = np.random.default_rng(nq + nt)
+    q = rng.integers(0, 256, (nq, 32), dtype=np.uint8)
+    t = rng.integers(0, 256, (nt, 32), dtype=np.uint8)
+    if nt > 10:
+        t[5] = t[7] = q[0]        # equal distances: scan order decides
+    np.testing.assert_array_equal(oracle.knn2(q, t), knn_brute(q, t))
+
+
+def test_filter_known_answers():
+    # ratio: d0/d1 <= r in float; (x, 999) passes, 999/999 and 0/0 do not
+    knn = np.array([[1, 10, 2, 20], [0, 999, 0, 999], [3, 5, 0, 999], [4, 0, 5, 0], [6, 14, 7, 20]], np.int32)
+    r = oracle.filter_ratio_threshold(knn, 0.7, 1000)
+    np.testing.assert_array_equal(r, [[0, 1, 10], [2, 3, 5], [4, 6, 14]])
+    # threshold swap-with-last: [a bad, b ok, c bad, d ok, e ok] -> [e, b, d]
+    knn = np.array([[0, 60, 1, 999], [1, 10, 2, 999], [2, 70, 3, 999], [3, 20, 4, 999], [4, 30, 5, 999]], np.int32)
+    r = oracle.filter_ratio_threshold(knn, 0.9, 50)
+    np.testing.assert_array_equal(r[:, 0], [4, 1, 3])
+    # bad tail: [a bad, b ok, c bad] -> [b]
+    knn = np.array([[0, 60, 1, 999], [1, 10, 2, 999], [2, 70, 3, 999]], np.int32)
+    np.testing.assert_array_equal(oracle.filter_ratio_threshold(knn, 0.9, 50)[:, 0], [1])
+
+
+def test_filter_keeps_exactly_the_passing_set():
+    rng = np.random.default_rng(1)
+    d0 = rng.integers(0, 120, 500)
+    d1 = np.maximum(d0, rng.integers(0, 200, 500))
+    knn = np.stack([np.arange(500), d0, np.arange(500), d1], 1).astype(np.int32)
+    r = oracle.filter_ratio_threshold(knn, 0.7, 50)
+    ok = (d0.astype(np.float32) / d1.astype(np.float32) <= np.float32(0.7)) & (d0 <= 50)
+    assert sorted(r[:, 0].tolist()) == np.nonzero(ok)[0].tolist()
+
+
+# ---------------------------------------------------------------- window search
+def window_brute(kxy, w, h, x, y, r):
+    """keypoints whose grid cell (round) lies in the scanned cell range and
+    that pass the strict window test, ordered by (cell x, cell y, index)"""
+    if x < 0 or y < 0 or x >= w or y >= h:
+        return []
+    inv_w, inv_h = np.float32(30.0 / w), np.float32(30.0 / h)
+    x, y, r = np.float32(x), np.float32(y), np.float32(r)
+    x0 = max(0, int(np.floor((x - r) * inv_w)))
+    x1 = min(29, int(np.ceil((x + r) * inv_w)))
+    y0 = max(0, int(np.floor((y - r) * inv_h)))
+    y1 = min(29, int(np.ceil((y + r) * inv_h)))
+    out = []
+    for i, (kx, ky) in enumerate(kxy):
+        # std::round: half away from zero
+        cx = int(np.floor(np.float32(kx) * inv_w + np.float32(0.5)))
+        cy = int(np.floor(np.float32(ky) * inv_h + np.float32(0.5)))
+        if not (0 <= cx < 30 and 0 <= cy < 30):
+            continue
+        if x0 <= cx <= x1 and y0 <= cy <= y1 and abs(np.float32(kx) - x) < r and abs(np.float32(ky) - y) < r:
+            out.append((cx, cy, i))
+    return [i for _, _, i in sorted(out)]
+
+
+def test_window_matches_brute_force():
+    rng = np.random.default_rng(2)
+    w, h = 640, 480
+    kxy = np.stack([rng.uniform(0, w, 3000), rng.uniform(0, h, 3000)], 1).astype(np.float32)
+    koct = np.zeros(3000, np.int32)
+    for x, y, r in [(100.3, 200.1, 15.0), (0.0, 0.0, 30.0), (639.9, 479.9, 20.0), (320, 240, 54.0),
+                    (-1.0, 10.0, 10.0), (700.0, 10.0, 10.0), (21.33333, 16.0, 8.0)]:
+        got = oracle.window(kxy, koct, w, h, x, y, r)
+        assert got.tolist() == window_brute(kxy, w, h, x, y, r)
+
+
+def test_window_round_not_floor():
+    """PosInGrid rounds half away from zero: x = 10.0 px of a 600 px image is
+    in cell round(0.5) = 1, x = 9.9 px in cell 0 (src/Object.cpp:249-257)."""
+    kxy = np.array([[10.0, 10.0], [9.9, 10.0], [30.0, 10.0]], np.float32)
+    got = oracle.window(kxy, np.zeros(3, np.int32), 600, 600, 10.0, 10.0, 25.0)
+    assert got.tolist() == [1, 0, 2]
+    assert got.tolist() == window_brute(kxy, 600, 600, 10.0, 10.0, 25.0)
+
+
+# ---------------------------------------------------------------- BA FP64 edge
+def _se3_exp(xi):
+    """SE3Quat::exp (se3quat.h:218-257): omega first, then upsilon."""
+    w, u = np.asarray(xi[:3], float), np.asarray(xi[3:], float)
+    th = np.linalg.norm(w)
+    W = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+    W2 = W @ W
+    if th < 1e-5:
+        R = np.eye(3) + W + 0.5 * W2
+        V = np.eye(3) + 0.5 * W + W2 / 6.0
+    else:
+        R = np.eye(3) + math.sin(th) / th * W + (1 - math.cos(th)) / th ** 2 * W2
+        V = np.eye(3) + (1 - math.cos(th)) / th ** 2 * W + (th - math.sin(th)) / th ** 3 * W2
+    return R, V @ u
+
+
+def _R_to_quat(R):
+    w = math.sqrt(max(0.0, 1 + R[0, 0] + R[1, 1] + R[2, 2])) / 2
+    x = (R[2, 1] - R[1, 2]) / (4 * w)
+    y = (R[0, 2] - R[2, 0]) / (4 * w)
+    z = (R[1, 0] - R[0, 1]) / (4 * w)
+    return np.array([x, y, z, w])
+
+
+def _quat_to_R(q):
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def _perturbed(prob, f, xi):
+    """frames[f] <- exp(xi) * frames[f]  (VertexSE3Expmap::oplusImpl)"""
+    p = synth.BAProblem(**{k: np.copy(v) for k, v in prob.__dict__.items()})
+    q = p.frames[f, :4]
+    R = _quat_to_R(q / np.linalg.norm(q))
+    t = p.frames[f, 4:]
+    dR, dt = _se3_exp(xi)
+    p.frames[f, :4] = _R_to_quat(dR @ R)
+    p.frames[f, 4:] = dR @ t + dt
+    return p
+
+
+def test_jacobians_match_central_differences():
+    prob = synth.ba_problem(seed=30, n_kf=3, n_points=200, target_obs=600, outlier_frac=0.0)
+    e, chi2, w, Jp, Jx = oracle.ba_linearize(prob)
+    h = 1e-4
+    rng = np.random.default_rng(0)
+    for i in rng.choice(len(prob.obs_pt), 25, replace=False):
+        f, p = prob.obs_frame[i], prob.obs_pt[i]
+        for k in range(6):
+            xi = np.zeros(6)
+            xi[k] = h
+            ep = oracle.ba_linearize(_perturbed(prob, f, xi))[0][i]
+            em = oracle.ba_linearize(_perturbed(prob, f, -xi))[0][i]
+            fd = (ep - em) / (2 * h)
+            np.testing.assert_allclose(Jp[i, :, k], fd, rtol=1e-5, atol=1e-6 * np.abs(Jp[i]).max())
+        for k in range(3):
+            pp = synth.BAProblem(**{kk: np.copy(v) for kk, v in prob.__dict__.items()})
+            pm = synth.BAProblem(**{kk: np.copy(v) for kk, v in prob.__dict__.items()})
+            pp.points[p, k] += h
+            pm.points[p, k] -= h
+            fd = (oracle.ba_linearize(pp)[0][i] - oracle.ba_linearize(pm)[0][i]) / (2 * h)
+            np.testing.assert_allclose(Jx[i, :, k], fd, rtol=1e-5, atol=1e-6 * np.abs(Jx[i]).max())
+
+
+def test_jacobian_equals_g2o_project_xyz_for_identity_extrinsics():
+    """With Tcb = I the edge is g2o's EdgeSE3ProjectXYZ, whose Jacobian
+    (types_six_dof_expmap.cpp:289-325) is written out in closed form."""
+    prob = synth.ba_problem(seed=3, n_kf=2, n_points=300, target_obs=400)
+    sel = prob.obs_cam == 0
+    e, chi2, w, Jp, Jx = oracle.ba_linearize(prob)
+    for i in np.nonzero(sel)[0][:50]:
+        q = prob.frames[prob.obs_frame[i], :4]
+        R = _quat_to_R(q / np.linalg.norm(q))
+        pc = R @ prob.points[prob.obs_pt[i]] + prob.frames[prob.obs_frame[i], 4:]
+        x, y, z = pc
+        fx, fy = prob.cams[0, 0], prob.cams[0, 1]
+        z2 = z * z
+        want = np.array([[x * y / z2 * fx, -(1 + x * x / z2) * fx, y / z * fx, -1 / z * fx, 0, x / z2 * fx],
+                         [(1 + y * y / z2) * fy, -x * y / z2 * fy, -x / z * fy, 0, -1 / z * fy, y / z2 * fy]])
+        np.testing.assert_allclose(Jp[i], want, rtol=1e-9, atol=1e-9)
+        jpt = -np.array([[fx / z, 0, -fx * x / z2], [0, fy / z, -fy * y / z2]]) @ R
+        np.testing.assert_allclose(Jx[i], jpt, rtol=1e-9, atol=1e-12)
+
+
+def test_huber_weight_closed_form():
+    prob = synth.ba_problem(seed=4, n_kf=3, n_points=500, target_obs=3000)
+    e, chi2, w, _, _ = oracle.ba_linearize(prob)
+    np.testing.assert_allclose(chi2, prob.obs_isig * (e ** 2).sum(1), rtol=1e-12)
+    d = prob.cams[prob.obs_cam, 11]
+    want = np.where(chi2 <= d * d, 1.0, d / np.sqrt(chi2))
+    np.testing.assert_allclose(w, want, rtol=1e-12)
+    assert (w < 1).mean() > 0.03
+
+
+def test_schur_matches_dense_elimination():
+    prob = synth.ba_problem(seed=5, n_kf=4, n_points=60, target_obs=500)
+    P, F = len(prob.points), len(prob.frames)
+    e, chi2, w, Jp, Jx = oracle.ba_linearize(prob)
+    lam = 1e-3
+    r = oracle.ba_schur(e, w, Jp, Jx, prob.obs_pt, prob.obs_frame, prob.obs_isig, P, F, prob.frame_fixed, lam)
+    free = [f for f in range(F) if not prob.frame_fixed[f]]
+    n = 6 * len(free) + 3 * P
+    H = np.zeros((n, n))
+    b = np.zeros(n)
+    for i in range(len(prob.obs_pt)):
+        J = np.zeros((2, n))
+        pi = 6 * len(free) + 3 * prob.obs_pt[i]
+        J[:, pi:pi + 3] = Jx[i]
+        if prob.obs_frame[i] in free:
+            fi = 6 * free.index(prob.obs_frame[i])
+            J[:, fi:fi + 6] = Jp[i]
+        Om = prob.obs_isig[i] * w[i] * np.eye(2)
+        H += J.T @ Om @ J
+        b += -J.T @ Om @ e[i]
+    H += lam * np.eye(n)
+    m = 6 * len(free)
+    A, B, C = H[:m, :m], H[:m, m:], H[m:, m:]
+    S = A - B @ np.linalg.solve(C, B.T)
+    bs = b[:m] - B @ np.linalg.solve(C, b[m:])
+    np.testing.assert_allclose(r["S"], S, rtol=1e-8, atol=1e-6 * np.abs(S).max())
+    np.testing.assert_allclose(r["bs"], bs, rtol=1e-8, atol=1e-8 * np.abs(bs).max())

@@ -1,0 +1,274 @@
+"""Pins the CPU oracle's ORB restatement (ORBextractor.cc) with checks that do
+not retype its code: closed forms, brute force on small inputs, known
+answers, invariants, and the reference's own pattern table when the
+reference tree is present.  OpenCV-internal bit choices (resize rounding,
+blur kernel quantisation, fastAtan2 polynomial) are pinned by invariants
+only: no OpenCV exists here to compare bits against."""
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from mcvslam_amd import synth
+
+REF = Path("/root/reference/modules/local_feature/ORB/orb3_extractor/ORBextractor.cc")
+
+
+# ---------------------------------------------------------------- init tables
+def test_tables_closed_form():
+    t = oracle.orb_tables(2000, 1.2, 8, 20, 7)
+    s = [np.float32(1.0)]
+    for _ in range(7):
+        s.append(np.float32(s[-1] * np.float32(1.2)))
+    np.testing.assert_array_equal(t["scale"], np.array(s, np.float32))
+    np.testing.assert_array_equal(t["sigma2"], np.array([x * x for x in s], np.float32))
+    np.testing.assert_array_equal(t["inv_scale"], np.float32(1) / np.array(s, np.float32))
+    # quotas: geometric series n(1-f)/(1-f^L) rounded, remainder on the last level
+    assert t["npl"].tolist() == [434, 362, 302, 251, 209, 175, 145, 122]
+    assert t["npl"].sum() == 2000
+    # umax: discrete quarter circle of radius 15, made symmetric
+    assert t["umax"].tolist() == [15, 15, 15, 15, 14, 14, 14, 13, 13, 12, 11, 10, 9, 8, 6, 3]
+    for v in range(16):   # symmetric: u <= umax[v] <=> v <= umax[u]
+        for u in range(16):
+            assert (u <= t["umax"][v]) == (v <= t["umax"][u])
+
+
+def test_gauss_taps():
+    g = oracle.orb_tables()["gauss"]
+    assert g.sum() == 256
+    assert g.tolist() == g[::-1].tolist()
+    exact = np.exp(-(np.arange(-3, 4) ** 2) / 8.0)
+    exact = exact / exact.sum() * 256
+    assert np.abs(g - exact).max() < 1.0
+    assert g.tolist() == [18, 34, 48, 56, 48, 34, 18]
+
+
+@pytest.mark.skipif(not REF.exists(), reason="reference tree not mounted")
+def test_pattern_table_is_the_reference_table():
+    text = REF.read_text()
+    start = text.index("bit_pattern_31_[256 * 4]")
+    body = re.sub(r"/\*.*?\*/", " ", text[text.index("{", start) + 1:text.index("};", start)], flags=re.S)
+    ref = [int(v) for v in body.replace(",", " ").split()]
+    hdr = (Path(__file__).resolve().parent.parent / "mcvslam_amd/csrc/orb_pattern.h").read_text()
+    ours = [int(v) for v in re.findall(r"-?\d+", hdr[hdr.index("= {"):])]
+    assert ours == ref
+
+
+# ---------------------------------------------------------------- resize
+def test_resize_invariants():
+    rng = np.random.default_rng(0)
+    for v in (0, 1, 77, 254, 255):
+        img = np.full((97, 131), v, np.uint8)
+        assert (oracle.resize(img, 109, 81) == v).all()
+    img = rng.integers(0, 256, (60, 80), dtype=np.uint8)
+    np.testing.assert_array_equal(oracle.resize(img, 80, 60), img)   # same size copies
+    # horizontal ramp stays a non-decreasing ramp, within the source range
+    ramp = np.tile(np.arange(0, 240, 2, dtype=np.uint8), (50, 1))
+    r = oracle.resize(ramp, 100, 42)
+    assert (np.diff(r.astype(int), axis=1) >= 0).all()
+    # rows differ by at most one level (the two vertical taps round separately)
+    assert np.abs(r.astype(int) - r[0].astype(int)).max() <= 1
+    # every output lies between the min and max of its 2x2 source support
+    src = rng.integers(0, 256, (240, 320), dtype=np.uint8)
+    out = oracle.resize(src, 267, 200)
+    sx = (np.arange(267) + 0.5) * (320 / 267) - 0.5
+    sy = (np.arange(200) + 0.5) * (240 / 200) - 0.5
+    x0 = np.clip(np.floor(sx).astype(int), 0, 319)
+    y0 = np.clip(np.floor(sy).astype(int), 0, 239)
+    x1 = np.minimum(x0 + 1, 319)
+    y1 = np.minimum(y0 + 1, 239)
+    q = np.stack([src[y0][:, x0], src[y0][:, x1], src[y1][:, x0], src[y1][:, x1]])
+    assert (out >= q.min(0)).all() and (out <= q.max(0)).all()
+    # and matches exact bilinear interpolation to within one grey level
+    fx, fy = sx - np.floor(sx), sy - np.floor(sy)
+    exact = (q[0] * (1 - fx) * (1 - fy[:, None]) + q[1] * fx * (1 - fy[:, None]) + q[2] * (1 - fx) * fy[:, None]
+             + q[3] * fx * fy[:, None])
+    assert np.abs(out - exact).max() <= 1.0
+
+
+# ---------------------------------------------------------------- blur
+def test_blur_impulse_and_invariants():
+    g = oracle.orb_tables()["gauss"].astype(np.int64)
+    img = np.zeros((41, 53), np.uint8)
+    img[20, 26] = 255
+    out = oracle.blur(img).astype(np.int64)
+    want = (np.outer(g, g) * 255 + (1 << 15)) >> 16
+    np.testing.assert_array_equal(out[17:24, 23:30], want)
+    assert out.sum() == want.sum()
+    for v in (0, 3, 128, 255):
+        assert (oracle.blur(np.full((30, 40), v, np.uint8)) == v).all()
+    rng = np.random.default_rng(1)
+    a = rng.integers(0, 256, (64, 64), dtype=np.uint8)
+    # REFLECT_101 + symmetric taps: mirroring commutes with the blur
+    np.testing.assert_array_equal(oracle.blur(a[:, ::-1]), oracle.blur(a)[:, ::-1])
+    np.testing.assert_array_equal(oracle.blur(a[::-1]), oracle.blur(a)[::-1])
+    # border pixel equals the blur of the explicitly reflected image
+    pad = np.pad(a, 3, mode="reflect")   # numpy 'reflect' == OpenCV REFLECT_101
+    full = oracle.blur(pad)
+    np.testing.assert_array_equal(oracle.blur(a)[0, :], full[3, 3:-3])
+
+
+# ---------------------------------------------------------------- FAST
+CIRCLE = [(0, 3), (1, 3), (2, 2), (3, 1), (3, 0), (3, -1), (2, -2), (1, -3), (0, -3), (-1, -3), (-2, -2),
+          (-3, -1), (-3, 0), (-3, 1), (-2, 2), (-1, 3)]
+
+
+def best_score(img, x, y):
+    """max over the 16 arcs of 9 of min(v - p) (darker) / min(p - v) (brighter)"""
+    v = int(img[y, x])
+    d = [v - int(img[y + dy, x + dx]) for dx, dy in CIRCLE]
+    b = -10 ** 9
+    for s in range(16):
+        arc = [d[(s + m) % 16] for m in range(9)]
+        b = max(b, min(arc), min(-a for a in arc))
+    return b
+
+
+def fast_brute(img, t):
+    """cv::FAST(img, t, nonmax=true) semantics restated as a closed form:
+    corner iff best > t, score = best - 1, 3x3 NMS where non-corners and
+    pixels outside [3, n-3) count as 0, output row-major."""
+    h, w = img.shape
+    sc = np.zeros((h, w), np.int64)
+    corner = np.zeros((h, w), bool)
+    for y in range(3, h - 3):
+        for x in range(3, w - 3):
+            b = best_score(img, x, y)
+            if b > t:
+                corner[y, x] = True
+                sc[y, x] = b - 1
+    out = []
+    for y in range(3, h - 3):
+        for x in range(3, w - 3):
+            if corner[y, x] and all(sc[y, x] > sc[y + dy, x + dx] for dy in (-1, 0, 1) for dx in (-1, 0, 1)
+                                    if dx or dy):
+                out.append((x, y, sc[y, x]))
+    return np.array(out, np.float32).reshape(-1, 3)
+
+
+@pytest.mark.parametrize("seed,th", [(0, 20), (1, 7), (2, 40), (3, 0)])
+def test_fast_matches_closed_form(seed, th):
+    rng = np.random.default_rng(seed)
+    img = synth.blob_image(seed, 48, 40)
+    if seed == 2:
+        img = rng.integers(0, 256, (30, 37), dtype=np.uint8)
+    np.testing.assert_array_equal(oracle.fast(img, th), fast_brute(img, th))
+
+
+def test_fast_known_answers():
+    img = np.full((21, 21), 100, np.uint8)
+    img[10, 10] = 180                      # isolated bright pixel: all 16 darker by 80
+    r = oracle.fast(img, 20)
+    np.testing.assert_array_equal(r, [[10, 10, 79]])
+    img[10, 10] = 110                      # contrast 10 <= threshold 20: no corner
+    assert len(oracle.fast(img, 20)) == 0
+    assert len(oracle.fast(np.zeros((6, 40), np.uint8), 0)) == 0   # too small for any detection row
+    sq = np.zeros((40, 40), np.uint8)
+    sq[20:, 20:] = 200                     # noiseless 90 degree corner: equal scores
+    assert len(oracle.fast(sq, 20)) == 0   # ... suppress each other (strict '>')
+    yy, xx = np.mgrid[0:20, 0:20]
+    sq[20:, 20:] = 200 - xx - yy           # brightest at the corner tip
+    r = oracle.fast(sq, 20)
+    np.testing.assert_array_equal(r[:, :2], [[20, 20]])
+
+
+# ---------------------------------------------------------------- octree
+def test_octree_extremes():
+    rng = np.random.default_rng(5)
+    pts = []
+    seen = set()
+    while len(pts) < 300:
+        x, y = int(rng.integers(0, 600)), int(rng.integers(0, 440))
+        if (x, y) not in seen:
+            seen.add((x, y))
+            pts.append((x, y, float(rng.integers(1, 100))))
+    xyr = np.array(pts, np.float32)
+    # N >= #candidates: every candidate survives (distinct positions)
+    r = oracle.octree(xyr, 16, 624, 16, 464, 400)
+    assert sorted(map(tuple, r.tolist())) == sorted(map(tuple, xyr.tolist()))
+    # N = 1: the single root, its first maximum-response keypoint
+    r = oracle.octree(xyr, 16, 624, 16, 464, 1)
+    assert len(r) == 1 and r[0, 2] == xyr[:, 2].max()
+    assert tuple(r[0]) == tuple(xyr[np.argmax(xyr[:, 2])])
+    # N = 0: every root emits its best (one root for 4:3)
+    assert len(oracle.octree(xyr, 16, 624, 16, 464, 0)) == 1
+    # empty input: nothing (the reference's behaviour is undefined)
+    assert len(oracle.octree(np.zeros((0, 3), np.float32), 16, 624, 16, 464, 100)) == 0
+    # output count stays within [min(N, n), N + 3)
+    for N in (5, 17, 64, 150):
+        r = oracle.octree(xyr, 16, 624, 16, 464, N)
+        assert N <= len(r) < N + 3
+        assert len(set(map(tuple, r.tolist()))) == len(r)
+
+
+# ---------------------------------------------------------------- angle / descriptor
+def test_fast_atan2_accuracy_and_axes():
+    for (y, x, a) in [(0, 1, 0.0), (1, 0, 90.0), (0, -1, 180.0), (-1, 0, 270.0)]:
+        assert abs(oracle.fast_atan2(y, x) - a) < 1e-3
+    rng = np.random.default_rng(2)
+    for y, x in rng.uniform(-1e6, 1e6, (2000, 2)):
+        ref = math.degrees(math.atan2(y, x)) % 360.0
+        got = oracle.fast_atan2(float(y), float(x))
+        assert 0 <= got <= 360
+        assert min(abs(got - ref), 360 - abs(got - ref)) < 0.3
+
+
+def test_ic_angle_ramp():
+    """I = a x + b y + c has moments m10 = a S, m01 = b S on the symmetric disk,
+    so the angle is atan2(b, a)."""
+    ys, xs = np.mgrid[0:64, 0:64]
+    for a, b in [(1, 0), (0, 1), (1, 1), (-1, 2), (2, -1), (-1, -1)]:
+        img = (100 + a * (xs - 32) + b * (ys - 32)).astype(np.uint8)
+        ang = oracle.ic_angle(img, 32, 32)
+        ref = math.degrees(math.atan2(b, a)) % 360
+        assert min(abs(ang - ref), 360 - abs(ang - ref)) < 0.3
+
+
+def _pattern():
+    hdr = (Path(__file__).resolve().parent.parent / "mcvslam_amd/csrc/orb_pattern.h").read_text()
+    return np.array([int(v) for v in re.findall(r"-?\d+", hdr[hdr.index("= {"):])]).reshape(256, 4)
+
+
+def test_descriptor_unrotated_and_quarter_turn():
+    rng = np.random.default_rng(3)
+    img = rng.integers(0, 256, (60, 60), dtype=np.uint8)
+    pat = _pattern()
+    cx, cy = 30, 30
+    # angle 0: a = 1, b = 0, taps are the pattern offsets themselves
+    bits = [int(img[cy + y0, cx + x0]) < int(img[cy + y1, cx + x1]) for x0, y0, x1, y1 in pat]
+    want = np.packbits(np.array(bits, np.uint8).reshape(32, 8)[:, ::-1], axis=1).ravel()
+    np.testing.assert_array_equal(oracle.descriptor(img, cx, cy, 0.0), want)
+    # 90 degrees: (x, y) -> row x, column -y
+    bits = [int(img[cy + x0, cx - y0]) < int(img[cy + x1, cx - y1]) for x0, y0, x1, y1 in pat]
+    want = np.packbits(np.array(bits, np.uint8).reshape(32, 8)[:, ::-1], axis=1).ravel()
+    np.testing.assert_array_equal(oracle.descriptor(img, cx, cy, 90.0), want)
+
+
+# ---------------------------------------------------------------- whole extractor
+def test_extract_properties():
+    img = synth.blob_image(1, 640, 480)
+    kps, desc, info, levels = oracle.orb_extract(img, with_levels=True)
+    t = oracle.orb_tables()
+    # level sizes: cvRound(cols * invScale)
+    for l in range(8):
+        assert info[l, 0] == int(np.rint(np.float32(640) * t["inv_scale"][l]))
+    # per level count within [N, N + 3) whenever there are enough candidates
+    for l in range(8):
+        n = int((kps["octave"] == l).sum())
+        if info[l, 2] >= t["npl"][l] + 3:
+            assert t["npl"][l] <= n < t["npl"][l] + 3
+    # levels appear in order, coordinates scaled back to level 0 frame
+    assert (np.diff(kps["octave"]) >= 0).all()
+    assert (kps["x"] >= 0).all() and (kps["x"] < 640).all() and (kps["y"] < 480).all()
+    assert (kps["size"] == np.array([np.float32(int(31 * s)) for s in t["scale"]])[kps["octave"]]).all()
+    # the descriptor of every keypoint is the descriptor function on its blurred level
+    for i in range(0, len(kps), 97):
+        k = kps[i]
+        l = k["octave"]
+        x, y = (k["x"], k["y"]) if l == 0 else (k["x"] / t["scale"][l], k["y"] / t["scale"][l])
+        np.testing.assert_array_equal(oracle.descriptor(levels[l][1], round(float(x)), round(float(y)), k["angle"]),
+                                      desc[i])
+    assert oracle.orb_extract(np.zeros((0, 0), np.uint8)) == -1

@@ -1,10 +1,11 @@
 """Benchmark of the MI355X hot path (one GPU).
 
 Headline: rig frames per second of batched ORB extraction + matching.  One
-step = one hipGraph launch over a batch of rig frames (L and R 1280x720, W
-640x480, 2000 features per image, 8 levels, scale 1.2, FAST 20/7), then
-brute-force Hamming top-2 + ratio 0.7 + threshold 50 for L-R, W-L and W-R.
-Inputs are resident in HBM before timing starts.
+step = one hipGraph launch over a batch of 64 rig frames (L and R 1280x720, W
+640x480; per-image ORB feature counts drawn from U{500..4000}, seed 40 + frame;
+8 levels, scale 1.2, FAST 20/7), then brute-force Hamming top-2 + ratio 0.7 +
+threshold 50 for L-R, W-L and W-R.  Inputs are resident in HBM before timing
+starts.
 
 Also reported on the same line: the plain path (one rig frame at a time, no
 graph), BA linearization throughput (50 keyframes x 3 cameras, 200k points,
@@ -68,21 +69,21 @@ def algorithmic_bytes(kernel, orb, frames, counts):
     if kernel == "k_keypoints":
         return 56 * nkp
     if kernel == "k_knn2":
-        return frames * 3 * (32 * 2 * 2000 + 16 * 2000)
+        return None
     return None
 
 
-def cpu_baseline(frames_np, budget_s=12.0):
+def cpu_baseline(frames_np, nfeat, budget_s=12.0):
     """Oracle on the host: per rig frame 3 extraction threads (one per camera,
     as src/Frame.cpp:120-125) then the three brute-force matches."""
     import oracle
     done = 0
     t0 = time.perf_counter()
-    for f in frames_np:
+    for i, f in enumerate(frames_np):
         res = [None] * 3
 
         def work(k):
-            res[k] = oracle.orb_extract(f[k])
+            res[k] = oracle.orb_extract(f[k], nfeatures=int(nfeat[3 * i + k]))
         th = [threading.Thread(target=work, args=(k,)) for k in range(3)]
         for t in th:
             t.start()
@@ -139,7 +140,9 @@ def main():
         ptrs += [L[f].data_ptr(), R[f].data_ptr(), W[f].data_ptr()]
     torch.cuda.synchronize()
 
-    rig = RigBatch(orb, n, ratio=0.7, th=50)
+    nfeat = np.concatenate([np.random.Generator(np.random.PCG64(40 + f)).integers(500, 4001, 3)
+                            for f in range(n)]).astype(np.int32)
+    rig = RigBatch(orb, n, ratio=0.7, th=50, nfeatures=nfeat)
     log("warmup (graph) ...")
     for _ in range(args.warmup):
         rig.run(ptrs, graph=True)
@@ -190,9 +193,10 @@ def main():
         "vs_baseline": None,
         "dtype": "u8/int32 (f32 angle, rotation)",
         "data": f"synthetic: seeded blob/rectangle/noise images, {len(uniq)} unique rig frames tiled to {n}",
-        "config": {"workload": f"{n} rig frames per step: L,R 1280x720 + W 640x480, 2000 ORB features, 8 levels, "
-                               "scale 1.2, FAST 20/7, match ratio 0.7 threshold 50",
-                   "frames_per_step": n, "parallelism": "single GPU"},
+        "config": {"workload": f"{n} rig frames per step: L,R 1280x720 + W 640x480, ORB features per image "
+                               "U{500..4000} (seed 40+frame), 8 levels, scale 1.2, FAST 20/7, match ratio 0.7 "
+                               "threshold 50",
+                   "frames_per_step": n, "features_per_step": int(nfeat.sum()), "parallelism": "single GPU"},
         "plain_frames_per_s": round(plain_fps, 2),
         "batched_over_plain": round(fps / plain_fps, 2),
         "kernels_ms_per_step": {k: round(v, 4) for k, v in kern.items()},
@@ -244,7 +248,7 @@ def main():
     if not args.no_cpu:
         log("CPU oracle baseline ...")
         fr = [uniq[i % len(uniq)] for i in range(8)]
-        cfps, done, dt = cpu_baseline(fr)
+        cfps, done, dt = cpu_baseline(fr, nfeat)
         result["cpu_baseline"] = {"value": round(cfps, 3), "unit": "rig frames/s", "cores": 3, "kind": "port",
                                   "sample": f"{done} rig frames (3 extraction threads + 3 brute-force matches) "
                                             f"in {dt:.1f}s on {os.cpu_count()} host cpus"}

@@ -97,8 +97,11 @@ int mcv_match_filter(mcv_ctx* ctx, const int32_t* knn, int nq, float ratio, int 
  * Frame::Frame extraction (src/Frame.cpp:118-125) followed by brute-force
  * KnnMatch -> FilterRatio -> FilterThreshold for the three camera pairs.
  * A rig object owns the plan, device outputs and the captured graph. */
-int mcv_rig_create(mcv_ctx* ctx, int n_frames, const int32_t* widths3, const int32_t* heights3, float ratio, int th,
-                   mcv_rig** rig);
+/* nfeatures: NULL (the configured count for every image) or 3 * n_frames
+ * per-image feature counts (frame-major L, R, W), each image then gets the
+ * mnFeaturesPerLevel quotas of its own count. */
+int mcv_rig_create(mcv_ctx* ctx, int n_frames, const int32_t* widths3, const int32_t* heights3,
+                   const int32_t* nfeatures, float ratio, int th, mcv_rig** rig);
 void mcv_rig_destroy(mcv_rig* rig);
 /* d_images: host array of 3 * n_frames device pointers (frame-major L, R, W),
  * each image dense (stride = width).  flags: MCV_FLAG_GRAPH replays the whole
@@ -170,6 +173,21 @@ int mcv_ba_copy_planes_dev(mcv_ba* ba, int first, int count, float* d_dst);
  * grad_point[P x 3] = sum J_point^T g over the last linearization, with
  * d_g an M x 2 device array; outputs are device arrays. */
 int mcv_ba_adjoint_dev(mcv_ba* ba, const float* d_g, float* d_grad_pose, float* d_grad_point);
+
+/* ---- Schur complement of the point blocks ------------------------------
+ * BlockSolver::buildSystem + the Schur part of BlockSolver::solve with the LM
+ * damping of setLambda (ThirdParty/g2o/g2o/core/block_solver.hpp:315-449,
+ * 463-520): from the last linearization, in FP32 with fixed summation
+ * orders, the reduced camera system S (n x n, n = 6 x free frames, frames
+ * numbered in order) and b_schur.  Fixed frames (VertexSE3Expmap::setFixed,
+ * src/BAoptimizer.cpp:31) get no pose block. */
+int mcv_ba_set_fixed(mcv_ba* ba, const int32_t* frame_fixed);
+int mcv_ba_schur(mcv_ba* ba, float lambda);
+/* Host copy of S (row-major, full symmetric) and b_schur as double. */
+int mcv_ba_read_schur(mcv_ba* ba, double* S, double* b_schur, int32_t* n);
+/* FP64 dense Cholesky solve S x = b on the host (stands in for g2o's
+ * LinearSolverEigen, solvers/eigen/linear_solver_eigen.h:55). */
+int mcv_solve_llt(int n, const double* S, const double* b, double* x);
 
 #ifdef __cplusplus
 }

@@ -47,7 +47,7 @@ SIGNATURES = {
     "mcv_match_knn2": (I, [P, P, I, P, I, P]),
     "mcv_match_knn2_dev": (I, [P, P, I, P, I, P]),
     "mcv_match_filter": (I, [P, P, I, F, I, P, P]),
-    "mcv_rig_create": (I, [P, I, P, P, F, I, P]),
+    "mcv_rig_create": (I, [P, I, P, P, P, F, I, P]),
     "mcv_rig_destroy": (None, [P]),
     "mcv_rig_run": (I, [P, P, I]),
     "mcv_rig_outputs": (I, [P, P, P, P, P, P, P, P]),
@@ -64,6 +64,10 @@ SIGNATURES = {
     "mcv_ba_read_planes": (I, [P, I, I, P]),
     "mcv_ba_copy_planes_dev": (I, [P, I, I, P]),
     "mcv_ba_adjoint_dev": (I, [P, P, P, P]),
+    "mcv_ba_set_fixed": (I, [P, P]),
+    "mcv_ba_schur": (I, [P, F]),
+    "mcv_ba_read_schur": (I, [P, P, P, P]),
+    "mcv_solve_llt": (I, [I, P, P, P]),
 }
 
 _lib = None

@@ -246,7 +246,7 @@ class RigBatch:
     """n rig frames (L, R, W) extracted and matched (L-R, W-L, W-R)."""
 
     def __init__(self, orb: ORB, n_frames: int, sizes=((1280, 720), (1280, 720), (640, 480)), ratio: float = 0.7,
-                 th: int = 50):
+                 th: int = 50, nfeatures=None):
         self.orb = orb
         self.ctx = orb.ctx
         self.n_frames = n_frames
@@ -254,7 +254,9 @@ class RigBatch:
         h = np.array([s[1] for s in sizes], np.int32)
         self.sizes = sizes
         hd = ctypes.c_void_p()
-        check(self.ctx.h, lib().mcv_rig_create(self.ctx.h, n_frames, _p(w), _p(h), ctypes.c_float(ratio), th,
+        nf = None if nfeatures is None else np.ascontiguousarray(nfeatures, np.int32).reshape(3 * n_frames)
+        self.nfeatures = nf
+        check(self.ctx.h, lib().mcv_rig_create(self.ctx.h, n_frames, _p(w), _p(h), _p(nf), ctypes.c_float(ratio), th,
                                                ctypes.byref(hd)))
         self.h = hd
         ptrs = [ctypes.c_void_p() for _ in range(6)]
@@ -322,6 +324,8 @@ class BALinearizer:
         b = cls(len(prob.obs_pt), len(prob.points), len(prob.frames), len(prob.cams), ctx)
         b.set_observations(prob.obs_uv, prob.obs_pt, prob.obs_frame, prob.obs_cam, prob.obs_isig)
         b.set_state(prob.points, prob.frames, prob.cams)
+        if getattr(prob, "frame_fixed", None) is not None:
+            b.set_fixed(prob.frame_fixed)
         return b
 
     def close(self):
@@ -364,6 +368,32 @@ class BALinearizer:
         pl = self.read()
         e = np.stack([pl[0], pl[1]], 1)
         return e, pl[2], pl[3], pl[4:16].T.reshape(-1, 2, 6), pl[16:22].T.reshape(-1, 2, 3)
+
+    def set_fixed(self, frame_fixed):
+        ff = np.ascontiguousarray(frame_fixed, np.int32)
+        check(self.ctx.h, lib().mcv_ba_set_fixed(self.h, _p(ff)))
+
+    def schur(self, lam: float = 0.0):
+        """Reduced camera system of the last linearization: (S, b_schur) as
+        FP64 host arrays (computed in FP32 on the GPU)."""
+        check(self.ctx.h, lib().mcv_ba_schur(self.h, ctypes.c_float(lam)))
+        n = ctypes.c_int32()
+        check(self.ctx.h, lib().mcv_ba_read_schur(self.h, None, None, ctypes.byref(n)))
+        S = np.zeros((n.value, n.value))
+        b = np.zeros(n.value)
+        check(self.ctx.h, lib().mcv_ba_read_schur(self.h, _p(S), _p(b), ctypes.byref(n)))
+        return S, b
+
+    @staticmethod
+    def solve_llt(S: np.ndarray, b: np.ndarray) -> np.ndarray:
+        """FP64 Cholesky solve on the host."""
+        S = np.ascontiguousarray(S, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros_like(b)
+        code = lib().mcv_solve_llt(len(b), _p(S), _p(b), _p(x))
+        if code != 0:
+            raise MCVError(code, "reduced camera system is not positive definite")
+        return x
 
     def copy_planes_dev(self, first: int, count: int, d_ptr: int):
         check(self.ctx.h, lib().mcv_ba_copy_planes_dev(self.h, first, count, ctypes.c_void_p(d_ptr)))

@@ -293,6 +293,21 @@ static void gauss_taps(int* taps) {
     taps[n2] = 256 - 2 * s;
 }
 
+// mnFeaturesPerLevel for nf features (ORBextractor.cc:426-436)
+static std::vector<int> feature_quotas(int nf, float sf, int nl) {
+    std::vector<int> npl(nl, 0);
+    const float factor = 1.0f / sf;
+    float nd = nf * (1 - factor) / (1 - (float)std::pow((double)factor, (double)nl));
+    int sumF = 0;
+    for (int l = 0; l < nl - 1; l++) {
+        npl[l] = (int)lrintf(nd);
+        sumF += npl[l];
+        nd *= factor;
+    }
+    npl[nl - 1] = std::max(nf - sumF, 0);
+    return npl;
+}
+
 static void orb_init(OrbConfig& c, int nf, float sf, int nl, int ini, int mn) {
     c.nfeatures = nf; c.scale_factor = sf; c.nlevels = nl; c.ini = ini; c.mn = mn;
     c.scale.assign(nl, 0.f); c.sigma2.assign(nl, 0.f); c.inv_scale.assign(nl, 0.f); c.inv_sigma2.assign(nl, 0.f);
@@ -305,16 +320,7 @@ static void orb_init(OrbConfig& c, int nf, float sf, int nl, int ini, int mn) {
         c.inv_scale[i] = 1.0f / c.scale[i];
         c.inv_sigma2[i] = 1.0f / c.sigma2[i];
     }
-    c.npl.assign(nl, 0);
-    const float factor = 1.0f / sf;
-    float nd = nf * (1 - factor) / (1 - (float)std::pow((double)factor, (double)nl));
-    int sumF = 0;
-    for (int l = 0; l < nl - 1; l++) {
-        c.npl[l] = (int)lrintf(nd);
-        sumF += c.npl[l];
-        nd *= factor;
-    }
-    c.npl[nl - 1] = std::max(nf - sumF, 0);
+    c.npl = feature_quotas(nf, sf, nl);
     int v, v0, vmax = (int)std::floor(15 * std::sqrt(2.f) / 2 + 1);
     int vmin = (int)std::ceil(15 * std::sqrt(2.f) / 2);
     const double hp2 = 15 * 15;
@@ -358,6 +364,7 @@ static int level_roots(int w, int h) {
 struct ImageSpec {
     int w, h;
     std::vector<std::vector<mcv_keypoint>> seeds;  // per level, in input order
+    int nfeatures = 0;                              // 0: the configured count
 };
 
 // A batch plan: geometry and work tables for a list of images.
@@ -395,6 +402,7 @@ struct OrbPlan {
             REQUIRE(im.w <= 4096 && im.h <= 4096, MCV_ERR_INVALID, "image larger than 4096 px");
             int first = (int)lv.size();
             int img_slots = 0;
+            const std::vector<int> npl = im.nfeatures > 0 ? feature_quotas(im.nfeatures, c.scale_factor, nlevels) : c.npl;
             for (int l = 0; l < nlevels; l++) {
                 LevelDesc d{};
                 d.img = i;
@@ -403,7 +411,7 @@ struct OrbPlan {
                 d.h = l == 0 ? im.h : (int)lrintf((float)im.h * c.inv_scale[l]);
                 REQUIRE(d.w >= 8 && d.h >= 8, MCV_ERR_INVALID, "pyramid level smaller than 8 px");
                 d.src = l == 0 ? -1 : (int)lv.size() - 1;
-                d.nfeat = c.npl[l];
+                d.nfeat = npl[l];
                 d.pix = pyr_bytes;
                 pyr_bytes += ((long long)d.w * d.h + 255) / 256 * 256;
                 d.coef_x = d.coef_y = 0;
@@ -671,8 +679,14 @@ struct mcv_ba {
     mcv_ctx* ctx;
     int M, P, F, C;
     DevBuf uv, pt, pair, isig, phi, plo, pairs, planes, fptr, fobs, pptr;
-    std::vector<int> h_pair;
+    std::vector<int> h_pt, h_fr, h_pptr;
+    std::vector<int> fixed;
     bool have_obs = false, have_state = false, linearized = false;
+    // Schur structure (depends on observations and fixed frames)
+    bool schur_ready = false, schur_done = false;
+    int npf = 0, Fp = 0, nblk = 0;
+    DevBuf pf_ptr, pf_obs_ptr, pf_obs, pf_point, sfptr, sfobs, fpf_ptr, fpf, blk_ptr, blk_items, blk_ij;
+    DevBuf Dinv, bl, db, Hll, Hpl, Hpp, bs, Sout;
 };
 
 static int fail(mcv_ctx* ctx, const Fail& f) {
@@ -971,7 +985,8 @@ int mcv_match_filter(mcv_ctx* c, const int32_t* knn, int nq, float ratio, int th
 }
 
 // ---- rig frames -------------------------------------------------------------
-int mcv_rig_create(mcv_ctx* c, int n_frames, const int32_t* w3, const int32_t* h3, float ratio, int th, mcv_rig** out) {
+int mcv_rig_create(mcv_ctx* c, int n_frames, const int32_t* w3, const int32_t* h3, const int32_t* nfeatures,
+                   float ratio, int th, mcv_rig** out) {
     API_BEGIN(c)
     REQUIRE(c && c->orb_ready, MCV_ERR_STATE, "ORB not configured");
     REQUIRE(out && n_frames > 0 && w3 && h3, MCV_ERR_INVALID, "bad rig arguments");
@@ -984,12 +999,21 @@ int mcv_rig_create(mcv_ctx* c, int n_frames, const int32_t* w3, const int32_t* h
     r->th = th;
     std::vector<ImageSpec> specs;
     for (int f = 0; f < n_frames; f++)
-        for (int k = 0; k < 3; k++)
-            specs.push_back(ImageSpec{w3[k], h3[k], std::vector<std::vector<mcv_keypoint>>(c->orb.nlevels)});
+        for (int k = 0; k < 3; k++) {
+            ImageSpec sp{w3[k], h3[k], std::vector<std::vector<mcv_keypoint>>(c->orb.nlevels)};
+            if (nfeatures) {
+                REQUIRE(nfeatures[3 * f + k] > 0, MCV_ERR_INVALID, "feature count must be positive");
+                sp.nfeatures = nfeatures[3 * f + k];
+            }
+            specs.push_back(sp);
+        }
     r->batch.reset(new OrbPlan());
     r->batch->build(c->orb, specs, pool_words_for(3 * n_frames));
     r->batch->upload_all(c->stream);
-    for (int k = 0; k < 3; k++) {
+    // plain path: one single-image plan per camera, or per image when the
+    // feature counts differ between images
+    const int nsingle = nfeatures ? 3 * n_frames : 3;
+    for (int k = 0; k < nsingle; k++) {
         r->single.emplace_back(new OrbPlan());
         r->single[k]->build(c->orb, {specs[k]}, pool_words_for(1));
         r->single[k]->upload_all(c->stream);
@@ -1106,7 +1130,7 @@ int mcv_rig_run(mcv_rig* r, const uint8_t* const* d_images, int flags) {
         for (int f = 0; f < r->n_frames; f++) {
             for (int k = 0; k < 3; k++) {
                 const int i = 3 * f + k;
-                OrbPlan& P = *r->single[k];
+                OrbPlan& P = *r->single[r->single.size() == 3 ? k : i];
                 HIP_OK(hipMemcpyAsync(P.d_src.p, &d_images[i], sizeof(void*), hipMemcpyHostToDevice, s));
                 HIP_OK(hipMemcpyAsync(P.d_stride.p, &r->w[k], sizeof(int), hipMemcpyHostToDevice, s));
                 P.enqueue(c->orb, tb, s, r->kps.as<mcv_keypoint>() + (size_t)i * r->cap,
@@ -1131,7 +1155,7 @@ int mcv_rig_check(mcv_rig* r) {
     REQUIRE(r, MCV_ERR_INVALID, "null rig");
     c->set_device();
     HIP_OK(hipStreamSynchronize(c->stream));
-    for (int k = -1; k < 3; k++) {
+    for (int k = -1; k < (int)r->single.size(); k++) {
         OrbPlan& P = k < 0 ? *r->batch : *r->single[k];
         unsigned long long st[2] = {0, 0};
         HIP_OK(hipMemcpy(st, P.d_used.p, 16, hipMemcpyDeviceToHost));
@@ -1330,6 +1354,10 @@ int mcv_ba_set_observations(mcv_ba* b, const float* uv, const int32_t* pt, const
     HIP_OK(hipStreamSynchronize(s));
     b->have_obs = sorted;
     REQUIRE(sorted, MCV_ERR_INVALID, "observations must be ordered by point");
+    b->h_pt.assign(pt, pt + M);
+    b->h_fr.assign(fr, fr + M);
+    b->h_pptr = pcount;
+    b->schur_ready = false;
     API_END
 }
 
@@ -1422,6 +1450,202 @@ int mcv_ba_linearize(mcv_ba* b, int residual_only) {
     }
     b->linearized = !residual_only;
     API_END
+}
+
+// Host side of the Schur structure: (point, free frame) groups of
+// observations, free-frame observation lists, and for every free-frame pair
+// block (i <= j) the list of (pf_i, pf_j) products, all in point order.
+static void ba_build_schur(mcv_ba* b, hipStream_t s) {
+    const int P = b->P, F = b->F, M = b->M;
+    if (b->fixed.empty()) b->fixed.assign(F, 0);
+    std::vector<int> fidx(F, -1);
+    int Fp = 0;
+    for (int f = 0; f < F; f++)
+        if (!b->fixed[f]) fidx[f] = Fp++;
+    std::vector<int> pf_ptr(P + 1, 0), pf_obs_ptr(1, 0), pf_obs, pf_point, pf_free;
+    std::vector<std::pair<int, int>> tmp;
+    for (int p = 0; p < P; p++) {
+        tmp.clear();
+        for (int i = b->h_pptr[p]; i < b->h_pptr[p + 1]; i++)
+            if (fidx[b->h_fr[i]] >= 0) tmp.push_back({fidx[b->h_fr[i]], i});
+        std::stable_sort(tmp.begin(), tmp.end(), [](const std::pair<int, int>& x, const std::pair<int, int>& y) {
+            return x.first < y.first;
+        });
+        for (size_t k = 0; k < tmp.size(); k++) {
+            if (k == 0 || tmp[k].first != tmp[k - 1].first) {
+                if (k) pf_obs_ptr.push_back((int)pf_obs.size());
+                pf_point.push_back(p);
+                pf_free.push_back(tmp[k].first);
+            }
+            pf_obs.push_back(tmp[k].second);
+        }
+        if (!tmp.empty()) pf_obs_ptr.push_back((int)pf_obs.size());
+        pf_ptr[p + 1] = (int)pf_point.size();
+    }
+    const int npf = (int)pf_point.size();
+    // free frame -> observations, free frame -> pf entries
+    std::vector<int> sfptr(Fp + 1, 0), fpf_ptr(Fp + 1, 0);
+    for (int i = 0; i < M; i++)
+        if (fidx[b->h_fr[i]] >= 0) sfptr[fidx[b->h_fr[i]] + 1]++;
+    for (int q = 0; q < npf; q++) fpf_ptr[pf_free[q] + 1]++;
+    for (int f = 0; f < Fp; f++) {
+        sfptr[f + 1] += sfptr[f];
+        fpf_ptr[f + 1] += fpf_ptr[f];
+    }
+    std::vector<int> sfobs(std::max(sfptr[Fp], 1)), fpf(std::max(npf, 1));
+    {
+        std::vector<int> a(sfptr.begin(), sfptr.end() - 1), c(fpf_ptr.begin(), fpf_ptr.end() - 1);
+        for (int i = 0; i < M; i++)
+            if (fidx[b->h_fr[i]] >= 0) sfobs[a[fidx[b->h_fr[i]]]++] = i;
+        for (int q = 0; q < npf; q++) fpf[c[pf_free[q]]++] = q;
+    }
+    // pair blocks, upper triangle row-major
+    const int nblk = Fp * (Fp + 1) / 2;
+    auto bidx = [Fp](int i, int j) { return i * Fp - i * (i - 1) / 2 + (j - i); };
+    std::vector<int> blk_ptr(nblk + 1, 0);
+    for (int p = 0; p < P; p++)
+        for (int qa = pf_ptr[p]; qa < pf_ptr[p + 1]; qa++)
+            for (int qb = qa; qb < pf_ptr[p + 1]; qb++) blk_ptr[bidx(pf_free[qa], pf_free[qb]) + 1]++;
+    for (int k = 0; k < nblk; k++) blk_ptr[k + 1] += blk_ptr[k];
+    std::vector<int2> items(std::max(blk_ptr[nblk], 1)), ij(std::max(nblk, 1));
+    {
+        std::vector<int> a(blk_ptr.begin(), blk_ptr.end() - 1);
+        for (int p = 0; p < P; p++)
+            for (int qa = pf_ptr[p]; qa < pf_ptr[p + 1]; qa++)
+                for (int qb = qa; qb < pf_ptr[p + 1]; qb++)
+                    items[a[bidx(pf_free[qa], pf_free[qb])]++] = make_int2(qa, qb);
+        for (int i = 0; i < Fp; i++)
+            for (int j = i; j < Fp; j++) ij[bidx(i, j)] = make_int2(i, j);
+    }
+    upload(b->pf_ptr, pf_ptr, s);
+    upload(b->pf_obs_ptr, pf_obs_ptr, s);
+    upload(b->pf_obs, pf_obs, s);
+    upload(b->pf_point, pf_point, s);
+    upload(b->sfptr, sfptr, s);
+    upload(b->sfobs, sfobs, s);
+    upload(b->fpf_ptr, fpf_ptr, s);
+    upload(b->fpf, fpf, s);
+    upload(b->blk_ptr, blk_ptr, s);
+    upload(b->blk_items, items, s);
+    upload(b->blk_ij, ij, s);
+    b->Dinv.ensure((size_t)P * 36);
+    b->bl.ensure((size_t)P * 12);
+    b->db.ensure((size_t)P * 12);
+    b->Hll.ensure((size_t)P * 36);
+    b->Hpl.ensure((size_t)std::max(npf, 1) * 72);
+    b->Hpp.ensure((size_t)std::max(Fp, 1) * 144);
+    b->bs.ensure((size_t)std::max(Fp, 1) * 24);
+    b->Sout.ensure((size_t)std::max(36 * Fp * Fp, 1) * 4);
+    HIP_OK(hipStreamSynchronize(s));
+    b->npf = npf;
+    b->Fp = Fp;
+    b->nblk = nblk;
+    b->schur_ready = true;
+}
+
+int mcv_ba_set_fixed(mcv_ba* b, const int32_t* frame_fixed) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && frame_fixed, MCV_ERR_INVALID, "null fixed flags");
+    b->fixed.assign(frame_fixed, frame_fixed + b->F);
+    b->schur_ready = false;
+    API_END
+}
+
+int mcv_ba_schur(mcv_ba* b, float lambda) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && b->linearized, MCV_ERR_STATE, "linearize before the Schur complement");
+    c->set_device();
+    hipStream_t s = c->stream;
+    if (!b->schur_ready) ba_build_schur(b, s);
+    SchurArgs a{};
+    a.planes = b->planes.as<float>();
+    a.S = b->M;
+    a.isig = b->isig.as<float>();
+    a.pptr = b->pptr.as<int>();
+    a.pf_ptr = b->pf_ptr.as<int>();
+    a.pf_obs_ptr = b->pf_obs_ptr.as<int>();
+    a.pf_obs = b->pf_obs.as<int>();
+    a.pf_point = b->pf_point.as<int>();
+    a.fptr = b->sfptr.as<int>();
+    a.fobs = b->sfobs.as<int>();
+    a.fpf_ptr = b->fpf_ptr.as<int>();
+    a.fpf = b->fpf.as<int>();
+    a.blk_ptr = b->blk_ptr.as<int>();
+    a.blk_items = b->blk_items.as<int2>();
+    a.blk_ij = b->blk_ij.as<int2>();
+    a.P = b->P;
+    a.Fp = b->Fp;
+    a.nblk = b->nblk;
+    a.lambda = lambda;
+    a.Dinv = b->Dinv.as<float>();
+    a.bl = b->bl.as<float>();
+    a.db = b->db.as<float>();
+    a.Hll = b->Hll.as<float>();
+    a.Hpl = b->Hpl.as<float>();
+    a.Hpp = b->Hpp.as<float>();
+    a.bs = b->bs.as<float>();
+    a.Sout = b->Sout.as<float>();
+    hipLaunchKernelGGL(k_schur_points, dim3((b->P + 255) / 256), dim3(256), 0, s, a);
+    if (b->Fp > 0) {
+        hipLaunchKernelGGL(k_schur_frames, dim3(b->Fp), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_schur_blocks, dim3(b->nblk), dim3(256), 0, s, a);
+    }
+    HIP_OK(hipGetLastError());
+    b->schur_done = true;
+    API_END
+}
+
+int mcv_ba_read_schur(mcv_ba* b, double* S, double* bs, int32_t* n) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && b->schur_done, MCV_ERR_STATE, "no Schur complement computed");
+    c->set_device();
+    const int m = 6 * b->Fp;
+    if (n) *n = m;
+    std::vector<float> hs((size_t)m * m), hb(m);
+    if (m) {
+        HIP_OK(hipMemcpyAsync(hs.data(), b->Sout.p, hs.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(hb.data(), b->bs.p, hb.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_OK(hipStreamSynchronize(c->stream));
+    if (S) for (size_t k = 0; k < hs.size(); k++) S[k] = hs[k];
+    if (bs) for (int k = 0; k < m; k++) bs[k] = hb[k];
+    API_END
+}
+
+// FP64 dense Cholesky solve of the reduced camera system on the host (the
+// part the reference leaves to g2o's LinearSolverEigen,
+// solvers/eigen/linear_solver_eigen.h:55).  Returns MCV_ERR_STATE when S is
+// not positive definite.
+int mcv_solve_llt(int n, const double* S, const double* b, double* x) {
+    if (n < 0 || (n && (!S || !b || !x))) return MCV_ERR_INVALID;
+    std::vector<double> L((size_t)n * n, 0.0);
+    for (int j = 0; j < n; j++) {
+        double d = S[(size_t)j * n + j];
+        for (int k = 0; k < j; k++) d -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+        if (!(d > 0)) return MCV_ERR_STATE;
+        const double ljj = std::sqrt(d);
+        L[(size_t)j * n + j] = ljj;
+        for (int i = j + 1; i < n; i++) {
+            double v = S[(size_t)i * n + j];
+            for (int k = 0; k < j; k++) v -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
+            L[(size_t)i * n + j] = v / ljj;
+        }
+    }
+    std::vector<double> y(n);
+    for (int i = 0; i < n; i++) {
+        double v = b[i];
+        for (int k = 0; k < i; k++) v -= L[(size_t)i * n + k] * y[k];
+        y[i] = v / L[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double v = y[i];
+        for (int k = i + 1; k < n; k++) v -= L[(size_t)k * n + i] * x[k];
+        x[i] = v / L[(size_t)i * n + i];
+    }
+    return MCV_OK;
 }
 
 int mcv_ba_planes(mcv_ba* b, float** d_planes, int64_t* stride) {

@@ -211,3 +211,202 @@ __global__ __launch_bounds__(256) void k_ba_pose_adj(const float* planes, long l
 }
 
 }  // namespace mcv
+
+namespace mcv {
+
+// ---------------------------------------------------------------------------
+// Schur complement of the point blocks (BlockSolver::buildSystem +
+// BlockSolver::solve, ThirdParty/g2o/g2o/core/block_solver.hpp:315-449,
+// 463-520; quadratic form of BaseBinaryEdge::constructQuadraticForm,
+// core/base_binary_edge.hpp:64-136), FP32, fixed summation orders:
+//   S1  per point: H_ll = sum w*isig A^T A, b_l = -sum w*isig A^T e,
+//       D^-1 = (H_ll + lambda I)^-1, db = D^-1 b_l, and per (point, free
+//       frame) H_pl = sum over its cameras of w*isig B^T A (6x3)
+//   S3  per free frame: H_pp = sum w*isig B^T B, b_p, b_schur = b_p - sum H_pl db
+//   S2  per free-frame pair block (i <= j): S_ij = [i == j](H_pp + lambda I)
+//       - sum over shared points H_pl,i D^-1 H_pl,j^T
+// A = J_point (2x3), B = J_pose (2x6); planes as written by k_ba_linearize.
+struct SchurArgs {
+    const float* planes;
+    long long S;            // plane stride (M)
+    const float* isig;
+    const int* pptr;        // P + 1, observations of point p
+    const int* pf_ptr;      // per point: first pf entry (P + 1)
+    const int* pf_obs_ptr;  // per pf entry: range in pf_obs (npf + 1)
+    const int* pf_obs;      // observation indices grouped by pf
+    const int* pf_point;    // point of each pf entry
+    const int* fptr;        // free frame -> observations (Fp + 1)
+    const int* fobs;
+    const int* fpf_ptr;     // free frame -> pf entries (Fp + 1)
+    const int* fpf;
+    const int* blk_ptr;     // block (i <= j, row-major upper) -> item range
+    const int2* blk_items;  // (pf_i, pf_j)
+    const int2* blk_ij;     // block -> (i, j)
+    int P, Fp, nblk;
+    float lambda;
+    float* Dinv;            // P x 9
+    float* bl;              // P x 3
+    float* db;              // P x 3
+    float* Hll;             // P x 9 (without lambda)
+    float* Hpl;             // npf x 18 (6x3 row-major)
+    float* Hpp;             // Fp x 36
+    float* bs;              // Fp x 6
+    float* Sout;            // (6 Fp)^2 row-major
+};
+
+__global__ __launch_bounds__(256) void k_schur_points(SchurArgs a) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.P) return;
+    const long long S = a.S;
+    float H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, b[3] = {0, 0, 0};
+    for (int i = a.pptr[p]; i < a.pptr[p + 1]; i++) {
+        const float W = a.planes[3 * S + i] * a.isig[i];
+        const float e0 = a.planes[i], e1 = a.planes[S + i];
+        float A[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) A[k] = a.planes[(16 + k) * S + i];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            b[r] -= W * (A[r] * e0 + A[3 + r] * e1);
+#pragma unroll
+            for (int c = 0; c < 3; c++) H[3 * r + c] += W * (A[r] * A[c] + A[3 + r] * A[3 + c]);
+        }
+    }
+    float D[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        a.Hll[9 * p + k] = H[k];
+        D[k] = H[k];
+    }
+    D[0] += a.lambda;
+    D[4] += a.lambda;
+    D[8] += a.lambda;
+    const float c00 = D[4] * D[8] - D[5] * D[7], c01 = D[5] * D[6] - D[3] * D[8], c02 = D[3] * D[7] - D[4] * D[6];
+    const float det = D[0] * c00 + D[1] * c01 + D[2] * c02;
+    const float id = 1.f / det;
+    const float Di[9] = {c00 * id, (D[2] * D[7] - D[1] * D[8]) * id, (D[1] * D[5] - D[2] * D[4]) * id,
+                         c01 * id, (D[0] * D[8] - D[2] * D[6]) * id, (D[2] * D[3] - D[0] * D[5]) * id,
+                         c02 * id, (D[1] * D[6] - D[0] * D[7]) * id, (D[0] * D[4] - D[1] * D[3]) * id};
+#pragma unroll
+    for (int k = 0; k < 9; k++) a.Dinv[9 * p + k] = Di[k];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        a.bl[3 * p + r] = b[r];
+        a.db[3 * p + r] = Di[3 * r] * b[0] + Di[3 * r + 1] * b[1] + Di[3 * r + 2] * b[2];
+    }
+    for (int q = a.pf_ptr[p]; q < a.pf_ptr[p + 1]; q++) {
+        float Hq[18];
+#pragma unroll
+        for (int k = 0; k < 18; k++) Hq[k] = 0.f;
+        for (int t = a.pf_obs_ptr[q]; t < a.pf_obs_ptr[q + 1]; t++) {
+            const int i = a.pf_obs[t];
+            const float W = a.planes[3 * S + i] * a.isig[i];
+            float A[6], B[12];
+#pragma unroll
+            for (int k = 0; k < 6; k++) A[k] = a.planes[(16 + k) * S + i];
+#pragma unroll
+            for (int k = 0; k < 12; k++) B[k] = a.planes[(4 + k) * S + i];
+#pragma unroll
+            for (int r = 0; r < 6; r++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) Hq[3 * r + c] += W * (B[r] * A[c] + B[6 + r] * A[3 + c]);
+        }
+#pragma unroll
+        for (int k = 0; k < 18; k++) a.Hpl[18 * (long long)q + k] = Hq[k];
+    }
+}
+
+__device__ __forceinline__ void block_sum36(float* acc, float (*red)[36]) {
+    // fixed-order tree reduction of 256 x 36 partial sums
+    for (int k = 0; k < 36; k++) red[threadIdx.x][k] = acc[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 36; k++) red[threadIdx.x][k] += red[threadIdx.x + s][k];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_schur_frames(SchurArgs a) {
+    __shared__ float red[256][36];
+    __shared__ float red6[256][6];
+    const int f = blockIdx.x;
+    const long long S = a.S;
+    float acc[36];
+    float bp[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 36; k++) acc[k] = 0.f;
+    for (int t = a.fptr[f] + threadIdx.x; t < a.fptr[f + 1]; t += 256) {
+        const int i = a.fobs[t];
+        const float W = a.planes[3 * S + i] * a.isig[i];
+        const float e0 = a.planes[i], e1 = a.planes[S + i];
+        float B[12];
+#pragma unroll
+        for (int k = 0; k < 12; k++) B[k] = a.planes[(4 + k) * S + i];
+#pragma unroll
+        for (int r = 0; r < 6; r++) {
+            bp[r] -= W * (B[r] * e0 + B[6 + r] * e1);
+#pragma unroll
+            for (int c = 0; c < 6; c++) acc[6 * r + c] += W * (B[r] * B[c] + B[6 + r] * B[6 + c]);
+        }
+    }
+    // b_schur contribution: - sum over the frame's pf entries of H_pl db
+    for (int t = a.fpf_ptr[f] + threadIdx.x; t < a.fpf_ptr[f + 1]; t += 256) {
+        const int q = a.fpf[t];
+        const int p = a.pf_point[q];
+        const float* Hq = a.Hpl + 18 * (long long)q;
+        const float d0 = a.db[3 * p], d1 = a.db[3 * p + 1], d2 = a.db[3 * p + 2];
+#pragma unroll
+        for (int r = 0; r < 6; r++) bp[r] -= Hq[3 * r] * d0 + Hq[3 * r + 1] * d1 + Hq[3 * r + 2] * d2;
+    }
+    block_sum36(acc, red);
+    if (threadIdx.x < 36) a.Hpp[36 * f + threadIdx.x] = red[0][threadIdx.x];
+    __syncthreads();
+    for (int k = 0; k < 6; k++) red6[threadIdx.x][k] = bp[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 6; k++) red6[threadIdx.x][k] += red6[threadIdx.x + s][k];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) a.bs[6 * f + threadIdx.x] = red6[0][threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_schur_blocks(SchurArgs a) {
+    __shared__ float red[256][36];
+    const int blk = blockIdx.x;
+    const int2 ij = a.blk_ij[blk];
+    float acc[36];
+#pragma unroll
+    for (int k = 0; k < 36; k++) acc[k] = 0.f;
+    for (int t = a.blk_ptr[blk] + threadIdx.x; t < a.blk_ptr[blk + 1]; t += 256) {
+        const int2 it = a.blk_items[t];
+        const int p = a.pf_point[it.x];
+        const float* Hi = a.Hpl + 18 * (long long)it.x;
+        const float* Hj = a.Hpl + 18 * (long long)it.y;
+        const float* Di = a.Dinv + 9 * p;
+        // Y = D^-1 H_j^T (3 x 6)
+        float Y[18];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 6; c++)
+                Y[6 * r + c] = Di[3 * r] * Hj[3 * c] + Di[3 * r + 1] * Hj[3 * c + 1] + Di[3 * r + 2] * Hj[3 * c + 2];
+#pragma unroll
+        for (int r = 0; r < 6; r++)
+#pragma unroll
+            for (int c = 0; c < 6; c++)
+                acc[6 * r + c] += Hi[3 * r] * Y[c] + Hi[3 * r + 1] * Y[6 + c] + Hi[3 * r + 2] * Y[12 + c];
+    }
+    block_sum36(acc, red);
+    if (threadIdx.x < 36) {
+        const int r = threadIdx.x / 6, c = threadIdx.x % 6;
+        const int n = 6 * a.Fp;
+        float v = -red[0][threadIdx.x];
+        if (ij.x == ij.y) v += a.Hpp[36 * ij.x + threadIdx.x] + (r == c ? a.lambda : 0.f);
+        a.Sout[(long long)(6 * ij.x + r) * n + 6 * ij.y + c] = v;
+        if (ij.x != ij.y) a.Sout[(long long)(6 * ij.y + c) * n + 6 * ij.x + r] = v;
+    }
+}
+
+}  // namespace mcv

@@ -68,3 +68,25 @@ def test_ba_residual_only(gpu_ctx):
     pl = b.read(0, 4)
     re, rchi2, rw, _, _ = oracle.ba_linearize(prob)
     assert rel_rows(np.stack([pl[0], pl[1]], 1), re, 1e-9).max() < TOL
+
+
+@pytest.mark.parametrize("lam", [0.0, 1e-2])
+def test_schur_vs_fp64(gpu_ctx, lam):
+    """FP32 reduced camera system against the FP64 oracle (dense elimination
+    is pinned in test_oracle_match_ba.py); the FP64 host Cholesky solves both
+    to the same pose step."""
+    prob = synth.ba_problem(seed=35, n_kf=8, n_points=6000, target_obs=60000)
+    b = BALinearizer.from_problem(prob, ctx=gpu_ctx)
+    b.linearize()
+    S, bs = b.schur(lam)
+    e, chi2, w, Jp, Jx = oracle.ba_linearize(prob)
+    r = oracle.ba_schur(e, w, Jp, Jx, prob.obs_pt, prob.obs_frame, prob.obs_isig, len(prob.points),
+                        len(prob.frames), prob.frame_fixed, lam)
+    assert S.shape == r["S"].shape
+    assert np.linalg.norm(S - r["S"]) / np.linalg.norm(r["S"]) < 1e-5
+    assert np.linalg.norm(bs - r["bs"]) / np.linalg.norm(r["bs"]) < 1e-4
+    if lam > 0:
+        x = b.solve_llt(S, bs)
+        xr = np.linalg.solve(r["S"], r["bs"])
+        assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-3
+    b.close()

@@ -38,3 +38,27 @@ def test_rig_two_frames(orb_default):
             want = oracle.filter_ratio_threshold(want_knn, 0.7, 50)
             assert mcounts[p] == len(want)
             np.testing.assert_array_equal(matches[p, :mcounts[p]], want)
+
+
+def test_rig_uneven_feature_counts(orb_default):
+    """Per-image feature counts (500-4000) in one batch: each image matches
+    the oracle run with its own count."""
+    import torch
+    frames = [synth.rig_frame(300 + 10 * f) for f in range(2)]
+    imgs = [torch.from_numpy(im).cuda() for fr in frames for im in fr]
+    nf = np.array([500, 4000, 1200, 3000, 800, 2500], np.int32)
+    rig = RigBatch(orb_default, 2, ratio=0.7, th=50, nfeatures=nf)
+    outs = []
+    for graph in (True, False):
+        rig.run([im.data_ptr() for im in imgs], graph=graph)
+        outs.append(rig.read())
+    for a, b in zip(outs[0], outs[1]):
+        np.testing.assert_array_equal(a, b)
+    kps, desc, counts, matches, mcounts, knn = outs[0]
+    ims = [im for fr in frames for im in fr]
+    for i, im in enumerate(ims):
+        rk, rd = oracle.orb_extract(im, nfeatures=int(nf[i]))
+        assert counts[i] == len(rk)
+        for f in ("x", "y", "size", "angle", "response", "octave"):
+            np.testing.assert_array_equal(kps[i, :counts[i]][f], rk[f])
+        np.testing.assert_array_equal(desc[i, :counts[i]], rd)

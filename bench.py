@@ -118,8 +118,10 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream()            # default priority
+    torch.cuda.set_stream(stream)
     ctx = Context(0)
-    ctx.use_torch_stream()
+    ctx.use_torch_stream()                  # library work and torch events share this stream
     orb = ORB(2000, 1.2, 8, 20, 7, ctx=ctx)
 
     log("generating frames ...")

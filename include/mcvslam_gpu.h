@@ -41,8 +41,10 @@ typedef struct {
 int mcv_create(mcv_ctx** ctx, int device);
 void mcv_destroy(mcv_ctx* ctx);
 const char* mcv_last_error(const mcv_ctx* ctx);
-/* hipStream_t to enqueue on (NULL: the context's own stream) */
+/* hipStream_t to enqueue on; NULL is the null (default) stream.  A new
+ * context enqueues on a non-blocking stream of its own. */
 int mcv_set_stream(mcv_ctx* ctx, void* hip_stream);
+int mcv_use_own_stream(mcv_ctx* ctx);
 int mcv_synchronize(mcv_ctx* ctx);
 /* Synchronous device -> host copy of `bytes` on the context's stream. */
 int mcv_copy_to_host(mcv_ctx* ctx, void* dst, const void* d_src, uint64_t bytes);
@@ -108,6 +110,15 @@ void mcv_rig_destroy(mcv_rig* rig);
  * batch as one hipGraph; without it every image and every pair is launched on
  * its own ("plain" path). */
 int mcv_rig_run(mcv_rig* rig, const uint8_t* const* d_images, int flags);
+/* Stereo matching of every frame's L-R pair on the pyramids of the last
+ * batched run (Frame::ComputeStereoMatch, src/Frame.cpp:150-328): row band
+ * r = 10 * scale, octave +-1, disparity in [1, min(bf / baseline, 1000)),
+ * ratio 0.70 / threshold 34, 11x11 SAD over 11 shifts, parabola sub-pixel
+ * fit, then the median filter (1.6x / 0.4x).  Outputs per left keypoint
+ * ([n_frames][capacity]): u_right and depth (-1 when unmatched) and the SAD
+ * distance (-1). */
+int mcv_rig_stereo(mcv_rig* rig, float bf, float baseline);
+int mcv_rig_stereo_outputs(mcv_rig* rig, float** d_u_right, float** d_depth, int32_t** d_sad);
 /* Waits for the last run and reports device-side overflow of the octree
  * node pool (MCV_ERR_STATE; the pool is grown for the next run). */
 int mcv_rig_check(mcv_rig* rig);

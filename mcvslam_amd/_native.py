@@ -36,6 +36,7 @@ SIGNATURES = {
     "mcv_destroy": (None, [P]),
     "mcv_last_error": (ctypes.c_char_p, [P]),
     "mcv_set_stream": (I, [P, P]),
+    "mcv_use_own_stream": (I, [P]),
     "mcv_synchronize": (I, [P]),
     "mcv_copy_to_host": (I, [P, P, P, ctypes.c_uint64]),
     "mcv_orb_configure": (I, [P, I, F, I, I, I]),
@@ -53,6 +54,8 @@ SIGNATURES = {
     "mcv_rig_outputs": (I, [P, P, P, P, P, P, P, P]),
     "mcv_rig_run_timed": (I, [P, P, P, P, I, P]),
     "mcv_rig_check": (I, [P]),
+    "mcv_rig_stereo": (I, [P, F, F]),
+    "mcv_rig_stereo_outputs": (I, [P, P, P, P]),
     "mcv_window_search": (I, [P, P, P, P, I, P, P, P, P, I, F, F, I, P, P, P, I, I, I, P]),
     "mcv_window_search_dev": (I, [P, I, P, P, P, I, P, P, P, P, I, F, F, I, P, P, P, P, P]),
     "mcv_ba_create": (I, [P, I, I, I, I, P]),

@@ -53,7 +53,11 @@ class Context:
             pass
 
     def set_stream(self, stream_handle: int | None):
+        """hipStream_t handle (0 / None: the null stream)."""
         check(self.h, lib().mcv_set_stream(self.h, ctypes.c_void_p(stream_handle or 0)))
+
+    def use_own_stream(self):
+        check(self.h, lib().mcv_use_own_stream(self.h))
 
     def use_torch_stream(self):
         import torch
@@ -292,6 +296,16 @@ class RigBatch:
         k = ctypes.c_int32()
         check(self.ctx.h, lib().mcv_rig_run_timed(self.h, arr, _p(ms), names, 64, ctypes.byref(k)))
         return [(names[i].decode(), float(ms[i])) for i in range(min(k.value, 64))]
+
+    def stereo(self, bf: float, baseline: float):
+        """L-R stereo of every frame of the last batched run; host copies of
+        (u_right, depth, sad), each [n_frames][cap]."""
+        check(self.ctx.h, lib().mcv_rig_stereo(self.h, ctypes.c_float(bf), ctypes.c_float(baseline)))
+        ptrs = [ctypes.c_void_p() for _ in range(3)]
+        check(self.ctx.h, lib().mcv_rig_stereo_outputs(self.h, *[ctypes.byref(p) for p in ptrs]))
+        shape = (self.n_frames, self.cap)
+        return (self.ctx.to_host(ptrs[0].value, np.float32, shape), self.ctx.to_host(ptrs[1].value, np.float32, shape),
+                self.ctx.to_host(ptrs[2].value, np.int32, shape))
 
     def read(self):
         """Host copies: (kps records [3n][cap], desc [3n][cap][32], counts [3n],

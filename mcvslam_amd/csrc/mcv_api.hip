@@ -27,6 +27,7 @@
 #include "mcv_match.hip"
 #include "mcv_orb.hip"
 #include "mcv_window.hip"
+#include "mcv_stereo.hip"
 
 using namespace mcv;
 
@@ -516,6 +517,7 @@ struct OrbPlan {
         d_oct.ensure(lv.size() * sizeof(OctLevel));
         d_pool.ensure((size_t)pool_cap * 4);
         d_used.ensure(64);
+        HIP_OK(hipMemsetAsync(d_used.p, 0, 64, s));  // set-up only, never captured
         d_nodes.ensure((size_t)std::max(n_slots, 1) * 4);
         d_src.ensure((size_t)n_images * sizeof(void*));
         d_stride.ensure((size_t)n_images * sizeof(int));
@@ -670,6 +672,8 @@ struct mcv_rig {
     hipGraphExec_t graph = nullptr;
     std::vector<const uint8_t*> graph_inputs;
     KernelTimer timer;
+    bool last_batched = false;
+    DevBuf st_u, st_d, st_sad, st_keys, st_jobs;
     ~mcv_rig() {
         if (graph) (void)hipGraphExecDestroy(graph);
     }
@@ -742,7 +746,14 @@ const char* mcv_last_error(const mcv_ctx* c) { return c ? c->err.c_str() : "null
 int mcv_set_stream(mcv_ctx* c, void* s) {
     API_BEGIN(c)
     REQUIRE(c, MCV_ERR_INVALID, "null context");
-    c->stream = s ? static_cast<hipStream_t>(s) : c->own;
+    c->stream = static_cast<hipStream_t>(s);   // NULL: the null (default) stream
+    API_END
+}
+
+int mcv_use_own_stream(mcv_ctx* c) {
+    API_BEGIN(c)
+    REQUIRE(c, MCV_ERR_INVALID, "null context");
+    c->stream = c->own;
     API_END
 }
 
@@ -1124,7 +1135,9 @@ int mcv_rig_run(mcv_rig* r, const uint8_t* const* d_images, int flags) {
             HIP_OK(hipGraphDestroy(gr));
         }
         HIP_OK(hipGraphLaunch(r->graph, s));
+        r->last_batched = true;
     } else {
+        r->last_batched = false;
         // plain path: one rig frame at a time, every image on its own launch
         // sequence (one single-image plan per camera), then the three pairs
         for (int f = 0; f < r->n_frames; f++) {
@@ -1149,13 +1162,70 @@ int mcv_rig_run(mcv_rig* r, const uint8_t* const* d_images, int flags) {
     API_END
 }
 
+int mcv_rig_stereo(mcv_rig* r, float bf, float baseline) {
+    mcv_ctx* c = r ? r->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(r && r->last_batched, MCV_ERR_STATE, "stereo needs the pyramids of a batched (graph) run");
+    REQUIRE(r->w[0] == r->w[1] && r->h[0] == r->h[1], MCV_ERR_INVALID, "left and right images differ in size");
+    REQUIRE(baseline > 0.f, MCV_ERR_INVALID, "baseline must be positive");
+    c->set_device();
+    hipStream_t s = c->stream;
+    const int nf = r->n_frames, cap = r->cap, nl = c->orb.nlevels;
+    r->st_u.ensure((size_t)nf * cap * 4);
+    r->st_d.ensure((size_t)nf * cap * 4);
+    r->st_sad.ensure((size_t)nf * cap * 4);
+    r->st_keys.ensure((size_t)nf * cap * 8);
+    std::vector<StereoJob> jobs(nf);
+    for (int f = 0; f < nf; f++) {
+        StereoJob& j = jobs[f];
+        const int il = 3 * f, ir = 3 * f + 1;
+        j.kl = reinterpret_cast<const float*>(r->kps.as<mcv_keypoint>() + (size_t)il * cap);
+        j.dl = r->desc.as<uint8_t>() + (size_t)il * cap * 32;
+        j.nl = r->counts.as<int>() + il;
+        j.kr = reinterpret_cast<const float*>(r->kps.as<mcv_keypoint>() + (size_t)ir * cap);
+        j.dr = r->desc.as<uint8_t>() + (size_t)ir * cap * 32;
+        j.nr = r->counts.as<int>() + ir;
+        j.lvl_l = il * nl;
+        j.lvl_r = ir * nl;
+        j.u_right = r->st_u.as<float>() + (size_t)f * cap;
+        j.depth = r->st_d.as<float>() + (size_t)f * cap;
+        j.sad = r->st_sad.as<int>() + (size_t)f * cap;
+        j.keys = r->st_keys.as<unsigned long long>() + (size_t)f * cap;
+    }
+    upload(r->st_jobs, jobs, s);
+    StereoParams sp{};
+    for (int l = 0; l < nl; l++) {
+        sp.scale[l] = c->orb.scale[l];
+        sp.inv_scale[l] = c->orb.inv_scale[l];
+    }
+    sp.bf = bf;
+    sp.b = baseline;
+    sp.cap = cap;
+    REQUIRE(cap <= kStereoSortCap, MCV_ERR_INVALID, "too many keypoints per image for the stereo median");
+    hipLaunchKernelGGL(k_stereo, dim3((cap + 3) / 4, nf), dim3(256), 0, s, r->st_jobs.as<StereoJob>(),
+                       r->batch->d_lv.as<LevelDesc>(), r->batch->d_pyr.as<uint8_t>(), sp);
+    hipLaunchKernelGGL(k_stereo_median, dim3(nf), dim3(256), 0, s, r->st_jobs.as<StereoJob>(), sp);
+    HIP_OK(hipGetLastError());
+    API_END
+}
+
+int mcv_rig_stereo_outputs(mcv_rig* r, float** u_right, float** depth, int32_t** sad) {
+    if (!r) return MCV_ERR_INVALID;
+    if (u_right) *u_right = r->st_u.as<float>();
+    if (depth) *depth = r->st_d.as<float>();
+    if (sad) *sad = r->st_sad.as<int32_t>();
+    return MCV_OK;
+}
+
 int mcv_rig_check(mcv_rig* r) {
     mcv_ctx* c = r ? r->ctx : nullptr;
     API_BEGIN(c)
     REQUIRE(r, MCV_ERR_INVALID, "null rig");
     c->set_device();
     HIP_OK(hipStreamSynchronize(c->stream));
-    for (int k = -1; k < (int)r->single.size(); k++) {
+    const int k0 = r->last_batched ? -1 : 0;
+    const int k1 = r->last_batched ? 0 : (int)r->single.size();
+    for (int k = k0; k < k1; k++) {
         OrbPlan& P = k < 0 ? *r->batch : *r->single[k];
         unsigned long long st[2] = {0, 0};
         HIP_OK(hipMemcpy(st, P.d_used.p, 16, hipMemcpyDeviceToHost));

@@ -30,6 +30,7 @@ _SIG = {
     "orc_fast_atan2": (F, [F, F]),
     "orc_octree": (I, [P, I, I, I, I, I, I, P, I]),
     "orc_orb_extract": (I, [P, I, I, I, F, I, I, I, P, I, P, P, I, P, P, P, P, I]),
+    "orc_stereo": (None, [P, P, I, P, P, I, P, P, P, P, I, P, P, F, F, P, P, P]),
     "orc_ic_angle": (F, [P, I, I, F, F]),
     "orc_descriptor": (None, [P, I, I, F, F, F, P]),
     "orc_libm_sincos": (None, [P, I, P, P]),
@@ -167,6 +168,29 @@ def orb_candidates(img: np.ndarray, nfeatures=2000, scale=1.2, nlevels=8, ini=20
     lib().orc_orb_extract(_p(im), w, h, nfeatures, scale, nlevels, ini, mn, None, 0, _p(kps), _p(desc), cap,
                           _p(info), None, None, _p(cand), ccap)
     return cand[:int(info[:, 2].sum())].copy(), info
+
+
+def _kp_rows(k):
+    return np.ascontiguousarray(np.stack([k["x"], k["y"], k["size"], k["angle"], k["response"],
+                                          k["octave"].astype(np.float32)], 1), np.float32)
+
+
+def stereo(kl, dl, kr, dr, levels_l, levels_r, scale, inv_scale, bf, b):
+    """Frame::ComputeStereoMatch: (u_right, depth, sad) per left keypoint.
+    levels_*: list of pyramid level arrays (as orb_extract(with_levels=True))."""
+    lw = np.array([l.shape[1] for l in levels_l], np.int32)
+    lh = np.array([l.shape[0] for l in levels_l], np.int32)
+    pl = np.ascontiguousarray(np.concatenate([l.ravel() for l in levels_l]), np.uint8)
+    pr = np.ascontiguousarray(np.concatenate([l.ravel() for l in levels_r]), np.uint8)
+    n = len(kl)
+    u = np.zeros(n, np.float32)
+    d = np.zeros(n, np.float32)
+    s = np.zeros(n, np.int32)
+    lib().orc_stereo(_p(_kp_rows(kl)), _p(np.ascontiguousarray(dl, np.uint8)), n, _p(_kp_rows(kr)),
+                     _p(np.ascontiguousarray(dr, np.uint8)), len(kr), _p(pl), _p(pr), _p(lw), _p(lh), len(lw),
+                     _p(np.ascontiguousarray(scale, np.float32)), _p(np.ascontiguousarray(inv_scale, np.float32)),
+                     bf, b, _p(u), _p(d), _p(s))
+    return u, d, s
 
 
 def ic_angle(img: np.ndarray, x: float, y: float) -> float:

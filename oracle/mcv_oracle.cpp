@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <array>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -963,6 +964,119 @@ int orc_orb_extract(const uint8_t* img, int w, int h, int nf, float sf, int nl, 
     return n;
 }
 
+// Frame::ComputeStereoMatch (src/Frame.cpp:150-328) on one left/right pair.
+// kl/kr: rows (x, y, size, angle, response, octave) in level-0 coordinates,
+// dl/dr: 32-byte descriptors, pyr_l/pyr_r: concatenated pyramid levels
+// (sizes lw/lh per level), scale/inv_scale: mvScaleFactor/mvInvScaleFactor.
+// Outputs per left keypoint: u_right (-1 when unmatched), depth (-1), and
+// the SAD distance of the accepted match (-1).  Where the reference would
+// index outside a pyramid level (colRange with a negative start) the match
+// is skipped.
+void orc_stereo(const float* kl, const uint8_t* dl, int nl, const float* kr, const uint8_t* dr, int nr,
+                const uint8_t* pyr_l, const uint8_t* pyr_r, const int* lw, const int* lh, int nlev,
+                const float* scale, const float* inv_scale, float bf, float b, float* u_right, float* depth,
+                int32_t* sad) {
+    std::vector<size_t> off(nlev + 1, 0);
+    for (int l = 0; l < nlev; l++) off[l + 1] = off[l] + (size_t)lw[l] * lh[l];
+    for (int i = 0; i < nl; i++) { u_right[i] = -1.f; depth[i] = -1.f; sad[i] = -1; }
+    const int nRows = lh[0];
+    std::vector<std::vector<size_t>> rows(nRows);
+    for (int iR = 0; iR < nr; iR++) {
+        const float kpY = kr[6 * iR + 1];
+        const float r = 10.f * scale[(int)kr[6 * iR + 5]];
+        const int maxr = (int)std::ceil(kpY + r);
+        const int minr = (int)std::floor(kpY - r);
+        for (int yi = minr; yi <= maxr; yi++)
+            if (yi >= 0 && yi < nRows) rows[yi].push_back(iR);
+    }
+    const float minZ = b, minD = 1;
+    const float maxD = std::min(bf / minZ, float(1000.0));
+    std::vector<std::pair<int, int>> vDistIdx;
+    std::vector<int> cand;
+    for (int iL = 0; iL < nl; iL++) {
+        const int levelL = (int)kl[6 * iL + 5];
+        const float vL = kl[6 * iL + 1], uL = kl[6 * iL];
+        const std::vector<size_t>& vc = rows[(size_t)vL];
+        if (vc.empty()) continue;
+        const float minU = uL - maxD, maxU = uL - minD;
+        if (maxU < 0) continue;
+        cand.clear();
+        for (size_t iR : vc) {
+            const int oR = (int)kr[6 * iR + 5];
+            if (oR < levelL - 1 || oR > levelL + 1) continue;
+            const float uR = kr[6 * iR];
+            if (uR >= minU && uR <= maxU) cand.push_back((int)iR);
+        }
+        if (cand.empty()) continue;
+        int32_t tp[4], m[3];
+        top2(dl + 32 * (size_t)iL, dr, (int)cand.size(), cand.data(), tp);
+        if (filter_ratio_threshold(tp, 1, 0.70f, (int)(46 * 0.75), m) != 1) continue;
+        const int bestIdxR = m[1];
+        const float uR0 = kr[6 * cand[bestIdxR]];
+        const float scaleFactor = inv_scale[levelL];
+        const float scaleduL = std::round(uL * scaleFactor);
+        const float scaledvL = std::round(vL * scaleFactor);
+        const float scaleduR0 = std::round(uR0 * scaleFactor);
+        const int w = 5, L = 5;
+        const int cols = lw[levelL], rws = lh[levelL];
+        if (scaledvL - w < 0 || scaledvL + w + 1 >= rws || scaleduL - w < 0 || scaleduL + w + 1 >= cols) continue;
+        const float iniu = scaleduR0 + L - w, endu = scaleduR0 + L + w + 1;
+        if (iniu < 0 || endu >= cols) continue;
+        if (scaleduR0 - L - w < 0) continue;   // the reference's colRange would assert
+        const uint8_t* IL = pyr_l + off[levelL];
+        const uint8_t* IR = pyr_r + off[levelL];
+        const int y0 = (int)scaledvL - w, xl = (int)scaleduL - w;
+        const int cl = IL[(size_t)(y0 + w) * cols + xl + w];
+        int bestDist = INT32_MAX, bestincR = 0;
+        float vDists[2 * L + 1];
+        for (int incR = -L; incR <= L; incR++) {
+            const int xr = (int)scaleduR0 + incR - w;
+            const int cr = IR[(size_t)(y0 + w) * cols + xr + w];
+            int acc = 0;
+            for (int yy = 0; yy < 2 * w + 1; yy++)
+                for (int xx = 0; xx < 2 * w + 1; xx++) {
+                    const int a = (int)IL[(size_t)(y0 + yy) * cols + xl + xx] - cl;
+                    const int bb = (int)IR[(size_t)(y0 + yy) * cols + xr + xx] - cr;
+                    acc += std::abs(a - bb);
+                }
+            const float dist = (float)acc;
+            if (dist < bestDist) {
+                bestDist = (int)dist;
+                bestincR = incR;
+            }
+            vDists[L + incR] = dist;
+        }
+        if (bestincR == -L || bestincR == L) continue;
+        const float dist1 = vDists[L + bestincR - 1], dist2 = vDists[L + bestincR], dist3 = vDists[L + bestincR + 1];
+        const float deltaR = (dist1 - dist3) / (2.0f * (dist1 + dist3 - 2.0f * dist2));
+        if (deltaR < -1 || deltaR > 1) continue;
+        const float bestuR = scale[levelL] * ((float)scaleduR0 + (float)bestincR + deltaR);
+        const float disparity = uL - bestuR;
+        if (disparity >= minD && disparity < maxD) {
+            depth[iL] = bf / disparity;
+            u_right[iL] = bestuR;
+            sad[iL] = bestDist;
+            vDistIdx.push_back({bestDist, iL});
+        }
+    }
+    if (vDistIdx.empty()) return;
+    std::sort(vDistIdx.begin(), vDistIdx.end());
+    const float median = (float)vDistIdx[(size_t)(vDistIdx.size() * 1.0 / 2)].first;
+    const float th_max = 1.6f * median;
+    const float th_min = 0.4 * median;
+    int l = 0, r = (int)vDistIdx.size() - 1;
+    for (; r >= 0 && vDistIdx[r].first > th_max; r--) {
+        u_right[vDistIdx[r].second] = -1;
+        depth[vDistIdx[r].second] = -1;
+        sad[vDistIdx[r].second] = -1;
+    }
+    for (; l < r && vDistIdx[l].first < th_min; l++) {
+        u_right[vDistIdx[l].second] = -1;
+        depth[vDistIdx[l].second] = -1;
+        sad[vDistIdx[l].second] = -1;
+    }
+}
+
 // IC_Angle of one keypoint on an image (umax of ORBextractor::init)
 float orc_ic_angle(const uint8_t* img, int w, int h, float x, float y) {
     Orb o;
@@ -1118,7 +1232,10 @@ int orc_ba_schur(const double* e, const double* w, const double* Jpose, const do
     if (bl_out) std::memcpy(bl_out, bl.data(), sizeof(double) * P * 3);
     for (int k = 0; k < n; k++) Hpp[(size_t)k * n + k] += lambda;
     std::vector<double> Ssum(Hpp), b(bp);
+    std::vector<int> nobs(P, 0);
+    for (int i = 0; i < M; i++) nobs[obs_pt[i]]++;
     for (int p = 0; p < P; p++) {
+        if (nobs[p] == 0) continue;  // a point without edges is not in the optimizer
         double D[9];
         for (int k = 0; k < 9; k++) D[k] = Hll[9 * p + k];
         D[0] += lambda; D[4] += lambda; D[8] += lambda;

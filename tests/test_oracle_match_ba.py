@@ -229,3 +229,29 @@ def test_schur_matches_dense_elimination():
     bs = b[:m] - B @ np.linalg.solve(C, b[m:])
     np.testing.assert_allclose(r["S"], S, rtol=1e-8, atol=1e-6 * np.abs(S).max())
     np.testing.assert_allclose(r["bs"], bs, rtol=1e-8, atol=1e-8 * np.abs(bs).max())
+
+
+# ---------------------------------------------------------------- stereo
+def test_stereo_known_disparity():
+    """Right image = left shifted by d pixels: level-0 matches recover the
+    disparity d (Frame::ComputeStereoMatch, src/Frame.cpp:150-328)."""
+    left = synth.blob_image(51, 640, 480)
+    d = 12
+    right = np.zeros_like(left)
+    right[:, :-d] = left[:, d:]
+    kl, dl, _, lv_l = oracle.orb_extract(left, with_levels=True)
+    kr, dr, _, lv_r = oracle.orb_extract(right, with_levels=True)
+    t = oracle.orb_tables()
+    bf, b = 500.0, 0.5
+    u, depth, sad = oracle.stereo(kl, dl, kr, dr, [x[0] for x in lv_l], [x[0] for x in lv_r], t["scale"],
+                                  t["inv_scale"], bf, b)
+    ok = u >= 0
+    assert ok.sum() > 100
+    disp = kl["x"][ok] - u[ok]
+    assert np.median(np.abs(disp - d)) < 0.5
+    assert (np.abs(disp - d) < 1.5).mean() > 0.9
+    np.testing.assert_allclose(depth[ok], bf / disp, rtol=1e-6)
+    assert ((u < 0) == (depth < 0)).all() and ((u < 0) == (sad < 0)).all()
+    # octave-0 pairs match with SAD 0 here, below 0.4 x the median SAD, so the
+    # median filter of the reference (src/Frame.cpp:307-323) drops them
+    assert not (ok & (kl["octave"] == 0)).any()

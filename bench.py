@@ -53,24 +53,37 @@ def orb_level_pixels(w, h, inv_scale):
 
 
 def algorithmic_bytes(kernel, orb, frames, counts):
-    """Compulsory HBM bytes of one launch of `kernel` over the batch."""
+    """Compulsory HBM bytes of one launch of `kernel` over the batch (None for
+    the latency-bound octree kernels)."""
     levels = [orb_level_pixels(w, h, orb.mvInvScaleFactor) for (w, h) in ((1280, 720), (1280, 720), (640, 480))]
+    sizes = [level_sizes(w, h, orb.mvInvScaleFactor) for (w, h) in ((1280, 720), (1280, 720), (640, 480))]
     P = sum(sum(l) for l in levels) * frames          # pyramid pixels
     P0 = sum(l[0] for l in levels) * frames
+    Plast = sum(l[-1] for l in levels) * frames
+    det = sum(max(w - 38, 0) * max(h - 38, 0) for s in sizes for (w, h) in s) * frames   # FAST cell detection areas
     nkp = int(counts.sum())
     if kernel == "k_blur":
-        return 2 * P
+        return 2 * P                                  # read level, write blurred level
+    if kernel == "k_fast_score":
+        return 2 * P                                  # read level, write score plane
     if kernel == "k_fast_cells":
-        return P
+        return det                                    # read score plane over the detection areas
     if kernel == "k_resize":
-        return 2 * (P - P0)
+        return (P - Plast) + (P - P0)                 # read level l-1, write level l
     if kernel == "k_copy_level0":
         return 2 * P0
     if kernel == "k_keypoints":
         return 56 * nkp
-    if kernel == "k_knn2":
-        return None
     return None
+
+
+def level_sizes(w, h, inv_scale):
+    out = []
+    for l, s in enumerate(inv_scale):
+        lw = w if l == 0 else int(np.rint(np.float32(w) * s))
+        lh = h if l == 0 else int(np.rint(np.float32(h) * s))
+        out.append((lw, lh))
+    return out
 
 
 def cpu_baseline(frames_np, nfeat, budget_s=12.0):
@@ -107,10 +120,11 @@ def main():
     ap.add_argument("--frames", type=int, default=FRAMES)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ba", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the one-frame-at-a-time comparison run")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (every box has one GPU; "
-              "the path would shard by rig frame and by BA observation, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path would shard by rig "
+              "frame and by BA observation, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -160,16 +174,18 @@ def main():
     log(f"batched: {ms_step:.3f} ms per {n}-frame step -> {fps:.0f} rig frames/s")
     _, _, counts, _, mcounts, _ = rig.read()
 
-    # plain path: one rig frame at a time, no graph
-    rig.run(ptrs, graph=False)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    plain_reps = 2
-    for _ in range(plain_reps):
+    plain_fps = None
+    if not args.no_plain:
+        # plain path: one rig frame at a time, no graph
         rig.run(ptrs, graph=False)
-    torch.cuda.synchronize()
-    plain_fps = n * plain_reps / (time.perf_counter() - t0)
-    log(f"plain: {plain_fps:.0f} rig frames/s (batched/plain = {fps / plain_fps:.1f}x)")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plain_reps = 2
+        for _ in range(plain_reps):
+            rig.run(ptrs, graph=False)
+        torch.cuda.synchronize()
+        plain_fps = n * plain_reps / (time.perf_counter() - t0)
+        log(f"plain: {plain_fps:.0f} rig frames/s (batched/plain = {fps / plain_fps:.1f}x)")
 
     # per-kernel durations (events after every launch, eager sequence)
     per = {}
@@ -199,8 +215,8 @@ def main():
                                "U{500..4000} (seed 40+frame), 8 levels, scale 1.2, FAST 20/7, match ratio 0.7 "
                                "threshold 50",
                    "frames_per_step": n, "features_per_step": int(nfeat.sum()), "parallelism": "single GPU"},
-        "plain_frames_per_s": round(plain_fps, 2),
-        "batched_over_plain": round(fps / plain_fps, 2),
+        "plain_frames_per_s": round(plain_fps, 2) if plain_fps else None,
+        "batched_over_plain": round(fps / plain_fps, 2) if plain_fps else None,
         "kernels_ms_per_step": {k: round(v, 4) for k, v in kern.items()},
         "keypoints_per_step": int(counts.sum()),
         "matches_per_step": int(mcounts.sum()),
@@ -244,7 +260,7 @@ def main():
                         "roofline": {"kernel": "k_ba_linearize", "bound": "hbm", "achieved": round(ach, 2),
                                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                                      "bytes_per_obs": round(ba_bytes / M, 2)}}
-        log(f"BA linearize: {ba_ms * 1e3:.1f} us per {M} obs -> {M / ba_ms / 1e3:.3e} obs/s, {ach:.0f} GB/s")
+        log(f"BA linearize: {ba_ms * 1e3:.1f} us per {M} obs -> {M / (ba_ms / 1e3):.3e} obs/s, {ach:.0f} GB/s")
         ba.close()
 
     if not args.no_cpu:

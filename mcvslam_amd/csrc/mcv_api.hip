@@ -373,14 +373,17 @@ struct OrbPlan {
     int n_images = 0, nlevels = 0, cap = 0, n_slots = 0;
     std::vector<LevelDesc> lv;
     std::vector<CellDesc> cells;
-    std::vector<Tile> tiles;
-    std::vector<int> tile_begin;  // per level index l: [tile_begin[l], tile_begin[l+1])
+    std::vector<Strip> strips;
+    std::vector<int> strip_begin;  // per level index l: [strip_begin[l], strip_begin[l+1])
     std::vector<int2> coef;
     std::vector<int> slot_lvl;
     std::vector<float> seeds;
     long long pyr_bytes = 0, pool_cap = 0;
     int n_cand = 0, max_slots_img = 0;
-    DevBuf d_lv, d_cells, d_tiles, d_coef, d_slot, d_seeds, d_pyr, d_blur, d_cell_cand, d_cell_n, d_lvl_cand, d_oct,
+    int heap_cap = 0;   // octree heap entries (dynamic LDS of k_octree_heap)
+    int node_cap = 0;   // node-table entries that k_octree_heap caches in LDS
+    std::vector<int> heap_order;  // levels by feature target, largest first
+    DevBuf d_heap_order, d_lv, d_cells, d_strips, d_coef, d_slot, d_seeds, d_pyr, d_blur, d_score, d_cell_cand, d_cell_n, d_lvl_cand, d_oct,
         d_pool, d_used, d_nodes, d_src, d_stride;
     hipGraphExec_t graph = nullptr;
     const void* graph_key = nullptr;
@@ -392,7 +395,7 @@ struct OrbPlan {
     void build(const OrbConfig& c, const std::vector<ImageSpec>& imgs, long long pool_words) {
         n_images = (int)imgs.size();
         nlevels = c.nlevels;
-        lv.clear(); cells.clear(); tiles.clear(); coef.clear(); slot_lvl.clear(); seeds.clear();
+        lv.clear(); cells.clear(); strips.clear(); coef.clear(); slot_lvl.clear(); seeds.clear();
         pyr_bytes = 0;
         n_cand = 0;
         int slot = 0;
@@ -413,8 +416,9 @@ struct OrbPlan {
                 REQUIRE(d.w >= 8 && d.h >= 8, MCV_ERR_INVALID, "pyramid level smaller than 8 px");
                 d.src = l == 0 ? -1 : (int)lv.size() - 1;
                 d.nfeat = npl[l];
+                d.pitch = (d.w + kPitchAlign - 1) / kPitchAlign * kPitchAlign;
                 d.pix = pyr_bytes;
-                pyr_bytes += ((long long)d.w * d.h + 255) / 256 * 256;
+                pyr_bytes += ((long long)d.pitch * d.h + 255) / 256 * 256;
                 d.coef_x = d.coef_y = 0;
                 if (l > 0) {
                     const LevelDesc& s = lv[d.src];
@@ -488,29 +492,38 @@ struct OrbPlan {
         }
         n_slots = slot;
         cap = max_slots_img;
-        // tiles grouped by level index, then image
-        tile_begin.assign(nlevels + 1, 0);
+        heap_cap = 16;
+        for (const LevelDesc& d : lv) heap_cap = std::max(heap_cap, std::max(d.nfeat, d.n_ini) + 16);
+        node_cap = std::max(0, std::min(4096, (65536 - 16 * heap_cap) / 8));
+        heap_order.resize(lv.size());
+        for (size_t k = 0; k < lv.size(); k++) heap_order[k] = (int)k;
+        std::stable_sort(heap_order.begin(), heap_order.end(),
+                         [&](int a, int b) { return lv[a].nfeat > lv[b].nfeat; });
+        // strips grouped by level index, then image
+        strip_begin.assign(nlevels + 1, 0);
         for (int l = 0; l < nlevels; l++) {
-            tile_begin[l] = (int)tiles.size();
+            strip_begin[l] = (int)strips.size();
             for (int i = 0; i < n_images; i++) {
                 const LevelDesc& d = lv[lvl_of[i][l]];
-                for (int y = 0; y < d.h; y += TH)
-                    for (int x = 0; x < d.w; x += TW) tiles.push_back(Tile{lvl_of[i][l], x, y});
+                for (int y = 0; y < d.h; y += kStripH)
+                    for (int x = 0; x < d.w; x += kStripW) strips.push_back(Strip{lvl_of[i][l], x, y});
             }
         }
-        tile_begin[nlevels] = (int)tiles.size();
+        strip_begin[nlevels] = (int)strips.size();
         pool_cap = pool_words;
     }
 
     void upload_all(hipStream_t s) {
         upload(d_lv, lv, s);
         upload(d_cells, cells, s);
-        upload(d_tiles, tiles, s);
+        upload(d_strips, strips, s);
+        upload(d_heap_order, heap_order, s);
         upload(d_coef, coef, s);
         upload(d_slot, slot_lvl, s);
         upload(d_seeds, seeds, s);
         d_pyr.ensure(pyr_bytes + 256);
         d_blur.ensure(pyr_bytes + 256);
+        d_score.ensure(pyr_bytes + 256);
         d_cell_cand.ensure((size_t)std::max(n_cand, 1) * 4);
         d_cell_n.ensure((size_t)std::max<size_t>(cells.size(), 1) * 4);
         d_lvl_cand.ensure((size_t)std::max(n_cand, 1) * 4);
@@ -529,6 +542,7 @@ struct OrbPlan {
         g.cells = d_cells.as<CellDesc>();
         g.pyr = d_pyr.as<uint8_t>();
         g.blur = d_blur.as<uint8_t>();
+        g.score = d_score.as<uint8_t>();
         g.cell_cand = d_cell_cand.as<uint32_t>();
         g.cell_n = d_cell_n.as<int>();
         g.lvl_cand = d_lvl_cand.as<uint32_t>();
@@ -586,33 +600,43 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
     const OrbGlobals g = globals(c);
     hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(d_used.p), 4);
     mark("k_zero_words");
-    const Tile* t = d_tiles.as<Tile>();
-    int n0 = tile_begin[1] - tile_begin[0];
+    const Strip* st = d_strips.as<Strip>();
+    auto strip_grid = [](int n) { return dim3((n + 3) / 4); };
+    const int n0 = strip_begin[1] - strip_begin[0];
     if (n0 > 0) {
-        hipLaunchKernelGGL(k_copy_level0, dim3(n0), dim3(256), 0, s, t + tile_begin[0], g.lv,
-                           d_src.as<const uint8_t*>(), d_stride.as<int>(), g.pyr);
+        hipLaunchKernelGGL(k_copy_level0, strip_grid(n0), dim3(256), 0, s, st, n0, g.lv, d_src.as<const uint8_t*>(),
+                           d_stride.as<int>(), g.pyr);
         mark("k_copy_level0");
     }
     for (int l = 1; l < nlevels; l++) {
-        const int n = tile_begin[l + 1] - tile_begin[l];
+        const int n = strip_begin[l + 1] - strip_begin[l];
         if (n > 0) {
-            hipLaunchKernelGGL(k_resize, dim3(n), dim3(256), 0, s, t + tile_begin[l], g.lv, d_coef.as<int2>(), g.pyr);
+            hipLaunchKernelGGL(k_resize, strip_grid(n), dim3(256), 0, s, st + strip_begin[l], n, g.lv,
+                               d_coef.as<int2>(), g.pyr);
             mark("k_resize");
         }
     }
-    BlurTaps taps;
+    BlurTaps taps{};
     for (int k = 0; k < 7; k++) taps.t[k] = c.gauss[k];
-    if (!tiles.empty()) {
-        hipLaunchKernelGGL(k_blur, dim3(tiles.size()), dim3(256), 0, s, t, g.lv, g.pyr, g.blur, taps);
+    taps.lo4 = (uint32_t)c.gauss[0] | (uint32_t)c.gauss[1] << 8 | (uint32_t)c.gauss[2] << 16 | (uint32_t)c.gauss[3] << 24;
+    taps.hi3 = (uint32_t)c.gauss[4] | (uint32_t)c.gauss[5] << 8 | (uint32_t)c.gauss[6] << 16;
+    const int ns = (int)strips.size();
+    if (ns > 0) {
+        hipLaunchKernelGGL(k_blur, strip_grid(ns), dim3(256), 0, s, st, ns, g.lv, g.pyr, g.blur, taps);
         mark("k_blur");
+        hipLaunchKernelGGL(k_fast_score, strip_grid(ns), dim3(256), 0, s, st, ns, g.lv, g.pyr, g.score,
+                           std::min(g.ini_th, g.min_th));
+        mark("k_fast_score");
     }
-    if (!cells.empty()) {
-        hipLaunchKernelGGL(k_fast_cells, dim3(cells.size()), dim3(256), 0, s, g);
+    const int nc = (int)cells.size();
+    if (nc > 0) {
+        hipLaunchKernelGGL(k_fast_cells, dim3((nc + 3) / 4), dim3(256), 0, s, g, nc);
         mark("k_fast_cells");
     }
     hipLaunchKernelGGL(k_octree_build, dim3(lv.size()), dim3(kOctThreads), 0, s, g);
     mark("k_octree_build");
-    hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), 0, s, g);
+    hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), sizeof(uint4) * heap_cap + 8 * node_cap, s, g,
+                       d_heap_order.as<int>(), heap_cap, node_cap);
     mark("k_octree_heap");
     hipLaunchKernelGGL(k_keypoints, dim3((n_slots + 3) / 4), dim3(256), 0, s, g, d_slot.as<int>(), n_slots, tb,
                        reinterpret_cast<float*>(d_kps), d_desc, out_cap, d_counts);
@@ -626,6 +650,7 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
 struct mcv_ctx {
     int device = 0;
     hipStream_t own = nullptr;
+    hipStream_t cap = nullptr;  // graph capture only (the null stream cannot capture)
     hipStream_t stream = nullptr;
     std::string err;
     OrbConfig orb;
@@ -640,6 +665,11 @@ struct mcv_ctx {
     std::map<std::string, std::unique_ptr<OrbPlan>> plans;
 
     void set_device() { HIP_OK(hipSetDevice(device)); }
+
+    hipStream_t capture_stream() {
+        if (!cap) HIP_OK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        return cap;
+    }
 
     const SinCosTables tables() {
         if (!sc.ready) {
@@ -738,6 +768,7 @@ void mcv_destroy(mcv_ctx* c) {
     c->plans.clear();
     c->single.reset();
     if (c->own) (void)hipStreamDestroy(c->own);
+    if (c->cap) (void)hipStreamDestroy(c->cap);
     delete c;
 }
 
@@ -882,8 +913,8 @@ int mcv_orb_pyramid(mcv_ctx* c, int level, uint8_t* out, int capacity, int32_t* 
     if (out) {
         REQUIRE(capacity >= d.w * d.h, MCV_ERR_CAPACITY, "pyramid buffer too small");
         c->set_device();
-        HIP_OK(hipMemcpyAsync(out, c->single->d_pyr.as<uint8_t>() + d.pix, (size_t)d.w * d.h, hipMemcpyDeviceToHost,
-                              c->stream));
+        HIP_OK(hipMemcpy2DAsync(out, (size_t)d.w, c->single->d_pyr.as<uint8_t>() + d.pix, (size_t)d.pitch, (size_t)d.w,
+                                (size_t)d.h, hipMemcpyDeviceToHost, c->stream));
         HIP_OK(hipStreamSynchronize(c->stream));
     }
     API_END
@@ -921,9 +952,10 @@ int mcv_orb_extract_batch_dev(mcv_ctx* c, int n, const uint8_t* const* d_images,
             if (P.graph) HIP_OK(hipGraphExecDestroy(P.graph));
             P.graph = nullptr;
             hipGraph_t gr;
-            HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            P.enqueue(c->orb, tb, c->stream, d_kps, d_desc, capacity, d_counts);
-            HIP_OK(hipStreamEndCapture(c->stream, &gr));
+            hipStream_t cs = c->capture_stream();
+            HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            P.enqueue(c->orb, tb, cs, d_kps, d_desc, capacity, d_counts);
+            HIP_OK(hipStreamEndCapture(cs, &gr));
             HIP_OK(hipGraphInstantiate(&P.graph, gr, nullptr, nullptr, 0));
             HIP_OK(hipGraphDestroy(gr));
             P.graph_key = gk;
@@ -1128,9 +1160,10 @@ int mcv_rig_run(mcv_rig* r, const uint8_t* const* d_images, int flags) {
         HIP_OK(hipMemcpyAsync(P.d_stride.p, st.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
         if (!r->graph) {
             hipGraph_t gr;
-            HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            rig_enqueue_batched(r, tb, s);
-            HIP_OK(hipStreamEndCapture(s, &gr));
+            hipStream_t cs = c->capture_stream();
+            HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            rig_enqueue_batched(r, tb, cs);
+            HIP_OK(hipStreamEndCapture(cs, &gr));
             HIP_OK(hipGraphInstantiate(&r->graph, gr, nullptr, nullptr, 0));
             HIP_OK(hipGraphDestroy(gr));
         }

@@ -16,16 +16,20 @@ namespace mcv {
 // ---------------------------------------------------------------------------
 // std::priority_queue<T, vector<T>, Less> emulation on a plain array, with the
 // exact sift order of libstdc++'s push_heap / pop_heap (bits/stl_heap.h:
-// __push_heap, __adjust_heap, __pop_heap).  Heap entries are u64 with the
-// ordering key in the high 32 bits; the low 32 bits are payload that never
-// takes part in a comparison, exactly like the shared_ptr payload of the
-// reference's node queue (ORBextractor.cc:530-531).
-MCV_HD bool heap_less(uint64_t a, uint64_t b) { return (uint32_t)(a >> 32) < (uint32_t)(b >> 32); }
+// __push_heap, __adjust_heap, __pop_heap).  The ordering key of an entry is
+// heap_key(); the rest of the entry is payload that never takes part in a
+// comparison, exactly like the shared_ptr payload of the reference's node
+// queue (ORBextractor.cc:530-531).  u64 entries keep the key in the high
+// 32 bits; uint4 entries in .x.
+MCV_HD uint32_t heap_key(uint64_t a) { return (uint32_t)(a >> 32); }
+#if defined(__HIPCC__)
+MCV_HD uint32_t heap_key(const uint4& a) { return a.x; }
+#endif
 
-template <typename Ptr>
-MCV_HD void heap_sift_up(Ptr h, int hole, int top, uint64_t value) {
+template <typename Ptr, typename T>
+MCV_HD void heap_sift_up(Ptr h, int hole, int top, T value) {
     int parent = (hole - 1) / 2;
-    while (hole > top && heap_less(h[parent], value)) {
+    while (hole > top && heap_key(h[parent]) < heap_key(value)) {
         h[hole] = h[parent];
         hole = parent;
         parent = (hole - 1) / 2;
@@ -33,20 +37,20 @@ MCV_HD void heap_sift_up(Ptr h, int hole, int top, uint64_t value) {
     h[hole] = value;
 }
 
-template <typename Ptr>
-MCV_HD void heap_push(Ptr h, int& len, uint64_t value) {
+template <typename Ptr, typename T>
+MCV_HD void heap_push(Ptr h, int& len, T value) {
     h[len] = value;
     heap_sift_up(h, len, 0, value);
     len++;
 }
 
-template <typename Ptr>
-MCV_HD void heap_adjust(Ptr h, int hole, int len, uint64_t value) {
+template <typename Ptr, typename T>
+MCV_HD void heap_adjust(Ptr h, int hole, int len, T value) {
     const int top = hole;
     int second = hole;
     while (second < (len - 1) / 2) {
         second = 2 * (second + 1);
-        if (heap_less(h[second], h[second - 1])) second--;
+        if (heap_key(h[second]) < heap_key(h[second - 1])) second--;
         h[hole] = h[second];
         hole = second;
     }
@@ -62,7 +66,7 @@ template <typename Ptr>
 MCV_HD void heap_pop(Ptr h, int& len) {
     if (len > 1) {
         const int last = len - 1;
-        uint64_t value = h[last];
+        const auto value = h[last];
         h[last] = h[0];
         heap_adjust(h, 0, last, value);
     }

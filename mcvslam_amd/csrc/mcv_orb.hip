@@ -22,14 +22,14 @@
 
 namespace mcv {
 
-struct Tile {
+struct Strip {
     int lvl;     // LevelDesc index
-    int x0, y0;  // tile origin (64 x 16 pixels)
+    int x0, y0;  // strip origin: kStripW x kStripH pixels, one wave
 };
-constexpr int TW = 64, TH = 16;
 
 struct BlurTaps {
     int t[7];
+    uint32_t lo4, hi3;  // taps 0..3 and 4..6 packed as u8x4 for udot4
 };
 
 struct SinCosTables {
@@ -47,6 +47,7 @@ struct OrbGlobals {
     const CellDesc* cells;
     uint8_t* pyr;
     uint8_t* blur;
+    uint8_t* score;         // FAST best-arc score per pixel (0 at or below the low threshold)
     uint32_t* cell_cand;    // per-cell candidate slots
     int* cell_n;            // candidates kept per cell
     uint32_t* lvl_cand;     // per-level contiguous candidates
@@ -61,84 +62,158 @@ struct OrbGlobals {
     int* err;               // bit 0: pool overflow, bit 1: level too large
 };
 
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_copy_level0(const Tile* tiles, const LevelDesc* lv,
-                                                     const uint8_t* const* src, const int* src_stride, uint8_t* pyr) {
-    const Tile t = tiles[blockIdx.x];
-    const LevelDesc L = lv[t.lvl];
-    const uint8_t* s = src[L.img];
-    const int stride = src_stride[L.img];
-    for (int i = threadIdx.x; i < TW * TH; i += 256) {
-        const int x = t.x0 + (i & (TW - 1)), y = t.y0 + i / TW;
-        if (x < L.w && y < L.h) pyr[L.pix + (long long)y * L.w + x] = s[(long long)y * stride + x];
-    }
-}
-
-// cv::resize INTER_LINEAR, CV_8U: horizontal int taps (11-bit), vertical
-// ((b0*(r0>>4))>>16) + ((b1*(r1>>4))>>16) + 2) >> 2.
-__global__ __launch_bounds__(256) void k_resize(const Tile* tiles, const LevelDesc* lv, const int2* coef,
-                                                uint8_t* pyr) {
-    const Tile t = tiles[blockIdx.x];
-    const LevelDesc L = lv[t.lvl];
-    const LevelDesc S = lv[L.src];
-    const uint8_t* sp = pyr + S.pix;
-    for (int i = threadIdx.x; i < TW * TH; i += 256) {
-        const int x = t.x0 + (i & (TW - 1)), y = t.y0 + i / TW;
-        if (x >= L.w || y >= L.h) continue;
-        const int2 cx = coef[L.coef_x + x];
-        const int2 cy = coef[L.coef_y + y];
-        const int a0 = cx.y & 0xFFFF, a1 = cx.y >> 16;
-        const int b0 = cy.y & 0xFFFF, b1 = cy.y >> 16;
-        const int sx = cx.x, sx1 = min(cx.x + 1, S.w - 1);
-        const int y0 = min(max(cy.x, 0), S.h - 1), y1 = min(max(cy.x + 1, 0), S.h - 1);
-        const uint8_t* r0p = sp + (long long)y0 * S.w;
-        const uint8_t* r1p = sp + (long long)y1 * S.w;
-        const int r0 = r0p[sx] * a0 + r0p[sx1] * a1;
-        const int r1 = r1p[sx] * a0 + r1p[sx1] * a1;
-        pyr[L.pix + (long long)y * L.w + x] = (uint8_t)((((b0 * (r0 >> 4)) >> 16) + ((b1 * (r1 >> 4)) >> 16) + 2) >> 2);
-    }
+// Orders this wave's LDS accesses across its lanes (wave-private LDS areas).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ int reflect101(int i, int n) {
-    i = i < 0 ? -i : i;
-    i = i >= n ? 2 * n - 2 - i : i;
+    if (i < 0) i = -i;
+    if (i >= n) i = 2 * n - 2 - i;
     return i;
 }
 
-// GaussianBlur 7x7 on CV_8U through the fixed-point path: Q8 taps,
-// h = sum t*src (<= 65280), out = (sum t*h + 2^15) >> 16.
-__global__ __launch_bounds__(256) void k_blur(const Tile* tiles, const LevelDesc* lv, const uint8_t* pyr,
-                                              uint8_t* blur, BlurTaps taps) {
-    __shared__ uint8_t in[(TH + 6) * (TW + 6)];
-    __shared__ uint32_t hb[(TH + 6) * TW];
-    const Tile t = tiles[blockIdx.x];
-    const LevelDesc L = lv[t.lvl];
+// ---------------------------------------------------------------------------
+// Strip kernels: a wave owns kStripW x kStripH pixels of one level, each lane
+// 4 horizontally adjacent pixels (one u32 of a pitched row).
+#define STRIP_SETUP(tab, n)                                         \
+    const int sidx = blockIdx.x * 4 + (threadIdx.x >> 6);           \
+    if (sidx >= (n)) return;                                        \
+    const int lane = threadIdx.x & 63;                              \
+    const Strip S = (tab)[sidx];                                    \
+    const LevelDesc L = lv[S.lvl];                                  \
+    const int x = S.x0 + 4 * lane;                                  \
+    const int yend = min(S.y0 + kStripH, L.h);
+
+__global__ __launch_bounds__(256) void k_copy_level0(const Strip* st, int n, const LevelDesc* lv,
+                                                     const uint8_t* const* src, const int* src_stride,
+                                                     uint8_t* __restrict__ pyr) {
+    STRIP_SETUP(st, n)
+    if (x >= L.w) return;
+    const uint8_t* s = src[L.img];
+    const long long stride = src_stride[L.img];
+    uint8_t* d = pyr + L.pix + x;
+    const bool vec = x + 4 <= L.w && ((reinterpret_cast<uintptr_t>(s) | (uintptr_t)stride | (uintptr_t)x) & 3) == 0;
+    for (int y = S.y0; y < yend; y++) {
+        const uint8_t* sr = s + y * stride + x;
+        uint32_t v;
+        if (vec) {
+            v = *reinterpret_cast<const uint32_t*>(sr);
+        } else {
+            v = 0;
+            for (int k = 0; k < 4 && x + k < L.w; k++) v |= (uint32_t)sr[k] << (8 * k);
+        }
+        *reinterpret_cast<uint32_t*>(d + (long long)y * L.pitch) = v;
+    }
+}
+
+// cv::resize INTER_LINEAR, CV_8U: horizontal int taps (coef 2048 scale) on
+// the two source rows, then the vertical pass
+// (((b0 * (r0 >> 4)) >> 16) + ((b1 * (r1 >> 4)) >> 16) + 2) >> 2.
+// Rows [y0, y1) of one strip: source and destination are different levels
+// of the same buffer, so they are passed as non-aliasing pointers and the
+// row loop is unrolled, which lets the loads of later rows run ahead of the
+// stores of earlier ones.
+__device__ __forceinline__ void resize_rows(const uint8_t* __restrict__ sp, uint8_t* __restrict__ dp,
+                                            const int2* __restrict__ coef, const LevelDesc& L,
+                                            const LevelDesc& SL, int x, int y0, int y1) {
+    int sx0[4], sx1[4], a0[4], a1[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int2 cx = coef[L.coef_x + min(x + j, L.w - 1)];
+        sx0[j] = cx.x;
+        sx1[j] = min(cx.x + 1, SL.w - 1);
+        a0[j] = cx.y & 0xFFFF;
+        a1[j] = cx.y >> 16;
+    }
+#pragma unroll 4
+    for (int y = y0; y < y1; y++) {
+        const int2 cy = coef[L.coef_y + y];
+        const int b0 = cy.y & 0xFFFF, b1 = cy.y >> 16;
+        const uint8_t* r0p = sp + (long long)min(max(cy.x, 0), SL.h - 1) * SL.pitch;
+        const uint8_t* r1p = sp + (long long)min(max(cy.x + 1, 0), SL.h - 1) * SL.pitch;
+        uint32_t v = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int r0 = r0p[sx0[j]] * a0[j] + r0p[sx1[j]] * a1[j];
+            const int r1 = r1p[sx0[j]] * a0[j] + r1p[sx1[j]] * a1[j];
+            const uint32_t o = (uint32_t)((((b0 * (r0 >> 4)) >> 16) + ((b1 * (r1 >> 4)) >> 16) + 2) >> 2);
+            v |= (o & 0xFF) << (8 * j);
+        }
+        *reinterpret_cast<uint32_t*>(dp + (long long)y * L.pitch + x) = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_resize(const Strip* st, int n, const LevelDesc* lv, const int2* coef,
+                                                uint8_t* pyr) {
+    STRIP_SETUP(st, n)
+    if (x >= L.w) return;
+    const LevelDesc SL = lv[L.src];
+    resize_rows(pyr + SL.pix, pyr + L.pix, coef, L, SL, x, S.y0, yend);
+}
+
+// Bytes x-4 .. x+7 of a row as three u32 (little endian).  Interior lanes
+// use aligned loads; lanes at the image border gather with REFLECT_101 on
+// indices clamped to [-3, w+2], the range any stored output can reach.
+__device__ __forceinline__ void row_words(const uint8_t* row, int x, int w, int pitch, uint32_t& w0, uint32_t& w1,
+                                          uint32_t& w2) {
+    if (x >= 4 && x + 8 <= pitch && x + 6 < w) {
+        const uint32_t* r = reinterpret_cast<const uint32_t*>(row + x);
+        w0 = r[-1];
+        w1 = r[0];
+        w2 = r[1];
+    } else {
+        uint32_t v[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+            const int i = reflect101(min(max(x - 4 + k, -3), w + 2), w);
+            v[k >> 2] |= (uint32_t)row[i] << (8 * (k & 3));
+        }
+        w0 = v[0];
+        w1 = v[1];
+        w2 = v[2];
+    }
+}
+
+// GaussianBlur 7x7, sigma 2, BORDER_REFLECT_101, CV_8U fixed point: the
+// 7 taps (sum 256) horizontally with udot4, then vertically, out =
+// (sum + 2^15) >> 16.  Each lane keeps the last 7 horizontal rows of its
+// 4 columns in registers while walking down the strip.
+__global__ __launch_bounds__(256) void k_blur(const Strip* st, int n, const LevelDesc* lv,
+                                              const uint8_t* __restrict__ pyr, uint8_t* __restrict__ blur,
+                                              BlurTaps taps) {
+    STRIP_SETUP(st, n)
+    if (x >= L.w) return;
     const uint8_t* src = pyr + L.pix;
-    for (int i = threadIdx.x; i < (TH + 6) * (TW + 6); i += 256) {
-        const int ly = i / (TW + 6), lx = i - ly * (TW + 6);
-        const int gx = reflect101(min(t.x0 + lx - 3, L.w + 2), L.w);
-        const int gy = reflect101(min(t.y0 + ly - 3, L.h + 2), L.h);
-        in[i] = src[(long long)gy * L.w + gx];
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < (TH + 6) * TW; i += 256) {
-        const int ly = i / TW, lx = i - ly * TW;
-        const uint8_t* r = in + ly * (TW + 6) + lx;
-        uint32_t acc = 0;
+    uint32_t hr[7][4];
 #pragma unroll
-        for (int k = 0; k < 7; k++) acc += (uint32_t)taps.t[k] * r[k];
-        hb[i] = acc;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < TH * TW; i += 256) {
-        const int ly = i / TW, lx = i - ly * TW;
-        const int x = t.x0 + lx, y = t.y0 + ly;
-        if (x >= L.w || y >= L.h) continue;
-        uint32_t acc = 0;
+    for (int r = 0; r < kStripH + 6; r++) {
+        const int yi = S.y0 - 3 + r;
+        if (yi - 3 >= yend) break;
+        const int ry = reflect101(min(max(yi, -3), L.h + 2), L.h);
+        uint32_t w0, w1, w2;
+        row_words(src + (long long)ry * L.pitch, x, L.w, L.pitch, w0, w1, w2);
+        uint32_t* h = hr[r % 7];
 #pragma unroll
-        for (int k = 0; k < 7; k++) acc += (uint32_t)taps.t[k] * hb[(ly + k) * TW + lx];
-        const uint32_t v = (acc + (1u << 15)) >> 16;
-        blur[L.pix + (long long)y * L.w + x] = (uint8_t)min(v, 255u);
+        for (int j = 0; j < 4; j++) {
+            const uint32_t lo = j == 3 ? w1 : __builtin_amdgcn_alignbyte(w1, w0, j + 1);
+            const uint32_t hi = j == 3 ? w2 : __builtin_amdgcn_alignbyte(w2, w1, j + 1);
+            h[j] = __builtin_amdgcn_udot4(lo, taps.lo4, __builtin_amdgcn_udot4(hi, taps.hi3, 0u, false), false);
+        }
+        if (r >= 6) {
+            const int y = yi - 3;
+            uint32_t v = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                uint32_t acc = 0;
+#pragma unroll
+                for (int k = 0; k < 7; k++) acc += (uint32_t)taps.t[k] * hr[(r - 6 + k) % 7][j];
+                v |= min((acc + (1u << 15)) >> 16, 255u) << (8 * j);
+            }
+            *reinterpret_cast<uint32_t*>(blur + L.pix + (long long)y * L.pitch + x) = v;
+        }
     }
 }
 
@@ -166,11 +241,6 @@ __device__ __forceinline__ int fast_best(const uint8_t* win, int W, int x, int y
     d[13] = v - c[W - 3];
     d[14] = v - c[2 * W - 2];
     d[15] = v - c[3 * W - 1];
-    // quick reject: any 9-arc covers two consecutive compass points (0,4,8,12)
-    int lo0 = min(d[0], d[4]), lo1 = min(d[4], d[8]), lo2 = min(d[8], d[12]), lo3 = min(d[12], d[0]);
-    int hi0 = max(d[0], d[4]), hi1 = max(d[4], d[8]), hi2 = max(d[8], d[12]), hi3 = max(d[12], d[0]);
-    const int bound = max(max(max(lo0, lo1), max(lo2, lo3)), -min(min(hi0, hi1), min(hi2, hi3)));
-    if (bound <= 0) return 0;
     int m2[16], m4[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) m2[k] = min(d[k], d[(k + 1) & 15]);
@@ -188,6 +258,248 @@ __device__ __forceinline__ int fast_best(const uint8_t* win, int W, int x, int y
     for (int k = 0; k < 16; k++) bright = max(bright, -max(max(m4[k], m4[(k + 4) & 15]), d[(k + 8) & 15]));
     const int b = max(dark, bright);
     return b < 0 ? 0 : b;
+}
+
+__device__ __forceinline__ int byte_of(uint32_t w, int j) { return (int)((w >> (8 * j)) & 0xFF); }
+
+// Score plane: best(p) where best > t_lo = min(iniTh, minTh) and the circle
+// fits the level, else 0 (values at or below both thresholds never change a
+// corner test or a 3x3 suppression).
+//
+// A wave walks its strip down row by row.  The rows it needs sit in an LDS
+// ring (16 rows x 264 bytes: the strip plus 4 bytes either side), each row
+// fetched with one coalesced u32 per lane one iteration ahead.  A compass
+// bound rejects most pixels: every 9-arc holds two consecutive compass
+// points (0,4,8,12), so a corner at t needs such a pair both darker than
+// v - t or both brighter than v + t; the test runs on 2 pixels per packed
+// u16 operation.  Survivors queue up in LDS and are scored 64 at a time by
+// the whole wave from the ring; an 8-row block of scores is assembled in LDS
+// and stored once its queue is drained.
+typedef unsigned short mcv_us2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ mcv_us2 as_us2(uint32_t v) { return __builtin_bit_cast(mcv_us2, v); }
+__device__ __forceinline__ uint32_t as_u32(mcv_us2 v) { return __builtin_bit_cast(uint32_t, v); }
+
+// compass test for two pixels held in the u16 lanes of the arguments
+__device__ __forceinline__ uint32_t compass2(mcv_us2 v, mcv_us2 s, mcv_us2 e, mcv_us2 n, mcv_us2 w, mcv_us2 t) {
+    const mcv_us2 vt = __builtin_elementwise_sub_sat(v, t);
+    const mcv_us2 vpt = v + t;
+    const mcv_us2 ds = __builtin_elementwise_sub_sat(vt, s), de = __builtin_elementwise_sub_sat(vt, e);
+    const mcv_us2 dn = __builtin_elementwise_sub_sat(vt, n), dw = __builtin_elementwise_sub_sat(vt, w);
+    const mcv_us2 bs = __builtin_elementwise_sub_sat(s, vpt), be = __builtin_elementwise_sub_sat(e, vpt);
+    const mcv_us2 bn = __builtin_elementwise_sub_sat(n, vpt), bw = __builtin_elementwise_sub_sat(w, vpt);
+    const mcv_us2 pd = __builtin_elementwise_max(
+        __builtin_elementwise_max(__builtin_elementwise_min(ds, de), __builtin_elementwise_min(de, dn)),
+        __builtin_elementwise_max(__builtin_elementwise_min(dn, dw), __builtin_elementwise_min(dw, ds)));
+    const mcv_us2 pb = __builtin_elementwise_max(
+        __builtin_elementwise_max(__builtin_elementwise_min(bs, be), __builtin_elementwise_min(be, bn)),
+        __builtin_elementwise_max(__builtin_elementwise_min(bn, bw), __builtin_elementwise_min(bw, bs)));
+    return as_u32(__builtin_elementwise_max(pd, pb));
+}
+
+// exact best of the pixel at byte c of the ring rows r[0..6] (rows -3..+3)
+__device__ __forceinline__ int fast_best_rows(const uint8_t* const* r, int c) {
+    const int v = r[3][c];
+    int d[16];
+    d[0] = v - r[6][c];
+    d[1] = v - r[6][c + 1];
+    d[2] = v - r[5][c + 2];
+    d[3] = v - r[4][c + 3];
+    d[4] = v - r[3][c + 3];
+    d[5] = v - r[2][c + 3];
+    d[6] = v - r[1][c + 2];
+    d[7] = v - r[0][c + 1];
+    d[8] = v - r[0][c];
+    d[9] = v - r[0][c - 1];
+    d[10] = v - r[1][c - 2];
+    d[11] = v - r[2][c - 3];
+    d[12] = v - r[3][c - 3];
+    d[13] = v - r[4][c - 3];
+    d[14] = v - r[5][c - 2];
+    d[15] = v - r[6][c - 1];
+    int m2[16], m4[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) m2[k] = min(d[k], d[(k + 1) & 15]);
+#pragma unroll
+    for (int k = 0; k < 16; k++) m4[k] = min(m2[k], m2[(k + 2) & 15]);
+    int dark = -1024;
+#pragma unroll
+    for (int k = 0; k < 16; k++) dark = max(dark, min(min(m4[k], m4[(k + 4) & 15]), d[(k + 8) & 15]));
+#pragma unroll
+    for (int k = 0; k < 16; k++) m2[k] = max(d[k], d[(k + 1) & 15]);
+#pragma unroll
+    for (int k = 0; k < 16; k++) m4[k] = max(m2[k], m2[(k + 2) & 15]);
+    int bright = -1024;
+#pragma unroll
+    for (int k = 0; k < 16; k++) bright = max(bright, -max(max(m4[k], m4[(k + 4) & 15]), d[(k + 8) & 15]));
+    const int b = max(dark, bright);
+    return b < 0 ? 0 : b;
+}
+
+constexpr int kRing = 16;                 // ring rows per wave (power of two)
+constexpr int kRingW = kStripW / 4 + 2;   // u32 per ring row: strip + one word either side
+constexpr int kBlk = 8;                   // rows per score block
+
+__global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, const LevelDesc* lv, const uint8_t* pyr,
+                                                    uint8_t* score, int t_lo) {
+    __shared__ uint32_t s_ring[4][kRing][kRingW];
+    __shared__ uint32_t s_tile[4][kBlk][kStripW / 4];
+    __shared__ uint32_t s_queue[4][64 + kStripW];
+    STRIP_SETUP(st, n)
+    const int wv = threadIdx.x >> 6;
+    uint32_t (*ring)[kRingW] = s_ring[wv];
+    uint32_t* queue = s_queue[wv];
+    const uint8_t* img = pyr + L.pix;
+    const bool own = x < L.w;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const mcv_us2 tt = {(unsigned short)t_lo, (unsigned short)t_lo};
+    // one level row into registers: this lane's u32, plus the word left of
+    // the strip (lane 0) or right of it (lane 63); rows clamp to the level
+    auto load_row = [&](int r, uint32_t& a, uint32_t& e) {
+        const uint8_t* row = img + (long long)min(max(r, 0), L.h - 1) * L.pitch;
+        a = x < L.pitch ? *reinterpret_cast<const uint32_t*>(row + x) : 0u;
+        e = 0u;
+        if (lane == 0 && S.x0 >= 4) e = *reinterpret_cast<const uint32_t*>(row + S.x0 - 4);
+        if (lane == 63 && S.x0 + kStripW < L.pitch) e = *reinterpret_cast<const uint32_t*>(row + S.x0 + kStripW);
+    };
+    auto store_row = [&](int r, uint32_t a, uint32_t e) {
+        uint32_t* R = ring[r & (kRing - 1)];
+        R[lane + 1] = a;
+        if (lane == 0) R[0] = e;
+        if (lane == 63) R[kRingW - 1] = e;
+    };
+    uint32_t pa, pe;
+    for (int r = -3; r < 3; r++) {
+        load_row(S.y0 + r, pa, pe);
+        store_row(S.y0 + r, pa, pe);
+    }
+    load_row(S.y0 + 3, pa, pe);
+    int qlen = 0;
+    for (int y = S.y0; y < yend; y++) {
+        store_row(y + 3, pa, pe);
+        if (y + 1 < yend) load_row(y + 4, pa, pe);
+        const int tb = (y - S.y0) & (kBlk - 1);
+        s_tile[wv][tb][lane] = 0u;
+        wave_lds_sync();
+        int surv = 0;
+        if (y >= 3 && y < L.h - 3) {
+            const uint32_t* C = ring[y & (kRing - 1)];
+            const uint32_t c0 = C[lane], c1 = C[lane + 1], c2 = C[lane + 2];
+            const uint32_t up = ring[(y - 3) & (kRing - 1)][lane + 1];
+            const uint32_t dn = ring[(y + 3) & (kRing - 1)][lane + 1];
+            const uint32_t rt = __builtin_amdgcn_alignbyte(c2, c1, 3);  // columns x+3 .. x+6
+            const uint32_t lf = __builtin_amdgcn_alignbyte(c1, c0, 1);  // columns x-3 .. x
+            const uint32_t m = 0x00FF00FFu;
+            const uint32_t ev = compass2(as_us2(c1 & m), as_us2(dn & m), as_us2(rt & m), as_us2(up & m),
+                                         as_us2(lf & m), tt);  // pixels 0, 2
+            const uint32_t od = compass2(as_us2((c1 >> 8) & m), as_us2((dn >> 8) & m), as_us2((rt >> 8) & m),
+                                         as_us2((up >> 8) & m), as_us2((lf >> 8) & m), tt);  // pixels 1, 3
+            surv = ((ev & 0xFFFFu) ? 1 : 0) | ((od & 0xFFFFu) ? 2 : 0) | ((ev >> 16) ? 4 : 0) | ((od >> 16) ? 8 : 0);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (x + j < 3 || x + j >= L.w - 3) surv &= ~(1 << j);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const unsigned long long b = __ballot((surv >> j) & 1);
+            if ((surv >> j) & 1) queue[qlen + __popcll(b & lt)] = ((uint32_t)tb << 8) | (uint32_t)(4 * lane + j);
+            qlen += __popcll(b);
+        }
+        const bool block_end = tb == kBlk - 1 || y == yend - 1;
+        while (qlen >= 64 || (block_end && qlen > 0)) {
+            wave_lds_sync();
+            const int cnt = min(qlen, 64);
+            if (lane < cnt) {
+                const uint32_t e = queue[qlen - cnt + lane];
+                const int et = (int)(e >> 8), pos = (int)(e & 0xFF);
+                const int yy = y - tb + et;
+                const uint8_t* rows[7];
+#pragma unroll
+                for (int k = 0; k < 7; k++)
+                    rows[k] = reinterpret_cast<const uint8_t*>(ring[(yy - 3 + k) & (kRing - 1)]);
+                const int b = fast_best_rows(rows, 4 + pos);
+                if (b > t_lo) reinterpret_cast<uint8_t*>(s_tile[wv][et])[pos] = (uint8_t)b;
+            }
+            qlen -= cnt;
+        }
+        if (block_end) {
+            wave_lds_sync();
+            if (own) {
+                for (int k = 0; k <= tb; k++)
+                    *reinterpret_cast<uint32_t*>(score + L.pix + (long long)(y - tb + k) * L.pitch + x) =
+                        s_tile[wv][k][lane];
+            }
+            wave_lds_sync();
+        }
+    }
+}
+
+// 3x3 suppression and compaction, one wave per FAST cell window.  Pixels of
+// the detection area [3, W-3) x [3, H-3) are visited in row-major order;
+// neighbours outside it count as 0, as in FAST_t's zeroed ring rows and
+// border columns.  A pass at iniTh keeps s = best - 1 where s exceeds every
+// neighbour's (best > t ? best - 1 : 0); a cell with no keeper is redone at
+// minTh (ORBextractor.cc:604-616).  The score bytes of 8 x 64 pixels are
+// loaded before any of them is examined.
+__device__ __forceinline__ int cell_pass(const uint8_t* __restrict__ sp, uint32_t* __restrict__ out, int pitch,
+                                         int DW, int DH, uint32_t magic, int t, int offx, int offy) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const int np = DW * DH;
+    int written = 0;
+    for (int p0 = 0; p0 < np; p0 += 8 * 64) {
+        int b[8], ry[8], rx[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int p = p0 + 64 * k + lane;
+            ry[k] = (int)(((uint32_t)p * magic) >> 20);
+            rx[k] = p - ry[k] * DW;
+            b[k] = p < np ? sp[(long long)ry[k] * pitch + rx[k]] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            bool keep = false;
+            uint32_t rec = 0;
+            if (b[k] > t) {
+                const int s = b[k] - 1;
+                keep = true;
+                const uint8_t* q = sp + (long long)ry[k] * pitch + rx[k];
+#pragma unroll
+                for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                    for (int dx = -1; dx <= 1; dx++) {
+                        if (dx == 0 && dy == 0) continue;
+                        const int nx = rx[k] + dx, ny = ry[k] + dy;
+                        const int bq = (nx >= 0 && nx < DW && ny >= 0 && ny < DH) ? q[dy * pitch + dx] : 0;
+                        const int mq = bq > t ? bq - 1 : 0;
+                        keep = keep && s > mq;
+                    }
+                rec = ((uint32_t)s << 24) | ((uint32_t)(ry[k] + 3 + offy) << 12) | (uint32_t)(rx[k] + 3 + offx);
+            }
+            const unsigned long long bal = __ballot(keep);
+            if (keep) out[written + __popcll(bal & lt)] = rec;
+            written += __popcll(bal);
+        }
+    }
+    return written;
+}
+
+__global__ __launch_bounds__(256) void k_fast_cells(OrbGlobals g, int ncells) {
+    const int ci = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (ci >= ncells) return;
+    const int lane = threadIdx.x & 63;
+    const CellDesc c = g.cells[ci];
+    const LevelDesc L = g.lv[c.lvl];
+    const int DW = c.x1 - c.x0 - 6, DH = c.y1 - c.y0 - 6;
+    int total = 0;
+    if (DW > 0 && DH > 0) {
+        const uint8_t* sp = g.score + L.pix + (long long)(c.y0 + 3) * L.pitch + (c.x0 + 3);
+        const uint32_t magic = ((1u << 20) + (uint32_t)DW - 1u) / (uint32_t)DW;  // p / DW for p < 2^20 / DW
+        uint32_t* out = g.cell_cand + c.cap_base;
+        total = cell_pass(sp, out, L.pitch, DW, DH, magic, g.ini_th, c.offx, c.offy);
+        if (total == 0) total = cell_pass(sp, out, L.pitch, DW, DH, magic, g.min_th, c.offx, c.offy);
+    }
+    if (lane == 0) g.cell_n[ci] = total;
 }
 
 __device__ __forceinline__ int block_excl_scan256(int v, int* ws, int& total) {
@@ -211,67 +523,6 @@ __device__ __forceinline__ int block_excl_scan256(int v, int* ws, int& total) {
     }
     __syncthreads();
     return base + x - v;
-}
-
-// One workgroup per FAST cell window.  The window is staged in LDS, best
-// scores computed for the detection area [3, W-3) x [3, H-3) (elsewhere 0,
-// as the 3-row ring buffer of FAST_t leaves them), then the NMS/compaction
-// pass runs at iniTh and, if it keeps nothing, again at minTh.
-__global__ __launch_bounds__(256) void k_fast_cells(OrbGlobals g) {
-    __shared__ uint8_t win[kMaxWin * kMaxWin];
-    __shared__ uint8_t best[kMaxWin * kMaxWin];
-    __shared__ int ws[4];
-    const CellDesc c = g.cells[blockIdx.x];
-    const LevelDesc L = g.lv[c.lvl];
-    const int W = c.x1 - c.x0, H = c.y1 - c.y0;
-    const uint8_t* base = g.pyr + L.pix + (long long)c.y0 * L.w + c.x0;
-    for (int i = threadIdx.x; i < W * H; i += 256) {
-        const int y = i / W, x = i - y * W;
-        win[i] = base[(long long)y * L.w + x];
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < W * H; i += 256) {
-        const int y = i / W, x = i - y * W;
-        int b = 0;
-        if (x >= 3 && x < W - 3 && y >= 3 && y < H - 3) b = fast_best(win, W, x, y);
-        best[i] = (uint8_t)b;
-    }
-    __syncthreads();
-    int total = 0;
-    for (int pass = 0; pass < 2; pass++) {
-        const int t = pass == 0 ? g.ini_th : g.min_th;
-        int written = 0;
-        for (int i0 = 0; i0 < W * H; i0 += 256) {
-            const int i = i0 + threadIdx.x;
-            int keep = 0;
-            uint32_t rec = 0;
-            if (i < W * H) {
-                const int y = i / W, x = i - y * W;
-                const int b = best[i];
-                if (b > t && x >= 3 && x < W - 3 && y >= 3 && y < H - 3) {
-                    const int s = b - 1;
-                    keep = 1;
-#pragma unroll
-                    for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-                        for (int dx = -1; dx <= 1; dx++) {
-                            if (dx == 0 && dy == 0) continue;
-                            const int bq = best[i + dy * W + dx];
-                            const int mq = bq > t ? bq - 1 : 0;
-                            keep &= (s > mq);
-                        }
-                    rec = ((uint32_t)s << 24) | ((uint32_t)(y + c.offy) << 12) | (uint32_t)(x + c.offx);
-                }
-            }
-            int chunk_total;
-            const int pos = block_excl_scan256(keep, ws, chunk_total);
-            if (keep) g.cell_cand[c.cap_base + written + pos] = rec;
-            written += chunk_total;
-        }
-        total = written;
-        if (total > 0) break;
-    }
-    if (threadIdx.x == 0) g.cell_n[blockIdx.x] = total;
 }
 
 // ---------------------------------------------------------------------------
@@ -329,20 +580,53 @@ __device__ void bitonic_sort(T* a, int n2) {
 
 constexpr int kOctThreads = 256;
 
+// Exclusive scan over the 256-thread block of every depth's count at once;
+// total[d] receives the block sum.  ws: 4 x (kOctreeDepth + 1) ints of LDS.
+__device__ __forceinline__ void block_excl_scan_depths(int (&v)[kOctreeDepth + 1], int (&total)[kOctreeDepth + 1],
+                                                       int (*ws)[kOctreeDepth + 1]) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x[kOctreeDepth + 1];
+#pragma unroll
+    for (int d = 0; d <= kOctreeDepth; d++) x[d] = v[d];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+        for (int d = 0; d <= kOctreeDepth; d++) {
+            const int y = __shfl_up(x[d], o, 64);
+            if (lane >= o) x[d] += y;
+        }
+    }
+    if (lane == 63) {
+#pragma unroll
+        for (int d = 0; d <= kOctreeDepth; d++) ws[wid][d] = x[d];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d <= kOctreeDepth; d++) {
+        int base = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            const int s = ws[w][d];
+            base += (w < wid) ? s : 0;
+            tot += s;
+        }
+        v[d] = base + x[d] - v[d];
+        total[d] = tot;
+    }
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g) {
     __shared__ unsigned long long s_keys[kSortCapLds];
     __shared__ uint8_t s_div[kSortCapLds];
     __shared__ int ws[4];
-    __shared__ int s_hist[kOctreeDepth + 2];
+    __shared__ int s_ws[4][kOctreeDepth + 1];
     __shared__ long long s_off;
-    __shared__ int s_n, s_maxdiv;
     const int li = blockIdx.x;
     const LevelDesc L = g.lv[li];
     OctLevel* O = g.oct + li;
     // 1. gather the level's candidates in cell order (cells are stored
     //    row-major, i then j, as the reference's nested loops visit them)
-    if (threadIdx.x == 0) s_n = 0;
-    __syncthreads();
     int run = 0;
     for (int c0 = 0; c0 < L.cell_count; c0 += kOctThreads) {
         const int ci = c0 + threadIdx.x;
@@ -409,86 +693,96 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g) {
     }
     __syncthreads();
     bitonic_sort(keys, n2);
-    // 3. divergence depth of each sorted position, histogram, deepest depth
-    if (threadIdx.x < kOctreeDepth + 2) s_hist[threadIdx.x] = 0;
-    if (threadIdx.x == 0) s_maxdiv = 0;
+    // 3. divergence depth of each sorted position
+    for (int i = threadIdx.x; i < n; i += kOctThreads)
+        dv[i] = (uint8_t)(i == 0 ? 0 : code_div(keys[i] >> 24, keys[i - 1] >> 24));
     __syncthreads();
-    int mymax = 0;
-    for (int i = threadIdx.x; i < n; i += kOctThreads) {
-        const int d = i == 0 ? 0 : code_div(keys[i] >> 24, keys[i - 1] >> 24);
-        dv[i] = (uint8_t)d;
-        atomicAdd(&s_hist[d], 1);
-        mymax = max(mymax, d);
+    // 4. node tables.  Position i starts a node at every depth d >= div[i];
+    //    its node index at depth d is rank_d(i) = #{j < i : div[j] <= d}.
+    //    Thread t owns the contiguous positions [i0, i1): per-depth counts,
+    //    one block scan of all depths, then the writes in position order.
+    const int C = (n + kOctThreads - 1) / kOctThreads;
+    const int i0 = min(n, (int)threadIdx.x * C), i1 = min(n, i0 + C);
+    int rank[kOctreeDepth + 1];
+#pragma unroll
+    for (int d = 0; d <= kOctreeDepth; d++) rank[d] = 0;
+    for (int i = i0; i < i1; i++) {
+        const int dvi = dv[i];
+#pragma unroll
+        for (int d = 0; d <= kOctreeDepth; d++) rank[d] += dvi <= d ? 1 : 0;
     }
-    atomicMax(&s_maxdiv, mymax);
-    __syncthreads();
-    const int D = s_maxdiv;
-    // K_d = #{i : div[i] <= d}; T = sum (K_d + 1)
     int K[kOctreeDepth + 1];
-    int T = 0;
-    {
-        int acc = 0;
-        for (int d = 0; d <= kOctreeDepth; d++) {
-            acc += s_hist[d];
-            K[d] = acc;
-        }
-        for (int d = 0; d <= D; d++) T += K[d] + 1;
+    block_excl_scan_depths(rank, K, s_ws);
+    // K[d] = nodes at depth d; the deepest depth D is the first with K == n
+    int D = kOctreeDepth, T = 0;
+    int off[kOctreeDepth + 1];
+#pragma unroll
+    for (int d = kOctreeDepth; d >= 0; d--)
+        if (K[d] == n) D = d;
+#pragma unroll
+    for (int d = 0; d <= kOctreeDepth; d++) {
+        off[d] = T;
+        if (d <= D) T += K[d] + 1;
     }
     if (threadIdx.x == 0) {
         const long long need = 2ll * T + n;
-        const unsigned long long off = atomicAdd(g.pool_used, (unsigned long long)need);
-        s_off = ((long long)off + need <= g.pool_cap) ? (long long)off : -1;
+        const unsigned long long o = atomicAdd(g.pool_used, (unsigned long long)need);
+        s_off = ((long long)o + need <= g.pool_cap) ? (long long)o : -1;
     }
     __syncthreads();
-    const long long off = s_off;
-    if (off < 0) {
+    const long long tab = s_off;
+    if (tab < 0) {
         if (threadIdx.x == 0) {
             O->n = n; O->failed = 1; O->n_roots = 0; O->depth = 0; O->total = 0; O->nout = 0;
             atomicOr(g.err, 1);
         }
         return;
     }
-    uint32_t* start = g.pool + off;
+    uint32_t* start = g.pool + tab;
     uint32_t* child = start + T;
     uint32_t* sorted = child + T;
     for (int i = threadIdx.x; i < n; i += kOctThreads) sorted[i] = (uint32_t)(keys[i] & 0xFFFFFF);
-    __syncthreads();
-    // 4. node tables, depth by depth.  rank[] (reusing the key storage) holds
-    //    each position's node index at the previous depth.
-    int* rank = reinterpret_cast<int*>(keys);
-    int doff = 0, prev_off = 0;
-    for (int d = 0; d <= D; d++) {
-        int run2 = 0;
-        for (int i0 = 0; i0 < n; i0 += kOctThreads) {
-            const int i = i0 + threadIdx.x;
-            const int dvi = i < n ? (int)dv[i] : 255;
-            const int f = dvi <= d ? 1 : 0;
-            int tot;
-            const int r = run2 + block_excl_scan256(f, ws, tot);
-            if (f) {
-                start[doff + r] = (uint32_t)i;
-                if (d > 0 && dvi <= d - 1) child[prev_off + rank[i]] = (uint32_t)(doff + r);
-                rank[i] = r;  // only read by the same thread, at the next depth
+    for (int i = i0; i < i1; i++) {
+        const int dvi = dv[i];
+#pragma unroll
+        for (int d = 0; d <= kOctreeDepth; d++) {
+            if (d <= D && dvi <= d) {
+                const int r = rank[d]++;
+                start[off[d] + r] = (uint32_t)i;
+                // the depth d-1 node starting at i was given rank[d-1] - 1 just before
+                if (d > 0 && dvi <= d - 1) child[off[d - 1] + rank[d - 1] - 1] = (uint32_t)(off[d] + r);
             }
-            run2 += tot;
         }
-        if (threadIdx.x == 0) {
-            start[doff + K[d]] = (uint32_t)n;
-            if (d > 0) child[prev_off + K[d - 1]] = (uint32_t)(doff + K[d]);
-        }
-        prev_off = doff;
-        doff += K[d] + 1;
     }
-    // deepest depth: no children
-    for (int k = threadIdx.x; k <= K[D]; k += kOctThreads) child[prev_off + k] = 0;
+    // sentinels: start[K_d] = n; child of the depth d-1 sentinel = depth d
+    // sentinel; the deepest depth has no children
+    if (threadIdx.x <= D) {
+        const int d = threadIdx.x;
+        int od = 0, Kd = 0, op = 0, Kp = 0;
+#pragma unroll
+        for (int e = 0; e <= kOctreeDepth; e++)
+            if (e == d) { od = off[e]; Kd = K[e]; }
+#pragma unroll
+        for (int e = 0; e <= kOctreeDepth; e++)
+            if (e == d - 1) { op = off[e]; Kp = K[e]; }
+        start[od + Kd] = (uint32_t)n;
+        if (d > 0) child[op + Kp] = (uint32_t)(od + Kd);
+    }
+    {
+        int oD = 0, KD = 0;
+#pragma unroll
+        for (int e = 0; e <= kOctreeDepth; e++)
+            if (e == D) { oD = off[e]; KD = K[e]; }
+        for (int k = threadIdx.x; k <= KD; k += kOctThreads) child[oD + k] = 0;
+    }
     if (threadIdx.x == 0) {
         O->n = n;
         O->failed = 0;
         O->n_roots = K[0];
         O->depth = D;
-        O->tab = off;
+        O->tab = tab;
         O->total = T;
-        O->sorted = off + 2ll * T;
+        O->sorted = tab + 2ll * T;
         O->nout = 0;
     }
 }
@@ -496,44 +790,99 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g) {
 // Replays DistributeOctTree's priority_queue (ORBextractor.cc:530-578) on
 // node counts: push the non-empty roots, split the largest node until the
 // queue holds N nodes or the top holds a single keypoint, then pop
-// everything.  One lane per level; the heap lives in LDS.
-constexpr int kHeapCap = 4096;
-__global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g) {
-    __shared__ unsigned long long heap[kHeapCap];
-    if (threadIdx.x != 0) return;
-    const int li = blockIdx.x;
+// everything.  Lane 0 runs the heap in LDS with entries {count, node, first
+// child, end child}; lanes 0..4 fetch the counts and child ranges of a
+// popped node's children.  The shallow part of the node tables (the first
+// `ncache` entries, depth-major, where the splits happen) is copied to LDS
+// first, so most fetches never leave the CU.  Workgroups take levels in
+// `order` (largest feature targets first).
+struct NodeTabs {
+    const uint32_t* start;
+    const uint32_t* child;
+    const uint32_t* s_start;
+    const uint32_t* s_child;
+    uint32_t ncache;
+    __device__ __forceinline__ uint32_t st(uint32_t i) const { return i < ncache ? s_start[i] : start[i]; }
+    __device__ __forceinline__ uint32_t ch(uint32_t i) const { return i < ncache ? s_child[i] : child[i]; }
+};
+
+__global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* order, int cap, int node_cap) {
+    extern __shared__ uint4 heap[];
+    uint32_t* s_start = reinterpret_cast<uint32_t*>(heap + cap);
+    uint32_t* s_child = s_start + node_cap;
+    const int lane = threadIdx.x;
+    const int li = order[blockIdx.x];
     const LevelDesc L = g.lv[li];
     OctLevel* O = g.oct + li;
     if (O->failed || O->n == 0) {
-        O->nout = 0;
+        if (lane == 0) O->nout = 0;
         return;
     }
-    const uint32_t* start = g.pool + O->tab;
-    const uint32_t* child = start + O->total;
-    int len = 0;
-    for (int k = 0; k < O->n_roots; k++) {
-        const uint32_t cnt = start[k + 1] - start[k];
-        heap_push(heap, len, ((unsigned long long)cnt << 32) | (unsigned)k);
+    NodeTabs nt;
+    nt.start = g.pool + O->tab;
+    nt.child = nt.start + O->total;
+    nt.s_start = s_start;
+    nt.s_child = s_child;
+    nt.ncache = (uint32_t)min(O->total, node_cap);
+    for (uint32_t i = lane; i < nt.ncache; i += 64) {
+        s_start[i] = nt.start[i];
+        s_child[i] = nt.child[i];
     }
-    const int N = min(L.nfeat, kHeapCap - 4);
+    wave_lds_sync();
+    const int nr = O->n_roots;
+    int len = 0;
+    {
+        uint32_t s_l = 0, c_l = 0;
+        if (lane <= nr) {
+            s_l = nt.st(lane);
+            c_l = nt.ch(lane);
+        }
+        for (int k = 0; k < nr; k++) {
+            const uint4 e = make_uint4(__shfl(s_l, k + 1) - __shfl(s_l, k), (uint32_t)k, __shfl(c_l, k),
+                                       __shfl(c_l, k + 1));
+            if (lane == 0) {
+                int l2 = len;
+                heap_push(heap, l2, e);
+            }
+            len++;
+        }
+    }
+    const int N = min(L.nfeat, cap - 8);
     while (len < N && len > 0) {
-        const unsigned long long top = heap[0];
-        if ((top >> 32) == 1) break;
-        heap_pop(heap, len);
-        const uint32_t gid = (uint32_t)top;
-        const uint32_t c0 = child[gid], c1 = child[gid + 1];
-        for (uint32_t c = c0; c < c1; c++) {
-            const uint32_t cnt = start[c + 1] - start[c];
-            heap_push(heap, len, ((unsigned long long)cnt << 32) | c);
+        wave_lds_sync();  // lane 0's heap writes before every lane reads the top
+        const uint4 top = heap[0];
+        if (top.x == 1) break;
+        const uint32_t c0 = top.z, nch = top.w - top.z;
+        uint32_t s_l = 0, c_l = 0;
+        if (lane <= (int)nch) {
+            s_l = nt.st(c0 + lane);
+            c_l = nt.ch(c0 + lane);
+        }
+        if (lane == 0) {
+            int l2 = len;
+            heap_pop(heap, l2);
+        }
+        len--;
+        for (uint32_t k = 0; k < nch; k++) {
+            const uint4 e = make_uint4(__shfl(s_l, k + 1) - __shfl(s_l, k), c0 + k, __shfl(c_l, k), __shfl(c_l, k + 1));
+            if (lane == 0) {
+                int l2 = len;
+                heap_push(heap, l2, e);
+            }
+            len++;
         }
     }
     int j = 0;
     while (len > 0) {
-        if (j < L.out_cap) g.nodes_out[L.kp_slot + j] = (uint32_t)heap[0];
+        if (lane == 0) {
+            if (j < L.out_cap) g.nodes_out[L.kp_slot + j] = heap[0].y;
+            int l2 = len;
+            heap_pop(heap, l2);
+        }
         j++;
-        heap_pop(heap, len);
+        len--;
     }
-    O->nout = min(j, L.out_cap);
+    if (lane == 0) O->nout = min(j, L.out_cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -583,40 +932,37 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (wave >= n_slots) return;
-    const int li = slot_lvl[2 * wave];
-    const int j = slot_lvl[2 * wave + 1];
+    const int2 sl = reinterpret_cast<const int2*>(slot_lvl)[wave];
+    const int li = sl.x, j = sl.y;
     const LevelDesc L = g.lv[li];
-    const OctLevel* O = g.oct + li;
-    const int nout = O->nout;
-    // image keypoint count, written once per image by slot 0 of level 0
-    if (j == 0 && L.level == 0 && lane == 0) {
-        int tot = 0;
-        for (int l = 0; l < L.nlevels; l++) {
-            const int lj = L.first_level + l;
-            tot += g.oct[lj].nout + g.lv[lj].seed_count;
-        }
-        out_count[L.img] = tot;
+    // keypoints per level of this image (octree output + seeds), one lane per level
+    int cnt_l = 0;
+    if (lane < L.nlevels) cnt_l = g.oct[L.first_level + lane].nout + g.lv[L.first_level + lane].seed_count;
+    const int nout = __shfl(cnt_l, L.level) - L.seed_count;
+    if (j == 0 && L.level == 0) {
+        const int tot = wave_sum(cnt_l);
+        if (lane == 0) out_count[L.img] = tot;
     }
+    const int off = wave_sum(lane < L.level ? cnt_l : 0);
     if (j >= nout + L.seed_count) return;
-    int off = 0;
-    for (int l = 0; l < L.level; l++) {
-        const int lj = L.first_level + l;
-        off += g.oct[lj].nout + g.lv[lj].seed_count;
-    }
     const int pos = off + j;
     if (pos >= cap) return;
     float x, y, resp, angle, size;
-    bool seed = j >= nout;
+    const bool seed = j >= nout;
     if (!seed) {
+        const OctLevel* O = g.oct + li;
         const uint32_t gid = g.nodes_out[L.kp_slot + j];
         const uint32_t* start = g.pool + O->tab;
         const uint32_t* sorted = g.pool + O->sorted;
         const uint32_t s0 = start[gid], s1 = start[gid + 1];
+        // key: response, then the lowest candidate index, then the record's
+        // packed coordinates (the winner's position travels with the max)
         unsigned long long bestk = 0;
         for (uint32_t p = s0 + lane; p < s1; p += 64) {
             const uint32_t idx = sorted[p];
             const uint32_t r = g.lvl_cand[L.cand_base + idx];
-            const unsigned long long k = ((unsigned long long)(r >> 24) << 32) | (0xFFFFFFu - idx);
+            const unsigned long long k = ((unsigned long long)(r >> 24) << 56) |
+                                         ((unsigned long long)(0xFFFFFFu - idx) << 24) | (r & 0xFFFFFFu);
             bestk = k > bestk ? k : bestk;
         }
 #pragma unroll
@@ -624,22 +970,23 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
             const unsigned long long v = __shfl_xor(bestk, o, 64);
             bestk = v > bestk ? v : bestk;
         }
-        const uint32_t idx = 0xFFFFFFu - (uint32_t)(bestk & 0xFFFFFFFFu);
-        const uint32_t r = g.lvl_cand[L.cand_base + idx];
+        const uint32_t r = (uint32_t)(bestk & 0xFFFFFFu);
         x = (float)((int)(r & 0xFFF) + L.minBX);
         y = (float)((int)((r >> 12) & 0xFFF) + L.minBY);
-        resp = (float)(r >> 24);
+        resp = (float)(uint32_t)(bestk >> 56);
         size = L.patch;
-        // IC_Angle: lanes 0..30 own rows v = lane - 15
-        const uint8_t* img = g.pyr + L.pix;
-        const int cx = (int)rintf(x), cy = (int)rintf(y);
+        // IC_Angle over the circular patch: lanes 0..30 take u = -15..15 of
+        // rows v = -15..0, lanes 32..62 the same columns of rows 1..15
+        const uint8_t* ctr = g.pyr + L.pix + (long long)(int)rintf(y) * L.pitch + (int)rintf(x);
+        const int u = (lane & 31) - 15;
+        const bool hi_half = lane >= 32;
         int m10 = 0, m01 = 0;
-        if (lane < 31) {
-            const int v = lane - 15;
-            const int du = g.umax[v < 0 ? -v : v];
-            const uint8_t* row = img + (long long)(cy + v) * L.w + cx;
-            for (int u = -du; u <= du; u++) {
-                const int val = row[u];
+#pragma unroll
+        for (int it = 0; it < 16; it++) {
+            const int v = hi_half ? it + 1 : it - 15;
+            const int du = hi_half ? (it < 15 ? g.umax[it + 1] : -1) : g.umax[15 - it];
+            if ((lane & 31) < 31 && u >= -du && u <= du) {
+                const int val = ctr[(long long)v * L.pitch + u];
                 m10 += u * val;
                 m01 += v * val;
             }
@@ -673,7 +1020,7 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
             int xx = cx + (int)rintf(px * a - py * b);
             yy = min(max(yy, 0), L.h - 1);
             xx = min(max(xx, 0), L.w - 1);
-            t[q] = bl[(long long)yy * L.w + xx];
+            t[q] = bl[(long long)yy * L.pitch + xx];
         }
         words[r] = __ballot(t[0] < t[1]);
     }
@@ -683,8 +1030,6 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
         dst[1] = words[1];
         dst[2] = words[2];
         dst[3] = words[3];
-    }
-    if (lane == 0) {
         if (L.level != 0) {
             x = x * L.scale;
             y = y * L.scale;

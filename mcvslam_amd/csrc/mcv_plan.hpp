@@ -9,13 +9,17 @@ namespace mcv {
 
 constexpr int kMaxLevels = 16;
 constexpr int kOctreeDepth = 15;        // quadtree depths encoded per candidate
-constexpr int kSortCapLds = 16384;      // candidates per level sorted in LDS
+constexpr int kSortCapLds = 4096;       // candidates per level sorted in LDS (more: pool memory)
 constexpr int kMaxWin = 96;             // max FAST cell window side (pixels)
+constexpr int kStripW = 256;            // strip kernels: one wave per 256 x 32 pixels,
+constexpr int kStripH = 32;             //   4 adjacent pixels per lane
+constexpr int kPitchAlign = 64;         // level rows start on 64-byte boundaries
 
 struct LevelDesc {
     int img;            // image index in the batch
     int level;
-    int w, h;           // level size (pitch == w)
+    int w, h;           // level size
+    int pitch;          // row stride in bytes: w rounded up to kPitchAlign
     int src;            // LevelDesc index of the previous level (-1 for level 0)
     int nfeat;          // octree target for this level (mnFeaturesPerLevel)
     long long pix;      // byte offset of the level in the pyramid / blur buffers

@@ -105,18 +105,18 @@ __global__ __launch_bounds__(256) void k_stereo(const StereoJob* jobs, const Lev
     const uint8_t* IL = pyr + DL.pix;
     const uint8_t* IR = pyr + DR.pix;
     const int y0 = (int)scaledvL - w, xl = (int)scaleduL - w;
-    const int cl = IL[(long long)(y0 + w) * DL.w + xl + w];
+    const int cl = IL[(long long)(y0 + w) * DL.pitch + xl + w];
     int bestDist = 0x7FFFFFFF, bestincR = 0;
     float vd[2 * 5 + 1];
 #pragma unroll
     for (int incR = -L; incR <= L; incR++) {
         const int xr = (int)scaleduR0 + incR - w;
-        const int cr = IR[(long long)(y0 + w) * DR.w + xr + w];
+        const int cr = IR[(long long)(y0 + w) * DR.pitch + xr + w];
         int acc = 0;
         for (int p = lane; p < 121; p += 64) {
             const int yy = p / 11, xx = p - 11 * yy;
-            const int a = (int)IL[(long long)(y0 + yy) * DL.w + xl + xx] - cl;
-            const int b = (int)IR[(long long)(y0 + yy) * DR.w + xr + xx] - cr;
+            const int a = (int)IL[(long long)(y0 + yy) * DL.pitch + xl + xx] - cl;
+            const int b = (int)IR[(long long)(y0 + yy) * DR.pitch + xr + xx] - cr;
             acc += abs(a - b);
         }
         acc = wave_sum(acc);

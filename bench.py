@@ -92,7 +92,9 @@ def cpu_baseline(frames_np, nfeat, budget_s=12.0):
     import oracle
     done = 0
     t0 = time.perf_counter()
-    for i, f in enumerate(frames_np):
+    for k in range(1 << 20):
+        i = k % len(frames_np)
+        f = frames_np[i]
         res = [None] * 3
 
         def work(k):
@@ -112,6 +114,21 @@ def cpu_baseline(frames_np, nfeat, budget_s=12.0):
     return done / dt, done, dt
 
 
+def ba_cpu_baseline(prob, n_obs=1500000):
+    """FP64 oracle linearization of the first n_obs observations, one thread
+    (g2o's per-edge loop with OpenMP off)."""
+    import copy
+    import oracle
+    sub = copy.copy(prob)
+    for f in ("obs_uv", "obs_pt", "obs_frame", "obs_cam", "obs_isig"):
+        setattr(sub, f, getattr(prob, f)[:n_obs])
+    t0 = time.perf_counter()
+    oracle.ba_linearize(sub)
+    dt = time.perf_counter() - t0
+    n = len(sub.obs_pt)
+    return n / dt, n, dt
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +138,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ba", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-frame-at-a-time comparison run")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch variants")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path would shard by rig "
@@ -159,6 +179,11 @@ def main():
     nfeat = np.concatenate([np.random.Generator(np.random.PCG64(40 + f)).integers(500, 4001, 3)
                             for f in range(n)]).astype(np.int32)
     rig = RigBatch(orb, n, ratio=0.7, th=50, nfeatures=nfeat)
+    tuning = ""
+    if not args.no_autotune:
+        log("autotune ...")
+        tuning = rig.autotune(ptrs, reps=3)
+        log(tuning.strip())
     log("warmup (graph) ...")
     for _ in range(args.warmup):
         rig.run(ptrs, graph=True)
@@ -218,6 +243,7 @@ def main():
         "plain_frames_per_s": round(plain_fps, 2) if plain_fps else None,
         "batched_over_plain": round(fps / plain_fps, 2) if plain_fps else None,
         "kernels_ms_per_step": {k: round(v, 4) for k, v in kern.items()},
+        "autotune": [ln for ln in tuning.strip().splitlines()],
         "keypoints_per_step": int(counts.sum()),
         "matches_per_step": int(mcounts.sum()),
     }
@@ -251,17 +277,36 @@ def main():
         e1.record()
         torch.cuda.synchronize()
         ba_ms = e0.elapsed_time(e1) / iters
+        # plain: one launch per (keyframe, camera) edge group
+        ba.linearize_plain()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(iters):
+            ba.linearize_plain()
+        e1.record()
+        torch.cuda.synchronize()
+        plain_ms = e0.elapsed_time(e1) / iters
         M = len(prob.obs_pt)
         P = len(prob.points)
         ba_bytes = 20 * M + 88 * M + 32 * P
         ach = ba_bytes / (ba_ms / 1e3) / 1e9
         result["ba"] = {"workload": "50 keyframes x 3 cams, 200k points, EdgeSE3XYZ2Camera linearize",
                         "observations": M, "obs_per_s": round(M / (ba_ms / 1e3), 1), "ms_per_launch": round(ba_ms, 5),
+                        "plain_obs_per_s": round(M / (plain_ms / 1e3), 1), "plain_ms": round(plain_ms, 5),
+                        "plain_launches": int(len(prob.frames) * len(prob.cams)),
+                        "batched_over_plain": round(plain_ms / ba_ms, 2),
                         "roofline": {"kernel": "k_ba_linearize", "bound": "hbm", "achieved": round(ach, 2),
                                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                                      "bytes_per_obs": round(ba_bytes / M, 2)}}
-        log(f"BA linearize: {ba_ms * 1e3:.1f} us per {M} obs -> {M / (ba_ms / 1e3):.3e} obs/s, {ach:.0f} GB/s")
+        log(f"BA linearize: {ba_ms * 1e3:.1f} us per {M} obs -> {M / (ba_ms / 1e3):.3e} obs/s, {ach:.0f} GB/s; "
+            f"plain {plain_ms * 1e3:.1f} us")
         ba.close()
+        if not args.no_cpu:
+            cps, n_cpu, dt = ba_cpu_baseline(prob)
+            result["ba"]["cpu_baseline"] = {"value": round(cps, 1), "unit": "obs/s", "cores": 1, "kind": "port",
+                                            "sample": f"first {n_cpu} observations of the same problem, FP64 "
+                                                      f"oracle edge linearization, {dt:.2f}s"}
+            log(f"BA CPU: {cps:.3e} obs/s")
 
     if not args.no_cpu:
         log("CPU oracle baseline ...")
@@ -272,6 +317,14 @@ def main():
                                             f"in {dt:.1f}s on {os.cpu_count()} host cpus"}
         log(f"CPU: {cfps:.2f} rig frames/s")
     rig.close()
+    if args.calib_copy_mb > 0:
+        nb = args.calib_copy_mb << 20
+        a = torch.empty(nb, dtype=torch.uint8, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        result["calib_copy_bytes"] = nb
     print(json.dumps(result), flush=True)
 
 

@@ -89,6 +89,10 @@ int mcv_orb_extract_batch_dev(mcv_ctx* ctx, int n, const uint8_t* const* d_image
  * per query row (trainIdx0, distance0, trainIdx1, distance1), sentinel
  * (0, 999) when fewer than two trains exist.  q: nq x 32, t: nt x 32 (host). */
 int mcv_match_knn2(mcv_ctx* ctx, const uint8_t* q, int nq, const uint8_t* t, int nt, int32_t* out);
+/* Launch variant of the top-2 kernel used by mcv_match_knn2* on this context
+ * (0 .. 4; no counterpart in the reference: the autotuner's choice, also
+ * settable for testing).  Every variant returns the same rows. */
+int mcv_match_set_variant(mcv_ctx* ctx, int variant);
 int mcv_match_knn2_dev(mcv_ctx* ctx, const uint8_t* d_q, int nq, const uint8_t* d_t, int nt, int32_t* d_out);
 /* Replaces MatchResKnn::FilterRatio(ratio).FilterThreshold(th)
  * (src/Matcher.cpp:100-111, :23-35): rows (queryIdx, trainIdx, distance) in
@@ -105,6 +109,13 @@ int mcv_match_filter(mcv_ctx* ctx, const int32_t* knn, int nq, float ratio, int 
 int mcv_rig_create(mcv_ctx* ctx, int n_frames, const int32_t* widths3, const int32_t* heights3,
                    const int32_t* nfeatures, float ratio, int th, mcv_rig** rig);
 void mcv_rig_destroy(mcv_rig* rig);
+/* Autotuner (no counterpart in the reference): times every launch variant
+ * of the tunable kernels of the batched sequence on these device images
+ * (median of `reps` runs), keeps the fastest for this rig and for rigs of the
+ * same shape created later on this context, and re-captures the graph on the
+ * next mcv_rig_run.  All variants produce identical outputs.  `report`
+ * (optional) receives one line per timed variant and the choice. */
+int mcv_rig_autotune(mcv_rig* rig, const uint8_t* const* d_images, int reps, char* report, int report_cap);
 /* d_images: host array of 3 * n_frames device pointers (frame-major L, R, W),
  * each image dense (stride = width).  flags: MCV_FLAG_GRAPH replays the whole
  * batch as one hipGraph; without it every image and every pair is launched on
@@ -174,6 +185,29 @@ int mcv_ba_set_state(mcv_ba* ba, const double* points, const double* frames, con
  * Huber weight, J_pose (2x6 row-major), J_point (2x3 row-major).
  * residual_only = 1 computes the first four planes only. */
 int mcv_ba_linearize(mcv_ba* ba, int residual_only);
+/* Same outputs with the plain launch pattern: one launch per (frame, camera)
+ * edge group, the granularity at which g2o's per-edge loop
+ * (core/block_solver.hpp:488-494) would be replaced group by group.  For the
+ * batched-vs-plain comparison. */
+int mcv_ba_linearize_plain(mcv_ba* ba, int residual_only);
+/* Stereo edges, EdgeStereoSE3ProjectXYZ (include/OptimizeTypes.hpp:119-157,
+ * src/OptimizeTypes.cpp:12-58; added by BAoptimizer::addEdgeStereo,
+ * src/BAoptimizer.cpp:67-87): obs Ms x 3 f32 (u, v, u_right), point and frame
+ * indices, inverse level sigma^2; cam = {fx, fy, cx, cy, bf, Huber delta}.
+ * The pose vertex is the frame state of mcv_ba_set_state (left camera = body). */
+int mcv_ba_set_stereo(mcv_ba* ba, int Ms, const float* obs, const int32_t* point, const int32_t* frame,
+                      const float* inv_sigma2, const double* cam);
+/* Linearize the stereo edges into 32 device planes of Ms floats: e_u, e_v,
+ * e_ur, chi2, Huber weight, J_pose (3x6 row-major), J_point (3x3).
+ * residual_only = 1 computes the first five planes only. */
+int mcv_ba_linearize_stereo(mcv_ba* ba, int residual_only);
+/* Copy stereo planes [first, first + count) to host (count x Ms floats). */
+int mcv_ba_read_stereo_planes(mcv_ba* ba, int first, int count, float* host);
+/* Outlier test of PoseEstimation::FilterCallBack_Chi2 (src/PoseEstimation.cpp:
+ * 132-151) on the last linearization: flag = chi2 >= threshold (the reference
+ * uses 5.991 for edges of the left camera object, 7.815 for the right).
+ * Host outputs of M (mono) and Ms (stereo) bytes; either may be null. */
+int mcv_ba_outliers(mcv_ba* ba, float thr_mono, float thr_stereo, uint8_t* flags_mono, uint8_t* flags_stereo);
 /* Device pointer of the planes (plane k at d_planes + k * stride). */
 int mcv_ba_planes(mcv_ba* ba, float** d_planes, int64_t* stride);
 /* Copy planes [first, first + count) to host (count x M floats). */

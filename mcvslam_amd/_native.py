@@ -46,6 +46,8 @@ SIGNATURES = {
     "mcv_orb_pyramid": (I, [P, I, P, I, P, P]),
     "mcv_orb_extract_batch_dev": (I, [P, I, P, P, P, P, P, P, I, P, I]),
     "mcv_match_knn2": (I, [P, P, I, P, I, P]),
+    "mcv_match_set_variant": (I, [P, I]),
+    "mcv_rig_autotune": (I, [P, P, I, P, I]),
     "mcv_match_knn2_dev": (I, [P, P, I, P, I, P]),
     "mcv_match_filter": (I, [P, P, I, F, I, P, P]),
     "mcv_rig_create": (I, [P, I, P, P, P, F, I, P]),
@@ -63,6 +65,11 @@ SIGNATURES = {
     "mcv_ba_set_observations": (I, [P, P, P, P, P, P]),
     "mcv_ba_set_state": (I, [P, P, P, P]),
     "mcv_ba_linearize": (I, [P, I]),
+    "mcv_ba_linearize_plain": (I, [P, I]),
+    "mcv_ba_set_stereo": (I, [P, I, P, P, P, P, P]),
+    "mcv_ba_linearize_stereo": (I, [P, I]),
+    "mcv_ba_read_stereo_planes": (I, [P, I, I, P]),
+    "mcv_ba_outliers": (I, [P, F, F, P, P]),
     "mcv_ba_planes": (I, [P, P, P]),
     "mcv_ba_read_planes": (I, [P, I, I, P]),
     "mcv_ba_copy_planes_dev": (I, [P, I, I, P]),

@@ -66,6 +66,11 @@ class Context:
     def synchronize(self):
         check(self.h, lib().mcv_synchronize(self.h))
 
+    def set_match_variant(self, variant: int):
+        """Launch variant of the top-2 kernel for Matcher calls (0..4; all
+        return the same rows)."""
+        check(self.h, lib().mcv_match_set_variant(self.h, int(variant)))
+
     def to_host(self, d_ptr: int, dtype, shape) -> np.ndarray:
         out = np.zeros(shape, dtype)
         check(self.h, lib().mcv_copy_to_host(self.h, _p(out), ctypes.c_void_p(d_ptr), out.nbytes))
@@ -287,6 +292,15 @@ class RigBatch:
         arr = (ctypes.c_void_p * n)(*image_ptrs)
         check(self.ctx.h, lib().mcv_rig_run(self.h, arr, _native.MCV_FLAG_GRAPH if graph else 0))
 
+    def autotune(self, image_ptrs, reps: int = 3) -> str:
+        """Time every launch variant of the tunable kernels on these images
+        and keep the fastest (outputs are identical); returns the report."""
+        n = 3 * self.n_frames
+        arr = (ctypes.c_void_p * n)(*image_ptrs)
+        buf = ctypes.create_string_buffer(4096)
+        check(self.ctx.h, lib().mcv_rig_autotune(self.h, arr, reps, buf, 4096))
+        return buf.value.decode()
+
     def run_timed(self, image_ptrs):
         """Eager run with an event after every launch: [(kernel, ms), ...]."""
         n = 3 * self.n_frames
@@ -366,6 +380,36 @@ class BALinearizer:
 
     def linearize(self, residual_only: bool = False):
         check(self.ctx.h, lib().mcv_ba_linearize(self.h, 1 if residual_only else 0))
+
+    def set_stereo(self, obs, point, frame, inv_sigma2, cam):
+        """Stereo edges (EdgeStereoSE3ProjectXYZ): obs Ms x 3 (u, v, u_right),
+        cam = (fx, fy, cx, cy, bf, delta)."""
+        a = [np.ascontiguousarray(obs, np.float32), np.ascontiguousarray(point, np.int32),
+             np.ascontiguousarray(frame, np.int32), np.ascontiguousarray(inv_sigma2, np.float32)]
+        c = np.ascontiguousarray(cam, np.float64)
+        self.Ms = len(a[1])
+        check(self.ctx.h, lib().mcv_ba_set_stereo(self.h, self.Ms, *[_p(x) for x in a], _p(c)))
+
+    def linearize_stereo(self, residual_only: bool = False):
+        check(self.ctx.h, lib().mcv_ba_linearize_stereo(self.h, 1 if residual_only else 0))
+
+    def stereo_results(self):
+        """(e Ms x 3, chi2, w, J_pose Ms x 3 x 6, J_point Ms x 3 x 3)"""
+        pl = np.zeros((32, self.Ms), np.float32)
+        check(self.ctx.h, lib().mcv_ba_read_stereo_planes(self.h, 0, 32, _p(pl)))
+        return (pl[0:3].T, pl[3], pl[4], pl[5:23].T.reshape(-1, 3, 6), pl[23:32].T.reshape(-1, 3, 3))
+
+    def outliers(self, thr_mono: float = 5.991, thr_stereo: float = 7.815, stereo: bool = False):
+        """chi2 >= threshold flags of the last linearization (mono, stereo)."""
+        fm = np.zeros(self.M, np.uint8)
+        fs = np.zeros(getattr(self, "Ms", 0), np.uint8) if stereo else None
+        check(self.ctx.h, lib().mcv_ba_outliers(self.h, ctypes.c_float(thr_mono), ctypes.c_float(thr_stereo),
+                                                _p(fm), _p(fs) if stereo else None))
+        return fm, fs
+
+    def linearize_plain(self, residual_only: bool = False):
+        """One launch per (frame, camera) edge group; same outputs."""
+        check(self.ctx.h, lib().mcv_ba_linearize_plain(self.h, 1 if residual_only else 0))
 
     def planes_ptr(self):
         p, s = ctypes.c_void_p(), ctypes.c_int64()

@@ -362,6 +362,13 @@ static int level_roots(int w, int h) {
     return (int)std::round(width / height);
 }
 
+// Launch choices that leave every output bit unchanged; picked by the
+// autotuner (mcv_rig_autotune) from timings on the real workload.
+struct Tuning {
+    int knn = 0;   // launch_knn2 variant
+    int blur = 0;  // 0: k_blur<false>, 1: k_blur<true>
+};
+
 struct ImageSpec {
     int w, h;
     std::vector<std::vector<mcv_keypoint>> seeds;  // per level, in input order
@@ -375,6 +382,7 @@ struct OrbPlan {
     std::vector<CellDesc> cells;
     std::vector<Strip> strips;
     std::vector<int> strip_begin;  // per level index l: [strip_begin[l], strip_begin[l+1])
+    std::vector<int> resize_rows, resize_rw;  // per level: staged source rows and bytes per row
     std::vector<int2> coef;
     std::vector<int> slot_lvl;
     std::vector<float> seeds;
@@ -387,6 +395,7 @@ struct OrbPlan {
         d_pool, d_used, d_nodes, d_src, d_stride;
     hipGraphExec_t graph = nullptr;
     const void* graph_key = nullptr;
+    Tuning tune;
 
     ~OrbPlan() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -494,7 +503,7 @@ struct OrbPlan {
         cap = max_slots_img;
         heap_cap = 16;
         for (const LevelDesc& d : lv) heap_cap = std::max(heap_cap, std::max(d.nfeat, d.n_ini) + 16);
-        node_cap = std::max(0, std::min(4096, (65536 - 16 * heap_cap) / 8));
+        node_cap = std::max(0, std::min(4096, (65536 - 8 * heap_cap) / 8));
         heap_order.resize(lv.size());
         for (size_t k = 0; k < lv.size(); k++) heap_order[k] = (int)k;
         std::stable_sort(heap_order.begin(), heap_order.end(),
@@ -510,6 +519,27 @@ struct OrbPlan {
             }
         }
         strip_begin[nlevels] = (int)strips.size();
+        // k_resize stages each strip's source rows and column window in LDS
+        resize_rows.assign(nlevels, 0);
+        resize_rw.assign(nlevels, 4);
+        for (int l = 1; l < nlevels; l++)
+            for (int k = strip_begin[l]; k < strip_begin[l + 1]; k++) {
+                const Strip& sp = strips[k];
+                const LevelDesc& d = lv[sp.lvl];
+                const LevelDesc& sl = lv[d.src];
+                const int yend = std::min(sp.y0 + kStripH, d.h);
+                const int xl = std::min(sp.x0 + kStripW - 1, d.w - 1);
+                const int cb = coef[d.coef_x + sp.x0].x & ~3;
+                const int c_hi = std::min(coef[d.coef_x + xl].x + 1, sl.w - 1);
+                const int r_lo = std::min(std::max(coef[d.coef_y + sp.y0].x, 0), sl.h - 1);
+                const int r_hi = std::min(std::max(coef[d.coef_y + yend - 1].x + 1, 0), sl.h - 1);
+                resize_rows[l] = std::max(resize_rows[l], r_hi - r_lo + 1);
+                resize_rw[l] = std::max(resize_rw[l], ((c_hi - cb) / 4 + 1) * 4);
+            }
+        for (int l = 1; l < nlevels; l++) {
+            REQUIRE(resize_rw[l] <= 4 * 3 * 64, MCV_ERR_INVALID, "scale factor too large for the resize window");
+            REQUIRE(resize_rows[l] * resize_rw[l] <= 65536, MCV_ERR_INVALID, "resize window exceeds LDS");
+        }
         pool_cap = pool_words;
     }
 
@@ -611,8 +641,8 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
     for (int l = 1; l < nlevels; l++) {
         const int n = strip_begin[l + 1] - strip_begin[l];
         if (n > 0) {
-            hipLaunchKernelGGL(k_resize, strip_grid(n), dim3(256), 0, s, st + strip_begin[l], n, g.lv,
-                               d_coef.as<int2>(), g.pyr);
+            hipLaunchKernelGGL(k_resize, dim3(n), dim3(64), resize_rows[l] * resize_rw[l], s, st + strip_begin[l], n,
+                               g.lv, d_coef.as<int2>(), g.pyr, resize_rw[l]);
             mark("k_resize");
         }
     }
@@ -622,7 +652,10 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
     taps.hi3 = (uint32_t)c.gauss[4] | (uint32_t)c.gauss[5] << 8 | (uint32_t)c.gauss[6] << 16;
     const int ns = (int)strips.size();
     if (ns > 0) {
-        hipLaunchKernelGGL(k_blur, strip_grid(ns), dim3(256), 0, s, st, ns, g.lv, g.pyr, g.blur, taps);
+        if (tune.blur)
+            hipLaunchKernelGGL(k_blur<true>, strip_grid(ns), dim3(256), 0, s, st, ns, g.lv, g.pyr, g.blur, taps);
+        else
+            hipLaunchKernelGGL(k_blur<false>, strip_grid(ns), dim3(256), 0, s, st, ns, g.lv, g.pyr, g.blur, taps);
         mark("k_blur");
         hipLaunchKernelGGL(k_fast_score, strip_grid(ns), dim3(256), 0, s, st, ns, g.lv, g.pyr, g.score,
                            std::min(g.ini_th, g.min_th));
@@ -635,7 +668,7 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
     }
     hipLaunchKernelGGL(k_octree_build, dim3(lv.size()), dim3(kOctThreads), 0, s, g);
     mark("k_octree_build");
-    hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), sizeof(uint4) * heap_cap + 8 * node_cap, s, g,
+    hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), sizeof(uint2) * heap_cap + 8 * node_cap, s, g,
                        d_heap_order.as<int>(), heap_cap, node_cap);
     mark("k_octree_heap");
     hipLaunchKernelGGL(k_keypoints, dim3((n_slots + 3) / 4), dim3(256), 0, s, g, d_slot.as<int>(), n_slots, tb,
@@ -651,6 +684,8 @@ struct mcv_ctx {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t cap = nullptr;  // graph capture only (the null stream cannot capture)
+    int knn_variant = 0;        // for mcv_match_knn2*: the last rig tuning's choice
+    std::map<std::string, Tuning> tuned;  // rig shape -> autotuned choices
     hipStream_t stream = nullptr;
     std::string err;
     OrbConfig orb;
@@ -703,6 +738,8 @@ struct mcv_rig {
     std::vector<const uint8_t*> graph_inputs;
     KernelTimer timer;
     bool last_batched = false;
+    Tuning tune;
+    std::string shape_key;
     DevBuf st_u, st_d, st_sad, st_keys, st_jobs;
     ~mcv_rig() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -712,7 +749,13 @@ struct mcv_rig {
 struct mcv_ba {
     mcv_ctx* ctx;
     int M, P, F, C;
-    DevBuf uv, pt, pair, isig, phi, plo, pairs, planes, fptr, fobs, pptr;
+    DevBuf uv, pt, pair, isig, phi, plo, pairs, planes, fptr, fobs, pptr, gobs;
+    // stereo edges (EdgeStereoSE3ProjectXYZ)
+    int Ms = 0;
+    StereoCam scam{};
+    bool have_stereo = false, stereo_linearized = false;
+    DevBuf s_obs, s_pt, s_fr, s_isig, s_planes, fposes, flags;
+    std::vector<int> h_gptr;  // observations of (frame, camera) group g: gobs[h_gptr[g] .. h_gptr[g + 1])
     std::vector<int> h_pt, h_fr, h_pptr;
     std::vector<int> fixed;
     bool have_obs = false, have_state = false, linearized = false;
@@ -968,10 +1011,10 @@ int mcv_orb_extract_batch_dev(mcv_ctx* c, int n, const uint8_t* const* d_images,
 }
 
 // ---- matching ---------------------------------------------------------------
-static void launch_knn(hipStream_t s, DevBuf& pbuf, const std::vector<MatchPair>& pairs, int max_nq) {
+static void launch_knn(hipStream_t s, DevBuf& pbuf, const std::vector<MatchPair>& pairs, int max_nq, int variant) {
     if (pairs.empty() || max_nq <= 0) return;
     upload(pbuf, pairs, s);
-    hipLaunchKernelGGL(k_knn2, dim3((max_nq + 63) / 64, pairs.size()), dim3(256), 0, s, pbuf.as<MatchPair>());
+    launch_knn2(variant, pbuf.as<MatchPair>(), (int)pairs.size(), max_nq, s);
     HIP_OK(hipGetLastError());
 }
 
@@ -980,7 +1023,14 @@ int mcv_match_knn2_dev(mcv_ctx* c, const uint8_t* d_q, int nq, const uint8_t* d_
     REQUIRE(c && nq >= 0 && nt >= 0 && nt < (1 << 22), MCV_ERR_INVALID, "bad sizes");
     c->set_device();
     MatchPair p{d_q, d_t, nullptr, nullptr, nq, nt, d_out};
-    launch_knn(c->stream, c->m_pairs, {p}, nq);
+    launch_knn(c->stream, c->m_pairs, {p}, nq, c->knn_variant);
+    API_END
+}
+
+int mcv_match_set_variant(mcv_ctx* c, int variant) {
+    API_BEGIN(c)
+    REQUIRE(c && variant >= 0 && variant < kKnnVariants, MCV_ERR_INVALID, "unknown matching variant");
+    c->knn_variant = variant;
     API_END
 }
 
@@ -996,7 +1046,7 @@ int mcv_match_knn2(mcv_ctx* c, const uint8_t* q, int nq, const uint8_t* t, int n
     HIP_OK(hipMemcpyAsync(c->m_q.p, q, (size_t)nq * 32, hipMemcpyHostToDevice, c->stream));
     if (nt) HIP_OK(hipMemcpyAsync(c->m_t.p, t, (size_t)nt * 32, hipMemcpyHostToDevice, c->stream));
     MatchPair p{c->m_q.as<uint8_t>(), c->m_t.as<uint8_t>(), nullptr, nullptr, nq, nt, c->m_out.as<int32_t>()};
-    launch_knn(c->stream, c->m_pairs, {p}, nq);
+    launch_knn(c->stream, c->m_pairs, {p}, nq, c->knn_variant);
     HIP_OK(hipMemcpyAsync(out, c->m_out.p, (size_t)nq * 16, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     API_END
@@ -1062,6 +1112,16 @@ int mcv_rig_create(mcv_ctx* c, int n_frames, const int32_t* w3, const int32_t* h
         r->single[k]->upload_all(c->stream);
     }
     r->cap = r->batch->cap;
+    {
+        std::string key = std::to_string(n_frames);
+        for (int k = 0; k < 3; k++) key += "," + std::to_string(w3[k]) + "x" + std::to_string(h3[k]);
+        if (nfeatures)
+            for (int i = 0; i < 3 * n_frames; i++) key += ":" + std::to_string(nfeatures[i]);
+        r->shape_key = key;
+        auto it = c->tuned.find(key);
+        if (it != c->tuned.end()) r->tune = it->second;
+        r->batch->tune = r->tune;
+    }
     const int n = 3 * n_frames;
     r->kps.ensure((size_t)n * r->cap * sizeof(mcv_keypoint));
     r->desc.ensure((size_t)n * r->cap * 32);
@@ -1108,7 +1168,7 @@ static void rig_enqueue_batched(mcv_rig* r, const SinCosTables& tb, hipStream_t 
     OrbPlan& P = *r->batch;
     P.enqueue(r->ctx->orb, tb, s, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>(), rec);
     const int np = (int)r->hp.size();
-    hipLaunchKernelGGL(k_knn2, dim3((r->cap + 63) / 64, np), dim3(256), 0, s, r->pairs.as<MatchPair>());
+    launch_knn2(r->tune.knn, r->pairs.as<MatchPair>(), np, r->cap, s);
     if (rec) rec->mark("k_knn2");
     hipLaunchKernelGGL(k_filter, dim3(np), dim3(1024), 0, s, r->jobs.as<FilterJob>());
     if (rec) rec->mark("k_filter");
@@ -1140,6 +1200,87 @@ int mcv_rig_run_timed(mcv_rig* r, const uint8_t* const* d_images, float* ms, con
         HIP_OK(hipEventElapsedTime(&t, r->timer.ev[i], r->timer.ev[i + 1]));
         ms[i] = t;
         if (names) names[i] = r->timer.names[i];
+    }
+    API_END
+}
+
+// Autotuner: each tunable launch of the batched sequence is timed in every
+// variant on the caller's images (median of `reps` eager runs, events
+// around each launch), the fastest is kept for this rig and for later rigs
+// of the same shape in this context, and the graph is re-captured on the
+// next run.  Variants compute identical bytes.
+int mcv_rig_autotune(mcv_rig* r, const uint8_t* const* d_images, int reps, char* report, int report_cap) {
+    mcv_ctx* c = r ? r->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(r && d_images && reps > 0, MCV_ERR_INVALID, "bad autotune arguments");
+    c->set_device();
+    const SinCosTables tb = c->tables();
+    hipStream_t s = c->stream;
+    OrbPlan& P = *r->batch;
+    const int ni = 3 * r->n_frames;
+    std::vector<int> st(ni);
+    for (int i = 0; i < ni; i++) st[i] = r->w[i % 3];
+    HIP_OK(hipMemcpyAsync(P.d_src.p, d_images, sizeof(void*) * ni, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(P.d_stride.p, st.data(), sizeof(int) * ni, hipMemcpyHostToDevice, s));
+    auto time_kernel = [&](const char* name) -> float {
+        std::vector<float> v;
+        for (int k = 0; k < reps + 1; k++) {
+            P.tune = r->tune;
+            r->timer.start(s);
+            rig_enqueue_batched(r, tb, s, &r->timer);
+            HIP_OK(hipStreamSynchronize(s));
+            float sum = 0;
+            for (size_t i = 0; i < r->timer.names.size(); i++)
+                if (strcmp(r->timer.names[i], name) == 0) {
+                    float t = 0;
+                    HIP_OK(hipEventElapsedTime(&t, r->timer.ev[i], r->timer.ev[i + 1]));
+                    sum += t;
+                }
+            if (k > 0) v.push_back(sum);  // the first run warms caches
+        }
+        std::sort(v.begin(), v.end());
+        return v[v.size() / 2];
+    };
+    std::string rep;
+    char line[160];
+    {
+        int best = 0;
+        float bt = 1e30f;
+        for (int v = 0; v < kKnnVariants; v++) {
+            r->tune.knn = v;
+            const float t = time_kernel("k_knn2");
+            snprintf(line, sizeof line, "k_knn2 %s %.4f ms\n", knn_variant_name(v), t);
+            rep += line;
+            if (t < bt) { bt = t; best = v; }
+        }
+        r->tune.knn = best;
+    }
+    {
+        int best = 0;
+        float bt = 1e30f;
+        for (int v = 0; v < 2; v++) {
+            r->tune.blur = v;
+            const float t = time_kernel("k_blur");
+            snprintf(line, sizeof line, "k_blur %s %.4f ms\n", v ? "fixed-walk" : "stop-at-end", t);
+            rep += line;
+            if (t < bt) { bt = t; best = v; }
+        }
+        r->tune.blur = best;
+    }
+    snprintf(line, sizeof line, "chosen: k_knn2 %s, k_blur %s\n", knn_variant_name(r->tune.knn),
+             r->tune.blur ? "fixed-walk" : "stop-at-end");
+    rep += line;
+    P.tune = r->tune;
+    for (auto& sp : r->single) sp->tune = r->tune;
+    c->tuned[r->shape_key] = r->tune;
+    c->knn_variant = r->tune.knn;
+    if (r->graph) {
+        HIP_OK(hipGraphExecDestroy(r->graph));
+        r->graph = nullptr;
+    }
+    if (report && report_cap > 0) {
+        strncpy(report, rep.c_str(), (size_t)report_cap - 1);
+        report[report_cap - 1] = 0;
     }
     API_END
 }
@@ -1184,7 +1325,7 @@ int mcv_rig_run(mcv_rig* r, const uint8_t* const* d_images, int flags) {
             }
             for (int k = 0; k < 3; k++) {
                 const int p = 3 * f + k;
-                hipLaunchKernelGGL(k_knn2, dim3((r->cap + 63) / 64, 1), dim3(256), 0, s, r->pairs.as<MatchPair>() + p);
+                launch_knn2(r->tune.knn, r->pairs.as<MatchPair>() + p, 1, r->cap, s);
                 hipLaunchKernelGGL(k_filter, dim3(1), dim3(1024), 0, s, r->jobs.as<FilterJob>() + p);
             }
             // the single-image plans are reused by the next frame
@@ -1445,12 +1586,23 @@ int mcv_ba_set_observations(mcv_ba* b, const float* uv, const int32_t* pt, const
         std::vector<int> pos(fcount.begin(), fcount.end() - 1);
         for (int i = 0; i < M; i++) fobs[pos[fr[i]]++] = i;
     }
+    const int G = b->F * b->C;
+    b->h_gptr.assign(G + 1, 0);
+    for (int i = 0; i < M; i++) b->h_gptr[pair[i] + 1]++;
+    for (int g = 0; g < G; g++) b->h_gptr[g + 1] += b->h_gptr[g];
+    std::vector<int> gobs(std::max(M, 1));
+    {
+        std::vector<int> pos(b->h_gptr.begin(), b->h_gptr.end() - 1);
+        for (int i = 0; i < M; i++) gobs[pos[pair[i]]++] = i;
+    }
+    b->gobs.ensure((size_t)std::max(M, 1) * 4);
     if (M) {
         HIP_OK(hipMemcpyAsync(b->uv.p, uv, (size_t)M * 8, hipMemcpyHostToDevice, s));
         HIP_OK(hipMemcpyAsync(b->pt.p, pt, (size_t)M * 4, hipMemcpyHostToDevice, s));
         HIP_OK(hipMemcpyAsync(b->pair.p, pair.data(), (size_t)M * 4, hipMemcpyHostToDevice, s));
         HIP_OK(hipMemcpyAsync(b->isig.p, isig, (size_t)M * 4, hipMemcpyHostToDevice, s));
         HIP_OK(hipMemcpyAsync(b->fobs.p, fobs.data(), (size_t)M * 4, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(b->gobs.p, gobs.data(), (size_t)M * 4, hipMemcpyHostToDevice, s));
     }
     HIP_OK(hipMemcpyAsync(b->fptr.p, fcount.data(), (size_t)(b->F + 1) * 4, hipMemcpyHostToDevice, s));
     if (sorted) HIP_OK(hipMemcpyAsync(b->pptr.p, pcount.data(), (size_t)(b->P + 1) * 4, hipMemcpyHostToDevice, s));
@@ -1495,6 +1647,7 @@ int mcv_ba_set_state(mcv_ba* b, const double* points, const double* frames, cons
         lo[p] = make_float4(l[0], l[1], l[2], 0.f);
     }
     std::vector<BAPair> pr((size_t)b->F * b->C);
+    std::vector<FramePose> fp(b->F);
     for (int f = 0; f < b->F; f++) {
         double Rbw[9];
         const double* fq = frames + 7 * f;
@@ -1504,6 +1657,8 @@ int mcv_ba_set_state(mcv_ba* b, const double* points, const double* frames, cons
         for (double& v : q) v /= qn;
         quat_to_R(q, Rbw);
         const double* tbw = fq + 4;
+        for (int i = 0; i < 9; i++) split(Rbw[i], fp[f].Rh[i], fp[f].Rl[i]);
+        for (int i = 0; i < 3; i++) split(tbw[i], fp[f].th[i], fp[f].tl[i]);
         for (int k = 0; k < b->C; k++) {
             const double* cm = cams + 12 * k;
             double Rcb[9];
@@ -1534,6 +1689,7 @@ int mcv_ba_set_state(mcv_ba* b, const double* points, const double* frames, cons
     upload(b->phi, hi, s);
     upload(b->plo, lo, s);
     upload(b->pairs, pr, s);
+    upload(b->fposes, fp, s);
     HIP_OK(hipStreamSynchronize(s));
     b->have_state = true;
     API_END
@@ -1547,10 +1703,122 @@ int mcv_ba_linearize(mcv_ba* b, int residual_only) {
     if (b->M > 0) {
         BAObs ob{b->uv.as<float2>(), b->pt.as<uint32_t>(), b->pair.as<uint32_t>(), b->isig.as<float>(), b->M};
         BAOut o{b->planes.as<float>(), (long long)b->M};
-        hipLaunchKernelGGL(k_ba_linearize, dim3((b->M + 255) / 256), dim3(256), 0, c->stream, ob, b->phi.as<float4>(),
-                           b->plo.as<float4>(), b->pairs.as<BAPair>(), o, residual_only);
+        const int np = b->F * b->C;
+        const int blocks = (b->M + 255) / 256;
+        if (np <= kLdsPairs)
+            hipLaunchKernelGGL(k_ba_linearize<true>, dim3(std::min(blocks, 1024)), dim3(256),
+                               (size_t)np * sizeof(BAPair), c->stream, ob,
+                               b->phi.as<float4>(), b->plo.as<float4>(), b->pairs.as<BAPair>(), np, o, residual_only);
+        else
+            hipLaunchKernelGGL(k_ba_linearize<false>, dim3(blocks), dim3(256), 0, c->stream, ob, b->phi.as<float4>(),
+                               b->plo.as<float4>(), b->pairs.as<BAPair>(), np, o, residual_only);
         HIP_OK(hipGetLastError());
     }
+    b->linearized = !residual_only;
+    API_END
+}
+
+int mcv_ba_set_stereo(mcv_ba* b, int Ms, const float* obs, const int32_t* pt, const int32_t* fr, const float* isig,
+                      const double* cam) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && Ms >= 0 && cam && (Ms == 0 || (obs && pt && fr && isig)), MCV_ERR_INVALID, "bad stereo edges");
+    c->set_device();
+    hipStream_t s = c->stream;
+    for (int i = 0; i < Ms; i++)
+        REQUIRE(pt[i] >= 0 && pt[i] < b->P && fr[i] >= 0 && fr[i] < b->F, MCV_ERR_INVALID,
+                "stereo edge index out of range");
+    b->Ms = Ms;
+    b->scam = StereoCam{(float)cam[0], (float)cam[1], (float)cam[2], (float)cam[3], (float)cam[4], (float)cam[5]};
+    const size_t m = (size_t)std::max(Ms, 1);
+    b->s_obs.ensure(m * 12);
+    b->s_pt.ensure(m * 4);
+    b->s_fr.ensure(m * 4);
+    b->s_isig.ensure(m * 4);
+    b->s_planes.ensure(m * 32 * 4);
+    if (Ms) {
+        HIP_OK(hipMemcpyAsync(b->s_obs.p, obs, (size_t)Ms * 12, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(b->s_pt.p, pt, (size_t)Ms * 4, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(b->s_fr.p, fr, (size_t)Ms * 4, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(b->s_isig.p, isig, (size_t)Ms * 4, hipMemcpyHostToDevice, s));
+    }
+    HIP_OK(hipStreamSynchronize(s));
+    b->have_stereo = true;
+    b->stereo_linearized = false;
+    API_END
+}
+
+int mcv_ba_linearize_stereo(mcv_ba* b, int residual_only) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && b->have_stereo && b->have_state, MCV_ERR_STATE, "stereo edges or state not set");
+    c->set_device();
+    if (b->Ms > 0) {
+        BAStereoObs ob{b->s_obs.as<float>(), b->s_pt.as<uint32_t>(), b->s_fr.as<uint32_t>(), b->s_isig.as<float>(),
+                       b->Ms};
+        BAOut o{b->s_planes.as<float>(), (long long)b->Ms};
+        hipLaunchKernelGGL(k_ba_stereo_linearize, dim3((b->Ms + 255) / 256), dim3(256), 0, c->stream, ob,
+                           b->phi.as<float4>(), b->plo.as<float4>(), b->fposes.as<FramePose>(), b->scam, o,
+                           residual_only);
+        HIP_OK(hipGetLastError());
+    }
+    b->stereo_linearized = true;
+    API_END
+}
+
+int mcv_ba_read_stereo_planes(mcv_ba* b, int first, int count, float* host) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && b->stereo_linearized, MCV_ERR_STATE, "no stereo linearization yet");
+    REQUIRE(host && first >= 0 && count >= 0 && first + count <= 32, MCV_ERR_INVALID, "plane range");
+    c->set_device();
+    if (b->Ms > 0 && count > 0)
+        HIP_OK(hipMemcpyAsync(host, b->s_planes.as<float>() + (size_t)first * b->Ms, (size_t)count * b->Ms * 4,
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    API_END
+}
+
+int mcv_ba_outliers(mcv_ba* b, float thr_mono, float thr_stereo, uint8_t* flags_mono, uint8_t* flags_stereo) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b, MCV_ERR_INVALID, "null BA object");
+    REQUIRE(!flags_mono || b->have_obs, MCV_ERR_STATE, "no mono edges");
+    REQUIRE(!flags_stereo || b->stereo_linearized, MCV_ERR_STATE, "no stereo linearization yet");
+    c->set_device();
+    hipStream_t s = c->stream;
+    b->flags.ensure((size_t)std::max(std::max(b->M, b->Ms), 1));
+    if (flags_mono && b->M > 0) {
+        hipLaunchKernelGGL(k_ba_outliers, dim3((b->M + 255) / 256), dim3(256), 0, s,
+                           b->planes.as<float>() + 2 * (size_t)b->M, b->M, thr_mono, b->flags.as<uint8_t>());
+        HIP_OK(hipMemcpyAsync(flags_mono, b->flags.p, (size_t)b->M, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+    }
+    if (flags_stereo && b->Ms > 0) {
+        hipLaunchKernelGGL(k_ba_outliers, dim3((b->Ms + 255) / 256), dim3(256), 0, s,
+                           b->s_planes.as<float>() + 3 * (size_t)b->Ms, b->Ms, thr_stereo, b->flags.as<uint8_t>());
+        HIP_OK(hipMemcpyAsync(flags_stereo, b->flags.p, (size_t)b->Ms, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+    }
+    HIP_OK(hipGetLastError());
+    API_END
+}
+
+int mcv_ba_linearize_plain(mcv_ba* b, int residual_only) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && b->have_obs && b->have_state, MCV_ERR_STATE, "observations or state not set");
+    c->set_device();
+    BAObs ob{b->uv.as<float2>(), b->pt.as<uint32_t>(), b->pair.as<uint32_t>(), b->isig.as<float>(), b->M};
+    BAOut o{b->planes.as<float>(), (long long)b->M};
+    for (size_t g = 0; g + 1 < b->h_gptr.size(); g++) {
+        const int n = b->h_gptr[g + 1] - b->h_gptr[g];
+        if (n == 0) continue;
+        hipLaunchKernelGGL(k_ba_linearize_group, dim3((n + 255) / 256), dim3(256), 0, c->stream, ob,
+                           b->phi.as<float4>(), b->plo.as<float4>(), b->pairs.as<BAPair>(),
+                           b->gobs.as<int>() + b->h_gptr[g], n, o, residual_only);
+    }
+    HIP_OK(hipGetLastError());
     b->linearized = !residual_only;
     API_END
 }

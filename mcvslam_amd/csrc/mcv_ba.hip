@@ -3,7 +3,10 @@
 //   k_ba_linearize  EdgeSE3XYZ2Camera::computeError + linearizeOplus
 //                   (include/OptimizeTypes.hpp:80-113), chi2 and the Huber
 //                   weight (ThirdParty/g2o/g2o/core/base_edge.h:94-97,
-//                   core/robust_kernel_impl.cpp:65-78)
+//                   core/robust_kernel_impl.cpp:65-78), all edges at once
+//   k_ba_linearize_group  the same for one (frame, camera) edge group
+//   k_ba_stereo_linearize EdgeStereoSE3ProjectXYZ (3-row stereo edge)
+//   k_ba_outliers   the chi2 outlier test of FilterCallBack_Chi2
 //   k_ba_point_adj  sum_obs J_point^T g per point (CSR by point)
 //   k_ba_pose_adj   sum_obs J_pose^T g per frame (observation list by frame)
 //
@@ -90,14 +93,26 @@ struct BAOut {
     long long stride;
 };
 
-__global__ __launch_bounds__(256) void k_ba_linearize(BAObs ob, const float4* __restrict__ pts_hi,
-                                                      const float4* __restrict__ pts_lo,
-                                                      const BAPair* __restrict__ pairs, BAOut out, int residual_only) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ob.M) return;
+// a / b as a float pair, from inv ~ 1 / b.hi (v_rcp_f32, 1 ulp): three
+// quotient digits, each remainder taken exactly, so the result carries the
+// accuracy of the pair arithmetic and no division is correctly rounded
+__device__ __forceinline__ f2 df_div_rcp(f2 a, f2 b, float inv) {
+    const float q1 = a.hi * inv;
+    f2 r = df_add(a, df_neg(df_mul_f(b, q1)));
+    const float q2 = r.hi * inv;
+    r = df_add(r, df_neg(df_mul_f(b, q2)));
+    const float q3 = r.hi * inv;
+    return df_add(fast_two_sum(q1, q2), f2{q3, 0.f});
+}
+
+constexpr int kPairWords = (int)(sizeof(BAPair) / 4);
+constexpr int kLdsPairs = 192;  // (frame, camera) pairs that fit the LDS copy
+
+__device__ __forceinline__ void ba_edge(int i, const BAObs& ob, const float4* __restrict__ pts_hi,
+                                        const float4* __restrict__ pts_lo, const BAPair& T, BAOut out,
+                                        int residual_only) {
     const float2 uv = ob.uv[i];
     const uint32_t pi = ob.pt[i];
-    const BAPair& T = pairs[ob.pair[i]];
     const float isig = ob.isig[i];
     const float4 ph = pts_hi[pi], pl = pts_lo[pi];
     const f2 P[3] = {{ph.x, pl.x}, {ph.y, pl.y}, {ph.z, pl.z}};
@@ -111,14 +126,15 @@ __global__ __launch_bounds__(256) void k_ba_linearize(BAObs ob, const float4* __
         C[r] = acc;
     }
     // e = (obs - c) - f * X / Z
-    const f2 qx = df_div(C[0], C[2]);
-    const f2 qy = df_div(C[1], C[2]);
+    const float iz = __builtin_amdgcn_rcpf(C[2].hi);
+    const f2 qx = df_div_rcp(C[0], C[2], iz);
+    const f2 qy = df_div_rcp(C[1], C[2], iz);
     const f2 eu = df_add(two_sum(uv.x, -T.cx), df_neg(df_mul_f(qx, T.fx)));
     const f2 ev = df_add(two_sum(uv.y, -T.cy), df_neg(df_mul_f(qy, T.fy)));
     const float e0 = eu.hi + eu.lo, e1 = ev.hi + ev.lo;
     const float chi2 = e0 * (isig * e0) + e1 * (isig * e1);
     const float d2 = T.delta * T.delta;
-    const float w = chi2 <= d2 ? 1.f : T.delta / sqrtf(chi2);
+    const float w = chi2 <= d2 ? 1.f : T.delta * __builtin_amdgcn_rsqf(chi2);
     const long long S = out.stride;
     float* o = out.planes + i;
     o[0] = e0;
@@ -127,10 +143,9 @@ __global__ __launch_bounds__(256) void k_ba_linearize(BAObs ob, const float4* __
     o[3 * S] = w;
     if (residual_only) return;
     // Jacobians (FP32).  Jpi = [[fx/z, 0, -fx x/z^2], [0, fy/z, -fy y/z^2]]
-    const float x = C[0].hi, y = C[1].hi, z = C[2].hi;
-    const float iz = 1.f / z;
-    const float j00 = T.fx * iz, j02 = -T.fx * x / (z * z);
-    const float j11 = T.fy * iz, j12 = -T.fy * y / (z * z);
+    const float x = C[0].hi, y = C[1].hi;
+    const float j00 = T.fx * iz, j02 = -T.fx * x * (iz * iz);
+    const float j11 = T.fy * iz, j12 = -T.fy * y * (iz * iz);
     // J_point = -Jpi R_cw
     float Jpt[6];
 #pragma unroll
@@ -164,6 +179,149 @@ __global__ __launch_bounds__(256) void k_ba_linearize(BAObs ob, const float4* __
     for (int k = 0; k < 12; k++) o[(4 + k) * S] = Jp[k];
 #pragma unroll
     for (int k = 0; k < 6; k++) o[(16 + k) * S] = Jpt[k];
+}
+
+// One observation per thread, grid-stride.  kLds: the (frame, camera) table
+// (npairs <= kLdsPairs) is copied to dynamic LDS once per workgroup, so the
+// per-observation parameter reads are LDS gathers (53-word records: odd
+// stride, distinct records fall in distinct banks) instead of ~50 scattered
+// global loads.
+template <bool kLds>
+__global__ __launch_bounds__(256) void k_ba_linearize(BAObs ob, const float4* __restrict__ pts_hi,
+                                                      const float4* __restrict__ pts_lo,
+                                                      const BAPair* __restrict__ pairs, int npairs, BAOut out,
+                                                      int residual_only) {
+    extern __shared__ float s_pairs[];  // npairs * kPairWords floats when kLds
+    if (kLds) {
+        const float* src = reinterpret_cast<const float*>(pairs);
+        for (int k = threadIdx.x; k < npairs * kPairWords; k += blockDim.x) s_pairs[k] = src[k];
+        __syncthreads();
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ob.M; i += gridDim.x * blockDim.x) {
+        const uint32_t pr = ob.pair[i];
+        const BAPair& T = kLds ? *reinterpret_cast<const BAPair*>(s_pairs + pr * kPairWords) : pairs[pr];
+        ba_edge(i, ob, pts_hi, pts_lo, T, out, residual_only);
+    }
+}
+
+// The plain launch pattern: the observations of one (frame, camera) edge
+// group (list[0..n)), one launch per group.
+__global__ __launch_bounds__(256) void k_ba_linearize_group(BAObs ob, const float4* __restrict__ pts_hi,
+                                                            const float4* __restrict__ pts_lo,
+                                                            const BAPair* __restrict__ pairs, const int* list, int n,
+                                                            BAOut out, int residual_only) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = list[k];
+    ba_edge(i, ob, pts_hi, pts_lo, pairs[ob.pair[i]], out, residual_only);
+}
+
+// ---------------------------------------------------------------------------
+// Stereo edge EdgeStereoSE3ProjectXYZ (include/OptimizeTypes.hpp:119-157,
+// src/OptimizeTypes.cpp:12-58).  The pose vertex is the keyframe pose (the
+// left camera is the body).  cam_project_uv_u rounds the inverse depth to
+// float and forms bf * invz as a float product; the rest is double in the
+// reference, float pairs here.  32 output planes of M floats:
+// e_u e_v e_ur chi2 w J_pose[3x6] J_point[3x3].
+struct FramePose {
+    float Rh[9], Rl[9];  // R_cw hi / lo
+    float th[3], tl[3];  // t_cw hi / lo
+};
+
+struct StereoCam {
+    float fx, fy, cx, cy, bf, delta;
+};
+
+struct BAStereoObs {
+    const float* obs;       // M x 3: u, v, u_right
+    const uint32_t* pt;
+    const uint32_t* fr;
+    const float* isig;
+    int M;
+};
+
+__global__ __launch_bounds__(256) void k_ba_stereo_linearize(BAStereoObs ob, const float4* __restrict__ pts_hi,
+                                                             const float4* __restrict__ pts_lo,
+                                                             const FramePose* __restrict__ poses, StereoCam cam,
+                                                             BAOut out, int residual_only) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ob.M) return;
+    const float ou = ob.obs[3 * i], ov = ob.obs[3 * i + 1], our = ob.obs[3 * i + 2];
+    const float isig = ob.isig[i];
+    const FramePose& T = poses[ob.fr[i]];
+    const uint32_t pi = ob.pt[i];
+    const float4 ph = pts_hi[pi], pl = pts_lo[pi];
+    const f2 P[3] = {{ph.x, pl.x}, {ph.y, pl.y}, {ph.z, pl.z}};
+    f2 C[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        f2 acc = {T.th[r], T.tl[r]};
+#pragma unroll
+        for (int k = 0; k < 3; k++) acc = df_add(acc, df_mul(f2{T.Rh[3 * r + k], T.Rl[3 * r + k]}, P[k]));
+        C[r] = acc;
+    }
+    // invz = (float)(1 / z), rounded from the pair quotient
+    const f2 q = df_div_rcp(f2{1.f, 0.f}, C[2], __builtin_amdgcn_rcpf(C[2].hi));
+    const float invz = q.hi + q.lo;
+    const f2 ru = df_add(df_mul_f(df_mul_f(C[0], invz), cam.fx), f2{cam.cx, 0.f});
+    const f2 rv = df_add(df_mul_f(df_mul_f(C[1], invz), cam.fy), f2{cam.cy, 0.f});
+    const float bfi = cam.bf * invz;
+    const f2 eu = df_add(f2{ou, 0.f}, df_neg(ru));
+    const f2 ev = df_add(f2{ov, 0.f}, df_neg(rv));
+    const f2 er = df_add(df_add(f2{our, 0.f}, df_neg(ru)), f2{bfi, 0.f});
+    const float e0 = eu.hi + eu.lo, e1 = ev.hi + ev.lo, e2 = er.hi + er.lo;
+    const float chi2 = isig * (e0 * e0 + e1 * e1 + e2 * e2);
+    const float w = chi2 <= cam.delta * cam.delta ? 1.f : cam.delta * __builtin_amdgcn_rsqf(chi2);
+    const long long S = out.stride;
+    float* o = out.planes + i;
+    o[0] = e0;
+    o[S] = e1;
+    o[2 * S] = e2;
+    o[3 * S] = chi2;
+    o[4 * S] = w;
+    if (residual_only) return;
+    const float x = C[0].hi, y = C[1].hi, z = C[2].hi;
+    const float iz = __builtin_amdgcn_rcpf(z), iz2 = iz * iz;
+    const float fx = cam.fx, fy = cam.fy, bf = cam.bf;
+    float J0[6], J1[6], J2[6];
+    J0[0] = x * y * iz2 * fx;
+    J0[1] = -(1.f + x * x * iz2) * fx;
+    J0[2] = y * iz * fx;
+    J0[3] = -iz * fx;
+    J0[4] = 0.f;
+    J0[5] = x * iz2 * fx;
+    J1[0] = (1.f + y * y * iz2) * fy;
+    J1[1] = -x * y * iz2 * fy;
+    J1[2] = -x * iz * fy;
+    J1[3] = 0.f;
+    J1[4] = -iz * fy;
+    J1[5] = y * iz2 * fy;
+    J2[0] = J0[0] - bf * y * iz2;
+    J2[1] = J0[1] + bf * x * iz2;
+    J2[2] = J0[2];
+    J2[3] = J0[3];
+    J2[4] = 0.f;
+    J2[5] = J0[5] - bf * iz2;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        o[(5 + k) * S] = J0[k];
+        o[(11 + k) * S] = J1[k];
+        o[(17 + k) * S] = J2[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float a = -fx * T.Rh[k] * iz + fx * x * T.Rh[6 + k] * iz2;
+        const float b = -fy * T.Rh[3 + k] * iz + fy * y * T.Rh[6 + k] * iz2;
+        o[(23 + k) * S] = a;
+        o[(26 + k) * S] = b;
+        o[(29 + k) * S] = a - bf * T.Rh[6 + k] * iz2;
+    }
+}
+
+// FilterCallBack_Chi2 (src/PoseEstimation.cpp:132-151): flag = chi2 >= threshold.
+__global__ __launch_bounds__(256) void k_ba_outliers(const float* chi2, int M, float threshold, uint8_t* flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < M) flags[i] = chi2[i] >= threshold ? 1 : 0;
 }
 
 // grad_point[p] = sum over the observations of p (CSR, observation order) of

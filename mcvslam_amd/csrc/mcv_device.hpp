@@ -20,58 +20,71 @@ namespace mcv {
 // heap_key(); the rest of the entry is payload that never takes part in a
 // comparison, exactly like the shared_ptr payload of the reference's node
 // queue (ORBextractor.cc:530-531).  u64 entries keep the key in the high
-// 32 bits; uint4 entries in .x.
+// 32 bits; uint2 entries in .x >> 3 (the low 3 bits of .x are payload).
+// Each step reads whole entries once (both children together on the way
+// down), so a step costs one dependent memory round trip.
 MCV_HD uint32_t heap_key(uint64_t a) { return (uint32_t)(a >> 32); }
 #if defined(__HIPCC__)
-MCV_HD uint32_t heap_key(const uint4& a) { return a.x; }
+MCV_HD uint32_t heap_key(const uint2& a) { return a.x >> 3; }
 #endif
 
-template <typename Ptr, typename T>
-MCV_HD void heap_sift_up(Ptr h, int hole, int top, T value) {
-    int parent = (hole - 1) / 2;
-    while (hole > top && heap_key(h[parent]) < heap_key(value)) {
-        h[hole] = h[parent];
+// `H` is any heap store with get(i) / set(i, v) (see ArrayHeap).
+template <typename H, typename T>
+MCV_HD void heap_sift_up(H& h, int hole, int top, T value) {
+    while (hole > top) {
+        const int parent = (hole - 1) / 2;
+        const T p = h.get(parent);
+        if (!(heap_key(p) < heap_key(value))) break;
+        h.set(hole, p);
         hole = parent;
-        parent = (hole - 1) / 2;
     }
-    h[hole] = value;
+    h.set(hole, value);
 }
 
-template <typename Ptr, typename T>
-MCV_HD void heap_push(Ptr h, int& len, T value) {
-    h[len] = value;
+template <typename H, typename T>
+MCV_HD void heap_push(H& h, int& len, T value) {
     heap_sift_up(h, len, 0, value);
     len++;
 }
 
-template <typename Ptr, typename T>
-MCV_HD void heap_adjust(Ptr h, int hole, int len, T value) {
+template <typename H, typename T>
+MCV_HD void heap_adjust(H& h, int hole, int len, T value) {
     const int top = hole;
     int second = hole;
     while (second < (len - 1) / 2) {
         second = 2 * (second + 1);
-        if (heap_key(h[second]) < heap_key(h[second - 1])) second--;
-        h[hole] = h[second];
+        const T r = h.get(second), l = h.get(second - 1);
+        const bool take_left = heap_key(r) < heap_key(l);
+        if (take_left) second--;
+        h.set(hole, take_left ? l : r);
         hole = second;
     }
     if ((len & 1) == 0 && second == (len - 2) / 2) {
         second = 2 * (second + 1);
-        h[hole] = h[second - 1];
+        h.set(hole, h.get(second - 1));
         hole = second - 1;
     }
     heap_sift_up(h, hole, top, value);
 }
 
-template <typename Ptr>
-MCV_HD void heap_pop(Ptr h, int& len) {
+template <typename H>
+MCV_HD void heap_pop(H& h, int& len) {
+    // libstdc++ also moves the old top to position len - 1, outside the
+    // remaining heap; nothing reads it here, so that store is left out
     if (len > 1) {
         const int last = len - 1;
-        const auto value = h[last];
-        h[last] = h[0];
-        heap_adjust(h, 0, last, value);
+        heap_adjust(h, 0, last, h.get(last));
     }
     len--;
 }
+
+// A plain array as heap store.
+template <typename T>
+struct ArrayHeap {
+    T* a;
+    MCV_HD T get(int i) const { return a[i]; }
+    MCV_HD void set(int i, T v) const { a[i] = v; }
+};
 
 // ---------------------------------------------------------------------------
 // Base sin/cos of a float angle in [0, 8): Cody-Waite reduction by pi/2 in

@@ -1,6 +1,6 @@
 // Hamming matching kernels for gfx950.
 //
-//   k_knn2        Matcher::KnnMatch(vector<Mat>, vector<Mat>) top-2
+//   k_knn2_s, k_knn2_lds  Matcher::KnnMatch(vector<Mat>, vector<Mat>) top-2
 //                 (src/Matcher.cpp:245-302, include/Matcher.hpp:19-33)
 //   k_filter      MatchResKnn::FilterRatio + MatchRes::FilterThreshold
 //                 (src/Matcher.cpp:100-111, :23-35)
@@ -9,8 +9,7 @@
 // '<' while scanning trains in order, which is the first two entries of a
 // stable sort by distance.  Here each candidate becomes the key
 // (distance << 22) | train_index; the two smallest keys are that same pair,
-// and (999 << 22) | 0 is the reference's sentinel.  Keys merge exactly across
-// waves, so the train set is split over the 4 waves of a workgroup.
+// and (999 << 22) | 0 is the reference's sentinel.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -29,61 +28,168 @@ struct MatchPair {
     int32_t* out;         // nq x (i0, d0, i1, d1)
 };
 
+// keeps the two smallest keys (k0 <= k1): the new second is the median of
+// {k0, k1, key}
+__device__ __forceinline__ uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 __device__ __forceinline__ void top2_insert(uint32_t& k0, uint32_t& k1, uint32_t key) {
-    k1 = min(k1, max(k0, key));
+    k1 = med3_u32(k0, k1, key);
     k0 = min(k0, key);
 }
 
-// 256 threads: lane = query, wave = quarter of the train set.  Train rows are
-// wave-uniform and read through scalar loads; the query row stays in VGPRs.
-__global__ __launch_bounds__(256) void k_knn2(const MatchPair* pairs) {
-    __shared__ uint32_t sk[2][4][64];
+// Variant S<QPL, UNR>.  256 threads; a lane holds QPL queries (64 QPL per
+// workgroup) and the 4 waves split the train set.  Train rows are
+// wave-uniform and come through scalar loads (SGPR operands of the XORs),
+// UNR rows per iteration; the four waves' top-2 keys merge through LDS.
+template <int QPL, int UNR>
+__global__ __launch_bounds__(256) void k_knn2_s(const MatchPair* pairs) {
+    __shared__ uint32_t sk[2][4][64 * QPL];
     const MatchPair P = pairs[blockIdx.y];
     const int nq = P.nq_dev ? min(*P.nq_dev, P.nq) : P.nq;
     const int nt = P.nt_dev ? min(*P.nt_dev, P.nt) : P.nt;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int qi = blockIdx.x * 64 + lane;
-    if (blockIdx.x * 64 >= nq) return;
-    uint32_t q[8];
-    const uint32_t* qrow = reinterpret_cast<const uint32_t*>(P.q + (long long)min(qi, nq - 1) * 32);
+    const int qb = blockIdx.x * 64 * QPL;
+    if (qb >= nq) return;
+    uint32_t q[QPL][8], k0[QPL], k1[QPL];
 #pragma unroll
-    for (int k = 0; k < 8; k++) q[k] = qrow[k];
+    for (int u = 0; u < QPL; u++) {
+        const uint4* qrow = reinterpret_cast<const uint4*>(P.q + (long long)min(qb + 64 * u + lane, nq - 1) * 32);
+        const uint4 a = qrow[0], b = qrow[1];
+        q[u][0] = a.x; q[u][1] = a.y; q[u][2] = a.z; q[u][3] = a.w;
+        q[u][4] = b.x; q[u][5] = b.y; q[u][6] = b.z; q[u][7] = b.w;
+        k0[u] = kSentinel;
+        k1[u] = kSentinel;
+    }
     const int per = (nt + 3) / 4;
     const int j0 = __builtin_amdgcn_readfirstlane(wave * per);
     const int j1 = __builtin_amdgcn_readfirstlane(min(nt, j0 + per));
-    uint32_t k0 = kSentinel, k1 = kSentinel;
     const uint32_t* tw = reinterpret_cast<const uint32_t*>(P.t);
     int j = j0;
-    for (; j + 1 < j1; j += 2) {
+    for (; j + UNR <= j1; j += UNR) {
         const uint32_t* ta = tw + (long long)j * 8;
-        uint32_t da = 0, db = 0;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            da += __builtin_popcount(q[k] ^ ta[k]);
-            db += __builtin_popcount(q[k] ^ ta[8 + k]);
+        for (int t = 0; t < UNR; t++)
+#pragma unroll
+            for (int u = 0; u < QPL; u++) {
+                uint32_t d = 0;
+#pragma unroll
+                for (int k = 0; k < 8; k++) d += __builtin_popcount(q[u][k] ^ ta[8 * t + k]);
+                top2_insert(k0[u], k1[u], (d << kIdxBits) | (uint32_t)(j + t));
+            }
+    }
+    for (; j < j1; j++) {
+        const uint32_t* ta = tw + (long long)j * 8;
+#pragma unroll
+        for (int u = 0; u < QPL; u++) {
+            uint32_t d = 0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) d += __builtin_popcount(q[u][k] ^ ta[k]);
+            top2_insert(k0[u], k1[u], (d << kIdxBits) | (uint32_t)j);
         }
-        top2_insert(k0, k1, (da << kIdxBits) | (uint32_t)j);
-        top2_insert(k0, k1, (db << kIdxBits) | (uint32_t)(j + 1));
     }
-    if (j < j1) {
-        const uint32_t* ta = tw + (long long)j * 8;
-        uint32_t da = 0;
 #pragma unroll
-        for (int k = 0; k < 8; k++) da += __builtin_popcount(q[k] ^ ta[k]);
-        top2_insert(k0, k1, (da << kIdxBits) | (uint32_t)j);
+    for (int u = 0; u < QPL; u++) {
+        sk[0][wave][64 * u + lane] = k0[u];
+        sk[1][wave][64 * u + lane] = k1[u];
     }
-    sk[0][wave][lane] = k0;
-    sk[1][wave][lane] = k1;
     __syncthreads();
-    if (wave == 0 && qi < nq) {
+    if (wave != 0) return;
+#pragma unroll
+    for (int u = 0; u < QPL; u++) {
+        const int qi = qb + 64 * u + lane;
+        uint32_t a0 = k0[u], a1 = k1[u];
 #pragma unroll
         for (int w = 1; w < 4; w++) {
-            const uint32_t b0 = sk[0][w][lane], b1 = sk[1][w][lane];
-            const uint32_t n1 = min(max(k0, b0), min(k1, b1));
-            k0 = min(k0, b0);
-            k1 = n1;
+            const uint32_t b0 = sk[0][w][64 * u + lane], b1 = sk[1][w][64 * u + lane];
+            const uint32_t n1 = min(max(a0, b0), min(a1, b1));
+            a0 = min(a0, b0);
+            a1 = n1;
         }
+        if (qi < nq) {
+            int4 o;
+            o.x = (int)(a0 & kIdxMask);
+            o.y = (int)(a0 >> kIdxBits);
+            o.z = (int)(a1 & kIdxMask);
+            o.w = (int)(a1 >> kIdxBits);
+            // the sentinel keeps index 0, as the reference's d_idx init does
+            reinterpret_cast<int4*>(P.out)[qi] = o;
+        }
+    }
+}
+
+// Variant L.  256 threads, lane = query: a workgroup holds 256 queries (one 32-byte
+// row in 8 VGPRs per lane) and streams the train set through LDS in tiles
+// of kKnnTile descriptors, double buffered, loaded with one coalesced 16-byte
+// read per thread.  Every wave reads each train row as a broadcast from LDS;
+// per (query, train): 8 XOR + 8 popcount-accumulate + key + 2 for the top-2.
+constexpr int kKnnTile = 128;
+constexpr int kKnnQ = 256;  // queries per workgroup
+
+__global__ __launch_bounds__(256) void k_knn2_lds(const MatchPair* pairs) {
+    __shared__ uint4 s_t[2][2 * kKnnTile];
+    const MatchPair P = pairs[blockIdx.y];
+    const int nq = P.nq_dev ? min(*P.nq_dev, P.nq) : P.nq;
+    const int nt = P.nt_dev ? min(*P.nt_dev, P.nt) : P.nt;
+    const int q0 = blockIdx.x * kKnnQ;
+    if (q0 >= nq) return;
+    const int qi = q0 + threadIdx.x;
+    uint32_t q[8];
+    {
+        const uint4* qrow = reinterpret_cast<const uint4*>(P.q + (long long)min(qi, nq - 1) * 32);
+        const uint4 a = qrow[0], b = qrow[1];
+        q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = a.w;
+        q[4] = b.x; q[5] = b.y; q[6] = b.z; q[7] = b.w;
+    }
+    uint32_t k0 = kSentinel, k1 = kSentinel;
+    const int ntiles = (nt + kKnnTile - 1) / kKnnTile;
+    const uint4* tsrc = reinterpret_cast<const uint4*>(P.t);
+    auto fetch = [&](int tile) -> uint4 {
+        const int ti = tile * kKnnTile + (threadIdx.x >> 1);
+        return ti < nt ? tsrc[2ll * ti + (threadIdx.x & 1)] : make_uint4(0u, 0u, 0u, 0u);
+    };
+    uint4 pf = make_uint4(0u, 0u, 0u, 0u);
+    if (ntiles > 0) {
+        s_t[0][threadIdx.x] = fetch(0);
+        if (ntiles > 1) pf = fetch(1);
+    }
+    __syncthreads();
+    for (int t = 0; t < ntiles; t++) {
+        const uint4* tb = s_t[t & 1];
+        const int cnt = min(kKnnTile, nt - t * kKnnTile);
+        const uint32_t base = (uint32_t)(t * kKnnTile);
+        auto dist = [&](int j) -> uint32_t {
+            const uint4 a = tb[2 * j], b = tb[2 * j + 1];
+            uint32_t d = __builtin_popcount(q[0] ^ a.x);
+            d += __builtin_popcount(q[1] ^ a.y);
+            d += __builtin_popcount(q[2] ^ a.z);
+            d += __builtin_popcount(q[3] ^ a.w);
+            d += __builtin_popcount(q[4] ^ b.x);
+            d += __builtin_popcount(q[5] ^ b.y);
+            d += __builtin_popcount(q[6] ^ b.z);
+            d += __builtin_popcount(q[7] ^ b.w);
+            return d;
+        };
+        int j = 0;
+        for (; j + 4 <= cnt; j += 4) {
+            const uint32_t d0 = dist(j), d1 = dist(j + 1), d2 = dist(j + 2), d3 = dist(j + 3);
+            top2_insert(k0, k1, (d0 << kIdxBits) | (base + j));
+            top2_insert(k0, k1, (d1 << kIdxBits) | (base + j + 1));
+            top2_insert(k0, k1, (d2 << kIdxBits) | (base + j + 2));
+            top2_insert(k0, k1, (d3 << kIdxBits) | (base + j + 3));
+        }
+        for (; j < cnt; j++) top2_insert(k0, k1, (dist(j) << kIdxBits) | (base + j));
+        if (t + 1 < ntiles) {
+            s_t[(t + 1) & 1][threadIdx.x] = pf;
+            if (t + 2 < ntiles) pf = fetch(t + 2);
+        }
+        __syncthreads();
+    }
+    if (qi < nq) {
         int4 o;
         o.x = (int)(k0 & kIdxMask);
         o.y = (int)(k0 >> kIdxBits);
@@ -91,6 +197,24 @@ __global__ __launch_bounds__(256) void k_knn2(const MatchPair* pairs) {
         o.w = (int)(k1 >> kIdxBits);
         // the sentinel keeps index 0, as the reference's d_idx init does
         reinterpret_cast<int4*>(P.out)[qi] = o;
+    }
+}
+
+// Variants of the same exact computation; the autotuner picks one per
+// workload (kKnnVariants of them).
+constexpr int kKnnVariants = 5;
+inline const char* knn_variant_name(int v) {
+    static const char* names[kKnnVariants] = {"S<1,2>", "S<1,4>", "S<2,2>", "S<2,4>", "L"};
+    return (v >= 0 && v < kKnnVariants) ? names[v] : "?";
+}
+
+inline void launch_knn2(int variant, const MatchPair* pairs, int npairs, int max_nq, hipStream_t s) {
+    switch (variant) {
+    case 1: hipLaunchKernelGGL((k_knn2_s<1, 4>), dim3((max_nq + 63) / 64, npairs), dim3(256), 0, s, pairs); break;
+    case 2: hipLaunchKernelGGL((k_knn2_s<2, 2>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;
+    case 3: hipLaunchKernelGGL((k_knn2_s<2, 4>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;
+    case 4: hipLaunchKernelGGL(k_knn2_lds, dim3((max_nq + kKnnQ - 1) / kKnnQ, npairs), dim3(256), 0, s, pairs); break;
+    default: hipLaunchKernelGGL((k_knn2_s<1, 2>), dim3((max_nq + 63) / 64, npairs), dim3(256), 0, s, pairs); break;
     }
 }
 

@@ -112,46 +112,78 @@ __global__ __launch_bounds__(256) void k_copy_level0(const Strip* st, int n, con
 // cv::resize INTER_LINEAR, CV_8U: horizontal int taps (coef 2048 scale) on
 // the two source rows, then the vertical pass
 // (((b0 * (r0 >> 4)) >> 16) + ((b1 * (r1 >> 4)) >> 16) + 2) >> 2.
-// Rows [y0, y1) of one strip: source and destination are different levels
-// of the same buffer, so they are passed as non-aliasing pointers and the
-// row loop is unrolled, which lets the loads of later rows run ahead of the
-// stores of earlier ones.
-__device__ __forceinline__ void resize_rows(const uint8_t* __restrict__ sp, uint8_t* __restrict__ dp,
-                                            const int2* __restrict__ coef, const LevelDesc& L,
-                                            const LevelDesc& SL, int x, int y0, int y1) {
-    int sx0[4], sx1[4], a0[4], a1[4];
+// One wave per strip (64-thread workgroups): every source row the strip
+// reads is staged first, over the strip's source column window, into
+// dynamic LDS (`rw` bytes per row, u32-aligned window start) with all the
+// loads in flight at once; the taps then gather bytes from LDS.
+__global__ __launch_bounds__(64) void k_resize(const Strip* st, int n, const LevelDesc* lv, const int2* coef,
+                                               uint8_t* pyr, int rw) {
+    extern __shared__ uint32_t s_rows[];
+    const int sidx = blockIdx.x;
+    if (sidx >= n) return;
+    const int lane = threadIdx.x;
+    const Strip S = st[sidx];
+    const LevelDesc L = lv[S.lvl];
+    const LevelDesc SL = lv[L.src];
+    const int x = S.x0 + 4 * lane;
+    const int yend = min(S.y0 + kStripH, L.h);
+    const uint8_t* sp = pyr + SL.pix;
+    // source window: columns [cb, c_hi], rows [r_lo, r_hi]
+    const int xl = min(S.x0 + kStripW - 1, L.w - 1);
+    const int cb = coef[L.coef_x + S.x0].x & ~3;
+    const int c_hi = min(coef[L.coef_x + xl].x + 1, SL.w - 1);
+    const int nw = (c_hi - cb) / 4 + 1;
+    const int r_lo = min(max(coef[L.coef_y + S.y0].x, 0), SL.h - 1);
+    const int r_hi = min(max(coef[L.coef_y + yend - 1].x + 1, 0), SL.h - 1);
+    const int nr = r_hi - r_lo + 1;
+    const int rwd = rw >> 2;
+    for (int k0 = 0; k0 < nr; k0 += 8) {
+        uint32_t v[8][3];
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++)
+#pragma unroll
+            for (int h = 0; h < 3; h++) {
+                const int d = lane + 64 * h;
+                v[kk][h] = (k0 + kk < nr && d < nw)
+                               ? *reinterpret_cast<const uint32_t*>(sp + (long long)(r_lo + k0 + kk) * SL.pitch + cb + 4 * d)
+                               : 0u;
+            }
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++)
+#pragma unroll
+            for (int h = 0; h < 3; h++) {
+                const int d = lane + 64 * h;
+                if (k0 + kk < nr && d < nw) s_rows[(k0 + kk) * rwd + d] = v[kk][h];
+            }
+    }
+    wave_lds_sync();
+    if (x >= L.w) return;
+    const uint8_t* rows = reinterpret_cast<const uint8_t*>(s_rows);
+    int o0[4], o1[4], a0[4], a1[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const int2 cx = coef[L.coef_x + min(x + j, L.w - 1)];
-        sx0[j] = cx.x;
-        sx1[j] = min(cx.x + 1, SL.w - 1);
+        o0[j] = cx.x - cb;
+        o1[j] = min(cx.x + 1, SL.w - 1) - cb;
         a0[j] = cx.y & 0xFFFF;
         a1[j] = cx.y >> 16;
     }
-#pragma unroll 4
-    for (int y = y0; y < y1; y++) {
+    uint8_t* dp = pyr + L.pix + x;
+    for (int y = S.y0; y < yend; y++) {
         const int2 cy = coef[L.coef_y + y];
         const int b0 = cy.y & 0xFFFF, b1 = cy.y >> 16;
-        const uint8_t* r0p = sp + (long long)min(max(cy.x, 0), SL.h - 1) * SL.pitch;
-        const uint8_t* r1p = sp + (long long)min(max(cy.x + 1, 0), SL.h - 1) * SL.pitch;
+        const uint8_t* r0p = rows + (min(max(cy.x, 0), SL.h - 1) - r_lo) * rw;
+        const uint8_t* r1p = rows + (min(max(cy.x + 1, 0), SL.h - 1) - r_lo) * rw;
         uint32_t v = 0;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const int r0 = r0p[sx0[j]] * a0[j] + r0p[sx1[j]] * a1[j];
-            const int r1 = r1p[sx0[j]] * a0[j] + r1p[sx1[j]] * a1[j];
+            const int r0 = r0p[o0[j]] * a0[j] + r0p[o1[j]] * a1[j];
+            const int r1 = r1p[o0[j]] * a0[j] + r1p[o1[j]] * a1[j];
             const uint32_t o = (uint32_t)((((b0 * (r0 >> 4)) >> 16) + ((b1 * (r1 >> 4)) >> 16) + 2) >> 2);
             v |= (o & 0xFF) << (8 * j);
         }
-        *reinterpret_cast<uint32_t*>(dp + (long long)y * L.pitch + x) = v;
+        *reinterpret_cast<uint32_t*>(dp + (long long)y * L.pitch) = v;
     }
-}
-
-__global__ __launch_bounds__(256) void k_resize(const Strip* st, int n, const LevelDesc* lv, const int2* coef,
-                                                uint8_t* pyr) {
-    STRIP_SETUP(st, n)
-    if (x >= L.w) return;
-    const LevelDesc SL = lv[L.src];
-    resize_rows(pyr + SL.pix, pyr + L.pix, coef, L, SL, x, S.y0, yend);
 }
 
 // Bytes x-4 .. x+7 of a row as three u32 (little endian).  Interior lanes
@@ -180,21 +212,30 @@ __device__ __forceinline__ void row_words(const uint8_t* row, int x, int w, int 
 // GaussianBlur 7x7, sigma 2, BORDER_REFLECT_101, CV_8U fixed point: the
 // 7 taps (sum 256) horizontally with udot4, then vertically, out =
 // (sum + 2^15) >> 16.  Each lane keeps the last 7 horizontal rows of its
-// 4 columns in registers while walking down the strip.
-__global__ __launch_bounds__(256) void k_blur(const Strip* st, int n, const LevelDesc* lv,
-                                              const uint8_t* __restrict__ pyr, uint8_t* __restrict__ blur,
-                                              BlurTaps taps) {
-    STRIP_SETUP(st, n)
-    if (x >= L.w) return;
-    const uint8_t* src = pyr + L.pix;
+// 4 columns in registers while walking down the strip.  Two walks compute
+// the same bytes: kFixed = false stops at the level's last row and picks
+// the border gather per row; kFixed = true always covers kStripH + 6 rows
+// (row indices reflect, stores past the level end are skipped) with the
+// lane's load kind fixed, so later rows' loads can be issued early.
+template <bool kFixed, bool kInterior>
+__device__ __forceinline__ void blur_strip(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
+                                           const LevelDesc& L, int x, int y0, int yend, const BlurTaps& taps) {
     uint32_t hr[7][4];
 #pragma unroll
     for (int r = 0; r < kStripH + 6; r++) {
-        const int yi = S.y0 - 3 + r;
-        if (yi - 3 >= yend) break;
+        const int yi = y0 - 3 + r;
+        if (!kFixed && yi - 3 >= yend) break;
         const int ry = reflect101(min(max(yi, -3), L.h + 2), L.h);
+        const uint8_t* row = src + (long long)ry * L.pitch;
         uint32_t w0, w1, w2;
-        row_words(src + (long long)ry * L.pitch, x, L.w, L.pitch, w0, w1, w2);
+        if (kFixed && kInterior) {
+            const uint32_t* rw = reinterpret_cast<const uint32_t*>(row + x);
+            w0 = rw[-1];
+            w1 = rw[0];
+            w2 = rw[1];
+        } else {
+            row_words(row, x, L.w, L.pitch, w0, w1, w2);
+        }
         uint32_t* h = hr[r % 7];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -212,9 +253,21 @@ __global__ __launch_bounds__(256) void k_blur(const Strip* st, int n, const Leve
                 for (int k = 0; k < 7; k++) acc += (uint32_t)taps.t[k] * hr[(r - 6 + k) % 7][j];
                 v |= min((acc + (1u << 15)) >> 16, 255u) << (8 * j);
             }
-            *reinterpret_cast<uint32_t*>(blur + L.pix + (long long)y * L.pitch + x) = v;
+            if (!kFixed || y < yend) *reinterpret_cast<uint32_t*>(dst + (long long)y * L.pitch + x) = v;
         }
     }
+}
+
+template <bool kFixed>
+__global__ __launch_bounds__(256) void k_blur(const Strip* st, int n, const LevelDesc* lv,
+                                              const uint8_t* __restrict__ pyr, uint8_t* __restrict__ blur,
+                                              BlurTaps taps) {
+    STRIP_SETUP(st, n)
+    if (x >= L.w) return;
+    if (kFixed && x >= 4 && x + 8 <= L.pitch && x + 6 < L.w)
+        blur_strip<kFixed, true>(pyr + L.pix, blur + L.pix, L, x, S.y0, yend, taps);
+    else
+        blur_strip<kFixed, false>(pyr + L.pix, blur + L.pix, L, x, S.y0, yend, taps);
 }
 
 // ---------------------------------------------------------------------------
@@ -790,12 +843,18 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g) {
 // Replays DistributeOctTree's priority_queue (ORBextractor.cc:530-578) on
 // node counts: push the non-empty roots, split the largest node until the
 // queue holds N nodes or the top holds a single keypoint, then pop
-// everything.  Lane 0 runs the heap in LDS with entries {count, node, first
-// child, end child}; lanes 0..4 fetch the counts and child ranges of a
-// popped node's children.  The shallow part of the node tables (the first
-// `ncache` entries, depth-major, where the splits happen) is copied to LDS
-// first, so most fetches never leave the CU.  Workgroups take levels in
-// `order` (largest feature targets first).
+// everything.  An entry is {count << 3 | children, first child} (a node
+// without children, at the deepest depth, carries its own index instead),
+// so a popped node's children are known without another lookup.
+//
+// The whole wave runs the heap in lockstep on uniform (scalar) values:
+// entries 0..63 live in lane i of two VGPRs (v_readlane / lane-select), the
+// rest in LDS (broadcast reads, lane 0 writes), and lanes 0..4 fetch the
+// counts and child ranges of a popped node's children together.  The
+// shallow part of the node tables (the first `ncache` entries, depth-major,
+// where the splits happen) is copied to LDS first.  Workgroups take levels
+// in `order` (largest feature targets first).  Output per keypoint slot:
+// the node's entry word (first child or own index) << 3 | children.
 struct NodeTabs {
     const uint32_t* start;
     const uint32_t* child;
@@ -806,9 +865,36 @@ struct NodeTabs {
     __device__ __forceinline__ uint32_t ch(uint32_t i) const { return i < ncache ? s_child[i] : child[i]; }
 };
 
+struct WaveHeap {
+    uint2* lds;
+    uint32_t rx, ry;  // entry i < 64: lane i of (rx, ry)
+    __device__ __forceinline__ uint2 get(int i) const {
+        if (i < 64) return make_uint2(__builtin_amdgcn_readlane(rx, i), __builtin_amdgcn_readlane(ry, i));
+        const uint2 v = lds[i];
+        return make_uint2(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y));
+    }
+    __device__ __forceinline__ void set(int i, uint2 v) {
+        if (i < 64) {
+            const bool mine = (int)threadIdx.x == i;
+            rx = mine ? v.x : rx;
+            ry = mine ? v.y : ry;
+        } else if (threadIdx.x == 0) {
+            lds[i] = v;
+        }
+    }
+};
+
+// heap entry of node c from start[c], start[c+1], child[c], child[c+1]
+__device__ __forceinline__ uint2 node_entry(uint32_t c, uint32_t s0, uint32_t s1, uint32_t c0, uint32_t c1) {
+    const uint32_t nch = c1 - c0;
+    return make_uint2(((s1 - s0) << 3) | nch, nch ? c0 : c);
+}
+
+__device__ __forceinline__ uint32_t lane_val(uint32_t v, int k) { return __builtin_amdgcn_readlane(v, k); }
+
 __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* order, int cap, int node_cap) {
-    extern __shared__ uint4 heap[];
-    uint32_t* s_start = reinterpret_cast<uint32_t*>(heap + cap);
+    extern __shared__ uint2 heap_lds[];
+    uint32_t* s_start = reinterpret_cast<uint32_t*>(heap_lds + cap);
     uint32_t* s_child = s_start + node_cap;
     const int lane = threadIdx.x;
     const int li = order[blockIdx.x];
@@ -829,6 +915,7 @@ __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* ord
         s_child[i] = nt.child[i];
     }
     wave_lds_sync();
+    WaveHeap H{heap_lds, 0u, 0u};
     const int nr = O->n_roots;
     int len = 0;
     {
@@ -837,50 +924,31 @@ __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* ord
             s_l = nt.st(lane);
             c_l = nt.ch(lane);
         }
-        for (int k = 0; k < nr; k++) {
-            const uint4 e = make_uint4(__shfl(s_l, k + 1) - __shfl(s_l, k), (uint32_t)k, __shfl(c_l, k),
-                                       __shfl(c_l, k + 1));
-            if (lane == 0) {
-                int l2 = len;
-                heap_push(heap, l2, e);
-            }
-            len++;
-        }
+        for (int k = 0; k < nr; k++)
+            heap_push(H, len, node_entry((uint32_t)k, lane_val(s_l, k), lane_val(s_l, k + 1), lane_val(c_l, k),
+                                         lane_val(c_l, k + 1)));
     }
     const int N = min(L.nfeat, cap - 8);
     while (len < N && len > 0) {
-        wave_lds_sync();  // lane 0's heap writes before every lane reads the top
-        const uint4 top = heap[0];
-        if (top.x == 1) break;
-        const uint32_t c0 = top.z, nch = top.w - top.z;
+        const uint2 top = H.get(0);
+        if ((top.x >> 3) == 1) break;
+        const uint32_t c0 = top.y, nch = top.x & 7u;
         uint32_t s_l = 0, c_l = 0;
         if (lane <= (int)nch) {
             s_l = nt.st(c0 + lane);
             c_l = nt.ch(c0 + lane);
         }
-        if (lane == 0) {
-            int l2 = len;
-            heap_pop(heap, l2);
-        }
-        len--;
-        for (uint32_t k = 0; k < nch; k++) {
-            const uint4 e = make_uint4(__shfl(s_l, k + 1) - __shfl(s_l, k), c0 + k, __shfl(c_l, k), __shfl(c_l, k + 1));
-            if (lane == 0) {
-                int l2 = len;
-                heap_push(heap, l2, e);
-            }
-            len++;
-        }
+        heap_pop(H, len);
+        for (uint32_t k = 0; k < nch; k++)
+            heap_push(H, len, node_entry(c0 + k, lane_val(s_l, k), lane_val(s_l, k + 1), lane_val(c_l, k),
+                                         lane_val(c_l, k + 1)));
     }
     int j = 0;
     while (len > 0) {
-        if (lane == 0) {
-            if (j < L.out_cap) g.nodes_out[L.kp_slot + j] = heap[0].y;
-            int l2 = len;
-            heap_pop(heap, l2);
-        }
+        const uint2 top = H.get(0);
+        if (lane == 0 && j < L.out_cap) g.nodes_out[L.kp_slot + j] = (top.y << 3) | (top.x & 7u);
         j++;
-        len--;
+        heap_pop(H, len);
     }
     if (lane == 0) O->nout = min(j, L.out_cap);
 }
@@ -951,10 +1019,13 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
     const bool seed = j >= nout;
     if (!seed) {
         const OctLevel* O = g.oct + li;
-        const uint32_t gid = g.nodes_out[L.kp_slot + j];
+        // node word: first child (or, without children, the node itself) << 3
+        // | children; the node's sorted range is [start[y], start[y + max(children, 1)])
+        const uint32_t nw = g.nodes_out[L.kp_slot + j];
+        const uint32_t ny = nw >> 3, nch = max(nw & 7u, 1u);
         const uint32_t* start = g.pool + O->tab;
         const uint32_t* sorted = g.pool + O->sorted;
-        const uint32_t s0 = start[gid], s1 = start[gid + 1];
+        const uint32_t s0 = start[ny], s1 = start[ny + nch];
         // key: response, then the lowest candidate index, then the record's
         // packed coordinates (the winner's position travels with the max)
         unsigned long long bestk = 0;
@@ -1002,24 +1073,48 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
         angle = sd[3];
         resp = sd[4];
     }
-    // rBRIEF on the blurred level
+    // rBRIEF on the blurred level.  Only the rounded pattern offsets
+    // cvRound(px * b + py * a), cvRound(px * a - py * b) depend on the
+    // rotation, so the correctly rounded base sin/cos (at most 1 ulp from the
+    // host libm's) gives the same offsets unless some product sum lies within
+    // 1e-5 of a rounding boundary (a 1-ulp change moves it by < 5e-6); the
+    // wave then redoes the offsets with the libm-patched values.
     const float factorPI = (float)(3.14159265358979323846 / 180.f);
+    const float xr = angle * factorPI;
     float a, b;
-    sincos_patched(angle * factorPI, tb, &a, &b);
+    sincos_base(fabsf(xr), &a, &b);
+    if (xr < 0.f) b = -b;
+    int oy[8], ox[8];
+    bool near = false;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const signed char* p = kOrbPattern[64 * (k >> 1) + lane] + 2 * (k & 1);
+        const float px = (float)p[0], py = (float)p[1];
+        const float vy = px * b + py * a, vx = px * a - py * b;
+        oy[k] = (int)rintf(vy);
+        ox[k] = (int)rintf(vx);
+        near = near || fabsf(fabsf(vy - truncf(vy)) - 0.5f) < 1e-5f || fabsf(fabsf(vx - truncf(vx)) - 0.5f) < 1e-5f;
+    }
+    if (__any(near)) {
+        sincos_patched(xr, tb, &a, &b);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const signed char* p = kOrbPattern[64 * (k >> 1) + lane] + 2 * (k & 1);
+            const float px = (float)p[0], py = (float)p[1];
+            oy[k] = (int)rintf(px * b + py * a);
+            ox[k] = (int)rintf(px * a - py * b);
+        }
+    }
     const uint8_t* bl = g.blur + L.pix;
     const int cx = (int)rintf(x), cy = (int)rintf(y);
     unsigned long long words[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-        const signed char* p = kOrbPattern[64 * r + lane];
         int t[2];
 #pragma unroll
         for (int q = 0; q < 2; q++) {
-            const float px = (float)p[2 * q], py = (float)p[2 * q + 1];
-            int yy = cy + (int)rintf(px * b + py * a);
-            int xx = cx + (int)rintf(px * a - py * b);
-            yy = min(max(yy, 0), L.h - 1);
-            xx = min(max(xx, 0), L.w - 1);
+            const int yy = min(max(cy + oy[2 * r + q], 0), L.h - 1);
+            const int xx = min(max(cx + ox[2 * r + q], 0), L.w - 1);
             t[q] = bl[(long long)yy * L.pitch + xx];
         }
         words[r] = __ballot(t[0] < t[1]);

@@ -299,3 +299,41 @@ def ba_problem(seed: int = 30, n_kf: int = 10, n_points: int = 20000, target_obs
     return BAProblem(uv[order].astype(np.float32), pid[order].astype(np.int32), fid[order].astype(np.int32),
                      cid[order].astype(np.int32), level[order], isig[order].astype(np.float32), est,
                      frames, cams, fixed)
+
+
+@dataclass
+class StereoProblem:
+    obs: np.ndarray         # (M, 3) f32: u, v, u_right
+    obs_pt: np.ndarray      # (M,) i32
+    obs_frame: np.ndarray   # (M,) i32
+    obs_isig: np.ndarray    # (M,) f32
+    points: np.ndarray      # (P, 3) f64
+    frames: np.ndarray      # (F, 7) f64 camera (= body) pose Tcw
+    cam: np.ndarray         # (6,) f64: fx fy cx cy bf delta
+
+
+def stereo_problem(seed: int = 50, n_kf: int = 6, n_points: int = 4000, target_obs: int = 30000,
+                   outlier_frac: float = 0.05) -> StereoProblem:
+    """Left-camera observations of ba_problem (the left camera is the body)
+    with a right-image coordinate u_r = u - bf / z (baseline 1.0 m, the R
+    camera of the rig), N(0, 1) px noise on u_r too, and outlier_frac of the
+    u_r replaced by uniform values; Huber delta sqrt(7.815) as thHuber3D."""
+    prob = ba_problem(seed=seed, n_kf=n_kf, n_points=n_points, target_obs=target_obs * 3,
+                      outlier_frac=outlier_frac)
+    rng = _rng(seed + 1)
+    sel = np.nonzero(prob.obs_cam == 0)[0][:target_obs]
+    fx, fy, cx, cy = prob.cams[0, :4]
+    bf = float(np.float32(fx * 1.0))
+    pts = prob.points[prob.obs_pt[sel]]
+    z = np.empty(len(sel))
+    for k, i in enumerate(sel):
+        f = prob.frames[prob.obs_frame[i]]
+        R = _quat_to_R(f[:4] / np.linalg.norm(f[:4]))
+        z[k] = (R @ pts[k] + f[4:])[2]
+    ur = prob.obs_uv[sel, 0].astype(np.float64) - bf / z + rng.normal(0, 1.0, len(sel))
+    n_out = int(round(outlier_frac * len(sel)))
+    ur[rng.choice(len(sel), size=n_out, replace=False)] = rng.uniform(0, NARROW_W, n_out)
+    obs = np.concatenate([prob.obs_uv[sel].astype(np.float64), ur[:, None]], 1).astype(np.float32)
+    delta = float(np.float32(math.sqrt(7.815)))
+    return StereoProblem(obs, prob.obs_pt[sel], prob.obs_frame[sel], prob.obs_isig[sel], prob.points,
+                         prob.frames, np.array([fx, fy, cx, cy, bf, delta], np.float64))

@@ -40,6 +40,8 @@ _SIG = {
     "orc_window": (I, [P, P, I, I, I, F, F, F, I, I, P, I]),
     "orc_project_search": (None, [P, P, P, I, P, P, P, P, F, F, I, P, P, P, I, I, I, P]),
     "orc_ba_linearize": (None, [P, P, P, P, P, I, P, P, P, P, P, P, P, P]),
+    "orc_ba_stereo_linearize": (None, [P, P, P, P, I, P, P, P, P, P, P, P, P]),
+    "orc_ba_outliers": (None, [P, I, F, P]),
     "orc_ba_schur": (I, [P, P, P, P, P, P, P, I, I, I, P, D, P, P, P, P, P, P]),
 }
 
@@ -269,6 +271,34 @@ def ba_linearize(prob):
                            _p(np.ascontiguousarray(prob.cams, np.float64)),
                            _p(e), _p(chi2), _p(w), _p(Jp), _p(Jx))
     return e, chi2, w, Jp.reshape(M, 2, 6), Jx.reshape(M, 2, 3)
+
+
+def ba_stereo_linearize(obs, obs_pt, obs_frame, obs_isig, points, frames, cam):
+    """EdgeStereoSE3ProjectXYZ for M edges; cam = (fx, fy, cx, cy, bf, delta).
+    Returns e (M x 3), chi2, w, J_pose (M x 3 x 6), J_point (M x 3 x 3)."""
+    M = len(obs_pt)
+    e = np.zeros((M, 3))
+    chi2 = np.zeros(M)
+    w = np.zeros(M)
+    Jp = np.zeros((M, 18))
+    Jx = np.zeros((M, 9))
+    lib().orc_ba_stereo_linearize(_p(np.ascontiguousarray(obs, np.float32)),
+                                  _p(np.ascontiguousarray(obs_pt, np.int32)),
+                                  _p(np.ascontiguousarray(obs_frame, np.int32)),
+                                  _p(np.ascontiguousarray(obs_isig, np.float32)), M,
+                                  _p(np.ascontiguousarray(points, np.float64)),
+                                  _p(np.ascontiguousarray(frames, np.float64)),
+                                  _p(np.ascontiguousarray(cam, np.float64)),
+                                  _p(e), _p(chi2), _p(w), _p(Jp), _p(Jx))
+    return e, chi2, w, Jp.reshape(M, 3, 6), Jx.reshape(M, 3, 3)
+
+
+def ba_outliers(chi2, threshold):
+    """FilterCallBack_Chi2's test: float(chi2) >= threshold."""
+    chi2 = np.ascontiguousarray(chi2, np.float64)
+    out = np.zeros(len(chi2), np.uint8)
+    lib().orc_ba_outliers(_p(chi2), len(chi2), ctypes.c_float(threshold), _p(out))
+    return out
 
 
 def ba_schur(e, w, Jpose, Jpoint, obs_pt, obs_frame, obs_isig, P, F, frame_fixed, lam=0.0):

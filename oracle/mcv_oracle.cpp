@@ -832,6 +832,61 @@ static void edge_linearize(const SE3& Tbw, const Cam& c, const double Pw[3], dou
         }
 }
 
+// Stereo edge (EdgeStereoSE3ProjectXYZ, include/OptimizeTypes.hpp:119-157,
+// src/OptimizeTypes.cpp:12-58): pose vertex Tcw (the keyframe pose; the left
+// camera is the body), residual obs(u, v, u_r) - cam_project_uv_u(P_c, bf),
+// where cam_project_uv_u takes a float inverse depth and a float bf:
+//   invz = (float)(1.0 / z), u = x * invz * fx + cx, v = y * invz * fy + cy,
+//   u_r = u - (float)(bf_f * invz)     (bf_f = (float)bf, a float product)
+// J_point 3x3 and J_pose 3x6 (rotation first) in FP64 from z, as in
+// linearizeOplus.  Information I * invSigma2, Huber on chi2.
+static void stereo_edge_linearize(const SE3& Tcw, double fx, double fy, double cx, double cy, double bf,
+                                  double delta, const double Pw[3], double u, double v, double ur, double invSigma2,
+                                  double* e, double* chi2, double* w, double* Jpose, double* Jpoint) {
+    double P[3];
+    se3map(Tcw, Pw, P);
+    const float invz = (float)(1.0 / P[2]);
+    const double ru = P[0] * (double)invz * fx + cx;
+    const double rv = P[1] * (double)invz * fy + cy;
+    const float bff = (float)bf;
+    const double rr = ru - (double)(bff * invz);
+    e[0] = u - ru;
+    e[1] = v - rv;
+    e[2] = ur - rr;
+    const double ch = invSigma2 * (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    *chi2 = ch;
+    *w = (ch <= delta * delta) ? 1.0 : delta / std::sqrt(ch);
+    double R[9];
+    qtoR(Tcw.r, R);
+    const double x = P[0], y = P[1], z = P[2], z2 = z * z;
+    for (int k = 0; k < 3; k++) {
+        Jpoint[k] = -fx * R[k] / z + fx * x * R[6 + k] / z2;
+        Jpoint[3 + k] = -fy * R[3 + k] / z + fy * y * R[6 + k] / z2;
+        Jpoint[6 + k] = Jpoint[k] - bf * R[6 + k] / z2;
+    }
+    double* J0 = Jpose;
+    double* J1 = Jpose + 6;
+    double* J2 = Jpose + 12;
+    J0[0] = x * y / z2 * fx;
+    J0[1] = -(1 + (x * x / z2)) * fx;
+    J0[2] = y / z * fx;
+    J0[3] = -1. / z * fx;
+    J0[4] = 0;
+    J0[5] = x / z2 * fx;
+    J1[0] = (1 + y * y / z2) * fy;
+    J1[1] = -x * y / z2 * fy;
+    J1[2] = -x / z * fy;
+    J1[3] = 0;
+    J1[4] = -1. / z * fy;
+    J1[5] = y / z2 * fy;
+    J2[0] = J0[0] - bf * y / z2;
+    J2[1] = J0[1] + bf * x / z2;
+    J2[2] = J0[2];
+    J2[3] = J0[3];
+    J2[4] = 0;
+    J2[5] = J0[5] - bf / z2;
+}
+
 }  // namespace orc
 
 // ===========================================================================
@@ -1181,6 +1236,29 @@ void orc_ba_linearize(const float* obs_uv, const int32_t* obs_pt, const int32_t*
         edge_linearize(Tbw, cam, points + 3 * obs_pt[i], (double)obs_uv[2 * i], (double)obs_uv[2 * i + 1],
                        (double)obs_isig[i], e + 2 * i, chi2 + i, w + i, Jpose + 12 * i, Jpoint + 6 * i);
     }
+}
+
+// Stereo edges: obs (M x 3 f32: u, v, u_right), point, frame, inverse level
+// sigma^2; frames F x 7 (qx qy qz qw tx ty tz, camera = body pose); cam
+// (fx, fy, cx, cy, bf, delta).  Outputs e (M x 3), chi2, w, Jpose (M x 18),
+// Jpoint (M x 9).
+void orc_ba_stereo_linearize(const float* obs, const int32_t* obs_pt, const int32_t* obs_frame, const float* obs_isig,
+                             int M, const double* points, const double* frames, const double* cam, double* e,
+                             double* chi2, double* w, double* Jpose, double* Jpoint) {
+    for (int i = 0; i < M; i++) {
+        const double* f = frames + 7 * obs_frame[i];
+        SE3 Tcw{Quat{f[0], f[1], f[2], f[3]}, {f[4], f[5], f[6]}};
+        qnormalize_pos(Tcw.r);  // SE3Quat keeps a unit quaternion with w >= 0
+        stereo_edge_linearize(Tcw, cam[0], cam[1], cam[2], cam[3], cam[4], cam[5], points + 3 * obs_pt[i],
+                              (double)obs[3 * i], (double)obs[3 * i + 1], (double)obs[3 * i + 2], (double)obs_isig[i],
+                              e + 3 * i, chi2 + i, w + i, Jpose + 18 * i, Jpoint + 9 * i);
+    }
+}
+
+// chi2 outlier test of FilterCallBack_Chi2 (src/PoseEstimation.cpp:132-151):
+// flag = chi2 >= threshold (float compare of the float chi2).
+void orc_ba_outliers(const double* chi2, int M, float threshold, uint8_t* flags) {
+    for (int i = 0; i < M; i++) flags[i] = ((float)chi2[i] >= threshold) ? 1 : 0;
 }
 
 // FP64 quadratic form + Schur complement (block_solver.hpp:315-449 with the

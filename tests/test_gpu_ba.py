@@ -61,6 +61,26 @@ def test_ba_reference_huber_deltas(gpu_ctx):
     check_against_oracle(prob, b.results())
 
 
+def test_ba_plain_equals_batched(gpu_ctx):
+    """One launch per (frame, camera) group gives the same bits as the
+    all-edges launch (the pair table comes from LDS in one, from global
+    memory in the other)."""
+    prob = synth.ba_problem(seed=34, n_kf=5, n_points=4000, target_obs=30000)
+    b = BALinearizer.from_problem(prob, ctx=gpu_ctx)
+    b.linearize()
+    batched = b.read()
+    b.linearize_plain()
+    np.testing.assert_array_equal(b.read(), batched)
+
+
+def test_ba_many_pairs(gpu_ctx):
+    """More (frame, camera) pairs than the LDS copy holds (70 x 3 = 210)."""
+    prob = synth.ba_problem(seed=35, n_kf=70, n_points=6000, target_obs=40000)
+    b = BALinearizer.from_problem(prob, ctx=gpu_ctx)
+    b.linearize()
+    check_against_oracle(prob, b.results())
+
+
 def test_ba_residual_only(gpu_ctx):
     prob = synth.ba_problem(seed=33, n_kf=3, n_points=1000, target_obs=8000)
     b = BALinearizer.from_problem(prob, ctx=gpu_ctx)
@@ -92,3 +112,42 @@ def test_schur_vs_fp64(gpu_ctx, lam):
         xr = np.linalg.solve(r["S"], r["bs"])
         assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-3
     b.close()
+
+
+def test_stereo_edge_vs_oracle(gpu_ctx):
+    """EdgeStereoSE3ProjectXYZ on the GPU against the FP64 oracle: residual,
+    chi2, Huber weight and the 3x6 / 3x3 Jacobians within 1e-5."""
+    sp = synth.stereo_problem(seed=54, n_kf=8, n_points=6000, target_obs=20000)
+    base = synth.ba_problem(seed=54, n_kf=8, n_points=6000, target_obs=60000, outlier_frac=0.05)
+    b = BALinearizer.from_problem(base, ctx=gpu_ctx)
+    b.set_stereo(sp.obs, sp.obs_pt, sp.obs_frame, sp.obs_isig, sp.cam)
+    b.linearize_stereo()
+    e, chi2, w, Jp, Jx = b.stereo_results()
+    re, rchi2, rw, rJp, rJx = oracle.ba_stereo_linearize(sp.obs, sp.obs_pt, sp.obs_frame, sp.obs_isig, sp.points,
+                                                         sp.frames, sp.cam)
+    assert rel_rows(e, re, 1e-9).max() < TOL
+    assert (np.abs(chi2 - rchi2) / np.maximum(rchi2, 1e-12)).max() < TOL
+    d2 = sp.cam[5] ** 2
+    ok = np.abs(rchi2 - d2) > 1e-6 * d2
+    assert (np.abs(w - rw)[ok] / rw[ok]).max() < TOL
+    assert (w < 1).sum() > 0
+    for r in range(3):
+        assert rel_rows(Jp[:, r], rJp[:, r]).max() < TOL, f"J_pose row {r}"
+        assert rel_rows(Jx[:, r], rJx[:, r]).max() < TOL, f"J_point row {r}"
+
+
+def test_outlier_flags(gpu_ctx):
+    """chi2 >= threshold on the GPU's chi2 planes, and the oracle's flags
+    wherever chi2 is not within 1e-5 of the threshold."""
+    prob = synth.ba_problem(seed=55, n_kf=5, n_points=3000, target_obs=20000)
+    b = BALinearizer.from_problem(prob, ctx=gpu_ctx)
+    b.linearize()
+    chi2 = b.results()[1]
+    thr = float(np.float32(5.991))
+    fm, _ = b.outliers(thr, 7.815)
+    np.testing.assert_array_equal(fm, (chi2 >= np.float32(thr)).astype(np.uint8))
+    _, rchi2, _, _, _ = oracle.ba_linearize(prob)
+    want = oracle.ba_outliers(rchi2, thr)
+    far = np.abs(rchi2 - thr) > 1e-5 * thr
+    np.testing.assert_array_equal(fm[far], want[far])
+    assert 0 < fm.sum() < len(fm)

@@ -21,6 +21,22 @@ def test_knn2_random(gpu_ctx, nq, nt):
     np.testing.assert_array_equal(got, oracle.knn2(q, t))
 
 
+@pytest.mark.parametrize("variant", range(5))
+@pytest.mark.parametrize("nq,nt", [(3, 1), (65, 5), (130, 257), (700, 1531)])
+def test_knn2_every_variant(gpu_ctx, variant, nq, nt):
+    """All launch variants the autotuner chooses from return the oracle's rows
+    (ragged query and train counts against the variants' block shapes)."""
+    rng = np.random.default_rng(10 * nq + nt + variant)
+    q, t = _desc(rng, nq), _desc(rng, nt)
+    t[: nt // 3] = t[nt // 3: 2 * (nt // 3)]          # duplicates: scan-order ties
+    try:
+        gpu_ctx.set_match_variant(variant)
+        got = Matcher.KnnMatch(q, t, ctx=gpu_ctx).rows
+    finally:
+        gpu_ctx.set_match_variant(0)
+    np.testing.assert_array_equal(got, oracle.knn2(q, t))
+
+
 def test_knn2_ties_keep_scan_order(gpu_ctx):
     """Duplicated trains: the first occurrence wins (strict '<' in order)."""
     rng = np.random.default_rng(3)

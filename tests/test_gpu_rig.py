@@ -78,3 +78,19 @@ def test_rig_uneven_feature_counts(orb_default):
         for f in ("x", "y", "size", "angle", "response", "octave"):
             np.testing.assert_array_equal(kps[i, :counts[i]][f], rk[f])
         np.testing.assert_array_equal(desc[i, :counts[i]], rd)
+
+
+def test_rig_autotune_keeps_outputs(orb_default):
+    """The autotuner only changes launch shapes: outputs after tuning (graph
+    re-captured with the chosen variants) equal the untuned ones."""
+    import torch
+    frames = [synth.rig_frame(500 + 10 * f) for f in range(2)]
+    imgs = [torch.from_numpy(im).cuda() for fr in frames for im in fr]
+    ptrs = [im.data_ptr() for im in imgs]
+    rig = RigBatch(orb_default, 2, ratio=0.7, th=50)
+    rig.run(ptrs, graph=True)
+    before = rig.read()
+    report = rig.autotune(ptrs, reps=1)
+    assert "chosen:" in report and report.count("k_knn2") >= 5
+    rig.run(ptrs, graph=True)
+    assert_same_outputs(before, rig.read())

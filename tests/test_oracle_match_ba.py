@@ -255,3 +255,90 @@ def test_stereo_known_disparity():
     # octave-0 pairs match with SAD 0 here, below 0.4 x the median SAD, so the
     # median filter of the reference (src/Frame.cpp:307-323) drops them
     assert not (ok & (kl["octave"] == 0)).any()
+
+
+# ------------------------------------------------------------ stereo edge
+def _stereo_mono_problem(sp):
+    """The same observations as a mono-edge problem with Tcb = I."""
+    M = len(sp.obs_pt)
+    fx, fy, cx, cy, bf, delta = sp.cam
+    cams = np.array([[fx, fy, cx, cy, 0, 0, 0, 1, 0, 0, 0, delta]], np.float64)
+    return synth.BAProblem(sp.obs[:, :2].copy(), sp.obs_pt, sp.obs_frame, np.zeros(M, np.int32),
+                           np.zeros(M, np.int32), sp.obs_isig, sp.points, sp.frames, cams,
+                           np.zeros(len(sp.frames), np.int32))
+
+
+def test_stereo_edge_rows_01_are_the_mono_edge():
+    """Rows u, v of EdgeStereoSE3ProjectXYZ's Jacobians (src/OptimizeTypes.cpp:12-58)
+    equal EdgeSE3XYZ2Camera's with Tcb = I (a separate code path); the u, v
+    residuals differ only through the float inverse depth (< 1e-3 px)."""
+    sp = synth.stereo_problem(seed=51)
+    e, chi2, w, Jp, Jx = oracle.ba_stereo_linearize(sp.obs, sp.obs_pt, sp.obs_frame, sp.obs_isig, sp.points,
+                                                    sp.frames, sp.cam)
+    me, _, _, mJp, mJx = oracle.ba_linearize(_stereo_mono_problem(sp))
+    np.testing.assert_allclose(Jp[:, :2], mJp, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(Jx[:, :2], mJx, rtol=1e-9, atol=1e-9)
+    assert np.abs(e[:, :2] - me).max() < 1e-3
+    assert np.abs(e[:, :2] - me).max() > 0          # the float invz is visible
+
+
+def test_stereo_edge_residual_closed_form():
+    """cam_project_uv_u (include/OptimizeTypes.hpp:143-152): float invz,
+    float bf * invz product, then double arithmetic."""
+    sp = synth.stereo_problem(seed=52)
+    e, chi2, w, _, _ = oracle.ba_stereo_linearize(sp.obs, sp.obs_pt, sp.obs_frame, sp.obs_isig, sp.points,
+                                                  sp.frames, sp.cam)
+    fx, fy, cx, cy, bf, delta = sp.cam
+    for i in range(0, len(sp.obs_pt), 37):
+        f = sp.frames[sp.obs_frame[i]]
+        R = _quat_to_R(f[:4] / np.linalg.norm(f[:4]))
+        x, y, z = R @ sp.points[sp.obs_pt[i]] + f[4:]
+        invz = np.float32(1.0 / z)
+        u = x * float(invz) * fx + cx
+        v = y * float(invz) * fy + cy
+        ur = u - float(np.float32(np.float32(bf) * invz))
+        want = np.array([sp.obs[i, 0], sp.obs[i, 1], sp.obs[i, 2]], np.float64) - [u, v, ur]
+        np.testing.assert_allclose(e[i], want, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(chi2, sp.obs_isig * (e ** 2).sum(1), rtol=1e-12)
+    d = sp.cam[5]
+    np.testing.assert_allclose(w, np.where(chi2 <= d * d, 1.0, d / np.sqrt(chi2)), rtol=1e-12)
+
+
+def test_stereo_edge_row2_central_differences():
+    """Third row against central differences of the exact (double) right-image
+    residual u_r - (fx x / z + cx - bf / z), h = 1e-4."""
+    sp = synth.stereo_problem(seed=53)
+    _, _, _, Jp, Jx = oracle.ba_stereo_linearize(sp.obs, sp.obs_pt, sp.obs_frame, sp.obs_isig, sp.points,
+                                                 sp.frames, sp.cam)
+    fx, fy, cx, cy, bf, _ = sp.cam
+
+    def er(R, t, P):
+        x, y, z = R @ P + t
+        return -(fx * x / z + cx - bf / z)
+    h = 1e-4
+    for i in range(0, len(sp.obs_pt), 97):
+        f = sp.frames[sp.obs_frame[i]]
+        R = _quat_to_R(f[:4] / np.linalg.norm(f[:4]))
+        t, P = f[4:], sp.points[sp.obs_pt[i]]
+        for k in range(6):
+            xi = np.zeros(6)
+            xi[k] = h
+            dRp, dtp = _se3_exp(xi)
+            dRm, dtm = _se3_exp(-xi)
+            fd = (er(dRp @ R, dRp @ t + dtp, P) - er(dRm @ R, dRm @ t + dtm, P)) / (2 * h)
+            np.testing.assert_allclose(Jp[i, 2, k], fd, rtol=1e-5, atol=1e-6 * np.abs(Jp[i, 2]).max())
+        for k in range(3):
+            d = np.zeros(3)
+            d[k] = h
+            fd = (er(R, t, P + d) - er(R, t, P - d)) / (2 * h)
+            np.testing.assert_allclose(Jx[i, 2, k], fd, rtol=1e-5, atol=1e-6 * np.abs(Jx[i, 2]).max())
+
+
+def test_outlier_flags():
+    """FilterCallBack_Chi2 (src/PoseEstimation.cpp:132-151): float(chi2) >=
+    threshold, with 5.991 (mono) and 7.815 (stereo) as floats."""
+    chi2 = np.array([0.0, 5.990999, 5.991, float(np.float32(5.991)), 5.9910002, 7.815, 1e9])
+    got = oracle.ba_outliers(chi2, float(np.float32(5.991)))
+    want = (chi2.astype(np.float32) >= np.float32(5.991)).astype(np.uint8)
+    np.testing.assert_array_equal(got, want)
+    assert got.tolist() == [0, 0, 1, 1, 1, 1, 1]

@@ -89,6 +89,21 @@ int mcv_orb_extract_batch_dev(mcv_ctx* ctx, int n, const uint8_t* const* d_image
  * per query row (trainIdx0, distance0, trainIdx1, distance1), sentinel
  * (0, 999) when fewer than two trains exist.  q: nq x 32, t: nt x 32 (host). */
 int mcv_match_knn2(mcv_ctx* ctx, const uint8_t* q, int nq, const uint8_t* t, int nt, int32_t* out);
+/* MatchRes::FilterOrientation (src/Matcher.cpp:44-75): matches n x (query,
+ * train, distance) (n <= 4096, host), angle1[query] and angle2[train] the
+ * keypoint angles (degrees).  out receives the matches of the three largest
+ * of 40 rotation bins, bins in the order std::sort leaves them (libstdc++'s
+ * tie order), list order within a bin; *out_n their count. */
+int mcv_match_filter_orientation(mcv_ctx* ctx, const int32_t* matches, int n, const float* angle1, int n1,
+                                 const float* angle2, int n2, int32_t* out, int32_t* out_n);
+/* MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cpp:101-150) for n
+ * map points at once: the observed descriptors of map point m are rows
+ * ptr[m] .. ptr[m+1] of desc (32 B each, host, at most 256 per map point);
+ * best[m] = index (within the map point's rows) of the descriptor with the
+ * least median distance to the others (first on ties), -1 when it has none;
+ * out_desc (optional, n x 32) receives that descriptor. */
+int mcv_distinctive_descriptors(mcv_ctx* ctx, const uint8_t* desc, const int32_t* ptr, int n, int32_t* best,
+                                uint8_t* out_desc);
 /* Launch variant of the top-2 kernel used by mcv_match_knn2* on this context
  * (0 .. 4; no counterpart in the reference: the autotuner's choice, also
  * settable for testing).  Every variant returns the same rows. */

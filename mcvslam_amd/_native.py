@@ -47,6 +47,8 @@ SIGNATURES = {
     "mcv_orb_extract_batch_dev": (I, [P, I, P, P, P, P, P, P, I, P, I]),
     "mcv_match_knn2": (I, [P, P, I, P, I, P]),
     "mcv_match_set_variant": (I, [P, I]),
+    "mcv_distinctive_descriptors": (I, [P, P, P, I, P, P]),
+    "mcv_match_filter_orientation": (I, [P, P, I, P, I, P, I, P, P]),
     "mcv_rig_autotune": (I, [P, P, I, P, I]),
     "mcv_match_knn2_dev": (I, [P, P, I, P, I, P]),
     "mcv_match_filter": (I, [P, P, I, F, I, P, P]),

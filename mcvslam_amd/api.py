@@ -335,6 +335,34 @@ class RigBatch:
         return kp_records(kps), desc, counts, matches, mcounts, knn
 
 
+def filter_orientation(matches, angles1, angles2, ctx: Context | None = None):
+    """MatchRes::FilterOrientation: rows (query, train, distance) of the three
+    largest rotation bins."""
+    ctx = ctx or default_context()
+    m = np.ascontiguousarray(matches, np.int32).reshape(-1, 3)
+    a1 = np.ascontiguousarray(angles1, np.float32)
+    a2 = np.ascontiguousarray(angles2, np.float32)
+    out = np.zeros_like(m)
+    k = ctypes.c_int32()
+    check(ctx.h, lib().mcv_match_filter_orientation(ctx.h, _p(m), len(m), _p(a1), len(a1), _p(a2), len(a2),
+                                                    _p(out), ctypes.byref(k)))
+    return out[:k.value]
+
+
+def distinctive_descriptors(desc, ptr, ctx: Context | None = None):
+    """MapPoint::ComputeDistinctiveDescriptors for many map points: desc
+    (total x 32 u8), ptr (n + 1) CSR offsets -> (best index per map point,
+    chosen descriptors n x 32)."""
+    ctx = ctx or default_context()
+    desc = np.ascontiguousarray(desc, np.uint8)
+    ptr = np.ascontiguousarray(ptr, np.int32)
+    n = len(ptr) - 1
+    best = np.zeros(n, np.int32)
+    out = np.zeros((n, 32), np.uint8)
+    check(ctx.h, lib().mcv_distinctive_descriptors(ctx.h, _p(desc), _p(ptr), n, _p(best), _p(out)))
+    return best, out
+
+
 class BALinearizer:
     """Batched linearization of EdgeSE3XYZ2Camera edges (see mcvslam_gpu.h)."""
 

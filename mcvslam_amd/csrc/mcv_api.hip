@@ -503,7 +503,18 @@ struct OrbPlan {
         cap = max_slots_img;
         heap_cap = 16;
         for (const LevelDesc& d : lv) heap_cap = std::max(heap_cap, std::max(d.nfeat, d.n_ini) + 16);
-        node_cap = std::max(0, std::min(4096, (65536 - 8 * heap_cap) / 8));
+        {
+            // every level's heap workgroup should be resident at once: split
+            // the CU's 160 KB of LDS over the workgroups each CU must hold
+            int dev = 0, ncu = 256;
+            if (hipGetDevice(&dev) == hipSuccess &&
+                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+                ncu = 256;
+            const int per_cu = std::max(1, ((int)lv.size() + ncu - 1) / std::max(ncu, 1));
+            // (LDS is allocated in blocks: keep 1 KB of margin per workgroup)
+            const int budget = std::min(65536, (160 * 1024 / per_cu) / 1024 * 1024 - 1024);
+            node_cap = std::max(0, std::min(4096, (budget - 8 * heap_cap) / 8));
+        }
         heap_order.resize(lv.size());
         for (size_t k = 0; k < lv.size(); k++) heap_order[k] = (int)k;
         std::stable_sort(heap_order.begin(), heap_order.end(),
@@ -668,7 +679,7 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
     }
     hipLaunchKernelGGL(k_octree_build, dim3(lv.size()), dim3(kOctThreads), 0, s, g);
     mark("k_octree_build");
-    hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), sizeof(uint2) * heap_cap + 8 * node_cap, s, g,
+    hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), 8 * (size_t)heap_cap + 8 * node_cap, s, g,
                        d_heap_order.as<int>(), heap_cap, node_cap);
     mark("k_octree_heap");
     hipLaunchKernelGGL(k_keypoints, dim3((n_slots + 3) / 4), dim3(256), 0, s, g, d_slot.as<int>(), n_slots, tb,
@@ -696,6 +707,8 @@ struct mcv_ctx {
     std::vector<int> single_key_seeds;
     DevBuf in_img, out_kps, out_desc, out_cnt;
     DevBuf m_q, m_t, m_out, m_pairs, f_knn, f_out, f_cnt, f_scr, f_jobs;
+    DevBuf d_desc_in, d_ptr, d_best, d_desc_out;
+    DevBuf o_m, o_a1, o_a2, o_out, o_cnt, o_jobs;
     DevBuf w_pts, w_pdesc, w_plev, w_kxy, w_kdesc, w_cs, w_ci, w_out, w_view, w_cams;
     std::map<std::string, std::unique_ptr<OrbPlan>> plans;
 
@@ -1024,6 +1037,72 @@ int mcv_match_knn2_dev(mcv_ctx* c, const uint8_t* d_q, int nq, const uint8_t* d_
     c->set_device();
     MatchPair p{d_q, d_t, nullptr, nullptr, nq, nt, d_out};
     launch_knn(c->stream, c->m_pairs, {p}, nq, c->knn_variant);
+    API_END
+}
+
+int mcv_distinctive_descriptors(mcv_ctx* c, const uint8_t* desc, const int32_t* ptr, int n, int32_t* best,
+                                uint8_t* out_desc) {
+    API_BEGIN(c)
+    REQUIRE(c && ptr && best && n >= 0, MCV_ERR_INVALID, "bad distinctive-descriptor arguments");
+    if (n == 0) return MCV_OK;
+    REQUIRE(ptr[0] == 0, MCV_ERR_INVALID, "ptr must start at 0");
+    for (int m = 0; m < n; m++)
+        REQUIRE(ptr[m + 1] >= ptr[m] && ptr[m + 1] - ptr[m] <= kDistinctMax, MCV_ERR_INVALID,
+                "descriptor lists must be ordered and hold at most 256 rows");
+    const int total = ptr[n];
+    REQUIRE(total == 0 || desc, MCV_ERR_INVALID, "null descriptors");
+    c->set_device();
+    hipStream_t s = c->stream;
+    c->d_desc_in.ensure((size_t)std::max(total, 1) * 32);
+    c->d_ptr.ensure((size_t)(n + 1) * 4);
+    c->d_best.ensure((size_t)n * 4);
+    c->d_desc_out.ensure((size_t)n * 32);
+    if (total) HIP_OK(hipMemcpyAsync(c->d_desc_in.p, desc, (size_t)total * 32, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(c->d_ptr.p, ptr, (size_t)(n + 1) * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_distinctive, dim3(n), dim3(64), 0, s, c->d_desc_in.as<uint8_t>(), c->d_ptr.as<int>(), n,
+                       c->d_best.as<int>(), out_desc ? c->d_desc_out.as<uint8_t>() : nullptr);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(best, c->d_best.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    if (out_desc) HIP_OK(hipMemcpyAsync(out_desc, c->d_desc_out.p, (size_t)n * 32, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    API_END
+}
+
+int mcv_match_filter_orientation(mcv_ctx* c, const int32_t* matches, int n, const float* angle1, int n1,
+                                 const float* angle2, int n2, int32_t* out, int32_t* out_n) {
+    API_BEGIN(c)
+    REQUIRE(c && out_n && n >= 0 && n <= kOrientMax, MCV_ERR_INVALID, "bad orientation-filter arguments");
+    if (n == 0) {
+        *out_n = 0;
+        return MCV_OK;
+    }
+    REQUIRE(matches && angle1 && angle2 && out, MCV_ERR_INVALID, "null orientation-filter arrays");
+    for (int i = 0; i < n; i++)
+        REQUIRE(matches[3 * i] >= 0 && matches[3 * i] < n1 && matches[3 * i + 1] >= 0 && matches[3 * i + 1] < n2,
+                MCV_ERR_INVALID, "match index out of range");
+    c->set_device();
+    hipStream_t s = c->stream;
+    c->o_m.ensure((size_t)n * 12);
+    c->o_a1.ensure((size_t)std::max(n1, 1) * 4);
+    c->o_a2.ensure((size_t)std::max(n2, 1) * 4);
+    c->o_out.ensure((size_t)n * 12);
+    c->o_cnt.ensure(4);
+    HIP_OK(hipMemcpyAsync(c->o_m.p, matches, (size_t)n * 12, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(c->o_a1.p, angle1, (size_t)n1 * 4, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(c->o_a2.p, angle2, (size_t)n2 * 4, hipMemcpyHostToDevice, s));
+    std::vector<OrientJob> jobs{OrientJob{c->o_m.as<int32_t>(), n, c->o_a1.as<float>(), c->o_a2.as<float>(),
+                                          c->o_out.as<int32_t>(), c->o_cnt.as<int>()}};
+    upload(c->o_jobs, jobs, s);
+    hipLaunchKernelGGL(k_filter_orientation, dim3(1), dim3(256), 0, s, c->o_jobs.as<OrientJob>());
+    HIP_OK(hipGetLastError());
+    int k = 0;
+    HIP_OK(hipMemcpyAsync(&k, c->o_cnt.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (k > 0) {
+        HIP_OK(hipMemcpyAsync(out, c->o_out.p, (size_t)k * 12, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+    }
+    *out_n = k;
     API_END
 }
 

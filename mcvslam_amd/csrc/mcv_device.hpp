@@ -13,6 +13,36 @@
 
 namespace mcv {
 
+#if defined(__HIPCC__)
+// Orders this wave's LDS accesses across its lanes (wave-private LDS areas).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Ascending bitonic sort of n2 (a power of two) keys by the whole workgroup.
+template <typename T>
+__device__ void bitonic_sort(T* a, int n2) {
+    for (int k = 2; k <= n2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const T x = a[i], y = a[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((x > y) == up) {
+                        a[i] = y;
+                        a[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+#endif
+
 // ---------------------------------------------------------------------------
 // std::priority_queue<T, vector<T>, Less> emulation on a plain array, with the
 // exact sift order of libstdc++'s push_heap / pop_heap (bits/stl_heap.h:
@@ -76,6 +106,156 @@ MCV_HD void heap_pop(H& h, int& len) {
         heap_adjust(h, 0, last, h.get(last));
     }
     len--;
+}
+
+// ---------------------------------------------------------------------------
+// std::sort(first, last, comp) with libstdc++'s exact element moves
+// (bits/stl_algo.h: __introsort_loop with threshold 16 and depth limit
+// 2 floor(log2 n), median-of-three to first, unguarded partition, heap sort
+// at depth 0, final insertion sort), so that elements comparing equal end in
+// the same order as with the reference's unstable sort.  `less(a, b)` is the
+// comparator; one thread runs it on a small array.
+template <typename T, typename Less>
+MCV_HD void stl_unguarded_linear_insert(T* last, Less less) {
+    T val = *last;
+    T* next = last - 1;
+    while (less(val, *next)) {
+        *last = *next;
+        last = next;
+        --next;
+    }
+    *last = val;
+}
+
+template <typename T, typename Less>
+MCV_HD void stl_insertion_sort(T* first, T* last, Less less) {
+    if (first == last) return;
+    for (T* i = first + 1; i != last; ++i) {
+        if (less(*i, *first)) {
+            T val = *i;
+            for (T* p = i; p != first; --p) *p = *(p - 1);
+            *first = val;
+        } else {
+            stl_unguarded_linear_insert(i, less);
+        }
+    }
+}
+
+template <typename T>
+MCV_HD void stl_swap(T* a, T* b) {
+    T t = *a;
+    *a = *b;
+    *b = t;
+}
+
+template <typename T, typename Less>
+MCV_HD void stl_move_median_to_first(T* result, T* a, T* b, T* c, Less less) {
+    if (less(*a, *b)) {
+        if (less(*b, *c)) stl_swap(result, b);
+        else if (less(*a, *c)) stl_swap(result, c);
+        else stl_swap(result, a);
+    } else if (less(*a, *c)) {
+        stl_swap(result, a);
+    } else if (less(*b, *c)) {
+        stl_swap(result, c);
+    } else {
+        stl_swap(result, b);
+    }
+}
+
+template <typename T, typename Less>
+MCV_HD T* stl_unguarded_partition(T* first, T* last, T* pivot, Less less) {
+    while (true) {
+        while (less(*first, *pivot)) ++first;
+        --last;
+        while (less(*pivot, *last)) --last;
+        if (!(first < last)) return first;
+        stl_swap(first, last);
+        ++first;
+    }
+}
+
+// __adjust_heap / __make_heap / __pop_heap on [first, last) with `less`
+template <typename T, typename Less>
+MCV_HD void stl_adjust_heap(T* first, int hole, int len, T value, Less less) {
+    const int top = hole;
+    int second = hole;
+    while (second < (len - 1) / 2) {
+        second = 2 * (second + 1);
+        if (less(first[second], first[second - 1])) second--;
+        first[hole] = first[second];
+        hole = second;
+    }
+    if ((len & 1) == 0 && second == (len - 2) / 2) {
+        second = 2 * (second + 1);
+        first[hole] = first[second - 1];
+        hole = second - 1;
+    }
+    int parent = (hole - 1) / 2;
+    while (hole > top && less(first[parent], value)) {
+        first[hole] = first[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    first[hole] = value;
+}
+
+template <typename T, typename Less>
+MCV_HD void stl_heap_sort_all(T* first, T* last, Less less) {
+    // __partial_sort(first, last, last): __heap_select (make_heap only, the
+    // middle is the end) then __sort_heap
+    const int len = (int)(last - first);
+    if (len >= 2)
+        for (int parent = (len - 2) / 2;; parent--) {
+            stl_adjust_heap(first, parent, len, first[parent], less);
+            if (parent == 0) break;
+        }
+    for (T* l = last; l - first > 1;) {
+        --l;
+        T value = *l;
+        *l = *first;
+        stl_adjust_heap(first, 0, (int)(l - first), value, less);
+    }
+}
+
+template <typename T, typename Less>
+MCV_HD void stl_sort(T* first, T* last, Less less) {
+    if (first == last) return;
+    int n = (int)(last - first), lg = 0;
+    while ((n >>= 1) != 0) lg++;
+    // __introsort_loop, with the recursion on the right part as an explicit stack
+    struct Range { T* f; T* l; int depth; };
+    Range stack[64];
+    int sp = 0;
+    stack[sp++] = Range{first, last, 2 * lg};
+    while (sp > 0) {
+        Range r = stack[--sp];
+        T* f = r.f;
+        T* l = r.l;
+        int depth = r.depth;
+        while (l - f > 16) {
+            if (depth == 0) {
+                stl_heap_sort_all(f, l, less);
+                break;
+            }
+            --depth;
+            T* mid = f + (l - f) / 2;
+            stl_move_median_to_first(f, f + 1, mid, l - 1, less);
+            T* cut = stl_unguarded_partition(f + 1, l, f, less);
+            // the reference recurses into [cut, l) before continuing with [f, cut)
+            stack[sp++] = Range{f, cut, depth};
+            f = cut;
+        }
+        // (the pushed left parts are processed after the right parts, as the
+        // reference's recursion order does)
+    }
+    // __final_insertion_sort
+    if (last - first > 16) {
+        stl_insertion_sort(first, first + 16, less);
+        for (T* i = first + 16; i != last; ++i) stl_unguarded_linear_insert(i, less);
+    } else {
+        stl_insertion_sort(first, last, less);
+    }
 }
 
 // A plain array as heap store.

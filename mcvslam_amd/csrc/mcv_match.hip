@@ -4,6 +4,8 @@
 //                 (src/Matcher.cpp:245-302, include/Matcher.hpp:19-33)
 //   k_filter      MatchResKnn::FilterRatio + MatchRes::FilterThreshold
 //                 (src/Matcher.cpp:100-111, :23-35)
+//   k_distinctive MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cpp:101-150)
+//   k_filter_orientation  MatchRes::FilterOrientation (src/Matcher.cpp:44-75)
 //
 // The reference keeps d = {999, 999}, idx = {0, 0} and replaces with strict
 // '<' while scanning trains in order, which is the first two entries of a
@@ -12,6 +14,8 @@
 // and (999 << 22) | 0 is the reference's sentinel.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "mcv_device.hpp"
 
 namespace mcv {
 
@@ -216,6 +220,134 @@ inline void launch_knn2(int variant, const MatchPair* pairs, int npairs, int max
     case 4: hipLaunchKernelGGL(k_knn2_lds, dim3((max_nq + kKnnQ - 1) / kKnnQ, npairs), dim3(256), 0, s, pairs); break;
     default: hipLaunchKernelGGL((k_knn2_s<1, 2>), dim3((max_nq + 63) / 64, npairs), dim3(256), 0, s, pairs); break;
     }
+}
+
+// ---------------------------------------------------------------------------
+// MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cpp:101-150), one
+// wave per map point with N <= kDistinctMax observed descriptors (staged in
+// LDS).  Lane i owns rows i, i + 64, ...; the row median (the value at
+// index floor(0.5 (N - 1)) of the sorted row, diagonal 0 included) is the
+// smallest v with #{j : d_ij <= v} > floor(0.5 (N - 1)), found by binary
+// search over v in [0, 256] without sorting.  The first row with the
+// smallest median wins: key (median << 16 | row), wave minimum.
+constexpr int kDistinctMax = 256;
+
+__global__ __launch_bounds__(64) void k_distinctive(const uint8_t* desc, const int* ptr, int n, int* best,
+                                                    uint8_t* out_desc) {
+    __shared__ uint4 s_d[2 * kDistinctMax];
+    const int m = blockIdx.x;
+    if (m >= n) return;
+    const int lane = threadIdx.x;
+    const int b = ptr[m], N = ptr[m + 1] - b;
+    if (N <= 0) {
+        if (lane == 0) best[m] = -1;
+        return;
+    }
+    const uint4* src = reinterpret_cast<const uint4*>(desc + 32ll * b);
+    for (int k = lane; k < 2 * N; k += 64) s_d[k] = src[k];
+    wave_lds_sync();
+    const int kth = (N - 1) / 2;  // (size_t)(0.5 * (N - 1))
+    uint32_t key = 0xFFFFFFFFu;
+    for (int i = lane; i < N; i += 64) {
+        const uint4 a0 = s_d[2 * i], a1 = s_d[2 * i + 1];
+        int lo = 0, hi = 256;  // answer in [lo, hi]
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            int cnt = 0;
+            for (int j = 0; j < N; j++) {
+                const uint4 c0 = s_d[2 * j], c1 = s_d[2 * j + 1];
+                const int d = __builtin_popcount(a0.x ^ c0.x) + __builtin_popcount(a0.y ^ c0.y) +
+                              __builtin_popcount(a0.z ^ c0.z) + __builtin_popcount(a0.w ^ c0.w) +
+                              __builtin_popcount(a1.x ^ c1.x) + __builtin_popcount(a1.y ^ c1.y) +
+                              __builtin_popcount(a1.z ^ c1.z) + __builtin_popcount(a1.w ^ c1.w);
+                cnt += d <= mid ? 1 : 0;
+            }
+            if (cnt > kth) hi = mid; else lo = mid + 1;
+        }
+        key = min(key, ((uint32_t)lo << 16) | (uint32_t)i);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) key = min(key, (uint32_t)__shfl_xor((int)key, o, 64));
+    const int bi = (int)(key & 0xFFFFu);
+    if (lane == 0) best[m] = bi;
+    if (out_desc && lane < 2)
+        reinterpret_cast<uint4*>(out_desc + 32ll * m)[lane] = s_d[2 * bi + lane];
+}
+
+// ---------------------------------------------------------------------------
+// MatchRes::FilterOrientation (src/Matcher.cpp:44-75), one workgroup per
+// match list (n <= kOrientMax).  Each match's rotation bin is computed with
+// the reference's float expressions; keys (bin << 16 | position) sorted in
+// LDS group every bin's matches in list order; the 40 (size, bin) records
+// are ordered by one thread with the libstdc++ std::sort replica (size
+// descending, the reference's unstable tie order), and the first three bins
+// are copied out.
+constexpr int kOrientMax = 4096;
+constexpr int kHistoLength = 40;
+
+struct OrientJob {
+    const int32_t* matches;  // n x (query, train, distance)
+    int n;
+    const float* angle1;     // by query index
+    const float* angle2;     // by train index
+    int32_t* out;            // up to n x 3
+    int* count;
+};
+
+struct OrientLess {
+    __device__ bool operator()(uint32_t a, uint32_t b) const { return (a >> 8) > (b >> 8); }
+};
+
+__global__ __launch_bounds__(256) void k_filter_orientation(const OrientJob* jobs) {
+    __shared__ uint32_t keys[kOrientMax];
+    __shared__ int cnt[kHistoLength];
+    __shared__ int start[kHistoLength];
+    __shared__ uint32_t rec[kHistoLength];
+    const OrientJob J = jobs[blockIdx.x];
+    const int n = J.n;
+    int n2 = 1;
+    while (n2 < n) n2 <<= 1;
+    if (threadIdx.x < kHistoLength) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const float factor = 1.0f / (360.0f / kHistoLength);
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+        uint32_t k = 0xFFFFFFFFu;
+        if (i < n) {
+            float rot = J.angle1[J.matches[3 * i]] - J.angle2[J.matches[3 * i + 1]];
+            if (rot < 0.0f) rot += 360.0f;
+            int bin = (int)roundf(rot * factor);
+            if (bin == kHistoLength) bin = 0;
+            bin = min(max(bin, 0), kHistoLength - 1);
+            k = ((uint32_t)bin << 16) | (uint32_t)i;
+            atomicAdd(&cnt[bin], 1);
+        }
+        keys[i] = k;
+    }
+    __syncthreads();
+    if (n2 > 1) bitonic_sort(keys, n2);
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int b = 0; b < kHistoLength; b++) {
+            start[b] = acc;
+            acc += cnt[b];
+            rec[b] = ((uint32_t)cnt[b] << 8) | (uint32_t)b;
+        }
+        stl_sort(rec, rec + kHistoLength, OrientLess());
+    }
+    __syncthreads();
+    int base = 0;
+    for (int r = 0; r < 3; r++) {
+        const int b = (int)(rec[r] & 0xFF), c = (int)(rec[r] >> 8);
+        for (int k = threadIdx.x; k < c; k += blockDim.x) {
+            const int i = (int)(keys[start[b] + k] & 0xFFFFu);
+            int32_t* o = J.out + 3 * (base + k);
+            o[0] = J.matches[3 * i];
+            o[1] = J.matches[3 * i + 1];
+            o[2] = J.matches[3 * i + 2];
+        }
+        base += c;
+    }
+    if (threadIdx.x == 0) *J.count = base;
 }
 
 struct FilterJob {

@@ -62,13 +62,6 @@ struct OrbGlobals {
     int* err;               // bit 0: pool overflow, bit 1: level too large
 };
 
-// Orders this wave's LDS accesses across its lanes (wave-private LDS areas).
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __device__ __forceinline__ int reflect101(int i, int n) {
     if (i < 0) i = -i;
     if (i >= n) i = 2 * n - 2 - i;
@@ -611,26 +604,6 @@ __device__ __forceinline__ int code_div(unsigned long long a, unsigned long long
     return kOctreeDepth - hb / 2;
 }
 
-template <typename T>
-__device__ void bitonic_sort(T* a, int n2) {
-    for (int k = 2; k <= n2; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < n2; i += blockDim.x) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const T x = a[i], y = a[ixj];
-                    const bool up = (i & k) == 0;
-                    if ((x > y) == up) {
-                        a[i] = y;
-                        a[ixj] = x;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
 constexpr int kOctThreads = 256;
 
 // Exclusive scan over the 256-thread block of every depth's count at once;
@@ -847,10 +820,9 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g) {
 // without children, at the deepest depth, carries its own index instead),
 // so a popped node's children are known without another lookup.
 //
-// The whole wave runs the heap in lockstep on uniform (scalar) values:
-// entries 0..63 live in lane i of two VGPRs (v_readlane / lane-select), the
-// rest in LDS (broadcast reads, lane 0 writes), and lanes 0..4 fetch the
-// counts and child ranges of a popped node's children together.  The
+// The whole wave runs the heap (ParHeap: one LDS round trip per gathered
+// chain or 5-level subtree), and lanes 0..4 fetch the counts and child
+// ranges of a popped node's children together.  The
 // shallow part of the node tables (the first `ncache` entries, depth-major,
 // where the splits happen) is copied to LDS first.  Workgroups take levels
 // in `order` (largest feature targets first).  Output per keypoint slot:
@@ -865,24 +837,110 @@ struct NodeTabs {
     __device__ __forceinline__ uint32_t ch(uint32_t i) const { return i < ncache ? s_child[i] : child[i]; }
 };
 
-struct WaveHeap {
-    uint2* lds;
-    uint32_t rx, ry;  // entry i < 64: lane i of (rx, ry)
-    __device__ __forceinline__ uint2 get(int i) const {
-        if (i < 64) return make_uint2(__builtin_amdgcn_readlane(rx, i), __builtin_amdgcn_readlane(ry, i));
-        const uint2 v = lds[i];
-        return make_uint2(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y));
-    }
-    __device__ __forceinline__ void set(int i, uint2 v) {
-        if (i < 64) {
-            const bool mine = (int)threadIdx.x == i;
-            rx = mine ? v.x : rx;
-            ry = mine ? v.y : ry;
-        } else if (threadIdx.x == 0) {
-            lds[i] = v;
-        }
-    }
+// libstdc++ binary heap (push_heap / pop_heap, keys compared alone) run by a
+// whole wave on LDS arrays: every sift gathers, in one round trip, either the
+// ancestor chain of the hole (sift-up) or the 5-level subtree below it
+// (62 nodes, sift-down), and the path is then decided in registers, so a pop
+// over a 1000-entry heap costs about three LDS round trips instead of one per
+// level.  Entries: key (count << 3 | children) and payload in two arrays.
+struct ParHeap {
+    uint32_t* key;
+    uint32_t* pay;
 };
+
+__device__ __forceinline__ uint32_t hkey(uint32_t k) { return k >> 3; }
+
+// push_heap's __push_heap(first, hole, top, value): move parents down while
+// parent < value, then place value.  Returns nothing; all lanes take part.
+__device__ __forceinline__ void par_sift_up(ParHeap& H, int hole, int top, uint32_t vk, uint32_t vp) {
+    const int lane = threadIdx.x & 63;
+    // c_l = ((hole + 1) >> l) - 1 is the l-th node of the chain (c_0 = hole);
+    // step l moves c_{l+1} into c_l when c_l > top and key(c_{l+1}) < key(value)
+    const int cl = ((hole + 1) >> lane) - 1;
+    const int cp = ((hole + 1) >> (lane + 1)) - 1;
+    bool go = false;
+    uint32_t pk = 0, pp = 0;
+    if (lane < 31 && cl > top && cl >= 1) {
+        pk = H.key[cp];
+        pp = H.pay[cp];
+        go = hkey(pk) < hkey(vk);
+    }
+    const unsigned long long stop = __ballot(!go);
+    const int L = (int)__builtin_ctzll(stop);  // first step that does not move
+    if (lane < L) {
+        H.key[cl] = pk;
+        H.pay[cl] = pp;
+    }
+    if (lane == 0) {
+        const int final_hole = ((hole + 1) >> L) - 1;
+        H.key[final_hole] = vk;
+        H.pay[final_hole] = vp;
+    }
+    wave_lds_sync();
+}
+
+__device__ __forceinline__ void par_push(ParHeap& H, int& len, uint32_t vk, uint32_t vp) {
+    par_sift_up(H, len, 0, vk, vp);
+    len++;
+}
+
+// __adjust_heap(first, 0, len, value) followed by its __push_heap
+__device__ __forceinline__ void par_adjust(ParHeap& H, int len, uint32_t vk, uint32_t vp) {
+    const int lane = threadIdx.x & 63;
+    // lane -> (depth d in 1..5, offset k) of the subtree below the hole
+    const int d = lane < 2 ? 1 : lane < 6 ? 2 : lane < 14 ? 3 : lane < 30 ? 4 : 5;
+    const int k = lane - ((1 << d) - 2);
+    int hole = 0, second = 0;
+    const int lim = (len - 1) / 2;
+    while (second < lim) {
+        // gather the subtree below the hole
+        const int idx = ((hole + 1) << d) - 1 + k;
+        uint32_t sk = 0, sp = 0;
+        if (lane < 62 && idx < len) {
+            sk = H.key[idx];
+            sp = H.pay[idx];
+        }
+        int dst = -1;  // this lane's entry moves to dst
+        int rel = 0;   // offset of the path node at the current depth
+        for (int dd = 1; dd <= 5 && second < lim; dd++) {
+            second = 2 * (second + 1);
+            const int lr = (1 << dd) - 2 + 2 * rel + 1;       // lane of the right child
+            const uint32_t kr = __builtin_amdgcn_readlane(sk, lr);
+            const uint32_t kl = __builtin_amdgcn_readlane(sk, lr - 1);
+            const bool take_left = hkey(kr) < hkey(kl);
+            const int src = take_left ? lr - 1 : lr;
+            if (take_left) second--;
+            if (lane == src) dst = hole;
+            hole = second;
+            rel = 2 * rel + (take_left ? 0 : 1);
+        }
+        if (dst >= 0) {
+            H.key[dst] = sk;
+            H.pay[dst] = sp;
+        }
+        wave_lds_sync();
+    }
+    if ((len & 1) == 0 && second == (len - 2) / 2) {
+        second = 2 * (second + 1);
+        if (lane == 0) {
+            H.key[hole] = H.key[second - 1];
+            H.pay[hole] = H.pay[second - 1];
+        }
+        hole = second - 1;
+        wave_lds_sync();
+    }
+    par_sift_up(H, hole, 0, vk, vp);
+}
+
+__device__ __forceinline__ void par_pop(ParHeap& H, int& len) {
+    if (len > 1) {
+        const int last = len - 1;
+        const uint32_t vk = __builtin_amdgcn_readfirstlane(H.key[last]);
+        const uint32_t vp = __builtin_amdgcn_readfirstlane(H.pay[last]);
+        par_adjust(H, last, vk, vp);
+    }
+    len--;
+}
 
 // heap entry of node c from start[c], start[c+1], child[c], child[c+1]
 __device__ __forceinline__ uint2 node_entry(uint32_t c, uint32_t s0, uint32_t s1, uint32_t c0, uint32_t c1) {
@@ -893,8 +951,9 @@ __device__ __forceinline__ uint2 node_entry(uint32_t c, uint32_t s0, uint32_t s1
 __device__ __forceinline__ uint32_t lane_val(uint32_t v, int k) { return __builtin_amdgcn_readlane(v, k); }
 
 __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* order, int cap, int node_cap) {
-    extern __shared__ uint2 heap_lds[];
-    uint32_t* s_start = reinterpret_cast<uint32_t*>(heap_lds + cap);
+    extern __shared__ uint32_t heap_lds[];
+    ParHeap H{heap_lds, heap_lds + cap};
+    uint32_t* s_start = heap_lds + 2 * cap;
     uint32_t* s_child = s_start + node_cap;
     const int lane = threadIdx.x;
     const int li = order[blockIdx.x];
@@ -915,7 +974,6 @@ __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* ord
         s_child[i] = nt.child[i];
     }
     wave_lds_sync();
-    WaveHeap H{heap_lds, 0u, 0u};
     const int nr = O->n_roots;
     int len = 0;
     {
@@ -924,31 +982,35 @@ __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* ord
             s_l = nt.st(lane);
             c_l = nt.ch(lane);
         }
-        for (int k = 0; k < nr; k++)
-            heap_push(H, len, node_entry((uint32_t)k, lane_val(s_l, k), lane_val(s_l, k + 1), lane_val(c_l, k),
-                                         lane_val(c_l, k + 1)));
+        for (int k = 0; k < nr; k++) {
+            const uint2 e = node_entry((uint32_t)k, lane_val(s_l, k), lane_val(s_l, k + 1), lane_val(c_l, k),
+                                       lane_val(c_l, k + 1));
+            par_push(H, len, e.x, e.y);
+        }
     }
     const int N = min(L.nfeat, cap - 8);
     while (len < N && len > 0) {
-        const uint2 top = H.get(0);
-        if ((top.x >> 3) == 1) break;
-        const uint32_t c0 = top.y, nch = top.x & 7u;
+        const uint32_t tk = __builtin_amdgcn_readfirstlane(H.key[0]);
+        const uint32_t tp = __builtin_amdgcn_readfirstlane(H.pay[0]);
+        if (hkey(tk) == 1) break;
+        const uint32_t c0 = tp, nch = tk & 7u;
         uint32_t s_l = 0, c_l = 0;
         if (lane <= (int)nch) {
             s_l = nt.st(c0 + lane);
             c_l = nt.ch(c0 + lane);
         }
-        heap_pop(H, len);
-        for (uint32_t k = 0; k < nch; k++)
-            heap_push(H, len, node_entry(c0 + k, lane_val(s_l, k), lane_val(s_l, k + 1), lane_val(c_l, k),
-                                         lane_val(c_l, k + 1)));
+        par_pop(H, len);
+        for (uint32_t k = 0; k < nch; k++) {
+            const uint2 e = node_entry(c0 + k, lane_val(s_l, k), lane_val(s_l, k + 1), lane_val(c_l, k),
+                                       lane_val(c_l, k + 1));
+            par_push(H, len, e.x, e.y);
+        }
     }
     int j = 0;
     while (len > 0) {
-        const uint2 top = H.get(0);
-        if (lane == 0 && j < L.out_cap) g.nodes_out[L.kp_slot + j] = (top.y << 3) | (top.x & 7u);
+        if (lane == 0 && j < L.out_cap) g.nodes_out[L.kp_slot + j] = (H.pay[0] << 3) | (H.key[0] & 7u);
         j++;
-        heap_pop(H, len);
+        par_pop(H, len);
     }
     if (lane == 0) O->nout = min(j, L.out_cap);
 }
@@ -1046,20 +1108,36 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
         y = (float)((int)((r >> 12) & 0xFFF) + L.minBY);
         resp = (float)(uint32_t)(bestk >> 56);
         size = L.patch;
-        // IC_Angle over the circular patch: lanes 0..30 take u = -15..15 of
-        // rows v = -15..0, lanes 32..62 the same columns of rows 1..15
-        const uint8_t* ctr = g.pyr + L.pix + (long long)(int)rintf(y) * L.pitch + (int)rintf(x);
-        const int u = (lane & 31) - 15;
-        const bool hi_half = lane >= 32;
+        // IC_Angle over the circular patch (|u| <= umax[|v|], |v| <= 15): the
+        // 31 rows are read as u32 words from the aligned column below cx - 15
+        // (at most 9 words a row, 279 words over the wave), each byte kept if
+        // inside the disk; the integer moments are order independent
+        const int cxi = (int)rintf(x), cyi = (int)rintf(y);
+        const int base = (cxi - 15) & ~3;
+        const int nwr = (cxi + 15 - base) / 4 + 1;
+        unsigned long long um = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) um |= (unsigned long long)(g.umax[k] & 15) << (4 * k);
+        const uint8_t* lvl = g.pyr + L.pix;
         int m10 = 0, m01 = 0;
 #pragma unroll
-        for (int it = 0; it < 16; it++) {
-            const int v = hi_half ? it + 1 : it - 15;
-            const int du = hi_half ? (it < 15 ? g.umax[it + 1] : -1) : g.umax[15 - it];
-            if ((lane & 31) < 31 && u >= -du && u <= du) {
-                const int val = ctr[(long long)v * L.pitch + u];
-                m10 += u * val;
-                m01 += v * val;
+        for (int it = 0; it < 5; it++) {
+            const int t = lane + 64 * it;
+            const int row = t / 9, w = t - 9 * row;
+            if (t < 31 * 9 && w < nwr) {
+                const int v = row - 15;
+                const int du = (int)((um >> (4 * (v < 0 ? -v : v))) & 15u);
+                const uint32_t dw =
+                    *reinterpret_cast<const uint32_t*>(lvl + (long long)(cyi + v) * L.pitch + base + 4 * w);
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const int u = base + 4 * w + b - cxi;
+                    if (u >= -du && u <= du) {
+                        const int val = (int)((dw >> (8 * b)) & 0xFFu);
+                        m10 += u * val;
+                        m01 += v * val;
+                    }
+                }
             }
         }
         m10 = wave_sum(m10);

@@ -42,6 +42,8 @@ _SIG = {
     "orc_ba_linearize": (None, [P, P, P, P, P, I, P, P, P, P, P, P, P, P]),
     "orc_ba_stereo_linearize": (None, [P, P, P, P, I, P, P, P, P, P, P, P, P]),
     "orc_ba_outliers": (None, [P, I, F, P]),
+    "orc_distinctive": (None, [P, P, I, P]),
+    "orc_filter_orientation": (I, [P, I, P, P, P]),
     "orc_ba_schur": (I, [P, P, P, P, P, P, P, I, I, I, P, D, P, P, P, P, P, P]),
 }
 
@@ -291,6 +293,28 @@ def ba_stereo_linearize(obs, obs_pt, obs_frame, obs_isig, points, frames, cam):
                                   _p(np.ascontiguousarray(cam, np.float64)),
                                   _p(e), _p(chi2), _p(w), _p(Jp), _p(Jx))
     return e, chi2, w, Jp.reshape(M, 3, 6), Jx.reshape(M, 3, 3)
+
+
+def distinctive(desc, ptr):
+    """ComputeDistinctiveDescriptors per map point: desc (total x 32 u8),
+    ptr (n + 1) CSR offsets -> best row index per map point (-1 if empty)."""
+    desc = np.ascontiguousarray(desc, np.uint8)
+    ptr = np.ascontiguousarray(ptr, np.int32)
+    out = np.zeros(len(ptr) - 1, np.int32)
+    lib().orc_distinctive(_p(desc), _p(ptr), len(ptr) - 1, _p(out))
+    return out
+
+
+def filter_orientation(matches, angles1, angles2):
+    """MatchRes::FilterOrientation: rows (query, train, distance) kept from
+    the three largest rotation bins, in libstdc++ std::sort bin order."""
+    m = np.ascontiguousarray(matches, np.int32).reshape(-1, 3)
+    out = np.zeros_like(m)
+    k = lib().orc_filter_orientation(_p(m), len(m), _p(np.ascontiguousarray(angles1, np.float32)),
+                                     _p(np.ascontiguousarray(angles2, np.float32)), _p(out))
+    if k < 0:
+        raise ValueError("rotation bin out of range")
+    return out[:k]
 
 
 def ba_outliers(chi2, threshold):

@@ -1255,6 +1255,69 @@ void orc_ba_stereo_linearize(const float* obs, const int32_t* obs_pt, const int3
     }
 }
 
+// MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cpp:101-150) for n
+// map points whose observed descriptors are desc[ptr[m] .. ptr[m+1]) (rows
+// of 32 B): all-pairs Hamming distances (diagonal 0), each row sorted, the
+// row's value at index (size_t)(0.5 * (N - 1)), and the first row with the
+// smallest such median.  best[m] is that row's index within the map point's
+// list, -1 for a map point without descriptors (the reference returns).
+void orc_distinctive(const uint8_t* desc, const int32_t* ptr, int n, int32_t* best) {
+    for (int m = 0; m < n; m++) {
+        const int b = ptr[m], N = ptr[m + 1] - ptr[m];
+        if (N <= 0) {
+            best[m] = -1;
+            continue;
+        }
+        std::vector<std::vector<int>> D(N, std::vector<int>(N, 0));
+        for (int i = 0; i < N; i++)
+            for (int j = i + 1; j < N; j++) {
+                const int d = hamming(desc + 32 * (size_t)(b + i), desc + 32 * (size_t)(b + j));
+                D[i][j] = d;
+                D[j][i] = d;
+            }
+        int bestMedian = INT_MAX, bestIdx = 0;
+        for (int i = 0; i < N; i++) {
+            std::sort(D[i].begin(), D[i].end());
+            const int median = D[i][(size_t)(0.5 * (N - 1))];
+            if (median < bestMedian) {
+                bestMedian = median;
+                bestIdx = i;
+            }
+        }
+        best[m] = bestIdx;
+    }
+}
+
+// MatchRes::FilterOrientation (src/Matcher.cpp:44-75): rotation histogram of
+// 40 bins over kp1.angle - kp2.angle, bins ordered by size with std::sort
+// (not stable: the tie order is libstdc++'s), the matches of the first three
+// bins kept, bin by bin, each in list order.  matches: n x (query, train,
+// distance); angles1 / angles2 indexed by query / train.  Returns the count.
+int orc_filter_orientation(const int32_t* matches, int n, const float* angles1, const float* angles2, int32_t* out) {
+    const int HISTO_LENGTH = 40;
+    const float HISTO_FACTOR = 1.0f / (360.0f / HISTO_LENGTH);
+    std::vector<uint32_t> rot_bins[HISTO_LENGTH];
+    for (int i = 0; i < n; i++) {
+        float rot = angles1[matches[3 * i]] - angles2[matches[3 * i + 1]];
+        if (rot < 0.0) rot += 360.0f;
+        int bin_id = (int)::round(rot * HISTO_FACTOR);
+        if (bin_id == HISTO_LENGTH) bin_id = 0;
+        if (bin_id < 0 || bin_id >= HISTO_LENGTH) return -1;
+        rot_bins[bin_id].push_back((uint32_t)i);
+    }
+    auto cmp = [](const std::vector<uint32_t>& a, const std::vector<uint32_t>& b) { return a.size() > b.size(); };
+    std::sort(&rot_bins[0], &rot_bins[0] + HISTO_LENGTH, cmp);
+    int k = 0;
+    for (int b = 0; b < 3; b++)
+        for (uint32_t idx : rot_bins[b]) {
+            out[3 * k] = matches[3 * idx];
+            out[3 * k + 1] = matches[3 * idx + 1];
+            out[3 * k + 2] = matches[3 * idx + 2];
+            k++;
+        }
+    return k;
+}
+
 // chi2 outlier test of FilterCallBack_Chi2 (src/PoseEstimation.cpp:132-151):
 // flag = chi2 >= threshold (float compare of the float chi2).
 void orc_ba_outliers(const double* chi2, int M, float threshold, uint8_t* flags) {

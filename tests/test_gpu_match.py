@@ -90,3 +90,46 @@ def test_reference_chain(gpu_ctx):
     t[:200] = q[:200]
     res = Matcher.KnnMatch(q, t, ctx=gpu_ctx).FilterRatio(0.7).FilterThreshold(50)
     np.testing.assert_array_equal(res, oracle.filter_ratio_threshold(oracle.knn2(q, t), 0.7, 50))
+
+
+def test_distinctive_descriptors(gpu_ctx):
+    """MapPoint::ComputeDistinctiveDescriptors batched over map points with
+    0..256 observations each, against the oracle (median index, first min)."""
+    from mcvslam_amd import distinctive_descriptors
+    rng = np.random.default_rng(21)
+    sizes = [0, 1, 2, 3, 5, 8, 31, 64, 65, 127, 256] + list(rng.integers(1, 40, 200))
+    ptr = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    desc = np.empty((ptr[-1], 32), np.uint8)
+    for m, n in enumerate(sizes):
+        base = _desc(rng, 1)[0]
+        for k in range(n):
+            d = base.copy()
+            for b in rng.integers(0, 256, rng.integers(0, 50)):
+                d[b >> 3] ^= np.uint8(1 << (b & 7))
+            desc[ptr[m] + k] = d
+    desc[ptr[5]:ptr[6]] = desc[ptr[5]]                 # all-equal list: ties
+    best, out = distinctive_descriptors(desc, ptr, ctx=gpu_ctx)
+    want = oracle.distinctive(desc, ptr)
+    np.testing.assert_array_equal(best, want)
+    for m in range(len(sizes)):
+        if want[m] >= 0:
+            np.testing.assert_array_equal(out[m], desc[ptr[m] + want[m]])
+
+
+@pytest.mark.parametrize("seed,n,used", [(1, 1, 40), (2, 37, 40), (3, 500, 40), (4, 2000, 6), (5, 4096, 40),
+                                         (6, 300, 3), (7, 120, 12)])
+def test_filter_orientation(gpu_ctx, seed, n, used):
+    """MatchRes::FilterOrientation against the oracle, whose bins are ordered
+    by the host's real std::sort: many equal bin sizes exercise the
+    replicated tie order."""
+    from mcvslam_amd import filter_orientation
+    rng = np.random.default_rng(seed)
+    a1 = rng.uniform(0, 360, n).astype(np.float32)
+    bins = rng.integers(0, used, n)
+    a2 = ((a1 - bins * np.float32(9.0) - rng.uniform(-4, 4, n).astype(np.float32)) % np.float32(360.0))
+    a2 = a2.astype(np.float32)
+    m = np.stack([np.arange(n), rng.permutation(n), rng.integers(0, 60, n)], 1).astype(np.int32)
+    a2p = np.empty_like(a2)
+    a2p[m[:, 1]] = a2
+    got = filter_orientation(m, a1, a2p, ctx=gpu_ctx)
+    np.testing.assert_array_equal(got, oracle.filter_orientation(m, a1, a2p))

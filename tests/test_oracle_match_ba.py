@@ -342,3 +342,90 @@ def test_outlier_flags():
     want = (chi2.astype(np.float32) >= np.float32(5.991)).astype(np.uint8)
     np.testing.assert_array_equal(got, want)
     assert got.tolist() == [0, 0, 1, 1, 1, 1, 1]
+
+
+# ------------------------------------------------ distinctive descriptors
+def _distinctive_brute(desc, ptr):
+    out = []
+    for m in range(len(ptr) - 1):
+        d = desc[ptr[m]:ptr[m + 1]]
+        N = len(d)
+        if N == 0:
+            out.append(-1)
+            continue
+        bits = np.unpackbits(d, axis=1).astype(np.int32)
+        D = (bits[:, None, :] != bits[None, :, :]).sum(2)
+        med = np.sort(D, axis=1)[:, int(0.5 * (N - 1))]
+        out.append(int(np.argmin(med)))           # first index of the minimum
+    return np.array(out, np.int32)
+
+
+def test_distinctive_descriptors_brute_force():
+    """MapPoint::ComputeDistinctiveDescriptors (src/MapPoint.cpp:101-150)
+    against numpy (sort per row, median index floor(0.5 (N-1)), first min)."""
+    rng = np.random.default_rng(7)
+    sizes = [0, 1, 2, 3, 4, 7, 8, 30, 64, 65, 130]
+    ptr = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    base = rng.integers(0, 256, (len(sizes), 32), dtype=np.uint8)
+    desc = []
+    for m, n in enumerate(sizes):
+        for k in range(n):
+            d = base[m].copy()
+            flips = rng.integers(0, 256, rng.integers(0, 40))
+            for b in flips:
+                d[b >> 3] ^= np.uint8(1 << (b & 7))
+            desc.append(d)
+    desc = np.array(desc, np.uint8).reshape(-1, 32)
+    got = oracle.distinctive(desc, ptr)
+    np.testing.assert_array_equal(got, _distinctive_brute(desc, ptr))
+    assert got[0] == -1 and got[1] == 0
+
+
+def test_distinctive_descriptors_ties_take_first():
+    """Identical descriptors: every median ties, the first row wins."""
+    d = np.tile(np.arange(32, dtype=np.uint8), (5, 1))
+    assert oracle.distinctive(d, np.array([0, 5], np.int32)).tolist() == [0]
+
+
+# ------------------------------------------------------ orientation filter
+def _orientation_case(rng, n, n_bins_used):
+    """matches whose angle differences fall in a few bins"""
+    a1 = rng.uniform(0, 360, n).astype(np.float32)
+    bins = rng.integers(0, n_bins_used, n) * 7 % 40
+    a2 = (a1 - bins * np.float32(9.0) - rng.uniform(-3, 3, n).astype(np.float32)) % np.float32(360.0)
+    m = np.stack([np.arange(n), np.arange(n), rng.integers(0, 60, n)], 1).astype(np.int32)
+    return m, a1, a2.astype(np.float32)
+
+
+def _bins(m, a1, a2):
+    rot = a1[m[:, 0]] - a2[m[:, 1]]
+    rot = np.where(rot < 0, rot + np.float32(360.0), rot).astype(np.float32)
+    b = np.floor(np.abs(rot * np.float32(1.0 / (360.0 / 40))).astype(np.float64) + 0.5).astype(int)
+    b[b == 40] = 0
+    return b
+
+
+def test_filter_orientation_keeps_three_largest_bins():
+    """MatchRes::FilterOrientation (src/Matcher.cpp:44-75): the output is the
+    matches of the three largest rotation bins, bin by bin, list order
+    within a bin; sizes here are distinct so std::sort's tie order is moot."""
+    rng = np.random.default_rng(8)
+    m, a1, a2 = _orientation_case(rng, 500, 6)
+    b = _bins(m, a1, a2)
+    sizes = np.bincount(b, minlength=40)
+    order = np.argsort(-sizes, kind="stable")
+    if len(set(sizes[order[:4]])) == 4:
+        got = oracle.filter_orientation(m, a1, a2)
+        want = np.concatenate([m[b == k] for k in order[:3]])
+        np.testing.assert_array_equal(got, want)
+    # any input: three bins' worth of matches, each bin contiguous and in order
+    m, a1, a2 = _orientation_case(rng, 300, 40)
+    got = oracle.filter_orientation(m, a1, a2)
+    b = _bins(m, a1, a2)
+    sizes = np.bincount(b, minlength=40)
+    assert len(got) == np.sort(sizes)[::-1][:3].sum()
+    gb = _bins(got, a1, a2)
+    runs = [gb[0]] + [x for i, x in enumerate(gb[1:]) if x != gb[i]]
+    assert len(runs) == len(set(runs)) <= 3
+    for k in set(runs):
+        np.testing.assert_array_equal(got[gb == k], m[b == k])

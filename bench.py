@@ -75,8 +75,32 @@ def algorithmic_bytes(kernel, orb, frames, counts):
     if kernel == "k_copy_level0":
         return 2 * P0
     if kernel == "k_keypoints":
-        return 56 * nkp
+        return 56 * nkp                               # keypoint record + descriptor written
+    if kernel == "k_octree_heap":
+        return 12 * nkp                               # node-table entries of each output node + its slot word
+    if kernel == "k_octree_build":
+        return 4 * nkp                                # candidate records gathered (lower bound)
+    if kernel == "k_knn2":
+        nq = counts.reshape(-1, 3)
+        return int(sum(32 * (nq[:, a] + nq[:, b]) + 16 * nq[:, a] for a, b in ((0, 1), (2, 0), (2, 1))).sum())
+    if kernel == "k_filter":
+        nq = counts.reshape(-1, 3)
+        return int(sum(16 * nq[:, a] for a in (0, 2, 2)).sum())
     return None
+
+
+def measured_traffic(path=ROOT / "profiles" / "hbm_traffic_rig64.txt"):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by the
+    calibration copy) from the committed rocprofv3 --pmc summary of this
+    workload; {} when it is absent."""
+    out = {}
+    if not path.exists():
+        return out
+    for line in path.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 5 and parts[0].startswith("k_") and parts[1].isdigit():
+            out[parts[0]] = int(parts[4])
+    return out
 
 
 def level_sizes(w, h, inv_scale):
@@ -288,18 +312,20 @@ def main():
         "keypoints_per_step": int(counts.sum()),
         "matches_per_step": int(mcounts.sum()),
     }
-    ab = algorithmic_bytes(dom, orb, n, counts)
-    per_launch_ms = kern[dom] / launches[dom]
-    if ab is not None:
-        per_launch_bytes = ab / launches[dom]
+    traffic = measured_traffic()
+    table = {}
+    for k in kern:
+        ab = algorithmic_bytes(k, orb, n, counts)
+        if ab is None:
+            continue
+        per_launch_ms = kern[k] / launches[k]
+        per_launch_bytes = ab / launches[k]
         ach = per_launch_bytes / (per_launch_ms / 1e3) / 1e9
-        result["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-                              "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
-                              "avg_launch_ms": round(per_launch_ms, 5)}
-    else:
-        result["roofline"] = {"kernel": dom, "bound": "latency", "achieved": None, "peak": HBM_PEAK_GBS,
-                              "unit": "GB/s", "frac": None, "traffic": None,
-                              "avg_launch_ms": round(per_launch_ms, 5)}
+        table[k] = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": traffic.get(k),
+                    "algorithmic_bytes": int(per_launch_bytes), "avg_launch_ms": round(per_launch_ms, 5)}
+    result["roofline"] = dict(kernel=dom, **table[dom])
+    result["kernels_roofline"] = table
 
     if not args.no_ba:
         log("BA problem ...")

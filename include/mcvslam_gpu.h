@@ -105,7 +105,7 @@ int mcv_match_filter_orientation(mcv_ctx* ctx, const int32_t* matches, int n, co
 int mcv_distinctive_descriptors(mcv_ctx* ctx, const uint8_t* desc, const int32_t* ptr, int n, int32_t* best,
                                 uint8_t* out_desc);
 /* Launch variant of the top-2 kernel used by mcv_match_knn2* on this context
- * (0 .. 4; no counterpart in the reference: the autotuner's choice, also
+ * (0 .. 5; no counterpart in the reference: the autotuner's choice, also
  * settable for testing).  Every variant returns the same rows. */
 int mcv_match_set_variant(mcv_ctx* ctx, int variant);
 int mcv_match_knn2_dev(mcv_ctx* ctx, const uint8_t* d_q, int nq, const uint8_t* d_t, int nt, int32_t* d_out);

@@ -67,7 +67,7 @@ class Context:
         check(self.h, lib().mcv_synchronize(self.h))
 
     def set_match_variant(self, variant: int):
-        """Launch variant of the top-2 kernel for Matcher calls (0..4; all
+        """Launch variant of the top-2 kernel for Matcher calls (0..5; all
         return the same rows)."""
         check(self.h, lib().mcv_match_set_variant(self.h, int(variant)))
 

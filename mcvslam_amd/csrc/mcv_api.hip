@@ -642,7 +642,7 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
     hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(d_used.p), 4);
     mark("k_zero_words");
     const Strip* st = d_strips.as<Strip>();
-    auto strip_grid = [](int n) { return dim3((n + 3) / 4); };
+    auto strip_grid = [](int n) { return dim3(xcd_grid((n + 3) / 4)); };
     const int n0 = strip_begin[1] - strip_begin[0];
     if (n0 > 0) {
         hipLaunchKernelGGL(k_copy_level0, strip_grid(n0), dim3(256), 0, s, st, n0, g.lv, d_src.as<const uint8_t*>(),
@@ -652,7 +652,8 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
     for (int l = 1; l < nlevels; l++) {
         const int n = strip_begin[l + 1] - strip_begin[l];
         if (n > 0) {
-            hipLaunchKernelGGL(k_resize, dim3(n), dim3(64), resize_rows[l] * resize_rw[l], s, st + strip_begin[l], n,
+            hipLaunchKernelGGL(k_resize, dim3(xcd_grid(n)), dim3(64), resize_rows[l] * resize_rw[l], s,
+                               st + strip_begin[l], n,
                                g.lv, d_coef.as<int2>(), g.pyr, resize_rw[l]);
             mark("k_resize");
         }
@@ -674,7 +675,7 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
     }
     const int nc = (int)cells.size();
     if (nc > 0) {
-        hipLaunchKernelGGL(k_fast_cells, dim3((nc + 3) / 4), dim3(256), 0, s, g, nc);
+        hipLaunchKernelGGL(k_fast_cells, dim3(xcd_grid((nc + 3) / 4)), dim3(256), 0, s, g, nc);
         mark("k_fast_cells");
     }
     hipLaunchKernelGGL(k_octree_build, dim3(lv.size()), dim3(kOctThreads), 0, s, g);
@@ -682,7 +683,7 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
     hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), 8 * (size_t)heap_cap + 8 * node_cap, s, g,
                        d_heap_order.as<int>(), heap_cap, node_cap);
     mark("k_octree_heap");
-    hipLaunchKernelGGL(k_keypoints, dim3((n_slots + 3) / 4), dim3(256), 0, s, g, d_slot.as<int>(), n_slots, tb,
+    hipLaunchKernelGGL(k_keypoints, dim3(xcd_grid((n_slots + 3) / 4)), dim3(256), 0, s, g, d_slot.as<int>(), n_slots, tb,
                        reinterpret_cast<float*>(d_kps), d_desc, out_cap, d_counts);
     mark("k_keypoints");
     HIP_OK(hipGetLastError());

@@ -21,6 +21,20 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b on XCD b % 8),
+// each XCD with its own L2.  For a grid padded to a multiple of 8 this maps
+// the physical block to a logical one so that every XCD works on one
+// contiguous eighth of the logical blocks (neighbouring strips, cells or
+// keypoints of the same image then share that XCD's L2); -1 for a padding
+// block past the logical count nb.
+__device__ __forceinline__ int xcd_block(int nb) {
+    const int per = (int)gridDim.x >> 3;
+    const int b = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    return b < nb ? b : -1;
+}
+
+inline int xcd_grid(int nb) { return (nb + 7) / 8 * 8; }
+
 // Ascending bitonic sort of n2 (a power of two) keys by the whole workgroup.
 template <typename T>
 __device__ void bitonic_sort(T* a, int n2) {

@@ -206,9 +206,9 @@ __global__ __launch_bounds__(256) void k_knn2_lds(const MatchPair* pairs) {
 
 // Variants of the same exact computation; the autotuner picks one per
 // workload (kKnnVariants of them).
-constexpr int kKnnVariants = 5;
+constexpr int kKnnVariants = 6;
 inline const char* knn_variant_name(int v) {
-    static const char* names[kKnnVariants] = {"S<1,2>", "S<1,4>", "S<2,2>", "S<2,4>", "L"};
+    static const char* names[kKnnVariants] = {"S<1,2>", "S<1,4>", "S<2,2>", "S<2,4>", "L", "S<2,8>"};
     return (v >= 0 && v < kKnnVariants) ? names[v] : "?";
 }
 
@@ -218,6 +218,7 @@ inline void launch_knn2(int variant, const MatchPair* pairs, int npairs, int max
     case 2: hipLaunchKernelGGL((k_knn2_s<2, 2>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;
     case 3: hipLaunchKernelGGL((k_knn2_s<2, 4>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;
     case 4: hipLaunchKernelGGL(k_knn2_lds, dim3((max_nq + kKnnQ - 1) / kKnnQ, npairs), dim3(256), 0, s, pairs); break;
+    case 5: hipLaunchKernelGGL((k_knn2_s<2, 8>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;
     default: hipLaunchKernelGGL((k_knn2_s<1, 2>), dim3((max_nq + 63) / 64, npairs), dim3(256), 0, s, pairs); break;
     }
 }

@@ -71,8 +71,13 @@ __device__ __forceinline__ int reflect101(int i, int n) {
 // ---------------------------------------------------------------------------
 // Strip kernels: a wave owns kStripW x kStripH pixels of one level, each lane
 // 4 horizontally adjacent pixels (one u32 of a pitched row).
-#define STRIP_SETUP(tab, n)                                         \
-    const int sidx = blockIdx.x * 4 + (threadIdx.x >> 6);           \
+// kXcd: consecutive workgroups land on one XCD (shared L2 for the row halos
+// of neighbouring strips); without it the dispatcher's round robin spreads
+// consecutive strips over the 8 XCDs, which balances uneven per-strip work.
+#define STRIP_SETUP(tab, n, kXcd)                                   \
+    const int xb = (kXcd) ? xcd_block(((n) + 3) / 4) : (int)blockIdx.x; \
+    if (xb < 0) return;                                             \
+    const int sidx = xb * 4 + (threadIdx.x >> 6);                   \
     if (sidx >= (n)) return;                                        \
     const int lane = threadIdx.x & 63;                              \
     const Strip S = (tab)[sidx];                                    \
@@ -83,7 +88,7 @@ __device__ __forceinline__ int reflect101(int i, int n) {
 __global__ __launch_bounds__(256) void k_copy_level0(const Strip* st, int n, const LevelDesc* lv,
                                                      const uint8_t* const* src, const int* src_stride,
                                                      uint8_t* __restrict__ pyr) {
-    STRIP_SETUP(st, n)
+    STRIP_SETUP(st, n, true)
     if (x >= L.w) return;
     const uint8_t* s = src[L.img];
     const long long stride = src_stride[L.img];
@@ -112,8 +117,8 @@ __global__ __launch_bounds__(256) void k_copy_level0(const Strip* st, int n, con
 __global__ __launch_bounds__(64) void k_resize(const Strip* st, int n, const LevelDesc* lv, const int2* coef,
                                                uint8_t* pyr, int rw) {
     extern __shared__ uint32_t s_rows[];
-    const int sidx = blockIdx.x;
-    if (sidx >= n) return;
+    const int sidx = xcd_block(n);
+    if (sidx < 0) return;
     const int lane = threadIdx.x;
     const Strip S = st[sidx];
     const LevelDesc L = lv[S.lvl];
@@ -211,24 +216,34 @@ __device__ __forceinline__ void row_words(const uint8_t* row, int x, int w, int 
 // (row indices reflect, stores past the level end are skipped) with the
 // lane's load kind fixed, so later rows' loads can be issued early.
 template <bool kFixed, bool kInterior>
+__device__ __forceinline__ void blur_load_row(const uint8_t* __restrict__ src, const LevelDesc& L, int x, int yi,
+                                              uint32_t& w0, uint32_t& w1, uint32_t& w2) {
+    const int ry = reflect101(min(max(yi, -3), L.h + 2), L.h);
+    const uint8_t* row = src + (long long)ry * L.pitch;
+    if (kFixed && kInterior) {
+        const uint32_t* rw = reinterpret_cast<const uint32_t*>(row + x);
+        w0 = rw[-1];
+        w1 = rw[0];
+        w2 = rw[1];
+    } else {
+        row_words(row, x, L.w, L.pitch, w0, w1, w2);
+    }
+}
+
+template <bool kFixed, bool kInterior>
 __device__ __forceinline__ void blur_strip(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
                                            const LevelDesc& L, int x, int y0, int yend, const BlurTaps& taps) {
     uint32_t hr[7][4];
+    // one row ahead: the next row's words are requested before this row's
+    // taps are computed
+    uint32_t n0, n1, n2;
+    blur_load_row<kFixed, kInterior>(src, L, x, y0 - 3, n0, n1, n2);
 #pragma unroll
     for (int r = 0; r < kStripH + 6; r++) {
         const int yi = y0 - 3 + r;
         if (!kFixed && yi - 3 >= yend) break;
-        const int ry = reflect101(min(max(yi, -3), L.h + 2), L.h);
-        const uint8_t* row = src + (long long)ry * L.pitch;
-        uint32_t w0, w1, w2;
-        if (kFixed && kInterior) {
-            const uint32_t* rw = reinterpret_cast<const uint32_t*>(row + x);
-            w0 = rw[-1];
-            w1 = rw[0];
-            w2 = rw[1];
-        } else {
-            row_words(row, x, L.w, L.pitch, w0, w1, w2);
-        }
+        const uint32_t w0 = n0, w1 = n1, w2 = n2;
+        if (r + 1 < kStripH + 6) blur_load_row<kFixed, kInterior>(src, L, x, yi + 1, n0, n1, n2);
         uint32_t* h = hr[r % 7];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -255,7 +270,7 @@ template <bool kFixed>
 __global__ __launch_bounds__(256) void k_blur(const Strip* st, int n, const LevelDesc* lv,
                                               const uint8_t* __restrict__ pyr, uint8_t* __restrict__ blur,
                                               BlurTaps taps) {
-    STRIP_SETUP(st, n)
+    STRIP_SETUP(st, n, true)
     if (x >= L.w) return;
     if (kFixed && x >= 4 && x + 8 <= L.pitch && x + 6 < L.w)
         blur_strip<kFixed, true>(pyr + L.pix, blur + L.pix, L, x, S.y0, yend, taps);
@@ -391,7 +406,7 @@ __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, cons
     __shared__ uint32_t s_ring[4][kRing][kRingW];
     __shared__ uint32_t s_tile[4][kBlk][kStripW / 4];
     __shared__ uint32_t s_queue[4][64 + kStripW];
-    STRIP_SETUP(st, n)
+    STRIP_SETUP(st, n, false)
     const int wv = threadIdx.x >> 6;
     uint32_t (*ring)[kRingW] = s_ring[wv];
     uint32_t* queue = s_queue[wv];
@@ -1059,7 +1074,9 @@ __device__ __forceinline__ void sincos_patched(float x, const SinCosTables& tb, 
 __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot_lvl, int n_slots,
                                                    SinCosTables tb, float* out_kps, uint8_t* out_desc, int cap,
                                                    int* out_count) {
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int xb = xcd_block((n_slots + 3) / 4);
+    if (xb < 0) return;
+    const int wave = xb * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (wave >= n_slots) return;
     const int2 sl = reinterpret_cast<const int2*>(slot_lvl)[wave];

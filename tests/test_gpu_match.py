@@ -21,7 +21,7 @@ def test_knn2_random(gpu_ctx, nq, nt):
     np.testing.assert_array_equal(got, oracle.knn2(q, t))
 
 
-@pytest.mark.parametrize("variant", range(5))
+@pytest.mark.parametrize("variant", range(6))
 @pytest.mark.parametrize("nq,nt", [(3, 1), (65, 5), (130, 257), (700, 1531)])
 def test_knn2_every_variant(gpu_ctx, variant, nq, nt):
     """All launch variants the autotuner chooses from return the oracle's rows

@@ -333,6 +333,7 @@ def main():
         prob = synth.ba_problem(seed=40, n_kf=50, n_points=200000, target_obs=1500000)
         ba = BALinearizer.from_problem(prob, ctx=ctx)
         log(f"  {len(prob.obs_pt)} observations in {time.perf_counter() - t0:.1f}s")
+        ba_tune = ba.autotune(reps=5).strip().splitlines() if not args.no_autotune else []
         for _ in range(3):
             ba.linearize()
         torch.cuda.synchronize()
@@ -361,7 +362,7 @@ def main():
                         "observations": M, "obs_per_s": round(M / (ba_ms / 1e3), 1), "ms_per_launch": round(ba_ms, 5),
                         "plain_obs_per_s": round(M / (plain_ms / 1e3), 1), "plain_ms": round(plain_ms, 5),
                         "plain_launches": int(len(prob.frames) * len(prob.cams)),
-                        "batched_over_plain": round(plain_ms / ba_ms, 2),
+                        "batched_over_plain": round(plain_ms / ba_ms, 2), "autotune": ba_tune,
                         "roofline": {"kernel": "k_ba_linearize", "bound": "hbm", "achieved": round(ach, 2),
                                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                                      "bytes_per_obs": round(ba_bytes / M, 2)}}

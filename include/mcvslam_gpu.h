@@ -105,7 +105,7 @@ int mcv_match_filter_orientation(mcv_ctx* ctx, const int32_t* matches, int n, co
 int mcv_distinctive_descriptors(mcv_ctx* ctx, const uint8_t* desc, const int32_t* ptr, int n, int32_t* best,
                                 uint8_t* out_desc);
 /* Launch variant of the top-2 kernel used by mcv_match_knn2* on this context
- * (0 .. 5; no counterpart in the reference: the autotuner's choice, also
+ * (0 .. 7; no counterpart in the reference: the autotuner's choice, also
  * settable for testing).  Every variant returns the same rows. */
 int mcv_match_set_variant(mcv_ctx* ctx, int variant);
 int mcv_match_knn2_dev(mcv_ctx* ctx, const uint8_t* d_q, int nq, const uint8_t* d_t, int nt, int32_t* d_out);
@@ -200,6 +200,15 @@ int mcv_ba_set_state(mcv_ba* ba, const double* points, const double* frames, con
  * Huber weight, J_pose (2x6 row-major), J_point (2x3 row-major).
  * residual_only = 1 computes the first four planes only. */
 int mcv_ba_linearize(mcv_ba* ba, int residual_only);
+/* Launch shape of mcv_ba_linearize (0 .. 7: workgroup size, grid cap,
+ * observations in flight per thread, streaming stores; no counterpart in the
+ * reference).  Every variant computes the same values per observation. */
+int mcv_ba_set_variant(mcv_ba* ba, int variant);
+/* Times every launch shape of mcv_ba_linearize on this problem (median of
+ * `reps` launches after one warm-up) and keeps the fastest; no counterpart in
+ * the reference.  Leaves the planes linearized.  `report` (optional) receives
+ * one line per variant and the choice. */
+int mcv_ba_autotune(mcv_ba* ba, int reps, char* report, int report_cap);
 /* Same outputs with the plain launch pattern: one launch per (frame, camera)
  * edge group, the granularity at which g2o's per-edge loop
  * (core/block_solver.hpp:488-494) would be replaced group by group.  For the

@@ -67,6 +67,8 @@ SIGNATURES = {
     "mcv_ba_set_observations": (I, [P, P, P, P, P, P]),
     "mcv_ba_set_state": (I, [P, P, P, P]),
     "mcv_ba_linearize": (I, [P, I]),
+    "mcv_ba_set_variant": (I, [P, I]),
+    "mcv_ba_autotune": (I, [P, I, P, I]),
     "mcv_ba_linearize_plain": (I, [P, I]),
     "mcv_ba_set_stereo": (I, [P, I, P, P, P, P, P]),
     "mcv_ba_linearize_stereo": (I, [P, I]),

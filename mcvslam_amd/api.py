@@ -66,9 +66,11 @@ class Context:
     def synchronize(self):
         check(self.h, lib().mcv_synchronize(self.h))
 
+    N_KNN_VARIANTS = 8
+
     def set_match_variant(self, variant: int):
-        """Launch variant of the top-2 kernel for Matcher calls (0..5; all
-        return the same rows)."""
+        """Launch variant of the top-2 kernel for Matcher calls
+        (0 .. N_KNN_VARIANTS - 1; all return the same rows)."""
         check(self.h, lib().mcv_match_set_variant(self.h, int(variant)))
 
     def to_host(self, d_ptr: int, dtype, shape) -> np.ndarray:
@@ -408,6 +410,19 @@ class BALinearizer:
 
     def linearize(self, residual_only: bool = False):
         check(self.ctx.h, lib().mcv_ba_linearize(self.h, 1 if residual_only else 0))
+
+    N_VARIANTS = 8
+
+    def set_variant(self, variant: int):
+        """Launch shape of the batched linearization (0 .. 7)."""
+        check(self.ctx.h, lib().mcv_ba_set_variant(self.h, variant))
+
+    def autotune(self, reps: int = 5) -> str:
+        """Time every launch shape on this problem, keep the fastest (values
+        per observation are the same); returns the report."""
+        buf = ctypes.create_string_buffer(2048)
+        check(self.ctx.h, lib().mcv_ba_autotune(self.h, reps, buf, 2048))
+        return buf.value.decode()
 
     def set_stereo(self, obs, point, frame, inv_sigma2, cam):
         """Stereo edges (EdgeStereoSE3ProjectXYZ): obs Ms x 3 (u, v, u_right),

@@ -772,6 +772,7 @@ struct mcv_ba {
     std::vector<int> h_gptr;  // observations of (frame, camera) group g: gobs[h_gptr[g] .. h_gptr[g + 1])
     std::vector<int> h_pt, h_fr, h_pptr;
     std::vector<int> fixed;
+    int variant = 0;  // k_ba_linearize launch shape (kBaVariant)
     bool have_obs = false, have_state = false, linearized = false;
     // Schur structure (depends on observations and fixed frames)
     bool schur_ready = false, schur_done = false;
@@ -779,6 +780,55 @@ struct mcv_ba {
     DevBuf pf_ptr, pf_obs_ptr, pf_obs, pf_point, sfptr, sfobs, fpf_ptr, fpf, blk_ptr, blk_items, blk_ij;
     DevBuf Dinv, bl, db, Hll, Hpl, Hpp, bs, Sout;
 };
+
+// Launch shapes of k_ba_linearize (same arithmetic per observation; the
+// autotuner mcv_ba_autotune picks one): workgroup size, grid cap,
+// observations in flight per thread, streaming stores.
+struct BAVariantDesc {
+    int threads, grid, ilp;
+    bool nt;
+    const char* name;
+};
+constexpr int kBaVariants = 8;
+static const BAVariantDesc kBaVariant[kBaVariants] = {
+    {256, 1024, 1, false, "256x1024 ilp1"},     {256, 1024, 2, false, "256x1024 ilp2"},
+    {512, 1024, 1, false, "512x1024 ilp1"},     {512, 1024, 2, false, "512x1024 ilp2"},
+    {512, 1024, 2, true, "512x1024 ilp2 nt"},   {1024, 512, 2, true, "1024x512 ilp2 nt"},
+    {256, 1024, 1, true, "256x1024 ilp1 nt"},   {512, 1024, 1, true, "512x1024 ilp1 nt"},
+};
+
+template <int kIlp, bool kNt>
+static void launch_ba_lds(int blocks, int threads, size_t lds, hipStream_t s, const BAObs& ob, const float4* phi,
+                          const float4* plo, const BAPair* pairs, int np, const BAOut& o, int residual_only) {
+    hipLaunchKernelGGL((k_ba_linearize<true, kIlp, kNt>), dim3(blocks), dim3(threads), lds, s, ob, phi, plo, pairs, np,
+                       o, residual_only);
+}
+
+static void ba_enqueue_linearize(mcv_ba* b, int variant, int residual_only, hipStream_t s) {
+    if (b->M <= 0) return;
+    BAObs ob{b->uv.as<float2>(), b->pt.as<uint32_t>(), b->pair.as<uint32_t>(), b->isig.as<float>(), b->M};
+    BAOut o{b->planes.as<float>(), (long long)b->M};
+    const int np = b->F * b->C;
+    const float4* phi = b->phi.as<float4>();
+    const float4* plo = b->plo.as<float4>();
+    const BAPair* pairs = b->pairs.as<BAPair>();
+    if (np <= kLdsPairs) {
+        const BAVariantDesc& V = kBaVariant[variant];
+        const long long per = (long long)V.threads * V.ilp;
+        const int blocks = (int)std::min<long long>((b->M + per - 1) / per, V.grid);
+        const size_t lds = (size_t)np * sizeof(BAPair);
+        switch (variant) {
+        case 1: case 3: launch_ba_lds<2, false>(blocks, V.threads, lds, s, ob, phi, plo, pairs, np, o, residual_only); break;
+        case 4: case 5: launch_ba_lds<2, true>(blocks, V.threads, lds, s, ob, phi, plo, pairs, np, o, residual_only); break;
+        case 6: case 7: launch_ba_lds<1, true>(blocks, V.threads, lds, s, ob, phi, plo, pairs, np, o, residual_only); break;
+        default: launch_ba_lds<1, false>(blocks, V.threads, lds, s, ob, phi, plo, pairs, np, o, residual_only); break;
+        }
+    } else {
+        hipLaunchKernelGGL((k_ba_linearize<false, 1, false>), dim3((b->M + 255) / 256), dim3(256), 0, s, ob, phi, plo,
+                           pairs, np, o, residual_only);
+    }
+    HIP_OK(hipGetLastError());
+}
 
 static int fail(mcv_ctx* ctx, const Fail& f) {
     if (ctx) ctx->err = f.msg;
@@ -1780,21 +1830,60 @@ int mcv_ba_linearize(mcv_ba* b, int residual_only) {
     API_BEGIN(c)
     REQUIRE(b && b->have_obs && b->have_state, MCV_ERR_STATE, "observations or state not set");
     c->set_device();
-    if (b->M > 0) {
-        BAObs ob{b->uv.as<float2>(), b->pt.as<uint32_t>(), b->pair.as<uint32_t>(), b->isig.as<float>(), b->M};
-        BAOut o{b->planes.as<float>(), (long long)b->M};
-        const int np = b->F * b->C;
-        const int blocks = (b->M + 255) / 256;
-        if (np <= kLdsPairs)
-            hipLaunchKernelGGL(k_ba_linearize<true>, dim3(std::min(blocks, 1024)), dim3(256),
-                               (size_t)np * sizeof(BAPair), c->stream, ob,
-                               b->phi.as<float4>(), b->plo.as<float4>(), b->pairs.as<BAPair>(), np, o, residual_only);
-        else
-            hipLaunchKernelGGL(k_ba_linearize<false>, dim3(blocks), dim3(256), 0, c->stream, ob, b->phi.as<float4>(),
-                               b->plo.as<float4>(), b->pairs.as<BAPair>(), np, o, residual_only);
-        HIP_OK(hipGetLastError());
-    }
+    ba_enqueue_linearize(b, b->variant, residual_only, c->stream);
     b->linearized = !residual_only;
+    API_END
+}
+
+int mcv_ba_set_variant(mcv_ba* b, int variant) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && variant >= 0 && variant < kBaVariants, MCV_ERR_INVALID, "BA launch variant out of range");
+    b->variant = variant;
+    API_END
+}
+
+int mcv_ba_autotune(mcv_ba* b, int reps, char* report, int report_cap) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && reps > 0, MCV_ERR_INVALID, "bad autotune arguments");
+    REQUIRE(b->have_obs && b->have_state, MCV_ERR_STATE, "observations or state not set");
+    c->set_device();
+    hipStream_t s = c->stream;
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    std::string rep;
+    char line[160];
+    int best = 0;
+    float bt = 1e30f;
+    for (int v = 0; v < kBaVariants; v++) {
+        std::vector<float> ts;
+        for (int k = 0; k < reps + 1; k++) {
+            HIP_OK(hipEventRecord(e0, s));
+            ba_enqueue_linearize(b, v, 0, s);
+            HIP_OK(hipEventRecord(e1, s));
+            HIP_OK(hipEventSynchronize(e1));
+            float t = 0;
+            HIP_OK(hipEventElapsedTime(&t, e0, e1));
+            if (k > 0) ts.push_back(t);  // the first launch warms caches
+        }
+        std::sort(ts.begin(), ts.end());
+        const float t = ts[ts.size() / 2];
+        snprintf(line, sizeof line, "k_ba_linearize %s %.4f ms\n", kBaVariant[v].name, t);
+        rep += line;
+        if (t < bt) { bt = t; best = v; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->variant = best;
+    snprintf(line, sizeof line, "chosen: k_ba_linearize %s\n", kBaVariant[best].name);
+    rep += line;
+    b->linearized = true;
+    if (report && report_cap > 0) {
+        strncpy(report, rep.c_str(), (size_t)report_cap - 1);
+        report[report_cap - 1] = 0;
+    }
     API_END
 }
 

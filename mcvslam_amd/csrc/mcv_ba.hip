@@ -108,13 +108,44 @@ __device__ __forceinline__ f2 df_div_rcp(f2 a, f2 b, float inv) {
 constexpr int kPairWords = (int)(sizeof(BAPair) / 4);
 constexpr int kLdsPairs = 192;  // (frame, camera) pairs that fit the LDS copy
 
-__device__ __forceinline__ void ba_edge(int i, const BAObs& ob, const float4* __restrict__ pts_hi,
-                                        const float4* __restrict__ pts_lo, const BAPair& T, BAOut out,
-                                        int residual_only) {
-    const float2 uv = ob.uv[i];
-    const uint32_t pi = ob.pt[i];
-    const float isig = ob.isig[i];
-    const float4 ph = pts_hi[pi], pl = pts_lo[pi];
+// Inputs of one observation: the streamed record and its gathered point.
+struct BAIn {
+    float2 uv;
+    uint32_t pt, pair;
+    float isig;
+    float4 ph, pl;
+};
+
+__device__ __forceinline__ void ba_load_obs(int i, const BAObs& ob, BAIn& in) {
+    in.uv = ob.uv[i];
+    in.pt = ob.pt[i];
+    in.pair = ob.pair[i];
+    in.isig = ob.isig[i];
+}
+
+__device__ __forceinline__ void ba_load_point(const float4* __restrict__ pts_hi, const float4* __restrict__ pts_lo,
+                                              BAIn& in) {
+    in.ph = pts_hi[in.pt];
+    in.pl = pts_lo[in.pt];
+}
+
+// kNt: streaming (non-temporal) stores for the output planes; this kernel
+// never reads them back
+// Plane k of observation i: the plane base is uniform (scalar registers) and
+// the byte offset 4 i a 32-bit lane value, so each store takes one address
+// register instead of a 64-bit pointer per plane.
+template <bool kNt>
+__device__ __forceinline__ void ba_store(const BAOut& out, int k, uint32_t off, float v) {
+    float* p = reinterpret_cast<float*>(reinterpret_cast<char*>(out.planes + (long long)k * out.stride) + off);
+    if (kNt) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <bool kNt>
+__device__ __forceinline__ void ba_edge(int i, const BAIn& in, const BAPair& T, BAOut out, int residual_only) {
+    const float2 uv = in.uv;
+    const float isig = in.isig;
+    const float4 ph = in.ph, pl = in.pl;
     const f2 P[3] = {{ph.x, pl.x}, {ph.y, pl.y}, {ph.z, pl.z}};
     // P_c = R_cw P_w + t_cw (float pairs)
     f2 C[3];
@@ -135,12 +166,11 @@ __device__ __forceinline__ void ba_edge(int i, const BAObs& ob, const float4* __
     const float chi2 = e0 * (isig * e0) + e1 * (isig * e1);
     const float d2 = T.delta * T.delta;
     const float w = chi2 <= d2 ? 1.f : T.delta * __builtin_amdgcn_rsqf(chi2);
-    const long long S = out.stride;
-    float* o = out.planes + i;
-    o[0] = e0;
-    o[S] = e1;
-    o[2 * S] = chi2;
-    o[3 * S] = w;
+    const uint32_t off = (uint32_t)i * 4u;
+    ba_store<kNt>(out, 0, off, e0);
+    ba_store<kNt>(out, 1, off, e1);
+    ba_store<kNt>(out, 2, off, chi2);
+    ba_store<kNt>(out, 3, off, w);
     if (residual_only) return;
     // Jacobians (FP32).  Jpi = [[fx/z, 0, -fx x/z^2], [0, fy/z, -fy y/z^2]]
     const float x = C[0].hi, y = C[1].hi;
@@ -176,31 +206,56 @@ __device__ __forceinline__ void ba_edge(int i, const BAObs& ob, const float4* __
         Jp[6 * r + 5] = -m2;
     }
 #pragma unroll
-    for (int k = 0; k < 12; k++) o[(4 + k) * S] = Jp[k];
+    for (int k = 0; k < 12; k++) ba_store<kNt>(out, 4 + k, off, Jp[k]);
 #pragma unroll
-    for (int k = 0; k < 6; k++) o[(16 + k) * S] = Jpt[k];
+    for (int k = 0; k < 6; k++) ba_store<kNt>(out, 16 + k, off, Jpt[k]);
 }
 
-// One observation per thread, grid-stride.  kLds: the (frame, camera) table
-// (npairs <= kLdsPairs) is copied to dynamic LDS once per workgroup, so the
-// per-observation parameter reads are LDS gathers (53-word records: odd
-// stride, distinct records fall in distinct banks) instead of ~50 scattered
-// global loads.
-template <bool kLds>
-__global__ __launch_bounds__(256) void k_ba_linearize(BAObs ob, const float4* __restrict__ pts_hi,
-                                                      const float4* __restrict__ pts_lo,
-                                                      const BAPair* __restrict__ pairs, int npairs, BAOut out,
-                                                      int residual_only) {
+// One observation per thread per step, grid-stride.  kLds: the (frame,
+// camera) table (npairs <= kLdsPairs) is copied to dynamic LDS once per
+// workgroup, so the per-observation parameter reads are LDS gathers
+// (53-word records: odd stride, distinct records fall in distinct banks)
+// instead of ~50 scattered global loads.  kIlp observations are in flight
+// per thread: all their loads are issued before the first is computed.
+template <bool kLds, int kIlp, bool kNt>
+__global__ __launch_bounds__(1024) void k_ba_linearize(BAObs ob, const float4* __restrict__ pts_hi,
+                                                       const float4* __restrict__ pts_lo,
+                                                       const BAPair* __restrict__ pairs, int npairs, BAOut out,
+                                                       int residual_only) {
     extern __shared__ float s_pairs[];  // npairs * kPairWords floats when kLds
     if (kLds) {
         const float* src = reinterpret_cast<const float*>(pairs);
         for (int k = threadIdx.x; k < npairs * kPairWords; k += blockDim.x) s_pairs[k] = src[k];
         __syncthreads();
     }
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ob.M; i += gridDim.x * blockDim.x) {
-        const uint32_t pr = ob.pair[i];
-        const BAPair& T = kLds ? *reinterpret_cast<const BAPair*>(s_pairs + pr * kPairWords) : pairs[pr];
-        ba_edge(i, ob, pts_hi, pts_lo, T, out, residual_only);
+    const int stride = gridDim.x * blockDim.x;
+    if (kIlp == 1) {
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ob.M; i += stride) {
+            const uint32_t pr = ob.pair[i];
+            const BAPair& T = kLds ? *reinterpret_cast<const BAPair*>(s_pairs + pr * kPairWords) : pairs[pr];
+            BAIn in;
+            ba_load_obs(i, ob, in);
+            ba_load_point(pts_hi, pts_lo, in);
+            ba_edge<kNt>(i, in, T, out, residual_only);
+        }
+        return;
+    }
+    for (int i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < ob.M; i0 += kIlp * stride) {
+        BAIn in[kIlp];
+#pragma unroll
+        for (int u = 0; u < kIlp; u++)
+            if (i0 + u * stride < ob.M) ba_load_obs(i0 + u * stride, ob, in[u]);
+#pragma unroll
+        for (int u = 0; u < kIlp; u++)
+            if (i0 + u * stride < ob.M) ba_load_point(pts_hi, pts_lo, in[u]);
+#pragma unroll
+        for (int u = 0; u < kIlp; u++) {
+            const int i = i0 + u * stride;
+            if (i >= ob.M) break;
+            const BAPair& T = kLds ? *reinterpret_cast<const BAPair*>(s_pairs + in[u].pair * kPairWords)
+                                   : pairs[in[u].pair];
+            ba_edge<kNt>(i, in[u], T, out, residual_only);
+        }
     }
 }
 
@@ -213,7 +268,10 @@ __global__ __launch_bounds__(256) void k_ba_linearize_group(BAObs ob, const floa
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int i = list[k];
-    ba_edge(i, ob, pts_hi, pts_lo, pairs[ob.pair[i]], out, residual_only);
+    BAIn in;
+    ba_load_obs(i, ob, in);
+    ba_load_point(pts_hi, pts_lo, in);
+    ba_edge<false>(i, in, pairs[in.pair], out, residual_only);
 }
 
 // ---------------------------------------------------------------------------

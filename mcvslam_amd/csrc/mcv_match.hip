@@ -40,6 +40,8 @@ __device__ __forceinline__ uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c)
     return r;
 }
 
+typedef __attribute__((address_space(4))) const uint32_t kconst_u32;
+
 __device__ __forceinline__ void top2_insert(uint32_t& k0, uint32_t& k1, uint32_t key) {
     k1 = med3_u32(k0, k1, key);
     k0 = min(k0, key);
@@ -72,10 +74,13 @@ __global__ __launch_bounds__(256) void k_knn2_s(const MatchPair* pairs) {
     const int per = (nt + 3) / 4;
     const int j0 = __builtin_amdgcn_readfirstlane(wave * per);
     const int j1 = __builtin_amdgcn_readfirstlane(min(nt, j0 + per));
-    const uint32_t* tw = reinterpret_cast<const uint32_t*>(P.t);
+    // train rows through the constant address space: s_load into SGPRs, which
+    // the XORs take as operands (wave-uniform data never occupies the vector
+    // memory path).  The rows are written by earlier launches only.
+    const kconst_u32* tw = (const kconst_u32*)P.t;
     int j = j0;
     for (; j + UNR <= j1; j += UNR) {
-        const uint32_t* ta = tw + (long long)j * 8;
+        const kconst_u32* ta = tw + (long long)j * 8;
 #pragma unroll
         for (int t = 0; t < UNR; t++)
 #pragma unroll
@@ -87,7 +92,7 @@ __global__ __launch_bounds__(256) void k_knn2_s(const MatchPair* pairs) {
             }
     }
     for (; j < j1; j++) {
-        const uint32_t* ta = tw + (long long)j * 8;
+        const kconst_u32* ta = tw + (long long)j * 8;
 #pragma unroll
         for (int u = 0; u < QPL; u++) {
             uint32_t d = 0;
@@ -206,9 +211,9 @@ __global__ __launch_bounds__(256) void k_knn2_lds(const MatchPair* pairs) {
 
 // Variants of the same exact computation; the autotuner picks one per
 // workload (kKnnVariants of them).
-constexpr int kKnnVariants = 6;
+constexpr int kKnnVariants = 8;
 inline const char* knn_variant_name(int v) {
-    static const char* names[kKnnVariants] = {"S<1,2>", "S<1,4>", "S<2,2>", "S<2,4>", "L", "S<2,8>"};
+    static const char* names[kKnnVariants] = {"S<1,2>", "S<1,4>", "S<2,2>", "S<2,4>", "L", "S<2,8>", "S<1,8>", "S<4,2>"};
     return (v >= 0 && v < kKnnVariants) ? names[v] : "?";
 }
 
@@ -219,6 +224,8 @@ inline void launch_knn2(int variant, const MatchPair* pairs, int npairs, int max
     case 3: hipLaunchKernelGGL((k_knn2_s<2, 4>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;
     case 4: hipLaunchKernelGGL(k_knn2_lds, dim3((max_nq + kKnnQ - 1) / kKnnQ, npairs), dim3(256), 0, s, pairs); break;
     case 5: hipLaunchKernelGGL((k_knn2_s<2, 8>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;
+    case 6: hipLaunchKernelGGL((k_knn2_s<1, 8>), dim3((max_nq + 63) / 64, npairs), dim3(256), 0, s, pairs); break;
+    case 7: hipLaunchKernelGGL((k_knn2_s<4, 2>), dim3((max_nq + 255) / 256, npairs), dim3(256), 0, s, pairs); break;
     default: hipLaunchKernelGGL((k_knn2_s<1, 2>), dim3((max_nq + 63) / 64, npairs), dim3(256), 0, s, pairs); break;
     }
 }

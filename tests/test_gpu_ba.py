@@ -73,6 +73,35 @@ def test_ba_plain_equals_batched(gpu_ctx):
     np.testing.assert_array_equal(b.read(), batched)
 
 
+def test_ba_every_launch_variant(gpu_ctx):
+    """Every launch shape (workgroup size, grid cap, observations in flight,
+    streaming stores) writes the same bits.  Before each variant the planes
+    are overwritten from a shifted state, so a variant that skips
+    observations fails."""
+    prob = synth.ba_problem(seed=36, n_kf=5, n_points=4000, target_obs=30011)
+    assert len(prob.obs_pt) % 256 != 0
+    b = BALinearizer.from_problem(prob, ctx=gpu_ctx)
+    b.set_variant(0)
+    b.linearize()
+    ref = b.read()
+    check_against_oracle(prob, b.results())
+    shifted = prob.points + 0.25
+    for v in range(1, BALinearizer.N_VARIANTS):
+        b.set_variant(0)
+        b.set_state(shifted, prob.frames, prob.cams)
+        b.linearize()
+        assert not np.array_equal(b.read(0, 1), ref[:1])
+        b.set_state(prob.points, prob.frames, prob.cams)
+        b.set_variant(v)
+        b.linearize()
+        np.testing.assert_array_equal(b.read(), ref, err_msg=f"variant {v}")
+    rep = b.autotune(reps=2)
+    assert rep.count("k_ba_linearize") == BALinearizer.N_VARIANTS + 1
+    np.testing.assert_array_equal(b.read(), ref)
+    with pytest.raises(Exception):
+        b.set_variant(BALinearizer.N_VARIANTS)
+
+
 def test_ba_many_pairs(gpu_ctx):
     """More (frame, camera) pairs than the LDS copy holds (70 x 3 = 210)."""
     prob = synth.ba_problem(seed=35, n_kf=70, n_points=6000, target_obs=40000)

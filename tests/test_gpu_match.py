@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle
-from mcvslam_amd import Matcher
+from mcvslam_amd import Context, Matcher
 
 pytestmark = pytest.mark.gpu
 
@@ -21,7 +21,7 @@ def test_knn2_random(gpu_ctx, nq, nt):
     np.testing.assert_array_equal(got, oracle.knn2(q, t))
 
 
-@pytest.mark.parametrize("variant", range(6))
+@pytest.mark.parametrize("variant", range(Context.N_KNN_VARIANTS))
 @pytest.mark.parametrize("nq,nt", [(3, 1), (65, 5), (130, 257), (700, 1531)])
 def test_knn2_every_variant(gpu_ctx, variant, nq, nt):
     """All launch variants the autotuner chooses from return the oracle's rows

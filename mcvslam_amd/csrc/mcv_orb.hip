@@ -500,52 +500,83 @@ __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, cons
 // neighbours outside it count as 0, as in FAST_t's zeroed ring rows and
 // border columns.  A pass at iniTh keeps s = best - 1 where s exceeds every
 // neighbour's (best > t ? best - 1 : 0); a cell with no keeper is redone at
-// minTh (ORBextractor.cc:604-616).  The score bytes of 8 x 64 pixels are
-// loaded before any of them is examined.
-__device__ __forceinline__ int cell_pass(const uint8_t* __restrict__ sp, uint32_t* __restrict__ out, int pitch,
-                                         int DW, int DH, uint32_t magic, int t, int offx, int offy) {
+// minTh (ORBextractor.cc:604-616).
+//
+// The area is walked in bands of kCellBand rows.  A band and its one-pixel
+// frame are staged in LDS with coalesced u32 loads that keep the global word
+// alignment (the frame and any byte outside the area are stored as 0), so
+// the centre and neighbour reads of the suppression are LDS reads with no
+// bounds tests.
+constexpr int kCellBand = 16;
+constexpr int kCellSW = kMaxWin;  // LDS bytes per staged row: >= 4 * ceil((3 + W - 4) / 4)
+
+__device__ __forceinline__ int cell_pass(const uint8_t* __restrict__ sp, uint32_t* __restrict__ out,
+                                         uint32_t* __restrict__ band, int pitch, int DW, int DH, uint32_t magic,
+                                         int t, int offx, int offy) {
     const int lane = threadIdx.x & 63;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    const int np = DW * DH;
+    // staged row: the words from the one holding detection column -1
+    const uint8_t* wbase = reinterpret_cast<const uint8_t*>(reinterpret_cast<uintptr_t>(sp - 1) & ~(uintptr_t)3);
+    const int lead = (int)((sp - 1) - wbase);  // byte of column -1 in the first word
+    const int nwr = (lead + DW + 2 + 3) >> 2;
+    const uint32_t wmagic = (65536u + (uint32_t)nwr - 1u) / (uint32_t)nwr;  // t / nwr for t < 18 * 24
+    const uint8_t* bb = reinterpret_cast<const uint8_t*>(band);
     int written = 0;
-    for (int p0 = 0; p0 < np; p0 += 8 * 64) {
-        int b[8], ry[8], rx[8];
+    for (int y0 = 0; y0 < DH; y0 += kCellBand) {
+        const int nb = min(kCellBand, DH - y0);
+        const int tot = (nb + 2) * nwr;
+        for (int k = lane; k < tot; k += 64) {
+            const int r = (int)(((uint32_t)k * wmagic) >> 16), w = k - r * nwr;
+            const int ry = y0 - 1 + r;
+            uint32_t v = 0;
+            if (ry >= 0 && ry < DH) {
+                v = *reinterpret_cast<const uint32_t*>(wbase + (long long)ry * pitch + 4 * w);
+                // keep the bytes of detection columns [0, DW)
+                const int lo = lead + 1 - 4 * w, hi = lead + 1 + DW - 4 * w;
+                uint32_t m = 0;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int p = p0 + 64 * k + lane;
-            ry[k] = (int)(((uint32_t)p * magic) >> 20);
-            rx[k] = p - ry[k] * DW;
-            b[k] = p < np ? sp[(long long)ry[k] * pitch + rx[k]] : 0;
+                for (int bt = 0; bt < 4; bt++)
+                    if (bt >= lo && bt < hi) m |= 0xFFu << (8 * bt);
+                v &= m;
+            }
+            band[r * (kCellSW / 4) + w] = v;
         }
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
+        wave_lds_sync();
+        const int npb = nb * DW;
+        for (int p0 = 0; p0 < npb; p0 += 64) {
+            const int p = p0 + lane;
             bool keep = false;
             uint32_t rec = 0;
-            if (b[k] > t) {
-                const int s = b[k] - 1;
-                keep = true;
-                const uint8_t* q = sp + (long long)ry[k] * pitch + rx[k];
+            if (p < npb) {
+                const int r = (int)(((uint32_t)p * magic) >> 20), x = p - r * DW;
+                const uint8_t* q = bb + (r + 1) * kCellSW + lead + 1 + x;
+                const int bc = q[0];
+                if (bc > t) {
+                    const int s = bc - 1;
+                    keep = true;
 #pragma unroll
-                for (int dy = -1; dy <= 1; dy++)
+                    for (int dy = -1; dy <= 1; dy++)
 #pragma unroll
-                    for (int dx = -1; dx <= 1; dx++) {
-                        if (dx == 0 && dy == 0) continue;
-                        const int nx = rx[k] + dx, ny = ry[k] + dy;
-                        const int bq = (nx >= 0 && nx < DW && ny >= 0 && ny < DH) ? q[dy * pitch + dx] : 0;
-                        const int mq = bq > t ? bq - 1 : 0;
-                        keep = keep && s > mq;
-                    }
-                rec = ((uint32_t)s << 24) | ((uint32_t)(ry[k] + 3 + offy) << 12) | (uint32_t)(rx[k] + 3 + offx);
+                        for (int dx = -1; dx <= 1; dx++) {
+                            if (dx == 0 && dy == 0) continue;
+                            const int bq = q[dy * kCellSW + dx];
+                            const int mq = bq > t ? bq - 1 : 0;
+                            keep = keep && s > mq;
+                        }
+                    rec = ((uint32_t)s << 24) | ((uint32_t)(y0 + r + 3 + offy) << 12) | (uint32_t)(x + 3 + offx);
+                }
             }
             const unsigned long long bal = __ballot(keep);
             if (keep) out[written + __popcll(bal & lt)] = rec;
             written += __popcll(bal);
         }
+        wave_lds_sync();
     }
     return written;
 }
 
 __global__ __launch_bounds__(256) void k_fast_cells(OrbGlobals g, int ncells) {
+    __shared__ uint32_t s_band[4][(kCellBand + 2) * kCellSW / 4];
     const int ci = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (ci >= ncells) return;
     const int lane = threadIdx.x & 63;
@@ -557,8 +588,9 @@ __global__ __launch_bounds__(256) void k_fast_cells(OrbGlobals g, int ncells) {
         const uint8_t* sp = g.score + L.pix + (long long)(c.y0 + 3) * L.pitch + (c.x0 + 3);
         const uint32_t magic = ((1u << 20) + (uint32_t)DW - 1u) / (uint32_t)DW;  // p / DW for p < 2^20 / DW
         uint32_t* out = g.cell_cand + c.cap_base;
-        total = cell_pass(sp, out, L.pitch, DW, DH, magic, g.ini_th, c.offx, c.offy);
-        if (total == 0) total = cell_pass(sp, out, L.pitch, DW, DH, magic, g.min_th, c.offx, c.offy);
+        uint32_t* band = s_band[threadIdx.x >> 6];
+        total = cell_pass(sp, out, band, L.pitch, DW, DH, magic, g.ini_th, c.offx, c.offy);
+        if (total == 0) total = cell_pass(sp, out, band, L.pitch, DW, DH, magic, g.min_th, c.offx, c.offy);
     }
     if (lane == 0) g.cell_n[ci] = total;
 }

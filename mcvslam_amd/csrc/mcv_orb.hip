@@ -1053,13 +1053,38 @@ __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* ord
             par_push(H, len, e.x, e.y);
         }
     }
-    int j = 0;
+    // drain: the j-th popped node's word goes to key[len] once the pop has
+    // freed that slot, i.e. node j ends at key[n0 - 1 - j]
+    const int n0 = len;
     while (len > 0) {
-        if (lane == 0 && j < L.out_cap) g.nodes_out[L.kp_slot + j] = (H.pay[0] << 3) | (H.key[0] & 7u);
-        j++;
+        const uint32_t w = (H.pay[0] << 3) | (H.key[0] & 7u);
         par_pop(H, len);
+        if (lane == 0) H.key[len] = w;
     }
-    if (lane == 0) O->nout = min(j, L.out_cap);
+    wave_lds_sync();
+    const int nkeep = min(n0, L.out_cap);
+    if (lane == 0) O->nout = nkeep;
+    // each kept node's keypoint, one node per lane: the candidate with the
+    // largest response, the first in candidate order among equals
+    // (DistributeOctTree's scan, ORBextractor.cc:569-574); its candidate
+    // word (response << 24 | y << 12 | x) is the slot's output
+    const uint32_t* sorted = g.pool + O->sorted;
+    const uint32_t* cand = g.lvl_cand + L.cand_base;
+    for (int jj = lane; jj < nkeep; jj += 64) {
+        const uint32_t nw = H.key[n0 - 1 - jj];
+        const uint32_t ny = nw >> 3, nch = max(nw & 7u, 1u);
+        const uint32_t s0 = nt.st(ny), s1 = nt.st(ny + nch);
+        uint32_t best = 0, bi = 0;
+        for (uint32_t p = s0; p < s1; p++) {
+            const uint32_t idx = sorted[p];
+            const uint32_t r = cand[idx];
+            if (p == s0 || (r >> 24) > (best >> 24) || ((r >> 24) == (best >> 24) && idx < bi)) {
+                best = r;
+                bi = idx;
+            }
+        }
+        g.nodes_out[L.kp_slot + jj] = best;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1103,9 +1128,15 @@ __device__ __forceinline__ void sincos_patched(float x, const SinCosTables& tb, 
 // the IC angle on the unblurred level (:75-98) and the rBRIEF descriptor on
 // the blurred level (:101-141); seed slots keep their angle (:845-847).
 // Level > 0 coordinates are then multiplied by the level scale (:886-890).
+constexpr int kPatchR = 18;                           // test points lie within 18 px of the centre
+constexpr int kPatchRows = 2 * kPatchR + 1;            // 37
+constexpr int kPatchWords = 10;                        // u32 per staged row (aligned column below cx - 18)
+constexpr int kPatchIt = (kPatchRows * kPatchWords + 63) / 64;
+
 __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot_lvl, int n_slots,
                                                    SinCosTables tb, float* out_kps, uint8_t* out_desc, int cap,
                                                    int* out_count) {
+    __shared__ uint32_t s_patch[4][kPatchRows * kPatchWords];
     const int xb = xcd_block((n_slots + 3) / 4);
     if (xb < 0) return;
     const int wave = xb * 4 + (threadIdx.x >> 6);
@@ -1129,41 +1160,47 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
     float x, y, resp, angle, size;
     const bool seed = j >= nout;
     if (!seed) {
-        const OctLevel* O = g.oct + li;
-        // node word: first child (or, without children, the node itself) << 3
-        // | children; the node's sorted range is [start[y], start[y + max(children, 1)])
-        const uint32_t nw = g.nodes_out[L.kp_slot + j];
-        const uint32_t ny = nw >> 3, nch = max(nw & 7u, 1u);
-        const uint32_t* start = g.pool + O->tab;
-        const uint32_t* sorted = g.pool + O->sorted;
-        const uint32_t s0 = start[ny], s1 = start[ny + nch];
-        // key: response, then the lowest candidate index, then the record's
-        // packed coordinates (the winner's position travels with the max)
-        unsigned long long bestk = 0;
-        for (uint32_t p = s0 + lane; p < s1; p += 64) {
-            const uint32_t idx = sorted[p];
-            const uint32_t r = g.lvl_cand[L.cand_base + idx];
-            const unsigned long long k = ((unsigned long long)(r >> 24) << 56) |
-                                         ((unsigned long long)(0xFFFFFFu - idx) << 24) | (r & 0xFFFFFFu);
-            bestk = k > bestk ? k : bestk;
-        }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const unsigned long long v = __shfl_xor(bestk, o, 64);
-            bestk = v > bestk ? v : bestk;
-        }
-        const uint32_t r = (uint32_t)(bestk & 0xFFFFFFu);
+        // the slot's keypoint as the heap kernel left it: response << 24 | y << 12 | x
+        const uint32_t r = g.nodes_out[L.kp_slot + j];
         x = (float)((int)(r & 0xFFF) + L.minBX);
         y = (float)((int)((r >> 12) & 0xFFF) + L.minBY);
-        resp = (float)(uint32_t)(bestk >> 56);
+        resp = (float)(r >> 24);
         size = L.patch;
+    } else {
+        const float* sd = g.seeds + 6 * (L.seed_first + (j - nout));
+        x = sd[0];
+        y = sd[1];
+        size = sd[2];
+        angle = sd[3];
+        resp = sd[4];
+    }
+    const int cx = (int)rintf(x), cy = (int)rintf(y);
+    // The rotated test points lie within 18 px of the centre (the pattern's
+    // largest radius is 18.38).  When that square is inside the level, the
+    // blurred 37 x 40-byte patch around it is staged in LDS with coalesced
+    // u32 loads issued together with the moment loads below, and the 512
+    // taps are LDS reads; otherwise (seeds near the border) the taps are
+    // clamped gathers from the level.
+    const bool staged = cx >= kPatchR && cx + kPatchR < L.w && cy >= kPatchR && cy + kPatchR < L.h;
+    const uint8_t* bl = g.blur + L.pix;
+    const int bx = (cx - kPatchR) & ~3;
+    uint32_t pw[kPatchIt];
+    if (staged) {
+#pragma unroll
+        for (int it = 0; it < kPatchIt; it++) {
+            const int k = lane + 64 * it, r = k / kPatchWords, w = k - kPatchWords * r;
+            pw[it] = 0;
+            if (k < kPatchRows * kPatchWords && bx + 4 * w + 4 <= L.pitch)
+                pw[it] = *reinterpret_cast<const uint32_t*>(bl + (long long)(cy - kPatchR + r) * L.pitch + bx + 4 * w);
+        }
+    }
+    if (!seed) {
         // IC_Angle over the circular patch (|u| <= umax[|v|], |v| <= 15): the
         // 31 rows are read as u32 words from the aligned column below cx - 15
         // (at most 9 words a row, 279 words over the wave), each byte kept if
         // inside the disk; the integer moments are order independent
-        const int cxi = (int)rintf(x), cyi = (int)rintf(y);
-        const int base = (cxi - 15) & ~3;
-        const int nwr = (cxi + 15 - base) / 4 + 1;
+        const int base = (cx - 15) & ~3;
+        const int nwr = (cx + 15 - base) / 4 + 1;
         unsigned long long um = 0;
 #pragma unroll
         for (int k = 0; k < 16; k++) um |= (unsigned long long)(g.umax[k] & 15) << (4 * k);
@@ -1177,10 +1214,10 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
                 const int v = row - 15;
                 const int du = (int)((um >> (4 * (v < 0 ? -v : v))) & 15u);
                 const uint32_t dw =
-                    *reinterpret_cast<const uint32_t*>(lvl + (long long)(cyi + v) * L.pitch + base + 4 * w);
+                    *reinterpret_cast<const uint32_t*>(lvl + (long long)(cy + v) * L.pitch + base + 4 * w);
 #pragma unroll
                 for (int b = 0; b < 4; b++) {
-                    const int u = base + 4 * w + b - cxi;
+                    const int u = base + 4 * w + b - cx;
                     if (u >= -du && u <= du) {
                         const int val = (int)((dw >> (8 * b)) & 0xFFu);
                         m10 += u * val;
@@ -1192,13 +1229,14 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
         m10 = wave_sum(m10);
         m01 = wave_sum(m01);
         angle = fast_atan2_deg((float)m01, (float)m10);
-    } else {
-        const float* sd = g.seeds + 6 * (L.seed_first + (j - nout));
-        x = sd[0];
-        y = sd[1];
-        size = sd[2];
-        angle = sd[3];
-        resp = sd[4];
+    }
+    uint32_t* patch = s_patch[threadIdx.x >> 6];
+    if (staged) {
+#pragma unroll
+        for (int it = 0; it < kPatchIt; it++) {
+            const int k = lane + 64 * it;
+            if (k < kPatchRows * kPatchWords) patch[k] = pw[it];
+        }
     }
     // rBRIEF on the blurred level.  Only the rounded pattern offsets
     // cvRound(px * b + py * a), cvRound(px * a - py * b) depend on the
@@ -1232,19 +1270,31 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
             ox[k] = (int)rintf(px * a - py * b);
         }
     }
-    const uint8_t* bl = g.blur + L.pix;
-    const int cx = (int)rintf(x), cy = (int)rintf(y);
     unsigned long long words[4];
+    if (staged) {
+        wave_lds_sync();
+        const uint8_t* pb = reinterpret_cast<const uint8_t*>(patch);
+        const int c0 = cx - bx;  // patch column of the centre
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        int t[2];
+        for (int r = 0; r < 4; r++) {
+            int t[2];
 #pragma unroll
-        for (int q = 0; q < 2; q++) {
-            const int yy = min(max(cy + oy[2 * r + q], 0), L.h - 1);
-            const int xx = min(max(cx + ox[2 * r + q], 0), L.w - 1);
-            t[q] = bl[(long long)yy * L.pitch + xx];
+            for (int q = 0; q < 2; q++)
+                t[q] = pb[(kPatchR + oy[2 * r + q]) * (4 * kPatchWords) + c0 + ox[2 * r + q]];
+            words[r] = __ballot(t[0] < t[1]);
         }
-        words[r] = __ballot(t[0] < t[1]);
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            int t[2];
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int yy = min(max(cy + oy[2 * r + q], 0), L.h - 1);
+                const int xx = min(max(cx + ox[2 * r + q], 0), L.w - 1);
+                t[q] = bl[(long long)yy * L.pitch + xx];
+            }
+            words[r] = __ballot(t[0] < t[1]);
+        }
     }
     if (lane == 0) {
         unsigned long long* dst = reinterpret_cast<unsigned long long*>(out_desc + ((long long)L.img * cap + pos) * 32);

@@ -5,9 +5,9 @@ step = one hipGraph launch over a batch of 64 rig frames (L and R 1280x720, W
 640x480; per-image ORB feature counts drawn from U{500..4000}, seed 40 + frame;
 8 levels, scale 1.2, FAST 20/7), then brute-force Hamming top-2 + ratio 0.7 +
 threshold 50 for L-R, W-L and W-R.  Inputs are resident in HBM before timing
-starts.  With --streams 2 (default) consecutive batches alternate between two
-rigs on two streams, so each batch's serial octree replay overlaps the next
-batch's pixel kernels; the one-stream rate is reported beside it.
+starts.  With --streams S (default 3) consecutive batches rotate over S rigs
+on S streams, so each batch's serial octree replay overlaps the other
+batches' pixel kernels; the one-stream rate is reported beside it.
 
 Also reported on the same line: the plain path (one rig frame at a time, no
 graph), BA linearization throughput (50 keyframes x 3 cameras, 200k points,
@@ -165,8 +165,8 @@ def main():
     ap.add_argument("--no-ba", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-frame-at-a-time comparison run")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch variants")
-    ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
-                    help="batches in flight: 2 alternates two rigs on two streams")
+    ap.add_argument("--streams", type=int, default=3, choices=(1, 2, 3, 4),
+                    help="batches in flight: S > 1 rotates S rigs on S streams")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -228,38 +228,46 @@ def main():
     _, _, counts, _, mcounts, _ = rig.read()
     ms_step, fps = ms_step1, fps1
     if args.streams > 1:
-        # consecutive batches alternate between two rigs on two streams, so one
-        # batch's serial octree replay overlaps the next batch's pixel kernels
-        stream_b = torch.cuda.Stream()
-        ctx_b = Context(0)
-        ctx_b.set_stream(stream_b.cuda_stream)
-        rig_b = RigBatch(ORB(2000, 1.2, 8, 20, 7, ctx=ctx_b), n, ratio=0.7, th=50, nfeatures=nfeat)
-        if not args.no_autotune:
-            rig_b.autotune(ptrs, reps=1)
-        rigs = [rig, rig_b]
-        for k in range(2 * args.warmup):
-            rigs[k % 2].run(ptrs, graph=True)
+        # consecutive batches rotate over S rigs on S streams, so one batch's
+        # serial octree replay overlaps the other batches' pixel kernels
+        S = args.streams
+        extra = []
+        rigs, streams = [rig], [stream]
+        for _ in range(S - 1):
+            st = torch.cuda.Stream()
+            cx = Context(0)
+            cx.set_stream(st.cuda_stream)
+            rg = RigBatch(ORB(2000, 1.2, 8, 20, 7, ctx=cx), n, ratio=0.7, th=50, nfeatures=nfeat)
+            if not args.no_autotune:
+                rg.autotune(ptrs, reps=1)
+            extra.append((rg, cx))
+            rigs.append(rg)
+            streams.append(st)
+        for k in range(S * args.warmup):
+            rigs[k % S].run(ptrs, graph=True)
         torch.cuda.synchronize()
         main = torch.cuda.Stream()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(main)
-        stream.wait_stream(main)
-        stream_b.wait_stream(main)
+        for st in streams:
+            st.wait_stream(main)
         for k in range(args.steps):
-            rigs[k % 2].run(ptrs, graph=True)
-        main.wait_stream(stream)
-        main.wait_stream(stream_b)
+            rigs[k % S].run(ptrs, graph=True)
+        for st in streams:
+            main.wait_stream(st)
         e1.record(main)
         torch.cuda.synchronize()
         ms_step = e0.elapsed_time(e1) / args.steps
         fps = n / (ms_step / 1e3)
-        log(f"batched, two streams: {ms_step:.3f} ms per {n}-frame step -> {fps:.0f} rig frames/s")
-        # both rigs produced the same outputs from the same images
+        log(f"batched, {S} streams: {ms_step:.3f} ms per {n}-frame step -> {fps:.0f} rig frames/s")
+        # every rig produced the same outputs from the same images
         ka, da, ca, ma, mca, _ = rig.read()
-        kb, db, cb, mb, mcb, _ = rig_b.read()
-        assert (ca == cb).all() and (mca == mcb).all(), "the two rigs disagree"
-        rig_b.close()
-        ctx_b.close()
+        for rg, cx in extra:
+            kb, db, cb, mb, mcb, _ = rg.read()
+            assert (ca == cb).all() and (mca == mcb).all(), "the rigs disagree"
+            assert np.array_equal(ka, kb) and np.array_equal(da, db) and np.array_equal(ma, mb), "the rigs disagree"
+            rg.close()
+            cx.close()
 
     plain_fps = None
     if not args.no_plain:

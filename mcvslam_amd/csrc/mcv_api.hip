@@ -389,7 +389,6 @@ struct OrbPlan {
     long long pyr_bytes = 0, pool_cap = 0;
     int n_cand = 0, max_slots_img = 0;
     int heap_cap = 0;   // octree heap entries (dynamic LDS of k_octree_heap)
-    int node_cap = 0;   // node-table entries that k_octree_heap caches in LDS
     std::vector<int> heap_order;  // levels by feature target, largest first
     DevBuf d_heap_order, d_lv, d_cells, d_strips, d_coef, d_slot, d_seeds, d_pyr, d_blur, d_score, d_cell_cand, d_cell_n, d_lvl_cand, d_oct,
         d_pool, d_used, d_nodes, d_src, d_stride;
@@ -503,18 +502,6 @@ struct OrbPlan {
         cap = max_slots_img;
         heap_cap = 16;
         for (const LevelDesc& d : lv) heap_cap = std::max(heap_cap, std::max(d.nfeat, d.n_ini) + 16);
-        {
-            // every level's heap workgroup should be resident at once: split
-            // the CU's 160 KB of LDS over the workgroups each CU must hold
-            int dev = 0, ncu = 256;
-            if (hipGetDevice(&dev) == hipSuccess &&
-                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-                ncu = 256;
-            const int per_cu = std::max(1, ((int)lv.size() + ncu - 1) / std::max(ncu, 1));
-            // (LDS is allocated in blocks: keep 1 KB of margin per workgroup)
-            const int budget = std::min(65536, (160 * 1024 / per_cu) / 1024 * 1024 - 1024);
-            node_cap = std::max(0, std::min(4096, (budget - 8 * heap_cap) / 8));
-        }
         heap_order.resize(lv.size());
         for (size_t k = 0; k < lv.size(); k++) heap_order[k] = (int)k;
         std::stable_sort(heap_order.begin(), heap_order.end(),
@@ -680,8 +667,8 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
     }
     hipLaunchKernelGGL(k_octree_build, dim3(lv.size()), dim3(kOctThreads), 0, s, g);
     mark("k_octree_build");
-    hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), 8 * (size_t)heap_cap + 8 * node_cap, s, g,
-                       d_heap_order.as<int>(), heap_cap, node_cap);
+    hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), 8 * (size_t)heap_cap, s, g,
+                       d_heap_order.as<int>(), heap_cap);
     mark("k_octree_heap");
     hipLaunchKernelGGL(k_keypoints, dim3(xcd_grid((n_slots + 3) / 4)), dim3(256), 0, s, g, d_slot.as<int>(), n_slots, tb,
                        reinterpret_cast<float*>(d_kps), d_desc, out_cap, d_counts);

@@ -815,15 +815,45 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g) {
     uint32_t* child = start + T;
     uint32_t* sorted = child + T;
     for (int i = threadIdx.x; i < n; i += kOctThreads) sorted[i] = (uint32_t)(keys[i] & 0xFFFFFF);
-    for (int i = i0; i < i1; i++) {
-        const int dvi = dv[i];
+    if (n <= kSortCapLds) {
+        // depth by depth: the entries of depth d (start) and of depth d - 1
+        // (child) are assembled in LDS over the dead sort keys, then stored
+        // as contiguous runs.  A thread's entries of one depth are the ranks
+        // rank[d] .. of its positions that start a node there.
+        __syncthreads();
+        uint32_t* st_start = reinterpret_cast<uint32_t*>(s_keys);
+        uint32_t* st_child = st_start + kSortCapLds;
 #pragma unroll
         for (int d = 0; d <= kOctreeDepth; d++) {
-            if (d <= D && dvi <= d) {
-                const int r = rank[d]++;
-                start[off[d] + r] = (uint32_t)i;
-                // the depth d-1 node starting at i was given rank[d-1] - 1 just before
-                if (d > 0 && dvi <= d - 1) child[off[d - 1] + rank[d - 1] - 1] = (uint32_t)(off[d] + r);
+            if (d <= D) {
+                int r = rank[d], rp = d > 0 ? rank[d - 1] : 0;
+                for (int i = i0; i < i1; i++) {
+                    const int dvi = dv[i];
+                    if (dvi <= d) {
+                        st_start[r] = (uint32_t)i;
+                        // the depth d-1 node starting at i has this depth-d node as first child
+                        if (d > 0 && dvi <= d - 1) st_child[rp++] = (uint32_t)(off[d] + r);
+                        r++;
+                    }
+                }
+                __syncthreads();
+                for (int k = threadIdx.x; k < K[d]; k += kOctThreads) start[off[d] + k] = st_start[k];
+                if (d > 0)
+                    for (int k = threadIdx.x; k < K[d - 1]; k += kOctThreads) child[off[d - 1] + k] = st_child[k];
+                __syncthreads();
+            }
+        }
+    } else {
+        for (int i = i0; i < i1; i++) {
+            const int dvi = dv[i];
+#pragma unroll
+            for (int d = 0; d <= kOctreeDepth; d++) {
+                if (d <= D && dvi <= d) {
+                    const int r = rank[d]++;
+                    start[off[d] + r] = (uint32_t)i;
+                    // the depth d-1 node starting at i was given rank[d-1] - 1 just before
+                    if (d > 0 && dvi <= d - 1) child[off[d - 1] + rank[d - 1] - 1] = (uint32_t)(off[d] + r);
+                }
             }
         }
     }
@@ -869,19 +899,16 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g) {
 //
 // The whole wave runs the heap (ParHeap: one LDS round trip per gathered
 // chain or 5-level subtree), and lanes 0..4 fetch the counts and child
-// ranges of a popped node's children together.  The
-// shallow part of the node tables (the first `ncache` entries, depth-major,
-// where the splits happen) is copied to LDS first.  Workgroups take levels
-// in `order` (largest feature targets first).  Output per keypoint slot:
-// the node's entry word (first child or own index) << 3 | children.
+// ranges of a popped node's children together from the node tables in
+// global memory.  The workgroup's LDS is the heap alone (8 bytes per entry),
+// so workgroups of other kernels still fit on a CU while the replays run.
+// Workgroups take levels in `order` (largest feature targets first).  Output
+// per keypoint slot: the node's winning candidate word.
 struct NodeTabs {
     const uint32_t* start;
     const uint32_t* child;
-    const uint32_t* s_start;
-    const uint32_t* s_child;
-    uint32_t ncache;
-    __device__ __forceinline__ uint32_t st(uint32_t i) const { return i < ncache ? s_start[i] : start[i]; }
-    __device__ __forceinline__ uint32_t ch(uint32_t i) const { return i < ncache ? s_child[i] : child[i]; }
+    __device__ __forceinline__ uint32_t st(uint32_t i) const { return start[i]; }
+    __device__ __forceinline__ uint32_t ch(uint32_t i) const { return child[i]; }
 };
 
 // libstdc++ binary heap (push_heap / pop_heap, keys compared alone) run by a
@@ -997,11 +1024,9 @@ __device__ __forceinline__ uint2 node_entry(uint32_t c, uint32_t s0, uint32_t s1
 
 __device__ __forceinline__ uint32_t lane_val(uint32_t v, int k) { return __builtin_amdgcn_readlane(v, k); }
 
-__global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* order, int cap, int node_cap) {
+__global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* order, int cap) {
     extern __shared__ uint32_t heap_lds[];
     ParHeap H{heap_lds, heap_lds + cap};
-    uint32_t* s_start = heap_lds + 2 * cap;
-    uint32_t* s_child = s_start + node_cap;
     const int lane = threadIdx.x;
     const int li = order[blockIdx.x];
     const LevelDesc L = g.lv[li];
@@ -1013,14 +1038,6 @@ __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* ord
     NodeTabs nt;
     nt.start = g.pool + O->tab;
     nt.child = nt.start + O->total;
-    nt.s_start = s_start;
-    nt.s_child = s_child;
-    nt.ncache = (uint32_t)min(O->total, node_cap);
-    for (uint32_t i = lane; i < nt.ncache; i += 64) {
-        s_start[i] = nt.start[i];
-        s_child[i] = nt.child[i];
-    }
-    wave_lds_sync();
     const int nr = O->n_roots;
     int len = 0;
     {

@@ -958,7 +958,12 @@ __device__ __forceinline__ void par_push(ParHeap& H, int& len, uint32_t vk, uint
     len++;
 }
 
-// __adjust_heap(first, 0, len, value) followed by its __push_heap
+// __adjust_heap(first, 0, len, value) followed by its __push_heap.  Each
+// round gathers the 5-level subtree below the hole (lane (2^d - 2) + k holds
+// node k of depth d), every right-child lane compares itself with its left
+// sibling (one lane swap), a ballot collects the choices, and the descent
+// (take the right child unless it is smaller than the left) is walked on
+// the scalar unit; the nodes on the path then move up one level at once.
 __device__ __forceinline__ void par_adjust(ParHeap& H, int len, uint32_t vk, uint32_t vp) {
     const int lane = threadIdx.x & 63;
     // lane -> (depth d in 1..5, offset k) of the subtree below the hole
@@ -974,23 +979,24 @@ __device__ __forceinline__ void par_adjust(ParHeap& H, int len, uint32_t vk, uin
             sk = H.key[idx];
             sp = H.pay[idx];
         }
-        int dst = -1;  // this lane's entry moves to dst
-        int rel = 0;   // offset of the path node at the current depth
+        // right children sit on odd lanes, their left siblings one lane below
+        // (lane swap within pairs: DPP quad_perm [1, 0, 3, 2])
+        const uint32_t sib = (uint32_t)__builtin_amdgcn_mov_dpp((int)sk, 0xB1, 0xF, 0xF, false);
+        const unsigned long long left = __ballot((lane & 1) && lane < 62 && hkey(sk) < hkey(sib));
+        unsigned long long path = 0;
+        int rel = 0;
         for (int dd = 1; dd <= 5 && second < lim; dd++) {
-            second = 2 * (second + 1);
-            const int lr = (1 << dd) - 2 + 2 * rel + 1;       // lane of the right child
-            const uint32_t kr = __builtin_amdgcn_readlane(sk, lr);
-            const uint32_t kl = __builtin_amdgcn_readlane(sk, lr - 1);
-            const bool take_left = hkey(kr) < hkey(kl);
-            const int src = take_left ? lr - 1 : lr;
-            if (take_left) second--;
-            if (lane == src) dst = hole;
-            hole = second;
-            rel = 2 * rel + (take_left ? 0 : 1);
+            const int lr = (1 << dd) - 2 + 2 * rel + 1;  // lane of the right child
+            const int tl = (int)((left >> lr) & 1ull);  // 1: the left child is taken
+            second = 2 * (second + 1) - tl;
+            path |= 1ull << (lr - tl);
+            rel = 2 * rel + 1 - tl;
         }
-        if (dst >= 0) {
-            H.key[dst] = sk;
-            H.pay[dst] = sp;
+        hole = second;
+        if ((path >> lane) & 1ull) {
+            const int parent = (idx - 1) >> 1;
+            H.key[parent] = sk;
+            H.pay[parent] = sp;
         }
         wave_lds_sync();
     }

@@ -294,6 +294,12 @@ class RigBatch:
         arr = (ctypes.c_void_p * n)(*image_ptrs)
         check(self.ctx.h, lib().mcv_rig_run(self.h, arr, _native.MCV_FLAG_GRAPH if graph else 0))
 
+    N_BLUR_VARIANTS = 2
+
+    def set_variants(self, knn: int, blur: int):
+        """Fix the launch variants (top-2 kernel, blur) instead of tuning."""
+        check(self.ctx.h, lib().mcv_rig_set_variants(self.h, int(knn), int(blur)))
+
     def autotune(self, image_ptrs, reps: int = 3) -> str:
         """Time every launch variant of the tunable kernels on these images
         and keep the fastest (outputs are identical); returns the report."""

@@ -1375,17 +1375,17 @@ int mcv_rig_autotune(mcv_rig* r, const uint8_t* const* d_images, int reps, char*
     {
         int best = 0;
         float bt = 1e30f;
-        for (int v = 0; v < 2; v++) {
+        for (int v = 0; v < kBlurVariants; v++) {
             r->tune.blur = v;
             const float t = time_kernel("k_blur");
-            snprintf(line, sizeof line, "k_blur %s %.4f ms\n", v ? "fixed-walk" : "stop-at-end", t);
+            snprintf(line, sizeof line, "k_blur %s %.4f ms\n", blur_variant_name(v), t);
             rep += line;
             if (t < bt) { bt = t; best = v; }
         }
         r->tune.blur = best;
     }
     snprintf(line, sizeof line, "chosen: k_knn2 %s, k_blur %s\n", knn_variant_name(r->tune.knn),
-             r->tune.blur ? "fixed-walk" : "stop-at-end");
+             blur_variant_name(r->tune.blur));
     rep += line;
     P.tune = r->tune;
     for (auto& sp : r->single) sp->tune = r->tune;
@@ -1398,6 +1398,22 @@ int mcv_rig_autotune(mcv_rig* r, const uint8_t* const* d_images, int reps, char*
     if (report && report_cap > 0) {
         strncpy(report, rep.c_str(), (size_t)report_cap - 1);
         report[report_cap - 1] = 0;
+    }
+    API_END
+}
+
+int mcv_rig_set_variants(mcv_rig* r, int knn_variant, int blur_variant) {
+    mcv_ctx* c = r ? r->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(r && knn_variant >= 0 && knn_variant < kKnnVariants && blur_variant >= 0 && blur_variant < kBlurVariants,
+            MCV_ERR_INVALID, "launch variant out of range");
+    r->tune.knn = knn_variant;
+    r->tune.blur = blur_variant;
+    r->batch->tune = r->tune;
+    for (auto& sp : r->single) sp->tune = r->tune;
+    if (r->graph) {
+        HIP_OK(hipGraphExecDestroy(r->graph));
+        r->graph = nullptr;
     }
     API_END
 }

@@ -278,6 +278,10 @@ __global__ __launch_bounds__(256) void k_blur(const Strip* st, int n, const Leve
         blur_strip<kFixed, false>(pyr + L.pix, blur + L.pix, L, x, S.y0, yend, taps);
 }
 
+// Launch variants of the blur (same bytes): the walk kind.
+constexpr int kBlurVariants = 2;
+inline const char* blur_variant_name(int v) { return v == 1 ? "fixed-walk" : "stop-at-end"; }
+
 // ---------------------------------------------------------------------------
 // FAST 9/16.  best(p) = max over the 16 arcs of 9 contiguous circle pixels
 // of min(v - p_k) (darker arc) or min(p_k - v) (brighter arc).  A pixel is a
@@ -506,71 +510,74 @@ __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, cons
 // frame are staged in LDS with coalesced u32 loads that keep the global word
 // alignment (the frame and any byte outside the area are stored as 0), so
 // the centre and neighbour reads of the suppression are LDS reads with no
-// bounds tests.
-constexpr int kCellBand = 16;
+// bounds tests.  A cell of at most kCellBand rows (the 35-pixel grid's cells
+// are 35-52 rows) is staged once and serves both the iniTh and the minTh pass.
+constexpr int kCellBand = 46;
 constexpr int kCellSW = kMaxWin;  // LDS bytes per staged row: >= 4 * ceil((3 + W - 4) / 4)
 
-__device__ __forceinline__ int cell_pass(const uint8_t* __restrict__ sp, uint32_t* __restrict__ out,
-                                         uint32_t* __restrict__ band, int pitch, int DW, int DH, uint32_t magic,
-                                         int t, int offx, int offy) {
+struct CellWin {
+    const uint8_t* wbase;  // word holding detection column -1 of row 0
+    int lead;              // byte of column -1 in that word
+    int nwr;               // u32 per staged row
+    uint32_t wmagic;       // t / nwr for t < (kCellBand + 2) * 24
+    int pitch, DW, DH;
+};
+
+__device__ __forceinline__ void cell_stage(const CellWin& C, uint32_t* __restrict__ band, int y0, int nb) {
+    const int lane = threadIdx.x & 63;
+    const int tot = (nb + 2) * C.nwr;
+    for (int k = lane; k < tot; k += 64) {
+        const int r = (int)(((uint32_t)k * C.wmagic) >> 16), w = k - r * C.nwr;
+        const int ry = y0 - 1 + r;
+        uint32_t v = 0;
+        if (ry >= 0 && ry < C.DH) {
+            v = *reinterpret_cast<const uint32_t*>(C.wbase + (long long)ry * C.pitch + 4 * w);
+            // keep the bytes of detection columns [0, DW)
+            const int lo = C.lead + 1 - 4 * w, hi = C.lead + 1 + C.DW - 4 * w;
+            uint32_t m = 0;
+#pragma unroll
+            for (int bt = 0; bt < 4; bt++)
+                if (bt >= lo && bt < hi) m |= 0xFFu << (8 * bt);
+            v &= m;
+        }
+        band[r * (kCellSW / 4) + w] = v;
+    }
+}
+
+// suppression over the staged rows y0 .. y0 + nb - 1; appends the keepers
+__device__ __forceinline__ int cell_scan(const CellWin& C, const uint32_t* __restrict__ band, int y0, int nb,
+                                         uint32_t magic, int t, int offx, int offy, uint32_t* __restrict__ out,
+                                         int written) {
     const int lane = threadIdx.x & 63;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    // staged row: the words from the one holding detection column -1
-    const uint8_t* wbase = reinterpret_cast<const uint8_t*>(reinterpret_cast<uintptr_t>(sp - 1) & ~(uintptr_t)3);
-    const int lead = (int)((sp - 1) - wbase);  // byte of column -1 in the first word
-    const int nwr = (lead + DW + 2 + 3) >> 2;
-    const uint32_t wmagic = (65536u + (uint32_t)nwr - 1u) / (uint32_t)nwr;  // t / nwr for t < 18 * 24
     const uint8_t* bb = reinterpret_cast<const uint8_t*>(band);
-    int written = 0;
-    for (int y0 = 0; y0 < DH; y0 += kCellBand) {
-        const int nb = min(kCellBand, DH - y0);
-        const int tot = (nb + 2) * nwr;
-        for (int k = lane; k < tot; k += 64) {
-            const int r = (int)(((uint32_t)k * wmagic) >> 16), w = k - r * nwr;
-            const int ry = y0 - 1 + r;
-            uint32_t v = 0;
-            if (ry >= 0 && ry < DH) {
-                v = *reinterpret_cast<const uint32_t*>(wbase + (long long)ry * pitch + 4 * w);
-                // keep the bytes of detection columns [0, DW)
-                const int lo = lead + 1 - 4 * w, hi = lead + 1 + DW - 4 * w;
-                uint32_t m = 0;
+    const int npb = nb * C.DW;
+    for (int p0 = 0; p0 < npb; p0 += 64) {
+        const int p = p0 + lane;
+        bool keep = false;
+        uint32_t rec = 0;
+        if (p < npb) {
+            const int r = (int)(((uint32_t)p * magic) >> 20), x = p - r * C.DW;
+            const uint8_t* q = bb + (r + 1) * kCellSW + C.lead + 1 + x;
+            const int bc = q[0];
+            if (bc > t) {
+                const int s = bc - 1;
+                keep = true;
 #pragma unroll
-                for (int bt = 0; bt < 4; bt++)
-                    if (bt >= lo && bt < hi) m |= 0xFFu << (8 * bt);
-                v &= m;
+                for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                    for (int dx = -1; dx <= 1; dx++) {
+                        if (dx == 0 && dy == 0) continue;
+                        const int bq = q[dy * kCellSW + dx];
+                        const int mq = bq > t ? bq - 1 : 0;
+                        keep = keep && s > mq;
+                    }
+                rec = ((uint32_t)s << 24) | ((uint32_t)(y0 + r + 3 + offy) << 12) | (uint32_t)(x + 3 + offx);
             }
-            band[r * (kCellSW / 4) + w] = v;
         }
-        wave_lds_sync();
-        const int npb = nb * DW;
-        for (int p0 = 0; p0 < npb; p0 += 64) {
-            const int p = p0 + lane;
-            bool keep = false;
-            uint32_t rec = 0;
-            if (p < npb) {
-                const int r = (int)(((uint32_t)p * magic) >> 20), x = p - r * DW;
-                const uint8_t* q = bb + (r + 1) * kCellSW + lead + 1 + x;
-                const int bc = q[0];
-                if (bc > t) {
-                    const int s = bc - 1;
-                    keep = true;
-#pragma unroll
-                    for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-                        for (int dx = -1; dx <= 1; dx++) {
-                            if (dx == 0 && dy == 0) continue;
-                            const int bq = q[dy * kCellSW + dx];
-                            const int mq = bq > t ? bq - 1 : 0;
-                            keep = keep && s > mq;
-                        }
-                    rec = ((uint32_t)s << 24) | ((uint32_t)(y0 + r + 3 + offy) << 12) | (uint32_t)(x + 3 + offx);
-                }
-            }
-            const unsigned long long bal = __ballot(keep);
-            if (keep) out[written + __popcll(bal & lt)] = rec;
-            written += __popcll(bal);
-        }
-        wave_lds_sync();
+        const unsigned long long bal = __ballot(keep);
+        if (keep) out[written + __popcll(bal & lt)] = rec;
+        written += __popcll(bal);
     }
     return written;
 }
@@ -586,11 +593,38 @@ __global__ __launch_bounds__(256) void k_fast_cells(OrbGlobals g, int ncells) {
     int total = 0;
     if (DW > 0 && DH > 0) {
         const uint8_t* sp = g.score + L.pix + (long long)(c.y0 + 3) * L.pitch + (c.x0 + 3);
+        CellWin C;
+        C.wbase = reinterpret_cast<const uint8_t*>(reinterpret_cast<uintptr_t>(sp - 1) & ~(uintptr_t)3);
+        C.lead = (int)((sp - 1) - C.wbase);
+        C.nwr = (C.lead + DW + 2 + 3) >> 2;
+        C.wmagic = (65536u + (uint32_t)C.nwr - 1u) / (uint32_t)C.nwr;
+        C.pitch = L.pitch;
+        C.DW = DW;
+        C.DH = DH;
         const uint32_t magic = ((1u << 20) + (uint32_t)DW - 1u) / (uint32_t)DW;  // p / DW for p < 2^20 / DW
         uint32_t* out = g.cell_cand + c.cap_base;
         uint32_t* band = s_band[threadIdx.x >> 6];
-        total = cell_pass(sp, out, band, L.pitch, DW, DH, magic, g.ini_th, c.offx, c.offy);
-        if (total == 0) total = cell_pass(sp, out, band, L.pitch, DW, DH, magic, g.min_th, c.offx, c.offy);
+        const bool once = DH <= kCellBand;
+        if (once) {
+            cell_stage(C, band, 0, DH);
+            wave_lds_sync();
+        }
+#pragma unroll
+        for (int pass = 0; pass < 2; pass++) {
+            const int t = pass == 0 ? g.ini_th : g.min_th;
+            int written = 0;
+            for (int y0 = 0; y0 < DH; y0 += kCellBand) {
+                const int nb = min(kCellBand, DH - y0);
+                if (!once) {
+                    cell_stage(C, band, y0, nb);
+                    wave_lds_sync();
+                }
+                written = cell_scan(C, band, y0, nb, magic, t, c.offx, c.offy, out, written);
+                if (!once) wave_lds_sync();
+            }
+            total = written;
+            if (total > 0) break;
+        }
     }
     if (lane == 0) g.cell_n[ci] = total;
 }

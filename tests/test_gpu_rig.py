@@ -92,5 +92,27 @@ def test_rig_autotune_keeps_outputs(orb_default):
     before = rig.read()
     report = rig.autotune(ptrs, reps=1)
     assert "chosen:" in report and report.count("k_knn2") >= 5
+    assert report.count("k_blur") == RigBatch.N_BLUR_VARIANTS + 1
     rig.run(ptrs, graph=True)
     assert_same_outputs(before, rig.read())
+
+
+@pytest.mark.parametrize("knn,blur", [(1, 1), (6, 0), (7, 1)])
+def test_rig_fixed_variants(orb_default, knn, blur):
+    """Fixed launch variants (top-2 kernel, blur walk) give the outputs of the
+    default ones, in the graph and in the plain path."""
+    import torch
+    frames = [synth.rig_frame(700 + 10 * f) for f in range(2)]
+    imgs = [torch.from_numpy(im).cuda() for fr in frames for im in fr]
+    ptrs = [im.data_ptr() for im in imgs]
+    rig = RigBatch(orb_default, 2, ratio=0.7, th=50)
+    rig.set_variants(0, 0)
+    rig.run(ptrs, graph=True)
+    before = rig.read()
+    rig.set_variants(knn, blur)
+    rig.run(ptrs, graph=True)
+    assert_same_outputs(before, rig.read())
+    rig.run(ptrs, graph=False)
+    assert_same_outputs(before, rig.read())
+    with pytest.raises(Exception):
+        rig.set_variants(0, RigBatch.N_BLUR_VARIANTS)

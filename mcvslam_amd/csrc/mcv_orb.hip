@@ -135,28 +135,11 @@ __global__ __launch_bounds__(64) void k_resize(const Strip* st, int n, const Lev
     const int r_hi = min(max(coef[L.coef_y + yend - 1].x + 1, 0), SL.h - 1);
     const int nr = r_hi - r_lo + 1;
     const int rwd = rw >> 2;
-    for (int k0 = 0; k0 < nr; k0 += 8) {
-        uint32_t v[8][3];
-#pragma unroll
-        for (int kk = 0; kk < 8; kk++)
-#pragma unroll
-            for (int h = 0; h < 3; h++) {
-                const int d = lane + 64 * h;
-                v[kk][h] = (k0 + kk < nr && d < nw)
-                               ? *reinterpret_cast<const uint32_t*>(sp + (long long)(r_lo + k0 + kk) * SL.pitch + cb + 4 * d)
-                               : 0u;
-            }
-#pragma unroll
-        for (int kk = 0; kk < 8; kk++)
-#pragma unroll
-            for (int h = 0; h < 3; h++) {
-                const int d = lane + 64 * h;
-                if (k0 + kk < nr && d < nw) s_rows[(k0 + kk) * rwd + d] = v[kk][h];
-            }
-    }
-    wave_lds_sync();
-    if (x >= L.w) return;
-    const uint8_t* rows = reinterpret_cast<const uint8_t*>(s_rows);
+    // the strip's row coefficients, one per lane, requested with the
+    // staging loads so that no output row waits on its own load
+    int2 cyl = make_int2(0, 0);
+    if (lane < yend - S.y0) cyl = coef[L.coef_y + S.y0 + lane];
+    // column taps of this lane's 4 outputs
     int o0[4], o1[4], a0[4], a1[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -166,9 +149,42 @@ __global__ __launch_bounds__(64) void k_resize(const Strip* st, int n, const Lev
         a0[j] = cx.y & 0xFFFF;
         a1[j] = cx.y >> 16;
     }
+    // source rows in batches of 8, the next batch requested before the
+    // current one is written to LDS
+    auto load8 = [&](int k0, uint32_t (&v)[8][3]) {
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++)
+#pragma unroll
+            for (int h = 0; h < 3; h++) {
+                const int d = lane + 64 * h;
+                v[kk][h] = (k0 + kk < nr && d < nw)
+                               ? *reinterpret_cast<const uint32_t*>(sp + (long long)(r_lo + k0 + kk) * SL.pitch + cb + 4 * d)
+                               : 0u;
+            }
+    };
+    auto store8 = [&](int k0, const uint32_t (&v)[8][3]) {
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++)
+#pragma unroll
+            for (int h = 0; h < 3; h++) {
+                const int d = lane + 64 * h;
+                if (k0 + kk < nr && d < nw) s_rows[(k0 + kk) * rwd + d] = v[kk][h];
+            }
+    };
+    uint32_t va[8][3], vb[8][3];
+    load8(0, va);
+    for (int k0 = 0; k0 < nr; k0 += 16) {
+        if (k0 + 8 < nr) load8(k0 + 8, vb);
+        store8(k0, va);
+        if (k0 + 16 < nr) load8(k0 + 16, va);
+        if (k0 + 8 < nr) store8(k0 + 8, vb);
+    }
+    wave_lds_sync();
+    const bool own = x < L.w;  // every lane stays for the coefficient shuffles
+    const uint8_t* rows = reinterpret_cast<const uint8_t*>(s_rows);
     uint8_t* dp = pyr + L.pix + x;
     for (int y = S.y0; y < yend; y++) {
-        const int2 cy = coef[L.coef_y + y];
+        const int2 cy = make_int2(__shfl(cyl.x, y - S.y0, 64), __shfl(cyl.y, y - S.y0, 64));
         const int b0 = cy.y & 0xFFFF, b1 = cy.y >> 16;
         const uint8_t* r0p = rows + (min(max(cy.x, 0), SL.h - 1) - r_lo) * rw;
         const uint8_t* r1p = rows + (min(max(cy.x + 1, 0), SL.h - 1) - r_lo) * rw;
@@ -180,7 +196,7 @@ __global__ __launch_bounds__(64) void k_resize(const Strip* st, int n, const Lev
             const uint32_t o = (uint32_t)((((b0 * (r0 >> 4)) >> 16) + ((b1 * (r1 >> 4)) >> 16) + 2) >> 2);
             v |= (o & 0xFF) << (8 * j);
         }
-        *reinterpret_cast<uint32_t*>(dp + (long long)y * L.pitch) = v;
+        if (own) *reinterpret_cast<uint32_t*>(dp + (long long)y * L.pitch) = v;
     }
 }
 
@@ -362,42 +378,30 @@ __device__ __forceinline__ uint32_t compass2(mcv_us2 v, mcv_us2 s, mcv_us2 e, mc
     return as_u32(__builtin_elementwise_max(pd, pb));
 }
 
-// exact best of the pixel at byte c of the ring rows r[0..6] (rows -3..+3)
+// exact best of the pixel at byte c of the ring rows r[0..6] (rows -3..+3).
+// Both polarities run in one packed-i16 min/max tree: lane half 0 carries
+// v - p_k (darker arcs), half 1 carries p_k - v (brighter arcs).
+typedef short mcv_s2 __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ int fast_best_rows(const uint8_t* const* r, int c) {
     const int v = r[3][c];
-    int d[16];
-    d[0] = v - r[6][c];
-    d[1] = v - r[6][c + 1];
-    d[2] = v - r[5][c + 2];
-    d[3] = v - r[4][c + 3];
-    d[4] = v - r[3][c + 3];
-    d[5] = v - r[2][c + 3];
-    d[6] = v - r[1][c + 2];
-    d[7] = v - r[0][c + 1];
-    d[8] = v - r[0][c];
-    d[9] = v - r[0][c - 1];
-    d[10] = v - r[1][c - 2];
-    d[11] = v - r[2][c - 3];
-    d[12] = v - r[3][c - 3];
-    d[13] = v - r[4][c - 3];
-    d[14] = v - r[5][c - 2];
-    d[15] = v - r[6][c - 1];
-    int m2[16], m4[16];
+    const int p[16] = {r[6][c],     r[6][c + 1], r[5][c + 2], r[4][c + 3], r[3][c + 3], r[2][c + 3],
+                       r[1][c + 2], r[0][c + 1], r[0][c],     r[0][c - 1], r[1][c - 2], r[2][c - 3],
+                       r[3][c - 3], r[4][c - 3], r[5][c - 2], r[6][c - 1]};
+    mcv_s2 e[16];
 #pragma unroll
-    for (int k = 0; k < 16; k++) m2[k] = min(d[k], d[(k + 1) & 15]);
+    for (int k = 0; k < 16; k++) e[k] = mcv_s2{(short)(v - p[k]), (short)(p[k] - v)};
+    mcv_s2 m2[16], m4[16];
 #pragma unroll
-    for (int k = 0; k < 16; k++) m4[k] = min(m2[k], m2[(k + 2) & 15]);
-    int dark = -1024;
+    for (int k = 0; k < 16; k++) m2[k] = __builtin_elementwise_min(e[k], e[(k + 1) & 15]);
 #pragma unroll
-    for (int k = 0; k < 16; k++) dark = max(dark, min(min(m4[k], m4[(k + 4) & 15]), d[(k + 8) & 15]));
+    for (int k = 0; k < 16; k++) m4[k] = __builtin_elementwise_min(m2[k], m2[(k + 2) & 15]);
+    mcv_s2 best = mcv_s2{(short)-1024, (short)-1024};
 #pragma unroll
-    for (int k = 0; k < 16; k++) m2[k] = max(d[k], d[(k + 1) & 15]);
-#pragma unroll
-    for (int k = 0; k < 16; k++) m4[k] = max(m2[k], m2[(k + 2) & 15]);
-    int bright = -1024;
-#pragma unroll
-    for (int k = 0; k < 16; k++) bright = max(bright, -max(max(m4[k], m4[(k + 4) & 15]), d[(k + 8) & 15]));
-    const int b = max(dark, bright);
+    for (int k = 0; k < 16; k++)
+        best = __builtin_elementwise_max(
+            best, __builtin_elementwise_min(__builtin_elementwise_min(m4[k], m4[(k + 4) & 15]), e[(k + 8) & 15]));
+    const int b = max((int)best.x, (int)best.y);
     return b < 0 ? 0 : b;
 }
 

@@ -741,14 +741,21 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g) {
     int run = 0;
     for (int c0 = 0; c0 < L.cell_count; c0 += kOctThreads) {
         const int ci = c0 + threadIdx.x;
-        const int cnt = ci < L.cell_count ? g.cell_n[L.cell_first + ci] : 0;
+        const bool has = ci < L.cell_count;
+        const int cnt = has ? g.cell_n[L.cell_first + ci] : 0;
+        const int cap_base = has ? g.cells[L.cell_first + ci].cap_base : 0;
         int tot;
         const int pos = block_excl_scan256(cnt, ws, tot);
-        if (ci < L.cell_count) {
-            const CellDesc& c = g.cells[L.cell_first + ci];
-            for (int k = 0; k < cnt; k++) {
-                const int dst = run + pos + k;
-                if (dst < L.cand_cap) g.lvl_cand[L.cand_base + dst] = g.cell_cand[c.cap_base + k];
+        // 8 candidate words in flight per step
+        for (int k0 = 0; k0 < cnt; k0 += 8) {
+            uint32_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (k0 + u < cnt) v[u] = g.cell_cand[cap_base + k0 + u];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int dst = run + pos + k0 + u;
+                if (k0 + u < cnt && dst < L.cand_cap) g.lvl_cand[L.cand_base + dst] = v[u];
             }
         }
         run += tot;
@@ -1203,9 +1210,51 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
     const int wave = xb * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (wave >= n_slots) return;
+    // slot `wave` is (level li, j) with L.kp_slot + j == wave, so the heap's
+    // winner word for it is requested together with the slot entry
     const int2 sl = reinterpret_cast<const int2*>(slot_lvl)[wave];
+    const uint32_t win = g.nodes_out[wave];
     const int li = sl.x, j = sl.y;
     const LevelDesc L = g.lv[li];
+    const uint8_t* bl = g.blur + L.pix;
+    const uint8_t* lvl = g.pyr + L.pix;
+    // blurred patch rows (37 x 10 words around the centre) and IC_Angle rows
+    // (31 x 9 words from the aligned column below cx - 15) of a centre
+    auto load_patch = [&](int cx, int cy, uint32_t (&pw)[kPatchIt]) {
+        const int bx = (cx - kPatchR) & ~3;
+#pragma unroll
+        for (int it = 0; it < kPatchIt; it++) {
+            const int k = lane + 64 * it, r = k / kPatchWords, w = k - kPatchWords * r;
+            pw[it] = 0;
+            if (k < kPatchRows * kPatchWords && bx + 4 * w + 4 <= L.pitch)
+                pw[it] = *reinterpret_cast<const uint32_t*>(bl + (long long)(cy - kPatchR + r) * L.pitch + bx + 4 * w);
+        }
+    };
+    auto load_ic = [&](int cx, int cy, uint32_t (&dw)[5]) {
+        const int base = (cx - 15) & ~3;
+        const int nwr = (cx + 15 - base) / 4 + 1;
+#pragma unroll
+        for (int it = 0; it < 5; it++) {
+            const int t = lane + 64 * it;
+            const int row = t / 9, w = t - 9 * row;
+            dw[it] = 0;
+            if (t < 31 * 9 && w < nwr)
+                dw[it] = *reinterpret_cast<const uint32_t*>(lvl + (long long)(cy + row - 15) * L.pitch + base + 4 * w);
+        }
+    };
+    // The heap's keypoints lie in [19, w - 19] x [19, h - 19].  Their patch
+    // and moment rows are requested before the level counts arrive, at the
+    // winner word's coordinates clamped to that range (exact for a keypoint;
+    // for a slot that turns out to be a seed or empty the loads stay inside
+    // the level and are dropped).
+    const bool spec = L.w >= 2 * kPatchR + 3 && L.h >= 2 * kPatchR + 3;
+    const int sx = min(max((int)(win & 0xFFF) + L.minBX, kPatchR + 1), L.w - kPatchR - 1);
+    const int sy = min(max((int)((win >> 12) & 0xFFF) + L.minBY, kPatchR + 1), L.h - kPatchR - 1);
+    uint32_t pw[kPatchIt], dw[5];
+    if (spec) {
+        load_patch(sx, sy, pw);
+        load_ic(sx, sy, dw);
+    }
     // keypoints per level of this image (octree output + seeds), one lane per level
     int cnt_l = 0;
     if (lane < L.nlevels) cnt_l = g.oct[L.first_level + lane].nout + g.lv[L.first_level + lane].seed_count;
@@ -1222,7 +1271,7 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
     const bool seed = j >= nout;
     if (!seed) {
         // the slot's keypoint as the heap kernel left it: response << 24 | y << 12 | x
-        const uint32_t r = g.nodes_out[L.kp_slot + j];
+        const uint32_t r = win;
         x = (float)((int)(r & 0xFFF) + L.minBX);
         y = (float)((int)((r >> 12) & 0xFFF) + L.minBY);
         resp = (float)(r >> 24);
@@ -1238,34 +1287,23 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
     const int cx = (int)rintf(x), cy = (int)rintf(y);
     // The rotated test points lie within 18 px of the centre (the pattern's
     // largest radius is 18.38).  When that square is inside the level, the
-    // blurred 37 x 40-byte patch around it is staged in LDS with coalesced
-    // u32 loads issued together with the moment loads below, and the 512
-    // taps are LDS reads; otherwise (seeds near the border) the taps are
-    // clamped gathers from the level.
+    // blurred 37 x 40-byte patch around it is staged in LDS and the 512 taps
+    // are LDS reads; otherwise (seeds near the border) the taps are clamped
+    // gathers from the level.
     const bool staged = cx >= kPatchR && cx + kPatchR < L.w && cy >= kPatchR && cy + kPatchR < L.h;
-    const uint8_t* bl = g.blur + L.pix;
     const int bx = (cx - kPatchR) & ~3;
-    uint32_t pw[kPatchIt];
-    if (staged) {
-#pragma unroll
-        for (int it = 0; it < kPatchIt; it++) {
-            const int k = lane + 64 * it, r = k / kPatchWords, w = k - kPatchWords * r;
-            pw[it] = 0;
-            if (k < kPatchRows * kPatchWords && bx + 4 * w + 4 <= L.pitch)
-                pw[it] = *reinterpret_cast<const uint32_t*>(bl + (long long)(cy - kPatchR + r) * L.pitch + bx + 4 * w);
-        }
-    }
+    const bool have = spec && cx == sx && cy == sy;  // the early loads are this centre's
+    if (staged && !have) load_patch(cx, cy, pw);
     if (!seed) {
-        // IC_Angle over the circular patch (|u| <= umax[|v|], |v| <= 15): the
-        // 31 rows are read as u32 words from the aligned column below cx - 15
-        // (at most 9 words a row, 279 words over the wave), each byte kept if
-        // inside the disk; the integer moments are order independent
+        if (!have) load_ic(cx, cy, dw);
+        // IC_Angle over the circular patch (|u| <= umax[|v|], |v| <= 15):
+        // each byte of the row words kept if inside the disk; the integer
+        // moments are order independent
         const int base = (cx - 15) & ~3;
         const int nwr = (cx + 15 - base) / 4 + 1;
         unsigned long long um = 0;
 #pragma unroll
         for (int k = 0; k < 16; k++) um |= (unsigned long long)(g.umax[k] & 15) << (4 * k);
-        const uint8_t* lvl = g.pyr + L.pix;
         int m10 = 0, m01 = 0;
 #pragma unroll
         for (int it = 0; it < 5; it++) {
@@ -1274,13 +1312,11 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
             if (t < 31 * 9 && w < nwr) {
                 const int v = row - 15;
                 const int du = (int)((um >> (4 * (v < 0 ? -v : v))) & 15u);
-                const uint32_t dw =
-                    *reinterpret_cast<const uint32_t*>(lvl + (long long)(cy + v) * L.pitch + base + 4 * w);
 #pragma unroll
                 for (int b = 0; b < 4; b++) {
                     const int u = base + 4 * w + b - cx;
                     if (u >= -du && u <= du) {
-                        const int val = (int)((dw >> (8 * b)) & 0xFFu);
+                        const int val = (int)((dw[it] >> (8 * b)) & 0xFFu);
                         m10 += u * val;
                         m01 += v * val;
                     }

@@ -390,7 +390,9 @@ struct OrbPlan {
     int n_cand = 0, max_slots_img = 0;
     int heap_cap = 0;   // octree heap entries (dynamic LDS of k_octree_heap)
     std::vector<int> heap_order;  // levels by feature target, largest first
-    DevBuf d_heap_order, d_lv, d_cells, d_strips, d_coef, d_slot, d_seeds, d_pyr, d_blur, d_score, d_cell_cand, d_cell_n, d_lvl_cand, d_oct,
+    std::vector<int> build_levels;  // large levels (1024-thread octree build) first, then the rest
+    int n_build_large = 0;
+    DevBuf d_heap_order, d_build_levels, d_lv, d_cells, d_strips, d_coef, d_slot, d_seeds, d_pyr, d_blur, d_score, d_cell_cand, d_cell_n, d_lvl_cand, d_oct,
         d_pool, d_used, d_nodes, d_src, d_stride;
     hipGraphExec_t graph = nullptr;
     const void* graph_key = nullptr;
@@ -502,6 +504,14 @@ struct OrbPlan {
         cap = max_slots_img;
         heap_cap = 16;
         for (const LevelDesc& d : lv) heap_cap = std::max(heap_cap, std::max(d.nfeat, d.n_ini) + 16);
+        // levels of at least kOctLargePixels pixels (the most FAST
+        // candidates) get 1024-thread octree workgroups
+        build_levels.clear();
+        for (size_t k = 0; k < lv.size(); k++)
+            if ((long long)lv[k].w * lv[k].h >= kOctLargePixels) build_levels.push_back((int)k);
+        n_build_large = (int)build_levels.size();
+        for (size_t k = 0; k < lv.size(); k++)
+            if ((long long)lv[k].w * lv[k].h < kOctLargePixels) build_levels.push_back((int)k);
         heap_order.resize(lv.size());
         for (size_t k = 0; k < lv.size(); k++) heap_order[k] = (int)k;
         std::stable_sort(heap_order.begin(), heap_order.end(),
@@ -546,6 +556,7 @@ struct OrbPlan {
         upload(d_cells, cells, s);
         upload(d_strips, strips, s);
         upload(d_heap_order, heap_order, s);
+        upload(d_build_levels, build_levels, s);
         upload(d_coef, coef, s);
         upload(d_slot, slot_lvl, s);
         upload(d_seeds, seeds, s);
@@ -665,7 +676,11 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
         hipLaunchKernelGGL(k_fast_cells, dim3(xcd_grid((nc + 3) / 4)), dim3(256), 0, s, g, nc);
         mark("k_fast_cells");
     }
-    hipLaunchKernelGGL(k_octree_build, dim3(lv.size()), dim3(kOctThreads), 0, s, g);
+    if (n_build_large > 0)
+        hipLaunchKernelGGL(k_octree_build<1024>, dim3(n_build_large), dim3(1024), 0, s, g, d_build_levels.as<int>());
+    if ((int)lv.size() > n_build_large)
+        hipLaunchKernelGGL(k_octree_build<256>, dim3((int)lv.size() - n_build_large), dim3(256), 0, s, g,
+                           d_build_levels.as<int>() + n_build_large);
     mark("k_octree_build");
     hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), 8 * (size_t)heap_cap, s, g,
                        d_heap_order.as<int>(), heap_cap);

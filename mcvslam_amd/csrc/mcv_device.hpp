@@ -36,20 +36,20 @@ __device__ __forceinline__ int xcd_block(int nb) {
 inline int xcd_grid(int nb) { return (nb + 7) / 8 * 8; }
 
 // Ascending bitonic sort of n2 (a power of two) keys by the whole workgroup.
-template <typename T>
+template <typename T, int kBatch = 8>
 __device__ void bitonic_sort(T* a, int n2) {
     // each thread takes the compare-exchange pairs p = threadIdx.x + m *
     // blockDim.x of a stage (pair p of distance j starts at element
-    // (p / j) * 2j + p % j); 8 pairs are read before any is written, so a
-    // stage costs one LDS round trip per thread, not one per pair
+    // (p / j) * 2j + p % j); kBatch pairs are read before any is written, so
+    // a stage costs one LDS round trip per kBatch pairs of a thread
     const int npairs = n2 >> 1;
     for (int k = 2; k <= n2; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int p0 = 0; p0 < npairs; p0 += 8 * (int)blockDim.x) {
-                T x[8], y[8];
-                int ia[8];
+            for (int p0 = 0; p0 < npairs; p0 += kBatch * (int)blockDim.x) {
+                T x[kBatch], y[kBatch];
+                int ia[kBatch];
 #pragma unroll
-                for (int u = 0; u < 8; u++) {
+                for (int u = 0; u < kBatch; u++) {
                     const int p = p0 + u * (int)blockDim.x + (int)threadIdx.x;
                     ia[u] = ((p & ~(j - 1)) << 1) | (p & (j - 1));
                     if (p < npairs) {
@@ -58,7 +58,7 @@ __device__ void bitonic_sort(T* a, int n2) {
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < 8; u++) {
+                for (int u = 0; u < kBatch; u++) {
                     const int p = p0 + u * (int)blockDim.x + (int)threadIdx.x;
                     const bool up = (ia[u] & k) == 0;
                     if (p < npairs && (x[u] > y[u]) == up) {

@@ -633,8 +633,10 @@ __global__ __launch_bounds__(256) void k_fast_cells(OrbGlobals g, int ncells) {
     if (lane == 0) g.cell_n[ci] = total;
 }
 
-__device__ __forceinline__ int block_excl_scan256(int v, int* ws, int& total) {
-    // ws: 4 ints of LDS; returns exclusive prefix of v over the block
+// Exclusive scan of v over a workgroup of kWaves waves; ws: kWaves ints of
+// LDS; total receives the block sum.
+template <int kWaves>
+__device__ __forceinline__ int block_excl_scan(int v, int* ws, int& total) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     int x = v;
 #pragma unroll
@@ -647,7 +649,7 @@ __device__ __forceinline__ int block_excl_scan256(int v, int* ws, int& total) {
     int base = 0;
     total = 0;
 #pragma unroll
-    for (int w = 0; w < 4; w++) {
+    for (int w = 0; w < kWaves; w++) {
         const int s = ws[w];
         base += (w < wid) ? s : 0;
         total += s;
@@ -689,10 +691,10 @@ __device__ __forceinline__ int code_div(unsigned long long a, unsigned long long
     return kOctreeDepth - hb / 2;
 }
 
-constexpr int kOctThreads = 256;
-
-// Exclusive scan over the 256-thread block of every depth's count at once;
-// total[d] receives the block sum.  ws: 4 x (kOctreeDepth + 1) ints of LDS.
+// Exclusive scan over a workgroup of kWaves waves of every depth's count at
+// once; total[d] receives the block sum.  ws: (kWaves + 1) x (kOctreeDepth +
+// 1) ints of LDS.
+template <int kWaves>
 __device__ __forceinline__ void block_excl_scan_depths(int (&v)[kOctreeDepth + 1], int (&total)[kOctreeDepth + 1],
                                                        int (*ws)[kOctreeDepth + 1]) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -712,28 +714,39 @@ __device__ __forceinline__ void block_excl_scan_depths(int (&v)[kOctreeDepth + 1
         for (int d = 0; d <= kOctreeDepth; d++) ws[wid][d] = x[d];
     }
     __syncthreads();
+    // one thread per depth turns the wave sums into exclusive wave offsets
+    // (row kWaves receives the totals)
+    if (threadIdx.x <= kOctreeDepth) {
+        const int d = threadIdx.x;
+        int run = 0;
+        for (int w = 0; w < kWaves; w++) {
+            const int t = ws[w][d];
+            ws[w][d] = run;
+            run += t;
+        }
+        ws[kWaves][d] = run;
+    }
+    __syncthreads();
 #pragma unroll
     for (int d = 0; d <= kOctreeDepth; d++) {
-        int base = 0, tot = 0;
-#pragma unroll
-        for (int w = 0; w < 4; w++) {
-            const int s = ws[w][d];
-            base += (w < wid) ? s : 0;
-            tot += s;
-        }
-        v[d] = base + x[d] - v[d];
-        total[d] = tot;
+        v[d] = ws[wid][d] + x[d] - v[d];
+        total[d] = ws[kWaves][d];
     }
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g) {
+// kOctThreads: 256 for most levels, 1024 for the large ones (the bitonic
+// stages and the per-thread chunks then spread over 16 waves); `levels`
+// lists the levels of this launch.
+template <int kOctThreads>
+__global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g, const int* levels) {
+    constexpr int kWaves = kOctThreads / 64;
     __shared__ unsigned long long s_keys[kSortCapLds];
     __shared__ uint8_t s_div[kSortCapLds];
-    __shared__ int ws[4];
-    __shared__ int s_ws[4][kOctreeDepth + 1];
+    __shared__ int ws[kWaves];
+    __shared__ int s_ws[kWaves + 1][kOctreeDepth + 1];
     __shared__ long long s_off;
-    const int li = blockIdx.x;
+    const int li = levels[blockIdx.x];
     const LevelDesc L = g.lv[li];
     OctLevel* O = g.oct + li;
     // 1. gather the level's candidates in cell order (cells are stored
@@ -745,7 +758,7 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g) {
         const int cnt = has ? g.cell_n[L.cell_first + ci] : 0;
         const int cap_base = has ? g.cells[L.cell_first + ci].cap_base : 0;
         int tot;
-        const int pos = block_excl_scan256(cnt, ws, tot);
+        const int pos = block_excl_scan<kWaves>(cnt, ws, tot);
         // 8 candidate words in flight per step
         for (int k0 = 0; k0 < cnt; k0 += 8) {
             uint32_t v[8];
@@ -810,7 +823,7 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g) {
         keys[i] = k;
     }
     __syncthreads();
-    bitonic_sort(keys, n2);
+    bitonic_sort<unsigned long long, (kSortCapLds / 2 + kOctThreads - 1) / kOctThreads>(keys, n2);
     // 3. divergence depth of each sorted position
     for (int i = threadIdx.x; i < n; i += kOctThreads)
         dv[i] = (uint8_t)(i == 0 ? 0 : code_div(keys[i] >> 24, keys[i - 1] >> 24));
@@ -830,7 +843,7 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g) {
         for (int d = 0; d <= kOctreeDepth; d++) rank[d] += dvi <= d ? 1 : 0;
     }
     int K[kOctreeDepth + 1];
-    block_excl_scan_depths(rank, K, s_ws);
+    block_excl_scan_depths<kWaves>(rank, K, s_ws);
     // K[d] = nodes at depth d; the deepest depth D is the first with K == n
     int D = kOctreeDepth, T = 0;
     int off[kOctreeDepth + 1];
@@ -1028,17 +1041,22 @@ __device__ __forceinline__ void par_adjust(ParHeap& H, int len, uint32_t vk, uin
         // (lane swap within pairs: DPP quad_perm [1, 0, 3, 2])
         const uint32_t sib = (uint32_t)__builtin_amdgcn_mov_dpp((int)sk, 0xB1, 0xF, 0xF, false);
         const unsigned long long left = __ballot((lane & 1) && lane < 62 && hkey(sk) < hkey(sib));
-        unsigned long long path = 0;
-        int rel = 0;
-        for (int dd = 1; dd <= 5 && second < lim; dd++) {
-            const int lr = (1 << dd) - 2 + 2 * rel + 1;  // lane of the right child
-            const int tl = (int)((left >> lr) & 1ull);  // 1: the left child is taken
-            second = 2 * (second + 1) - tl;
-            path |= 1ull << (lr - tl);
-            rel = 2 * rel + 1 - tl;
+        // the descent: rel is the offset of the current node at depth dd
+        // (its lane is (2^dd - 2) + rel), second its heap position
+        // (unrolled: once second reaches lim it stays there)
+        int rel = 0, dd = 0;
+#pragma unroll
+        for (int e = 1; e <= 5; e++) {
+            if (second < lim) {
+                const int tl = (int)((left >> ((1 << e) - 1 + 2 * rel)) & 1ull);  // 1: left child taken
+                second = 2 * second + 2 - tl;
+                rel = 2 * rel + 1 - tl;
+                dd = e;
+            }
         }
         hole = second;
-        if ((path >> lane) & 1ull) {
+        // the path holds, per depth d <= dd, the node at offset rel >> (dd - d)
+        if (lane < 62 && d <= dd && k == (rel >> (dd - d))) {
             const int parent = (idx - 1) >> 1;
             H.key[parent] = sk;
             H.pay[parent] = sp;
@@ -1216,45 +1234,6 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
     const uint32_t win = g.nodes_out[wave];
     const int li = sl.x, j = sl.y;
     const LevelDesc L = g.lv[li];
-    const uint8_t* bl = g.blur + L.pix;
-    const uint8_t* lvl = g.pyr + L.pix;
-    // blurred patch rows (37 x 10 words around the centre) and IC_Angle rows
-    // (31 x 9 words from the aligned column below cx - 15) of a centre
-    auto load_patch = [&](int cx, int cy, uint32_t (&pw)[kPatchIt]) {
-        const int bx = (cx - kPatchR) & ~3;
-#pragma unroll
-        for (int it = 0; it < kPatchIt; it++) {
-            const int k = lane + 64 * it, r = k / kPatchWords, w = k - kPatchWords * r;
-            pw[it] = 0;
-            if (k < kPatchRows * kPatchWords && bx + 4 * w + 4 <= L.pitch)
-                pw[it] = *reinterpret_cast<const uint32_t*>(bl + (long long)(cy - kPatchR + r) * L.pitch + bx + 4 * w);
-        }
-    };
-    auto load_ic = [&](int cx, int cy, uint32_t (&dw)[5]) {
-        const int base = (cx - 15) & ~3;
-        const int nwr = (cx + 15 - base) / 4 + 1;
-#pragma unroll
-        for (int it = 0; it < 5; it++) {
-            const int t = lane + 64 * it;
-            const int row = t / 9, w = t - 9 * row;
-            dw[it] = 0;
-            if (t < 31 * 9 && w < nwr)
-                dw[it] = *reinterpret_cast<const uint32_t*>(lvl + (long long)(cy + row - 15) * L.pitch + base + 4 * w);
-        }
-    };
-    // The heap's keypoints lie in [19, w - 19] x [19, h - 19].  Their patch
-    // and moment rows are requested before the level counts arrive, at the
-    // winner word's coordinates clamped to that range (exact for a keypoint;
-    // for a slot that turns out to be a seed or empty the loads stay inside
-    // the level and are dropped).
-    const bool spec = L.w >= 2 * kPatchR + 3 && L.h >= 2 * kPatchR + 3;
-    const int sx = min(max((int)(win & 0xFFF) + L.minBX, kPatchR + 1), L.w - kPatchR - 1);
-    const int sy = min(max((int)((win >> 12) & 0xFFF) + L.minBY, kPatchR + 1), L.h - kPatchR - 1);
-    uint32_t pw[kPatchIt], dw[5];
-    if (spec) {
-        load_patch(sx, sy, pw);
-        load_ic(sx, sy, dw);
-    }
     // keypoints per level of this image (octree output + seeds), one lane per level
     int cnt_l = 0;
     if (lane < L.nlevels) cnt_l = g.oct[L.first_level + lane].nout + g.lv[L.first_level + lane].seed_count;
@@ -1287,23 +1266,34 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
     const int cx = (int)rintf(x), cy = (int)rintf(y);
     // The rotated test points lie within 18 px of the centre (the pattern's
     // largest radius is 18.38).  When that square is inside the level, the
-    // blurred 37 x 40-byte patch around it is staged in LDS and the 512 taps
-    // are LDS reads; otherwise (seeds near the border) the taps are clamped
-    // gathers from the level.
+    // blurred 37 x 40-byte patch around it is staged in LDS with coalesced
+    // u32 loads issued together with the moment loads below, and the 512
+    // taps are LDS reads; otherwise (seeds near the border) the taps are
+    // clamped gathers from the level.
     const bool staged = cx >= kPatchR && cx + kPatchR < L.w && cy >= kPatchR && cy + kPatchR < L.h;
+    const uint8_t* bl = g.blur + L.pix;
     const int bx = (cx - kPatchR) & ~3;
-    const bool have = spec && cx == sx && cy == sy;  // the early loads are this centre's
-    if (staged && !have) load_patch(cx, cy, pw);
+    uint32_t pw[kPatchIt];
+    if (staged) {
+#pragma unroll
+        for (int it = 0; it < kPatchIt; it++) {
+            const int k = lane + 64 * it, r = k / kPatchWords, w = k - kPatchWords * r;
+            pw[it] = 0;
+            if (k < kPatchRows * kPatchWords && bx + 4 * w + 4 <= L.pitch)
+                pw[it] = *reinterpret_cast<const uint32_t*>(bl + (long long)(cy - kPatchR + r) * L.pitch + bx + 4 * w);
+        }
+    }
     if (!seed) {
-        if (!have) load_ic(cx, cy, dw);
-        // IC_Angle over the circular patch (|u| <= umax[|v|], |v| <= 15):
-        // each byte of the row words kept if inside the disk; the integer
-        // moments are order independent
+        // IC_Angle over the circular patch (|u| <= umax[|v|], |v| <= 15): the
+        // 31 rows are read as u32 words from the aligned column below cx - 15
+        // (at most 9 words a row, 279 words over the wave), each byte kept if
+        // inside the disk; the integer moments are order independent
         const int base = (cx - 15) & ~3;
         const int nwr = (cx + 15 - base) / 4 + 1;
         unsigned long long um = 0;
 #pragma unroll
         for (int k = 0; k < 16; k++) um |= (unsigned long long)(g.umax[k] & 15) << (4 * k);
+        const uint8_t* lvl = g.pyr + L.pix;
         int m10 = 0, m01 = 0;
 #pragma unroll
         for (int it = 0; it < 5; it++) {
@@ -1312,11 +1302,13 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
             if (t < 31 * 9 && w < nwr) {
                 const int v = row - 15;
                 const int du = (int)((um >> (4 * (v < 0 ? -v : v))) & 15u);
+                const uint32_t dw =
+                    *reinterpret_cast<const uint32_t*>(lvl + (long long)(cy + v) * L.pitch + base + 4 * w);
 #pragma unroll
                 for (int b = 0; b < 4; b++) {
                     const int u = base + 4 * w + b - cx;
                     if (u >= -du && u <= du) {
-                        const int val = (int)((dw[it] >> (8 * b)) & 0xFFu);
+                        const int val = (int)((dw >> (8 * b)) & 0xFFu);
                         m10 += u * val;
                         m01 += v * val;
                     }

@@ -1022,21 +1022,32 @@ __device__ __forceinline__ void par_push(ParHeap& H, int& len, uint32_t vk, uint
 // sibling (one lane swap), a ballot collects the choices, and the descent
 // (take the right child unless it is smaller than the left) is walked on
 // the scalar unit; the nodes on the path then move up one level at once.
-__device__ __forceinline__ void par_adjust(ParHeap& H, int len, uint32_t vk, uint32_t vp) {
+// lane -> (depth d in 1..5, offset k) of a 5-level subtree
+__device__ __forceinline__ int subtree_depth(int lane) {
+    return lane < 2 ? 1 : lane < 6 ? 2 : lane < 14 ? 3 : lane < 30 ? 4 : 5;
+}
+
+// (sk, sp): this lane's node of the subtree below the root, already read
+// (0 where the position is at or past len)
+__device__ __forceinline__ void par_adjust(ParHeap& H, int len, uint32_t vk, uint32_t vp, uint32_t sk, uint32_t sp) {
     const int lane = threadIdx.x & 63;
-    // lane -> (depth d in 1..5, offset k) of the subtree below the hole
-    const int d = lane < 2 ? 1 : lane < 6 ? 2 : lane < 14 ? 3 : lane < 30 ? 4 : 5;
+    const int d = subtree_depth(lane);
     const int k = lane - ((1 << d) - 2);
     int hole = 0, second = 0;
     const int lim = (len - 1) / 2;
+    bool first = true;
     while (second < lim) {
-        // gather the subtree below the hole
+        // gather the subtree below the hole (the root's came with the pop)
         const int idx = ((hole + 1) << d) - 1 + k;
-        uint32_t sk = 0, sp = 0;
-        if (lane < 62 && idx < len) {
-            sk = H.key[idx];
-            sp = H.pay[idx];
+        if (!first) {
+            sk = 0;
+            sp = 0;
+            if (lane < 62 && idx < len) {
+                sk = H.key[idx];
+                sp = H.pay[idx];
+            }
         }
+        first = false;
         // right children sit on odd lanes, their left siblings one lane below
         // (lane swap within pairs: DPP quad_perm [1, 0, 3, 2])
         const uint32_t sib = (uint32_t)__builtin_amdgcn_mov_dpp((int)sk, 0xB1, 0xF, 0xF, false);
@@ -1075,13 +1086,38 @@ __device__ __forceinline__ void par_adjust(ParHeap& H, int len, uint32_t vk, uin
     par_sift_up(H, hole, 0, vk, vp);
 }
 
-__device__ __forceinline__ void par_pop(ParHeap& H, int& len) {
-    if (len > 1) {
-        const int last = len - 1;
-        const uint32_t vk = __builtin_amdgcn_readfirstlane(H.key[last]);
-        const uint32_t vp = __builtin_amdgcn_readfirstlane(H.pay[last]);
-        par_adjust(H, last, vk, vp);
+// The reads a pop starts with, in one LDS round trip: the top, the last
+// entry and the 5-level subtree below the root.
+struct PopReads {
+    uint32_t tk, tp, lk, lp, sk, sp;
+};
+
+__device__ __forceinline__ PopReads par_pop_reads(const ParHeap& H, int len) {
+    const int lane = threadIdx.x & 63;
+    const int d = subtree_depth(lane);
+    const int idx = (1 << d) - 1 + (lane - ((1 << d) - 2));
+    const int last = len - 1;
+    PopReads r;
+    r.tk = H.key[0];
+    r.tp = H.pay[0];
+    r.lk = H.key[last];
+    r.lp = H.pay[last];
+    r.sk = 0;
+    r.sp = 0;
+    if (lane < 62 && idx < last) {
+        r.sk = H.key[idx];
+        r.sp = H.pay[idx];
     }
+    r.tk = __builtin_amdgcn_readfirstlane(r.tk);
+    r.tp = __builtin_amdgcn_readfirstlane(r.tp);
+    r.lk = __builtin_amdgcn_readfirstlane(r.lk);
+    r.lp = __builtin_amdgcn_readfirstlane(r.lp);
+    return r;
+}
+
+// pop_heap with its reads done
+__device__ __forceinline__ void par_pop(ParHeap& H, int& len, const PopReads& r) {
+    if (len > 1) par_adjust(H, len - 1, r.lk, r.lp, r.sk, r.sp);
     len--;
 }
 
@@ -1123,8 +1159,8 @@ __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* ord
     }
     const int N = min(L.nfeat, cap - 8);
     while (len < N && len > 0) {
-        const uint32_t tk = __builtin_amdgcn_readfirstlane(H.key[0]);
-        const uint32_t tp = __builtin_amdgcn_readfirstlane(H.pay[0]);
+        const PopReads pr = par_pop_reads(H, len);
+        const uint32_t tk = pr.tk, tp = pr.tp;
         if (hkey(tk) == 1) break;
         const uint32_t c0 = tp, nch = tk & 7u;
         uint32_t s_l = 0, c_l = 0;
@@ -1132,7 +1168,7 @@ __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* ord
             s_l = nt.st(c0 + lane);
             c_l = nt.ch(c0 + lane);
         }
-        par_pop(H, len);
+        par_pop(H, len, pr);
         for (uint32_t k = 0; k < nch; k++) {
             const uint2 e = node_entry(c0 + k, lane_val(s_l, k), lane_val(s_l, k + 1), lane_val(c_l, k),
                                        lane_val(c_l, k + 1));
@@ -1143,8 +1179,9 @@ __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* ord
     // freed that slot, i.e. node j ends at key[n0 - 1 - j]
     const int n0 = len;
     while (len > 0) {
-        const uint32_t w = (H.pay[0] << 3) | (H.key[0] & 7u);
-        par_pop(H, len);
+        const PopReads pr = par_pop_reads(H, len);
+        const uint32_t w = (pr.tp << 3) | (pr.tk & 7u);
+        par_pop(H, len, pr);
         if (lane == 0) H.key[len] = w;
     }
     wave_lds_sync();

@@ -94,16 +94,25 @@ __global__ __launch_bounds__(256) void k_copy_level0(const Strip* st, int n, con
     const long long stride = src_stride[L.img];
     uint8_t* d = pyr + L.pix + x;
     const bool vec = x + 4 <= L.w && ((reinterpret_cast<uintptr_t>(s) | (uintptr_t)stride | (uintptr_t)x) & 3) == 0;
-    for (int y = S.y0; y < yend; y++) {
-        const uint8_t* sr = s + y * stride + x;
-        uint32_t v;
-        if (vec) {
-            v = *reinterpret_cast<const uint32_t*>(sr);
-        } else {
-            v = 0;
-            for (int k = 0; k < 4 && x + k < L.w; k++) v |= (uint32_t)sr[k] << (8 * k);
+    // 8 rows in flight: all loads of a group before its stores
+    for (int y0 = S.y0; y0 < yend; y0 += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int y = y0 + u;
+            v[u] = 0;
+            if (y < yend) {
+                const uint8_t* sr = s + y * stride + x;
+                if (vec) {
+                    v[u] = *reinterpret_cast<const uint32_t*>(sr);
+                } else {
+                    for (int k = 0; k < 4 && x + k < L.w; k++) v[u] |= (uint32_t)sr[k] << (8 * k);
+                }
+            }
         }
-        *reinterpret_cast<uint32_t*>(d + (long long)y * L.pitch) = v;
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (y0 + u < yend) *reinterpret_cast<uint32_t*>(d + (long long)(y0 + u) * L.pitch) = v[u];
     }
 }
 
@@ -200,66 +209,81 @@ __global__ __launch_bounds__(64) void k_resize(const Strip* st, int n, const Lev
     }
 }
 
-// Bytes x-4 .. x+7 of a row as three u32 (little endian).  Interior lanes
-// use aligned loads; lanes at the image border gather with REFLECT_101 on
-// indices clamped to [-3, w+2], the range any stored output can reach.
-__device__ __forceinline__ void row_words(const uint8_t* row, int x, int w, int pitch, uint32_t& w0, uint32_t& w1,
-                                          uint32_t& w2) {
-    if (x >= 4 && x + 8 <= pitch && x + 6 < w) {
-        const uint32_t* r = reinterpret_cast<const uint32_t*>(row + x);
-        w0 = r[-1];
-        w1 = r[0];
-        w2 = r[1];
-    } else {
-        uint32_t v[3] = {0, 0, 0};
+// Lanes near a level's left or right edge: the 12 source bytes x-4 .. x+7
+// of a row (REFLECT_101 on indices clamped to [-3, w+2], the range any
+// stored output reaches) lie in the 3 aligned words at `base` (x - 4 clamped
+// into [0, pitch - 12]); each of the lane's 3 words is one byte permute of
+// two consecutive loaded words (pair 0-1 or 1-2).  Fixed per lane for the
+// whole strip, so an edge wave runs the same instructions on every lane.
+struct EdgeGather {
+    int base;
+    uint32_t sel[3];
+    bool hi[3];  // pair 1-2 instead of 0-1
+};
+
+__device__ __forceinline__ EdgeGather edge_gather(int x, int w, int pitch) {
+    EdgeGather e;
+    e.base = min(max(x - 4, 0), pitch - 12);
 #pragma unroll
-        for (int k = 0; k < 12; k++) {
-            const int i = reflect101(min(max(x - 4 + k, -3), w + 2), w);
-            v[k >> 2] |= (uint32_t)row[i] << (8 * (k & 3));
+    for (int o = 0; o < 3; o++) {
+        int p[4], pmin = 12;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            p[b] = reflect101(min(max(x - 4 + 4 * o + b, -3), w + 2), w) - e.base;
+            pmin = min(pmin, p[b]);
         }
-        w0 = v[0];
-        w1 = v[1];
-        w2 = v[2];
+        const int j = min(pmin >> 2, 1);
+        uint32_t sel = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) sel |= (uint32_t)(p[b] - 4 * j) << (8 * b);
+        e.sel[o] = sel;
+        e.hi[o] = j == 1;
     }
+    return e;
 }
 
 // GaussianBlur 7x7, sigma 2, BORDER_REFLECT_101, CV_8U fixed point: the
 // 7 taps (sum 256) horizontally with udot4, then vertically, out =
 // (sum + 2^15) >> 16.  Each lane keeps the last 7 horizontal rows of its
 // 4 columns in registers while walking down the strip.  Two walks compute
-// the same bytes: kFixed = false stops at the level's last row and picks
-// the border gather per row; kFixed = true always covers kStripH + 6 rows
-// (row indices reflect, stores past the level end are skipped) with the
-// lane's load kind fixed, so later rows' loads can be issued early.
-template <bool kFixed, bool kInterior>
+// the same bytes: kFixed = false stops at the level's last row; kFixed =
+// true always covers kStripH + 6 rows (row indices reflect, stores past the
+// level end are skipped).  kEdge: the wave has a lane near a level edge and
+// reads its rows through EdgeGather.
+template <bool kFixed, bool kEdge>
 __device__ __forceinline__ void blur_load_row(const uint8_t* __restrict__ src, const LevelDesc& L, int x, int yi,
-                                              uint32_t& w0, uint32_t& w1, uint32_t& w2) {
+                                              const EdgeGather& eg, uint32_t& w0, uint32_t& w1, uint32_t& w2) {
     const int ry = reflect101(min(max(yi, -3), L.h + 2), L.h);
     const uint8_t* row = src + (long long)ry * L.pitch;
-    if (kFixed && kInterior) {
+    if (!kEdge) {
         const uint32_t* rw = reinterpret_cast<const uint32_t*>(row + x);
         w0 = rw[-1];
         w1 = rw[0];
         w2 = rw[1];
     } else {
-        row_words(row, x, L.w, L.pitch, w0, w1, w2);
+        const uint32_t* rw = reinterpret_cast<const uint32_t*>(row + eg.base);
+        const uint32_t r0 = rw[0], r1 = rw[1], r2 = rw[2];
+        w0 = __builtin_amdgcn_perm(eg.hi[0] ? r2 : r1, eg.hi[0] ? r1 : r0, eg.sel[0]);
+        w1 = __builtin_amdgcn_perm(eg.hi[1] ? r2 : r1, eg.hi[1] ? r1 : r0, eg.sel[1]);
+        w2 = __builtin_amdgcn_perm(eg.hi[2] ? r2 : r1, eg.hi[2] ? r1 : r0, eg.sel[2]);
     }
 }
 
-template <bool kFixed, bool kInterior>
+template <bool kFixed, bool kEdge>
 __device__ __forceinline__ void blur_strip(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
-                                           const LevelDesc& L, int x, int y0, int yend, const BlurTaps& taps) {
+                                           const LevelDesc& L, int x, int y0, int yend, const BlurTaps& taps,
+                                           const EdgeGather& eg) {
     uint32_t hr[7][4];
     // one row ahead: the next row's words are requested before this row's
     // taps are computed
     uint32_t n0, n1, n2;
-    blur_load_row<kFixed, kInterior>(src, L, x, y0 - 3, n0, n1, n2);
+    blur_load_row<kFixed, kEdge>(src, L, x, y0 - 3, eg, n0, n1, n2);
 #pragma unroll
     for (int r = 0; r < kStripH + 6; r++) {
         const int yi = y0 - 3 + r;
         if (!kFixed && yi - 3 >= yend) break;
         const uint32_t w0 = n0, w1 = n1, w2 = n2;
-        if (r + 1 < kStripH + 6) blur_load_row<kFixed, kInterior>(src, L, x, yi + 1, n0, n1, n2);
+        if (r + 1 < kStripH + 6) blur_load_row<kFixed, kEdge>(src, L, x, yi + 1, eg, n0, n1, n2);
         uint32_t* h = hr[r % 7];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -288,10 +312,14 @@ __global__ __launch_bounds__(256) void k_blur(const Strip* st, int n, const Leve
                                               BlurTaps taps) {
     STRIP_SETUP(st, n, true)
     if (x >= L.w) return;
-    if (kFixed && x >= 4 && x + 8 <= L.pitch && x + 6 < L.w)
-        blur_strip<kFixed, true>(pyr + L.pix, blur + L.pix, L, x, S.y0, yend, taps);
-    else
-        blur_strip<kFixed, false>(pyr + L.pix, blur + L.pix, L, x, S.y0, yend, taps);
+    // a wave with no lane near a level edge reads its 3 words directly
+    const bool interior = x >= 4 && x + 8 <= L.pitch && x + 6 < L.w;
+    if (__all(interior)) {
+        blur_strip<kFixed, false>(pyr + L.pix, blur + L.pix, L, x, S.y0, yend, taps, EdgeGather{});
+    } else {
+        const EdgeGather eg = edge_gather(x, L.w, L.pitch);
+        blur_strip<kFixed, true>(pyr + L.pix, blur + L.pix, L, x, S.y0, yend, taps, eg);
+    }
 }
 
 // Launch variants of the blur (same bytes): the walk kind.

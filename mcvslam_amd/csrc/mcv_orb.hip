@@ -576,40 +576,74 @@ __device__ __forceinline__ void cell_stage(const CellWin& C, uint32_t* __restric
     }
 }
 
-// suppression over the staged rows y0 .. y0 + nb - 1; appends the keepers
+// suppression over the staged rows y0 .. y0 + nb - 1; appends the keepers.
+// A lane takes one staged word (4 columns) of a row, words in row-major
+// order; a word is examined byte by byte only if some byte exceeds t (bytes
+// outside the area are 0), and the keepers of a step are compacted in
+// (word, byte) order: the lane's offset is the count of keep bits in the
+// lanes below (one ballot per byte position).
 __device__ __forceinline__ int cell_scan(const CellWin& C, const uint32_t* __restrict__ band, int y0, int nb,
-                                         uint32_t magic, int t, int offx, int offy, uint32_t* __restrict__ out,
-                                         int written) {
+                                         int t, int offx, int offy, uint32_t* __restrict__ out, int written) {
     const int lane = threadIdx.x & 63;
     const unsigned long long lt = (1ull << lane) - 1ull;
     const uint8_t* bb = reinterpret_cast<const uint8_t*>(band);
-    const int npb = nb * C.DW;
-    for (int p0 = 0; p0 < npb; p0 += 64) {
-        const int p = p0 + lane;
-        bool keep = false;
-        uint32_t rec = 0;
-        if (p < npb) {
-            const int r = (int)(((uint32_t)p * magic) >> 20), x = p - r * C.DW;
-            const uint8_t* q = bb + (r + 1) * kCellSW + C.lead + 1 + x;
-            const int bc = q[0];
-            if (bc > t) {
-                const int s = bc - 1;
-                keep = true;
+    const int wf = (C.lead + 1) >> 2;                 // first word holding an area byte
+    const int nwa = ((C.lead + C.DW) >> 2) - wf + 1;  // words per row holding area bytes
+    const uint32_t amagic = (65536u + (uint32_t)nwa - 1u) / (uint32_t)nwa;  // i / nwa, i < (kCellBand + 2) * 24
+    const uint32_t addt = 0x01010101u * (uint32_t)(0x7F - t);  // t <= 127: a byte > t sets its high bit
+    const int nit = nb * nwa;
+    for (int i0 = 0; i0 < nit; i0 += 64) {
+        const int i = i0 + lane;
+        int kb = 0;          // keep bits of the word's 4 bytes
+        uint32_t rec[4] = {0, 0, 0, 0};
+        int r = 0, wb = 0;   // area row, byte offset of the word in the staged row
+        uint32_t W = 0;
+        if (i < nit) {
+            r = (int)(((uint32_t)i * amagic) >> 16);
+            wb = 4 * (wf + i - r * nwa);
+            W = band[(r + 1) * (kCellSW / 4) + (wb >> 2)];
+        }
+        const bool any = t <= 127 ? ((((W + addt) | W) & 0x80808080u) != 0u)
+                                  : (int)max(max(W & 0xFFu, (W >> 8) & 0xFFu), max((W >> 16) & 0xFFu, W >> 24)) > t;
+        if (any) {
 #pragma unroll
-                for (int dy = -1; dy <= 1; dy++)
+            for (int bt = 0; bt < 4; bt++) {
+                const int bc = (int)((W >> (8 * bt)) & 0xFFu);
+                if (bc > t) {
+                    const uint8_t* q = bb + (r + 1) * kCellSW + wb + bt;
+                    const int s = bc - 1;
+                    bool keep = true;
 #pragma unroll
-                    for (int dx = -1; dx <= 1; dx++) {
-                        if (dx == 0 && dy == 0) continue;
-                        const int bq = q[dy * kCellSW + dx];
-                        const int mq = bq > t ? bq - 1 : 0;
-                        keep = keep && s > mq;
+                    for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                        for (int dx = -1; dx <= 1; dx++) {
+                            if (dx == 0 && dy == 0) continue;
+                            const int bq = q[dy * kCellSW + dx];
+                            const int mq = bq > t ? bq - 1 : 0;
+                            keep = keep && s > mq;
+                        }
+                    if (keep) {
+                        kb |= 1 << bt;
+                        const int x = wb + bt - (C.lead + 1);  // detection column
+                        rec[bt] = ((uint32_t)s << 24) | ((uint32_t)(y0 + r + 3 + offy) << 12) | (uint32_t)(x + 3 + offx);
                     }
-                rec = ((uint32_t)s << 24) | ((uint32_t)(y0 + r + 3 + offy) << 12) | (uint32_t)(x + 3 + offx);
+                }
             }
         }
-        const unsigned long long bal = __ballot(keep);
-        if (keep) out[written + __popcll(bal & lt)] = rec;
-        written += __popcll(bal);
+        if (__any(kb != 0)) {
+            int below = 0, total = 0;
+#pragma unroll
+            for (int bt = 0; bt < 4; bt++) {
+                const unsigned long long bm = __ballot((kb >> bt) & 1);
+                below += __popcll(bm & lt);
+                total += __popcll(bm);
+            }
+            int k = written + below;
+#pragma unroll
+            for (int bt = 0; bt < 4; bt++)
+                if ((kb >> bt) & 1) out[k++] = rec[bt];
+            written += total;
+        }
     }
     return written;
 }
@@ -633,7 +667,6 @@ __global__ __launch_bounds__(256) void k_fast_cells(OrbGlobals g, int ncells) {
         C.pitch = L.pitch;
         C.DW = DW;
         C.DH = DH;
-        const uint32_t magic = ((1u << 20) + (uint32_t)DW - 1u) / (uint32_t)DW;  // p / DW for p < 2^20 / DW
         uint32_t* out = g.cell_cand + c.cap_base;
         uint32_t* band = s_band[threadIdx.x >> 6];
         const bool once = DH <= kCellBand;
@@ -651,7 +684,7 @@ __global__ __launch_bounds__(256) void k_fast_cells(OrbGlobals g, int ncells) {
                     cell_stage(C, band, y0, nb);
                     wave_lds_sync();
                 }
-                written = cell_scan(C, band, y0, nb, magic, t, c.offx, c.offy, out, written);
+                written = cell_scan(C, band, y0, nb, t, c.offx, c.offy, out, written);
                 if (!once) wave_lds_sync();
             }
             total = written;
@@ -1351,8 +1384,10 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
     if (!seed) {
         // IC_Angle over the circular patch (|u| <= umax[|v|], |v| <= 15): the
         // 31 rows are read as u32 words from the aligned column below cx - 15
-        // (at most 9 words a row, 279 words over the wave), each byte kept if
-        // inside the disk; the integer moments are order independent
+        // (at most 9 words a row, 279 words over the wave); the bytes of a
+        // word inside the disk are masked and summed with two udot4 (sum of
+        // values, sum of byte index times value); the integer moments are
+        // order independent
         const int base = (cx - 15) & ~3;
         const int nwr = (cx + 15 - base) / 4 + 1;
         unsigned long long um = 0;
@@ -1369,14 +1404,14 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
                 const int du = (int)((um >> (4 * (v < 0 ? -v : v))) & 15u);
                 const uint32_t dw =
                     *reinterpret_cast<const uint32_t*>(lvl + (long long)(cy + v) * L.pitch + base + 4 * w);
-#pragma unroll
-                for (int b = 0; b < 4; b++) {
-                    const int u = base + 4 * w + b - cx;
-                    if (u >= -du && u <= du) {
-                        const int val = (int)((dw >> (8 * b)) & 0xFFu);
-                        m10 += u * val;
-                        m01 += v * val;
-                    }
+                const int u0 = base + 4 * w - cx;  // u of byte 0
+                const int blo = max(-du - u0, 0), bhi = min(du - u0, 3);
+                if (blo <= bhi) {
+                    const uint32_t m = dw & (0xFFFFFFFFu << (8 * blo)) & (0xFFFFFFFFu >> (8 * (3 - bhi)));
+                    const int s1 = (int)__builtin_amdgcn_udot4(m, 0x01010101u, 0u, false);
+                    const int sb = (int)__builtin_amdgcn_udot4(m, 0x03020100u, 0u, false);
+                    m10 += u0 * s1 + sb;
+                    m01 += v * s1;
                 }
             }
         }
@@ -1410,9 +1445,11 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
         const signed char* p = kOrbPattern[64 * (k >> 1) + lane] + 2 * (k & 1);
         const float px = (float)p[0], py = (float)p[1];
         const float vy = px * b + py * a, vx = px * a - py * b;
-        oy[k] = (int)rintf(vy);
-        ox[k] = (int)rintf(vx);
-        near = near || fabsf(fabsf(vy - truncf(vy)) - 0.5f) < 1e-5f || fabsf(fabsf(vx - truncf(vx)) - 0.5f) < 1e-5f;
+        const float ry = rintf(vy), rx = rintf(vx);
+        oy[k] = (int)ry;
+        ox[k] = (int)rx;
+        // within 1e-5 of a half-integer: |v - rint(v)| is exact and <= 0.5
+        near = near || fabsf(vy - ry) > 0.5f - 1e-5f || fabsf(vx - rx) > 0.5f - 1e-5f;
     }
     if (__any(near)) {
         sincos_patched(xr, tb, &a, &b);

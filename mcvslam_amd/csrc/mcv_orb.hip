@@ -547,6 +547,12 @@ __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, cons
 constexpr int kCellBand = 46;
 constexpr int kCellSW = kMaxWin;  // LDS bytes per staged row: >= 4 * ceil((3 + W - 4) / 4)
 
+// ceil(2^16 / n) for n = 1 .. 24 (u32 per staged row): i / n ==
+// (i * kDiv16[n]) >> 16 for every index i < (kCellBand + 2) * 24 used here
+__constant__ uint32_t kDiv16[25] = {0,    65536, 32768, 21846, 16384, 13108, 10923, 9363, 8192,
+                                    7282, 6554,  5958,  5462,  5042,  4682,  4370,  4096, 3856,
+                                    3641, 3450,  3277,  3121,  2979,  2850,  2731};
+
 struct CellWin {
     const uint8_t* wbase;  // word holding detection column -1 of row 0
     int lead;              // byte of column -1 in that word
@@ -589,7 +595,7 @@ __device__ __forceinline__ int cell_scan(const CellWin& C, const uint32_t* __res
     const uint8_t* bb = reinterpret_cast<const uint8_t*>(band);
     const int wf = (C.lead + 1) >> 2;                 // first word holding an area byte
     const int nwa = ((C.lead + C.DW) >> 2) - wf + 1;  // words per row holding area bytes
-    const uint32_t amagic = (65536u + (uint32_t)nwa - 1u) / (uint32_t)nwa;  // i / nwa, i < (kCellBand + 2) * 24
+    const uint32_t amagic = kDiv16[nwa];  // i / nwa
     const uint32_t addt = 0x01010101u * (uint32_t)(0x7F - t);  // t <= 127: a byte > t sets its high bit
     const int nit = nb * nwa;
     for (int i0 = 0; i0 < nit; i0 += 64) {
@@ -663,7 +669,7 @@ __global__ __launch_bounds__(256) void k_fast_cells(OrbGlobals g, int ncells) {
         C.wbase = reinterpret_cast<const uint8_t*>(reinterpret_cast<uintptr_t>(sp - 1) & ~(uintptr_t)3);
         C.lead = (int)((sp - 1) - C.wbase);
         C.nwr = (C.lead + DW + 2 + 3) >> 2;
-        C.wmagic = (65536u + (uint32_t)C.nwr - 1u) / (uint32_t)C.nwr;
+        C.wmagic = kDiv16[C.nwr];
         C.pitch = L.pitch;
         C.DW = DW;
         C.DH = DH;

@@ -90,16 +90,25 @@ def algorithmic_bytes(kernel, orb, frames, counts):
 
 
 def measured_traffic(path=ROOT / "profiles" / "hbm_traffic_rig64.txt"):
-    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by the
+    """HBM bytes per rig step (FETCH_SIZE + WRITE_SIZE, scaled by the
     calibration copy) from the committed rocprofv3 --pmc summary of this
-    workload; {} when it is absent."""
-    out = {}
+    workload, keyed by the names the timed run reports; {} when it is absent.
+    The summary lists bytes per dispatch and the dispatch count of the run;
+    k_fast_score runs once per step, so its count is the number of steps."""
+    rows = {}
     if not path.exists():
-        return out
+        return rows
     for line in path.read_text().splitlines():
         parts = line.split()
         if len(parts) == 5 and parts[0].startswith("k_") and parts[1].isdigit():
-            out[parts[0]] = int(parts[4])
+            rows[parts[0]] = (int(parts[1]), int(parts[4]))
+    if "k_fast_score" not in rows:
+        return {}
+    steps = rows["k_fast_score"][0]
+    out = {}
+    for name, (calls, per_call) in rows.items():
+        key = "k_knn2" if name.startswith("k_knn2") else name
+        out[key] = out.get(key, 0) + calls * per_call / steps
     return out
 
 
@@ -330,7 +339,8 @@ def main():
         per_launch_bytes = ab / launches[k]
         ach = per_launch_bytes / (per_launch_ms / 1e3) / 1e9
         table[k] = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": traffic.get(k),
+                    "frac": round(ach / HBM_PEAK_GBS, 5),
+                    "traffic": int(traffic[k] / launches[k]) if k in traffic else None,
                     "algorithmic_bytes": int(per_launch_bytes), "avg_launch_ms": round(per_launch_ms, 5)}
     result["roofline"] = dict(kernel=dom, **table[dom])
     result["kernels_roofline"] = table

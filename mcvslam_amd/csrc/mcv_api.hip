@@ -527,7 +527,7 @@ struct OrbPlan {
             }
         }
         strip_begin[nlevels] = (int)strips.size();
-        // k_resize stages each strip's source rows and column window in LDS
+        // k_resize stages each pass's source rows and the strip's column window in LDS
         resize_rows.assign(nlevels, 0);
         resize_rw.assign(nlevels, 4);
         for (int l = 1; l < nlevels; l++)
@@ -539,9 +539,12 @@ struct OrbPlan {
                 const int xl = std::min(sp.x0 + kStripW - 1, d.w - 1);
                 const int cb = coef[d.coef_x + sp.x0].x & ~3;
                 const int c_hi = std::min(coef[d.coef_x + xl].x + 1, sl.w - 1);
-                const int r_lo = std::min(std::max(coef[d.coef_y + sp.y0].x, 0), sl.h - 1);
-                const int r_hi = std::min(std::max(coef[d.coef_y + yend - 1].x + 1, 0), sl.h - 1);
-                resize_rows[l] = std::max(resize_rows[l], r_hi - r_lo + 1);
+                for (int ys = sp.y0; ys < yend; ys += kResizePass) {  // the kernel's passes
+                    const int ye = std::min(ys + kResizePass, yend);
+                    const int r_lo = std::min(std::max(coef[d.coef_y + ys].x, 0), sl.h - 1);
+                    const int r_hi = std::min(std::max(coef[d.coef_y + ye - 1].x + 1, 0), sl.h - 1);
+                    resize_rows[l] = std::max(resize_rows[l], r_hi - r_lo + 1);
+                }
                 resize_rw[l] = std::max(resize_rw[l], ((c_hi - cb) / 4 + 1) * 4);
             }
         for (int l = 1; l < nlevels; l++) {

@@ -119,10 +119,11 @@ __global__ __launch_bounds__(256) void k_copy_level0(const Strip* st, int n, con
 // cv::resize INTER_LINEAR, CV_8U: horizontal int taps (coef 2048 scale) on
 // the two source rows, then the vertical pass
 // (((b0 * (r0 >> 4)) >> 16) + ((b1 * (r1 >> 4)) >> 16) + 2) >> 2.
-// One wave per strip (64-thread workgroups): every source row the strip
-// reads is staged first, over the strip's source column window, into
-// dynamic LDS (`rw` bytes per row, u32-aligned window start) with all the
-// loads in flight at once; the taps then gather bytes from LDS.
+// One wave per strip (64-thread workgroups), in passes of kResizePass
+// output rows: every source row a pass reads is staged first, over the
+// strip's source column window, into dynamic LDS (`rw` bytes per row,
+// u32-aligned window start) with the loads in flight in batches of 8; the
+// taps then gather bytes from LDS.
 __global__ __launch_bounds__(64) void k_resize(const Strip* st, int n, const LevelDesc* lv, const int2* coef,
                                                uint8_t* pyr, int rw) {
     extern __shared__ uint32_t s_rows[];
@@ -140,9 +141,6 @@ __global__ __launch_bounds__(64) void k_resize(const Strip* st, int n, const Lev
     const int cb = coef[L.coef_x + S.x0].x & ~3;
     const int c_hi = min(coef[L.coef_x + xl].x + 1, SL.w - 1);
     const int nw = (c_hi - cb) / 4 + 1;
-    const int r_lo = min(max(coef[L.coef_y + S.y0].x, 0), SL.h - 1);
-    const int r_hi = min(max(coef[L.coef_y + yend - 1].x + 1, 0), SL.h - 1);
-    const int nr = r_hi - r_lo + 1;
     const int rwd = rw >> 2;
     // the strip's row coefficients, one per lane, requested with the
     // staging loads so that no output row waits on its own load
@@ -158,54 +156,63 @@ __global__ __launch_bounds__(64) void k_resize(const Strip* st, int n, const Lev
         a0[j] = cx.y & 0xFFFF;
         a1[j] = cx.y >> 16;
     }
-    // source rows in batches of 8, the next batch requested before the
-    // current one is written to LDS
-    auto load8 = [&](int k0, uint32_t (&v)[8][3]) {
-#pragma unroll
-        for (int kk = 0; kk < 8; kk++)
-#pragma unroll
-            for (int h = 0; h < 3; h++) {
-                const int d = lane + 64 * h;
-                v[kk][h] = (k0 + kk < nr && d < nw)
-                               ? *reinterpret_cast<const uint32_t*>(sp + (long long)(r_lo + k0 + kk) * SL.pitch + cb + 4 * d)
-                               : 0u;
-            }
-    };
-    auto store8 = [&](int k0, const uint32_t (&v)[8][3]) {
-#pragma unroll
-        for (int kk = 0; kk < 8; kk++)
-#pragma unroll
-            for (int h = 0; h < 3; h++) {
-                const int d = lane + 64 * h;
-                if (k0 + kk < nr && d < nw) s_rows[(k0 + kk) * rwd + d] = v[kk][h];
-            }
-    };
-    uint32_t va[8][3], vb[8][3];
-    load8(0, va);
-    for (int k0 = 0; k0 < nr; k0 += 16) {
-        if (k0 + 8 < nr) load8(k0 + 8, vb);
-        store8(k0, va);
-        if (k0 + 16 < nr) load8(k0 + 16, va);
-        if (k0 + 8 < nr) store8(k0 + 8, vb);
-    }
-    wave_lds_sync();
     const bool own = x < L.w;  // every lane stays for the coefficient shuffles
     const uint8_t* rows = reinterpret_cast<const uint8_t*>(s_rows);
     uint8_t* dp = pyr + L.pix + x;
-    for (int y = S.y0; y < yend; y++) {
-        const int2 cy = make_int2(__shfl(cyl.x, y - S.y0, 64), __shfl(cyl.y, y - S.y0, 64));
-        const int b0 = cy.y & 0xFFFF, b1 = cy.y >> 16;
-        const uint8_t* r0p = rows + (min(max(cy.x, 0), SL.h - 1) - r_lo) * rw;
-        const uint8_t* r1p = rows + (min(max(cy.x + 1, 0), SL.h - 1) - r_lo) * rw;
-        uint32_t v = 0;
+    // the strip in passes of kResizePass output rows, each staging only the
+    // source rows it reads (a smaller LDS window per wave)
+    for (int ys = S.y0; ys < yend; ys += kResizePass) {
+        const int ye = min(ys + kResizePass, yend);
+        const int r_lo = min(max(__shfl(cyl.x, ys - S.y0, 64), 0), SL.h - 1);
+        const int r_hi = min(max(__shfl(cyl.x, ye - 1 - S.y0, 64) + 1, 0), SL.h - 1);
+        const int nr = r_hi - r_lo + 1;
+        // source rows in batches of 8, the next batch requested before the
+        // current one is written to LDS
+        auto load8 = [&](int k0, uint32_t (&v)[8][3]) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int r0 = r0p[o0[j]] * a0[j] + r0p[o1[j]] * a1[j];
-            const int r1 = r1p[o0[j]] * a0[j] + r1p[o1[j]] * a1[j];
-            const uint32_t o = (uint32_t)((((b0 * (r0 >> 4)) >> 16) + ((b1 * (r1 >> 4)) >> 16) + 2) >> 2);
-            v |= (o & 0xFF) << (8 * j);
+            for (int kk = 0; kk < 8; kk++)
+#pragma unroll
+                for (int h = 0; h < 3; h++) {
+                    const int d = lane + 64 * h;
+                    v[kk][h] = (k0 + kk < nr && d < nw) ? *reinterpret_cast<const uint32_t*>(
+                                                              sp + (long long)(r_lo + k0 + kk) * SL.pitch + cb + 4 * d)
+                                                        : 0u;
+                }
+        };
+        auto store8 = [&](int k0, const uint32_t (&v)[8][3]) {
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++)
+#pragma unroll
+                for (int h = 0; h < 3; h++) {
+                    const int d = lane + 64 * h;
+                    if (k0 + kk < nr && d < nw) s_rows[(k0 + kk) * rwd + d] = v[kk][h];
+                }
+        };
+        if (ys != S.y0) wave_lds_sync();  // the previous pass's reads are done
+        uint32_t va[8][3], vb[8][3];
+        load8(0, va);
+        for (int k0 = 0; k0 < nr; k0 += 16) {
+            if (k0 + 8 < nr) load8(k0 + 8, vb);
+            store8(k0, va);
+            if (k0 + 16 < nr) load8(k0 + 16, va);
+            if (k0 + 8 < nr) store8(k0 + 8, vb);
         }
-        if (own) *reinterpret_cast<uint32_t*>(dp + (long long)y * L.pitch) = v;
+        wave_lds_sync();
+        for (int y = ys; y < ye; y++) {
+            const int2 cy = make_int2(__shfl(cyl.x, y - S.y0, 64), __shfl(cyl.y, y - S.y0, 64));
+            const int b0 = cy.y & 0xFFFF, b1 = cy.y >> 16;
+            const uint8_t* r0p = rows + (min(max(cy.x, 0), SL.h - 1) - r_lo) * rw;
+            const uint8_t* r1p = rows + (min(max(cy.x + 1, 0), SL.h - 1) - r_lo) * rw;
+            uint32_t v = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int r0 = r0p[o0[j]] * a0[j] + r0p[o1[j]] * a1[j];
+                const int r1 = r1p[o0[j]] * a0[j] + r1p[o1[j]] * a1[j];
+                const uint32_t o = (uint32_t)((((b0 * (r0 >> 4)) >> 16) + ((b1 * (r1 >> 4)) >> 16) + 2) >> 2);
+                v |= (o & 0xFF) << (8 * j);
+            }
+            if (own) *reinterpret_cast<uint32_t*>(dp + (long long)y * L.pitch) = v;
+        }
     }
 }
 
@@ -450,6 +457,11 @@ __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, cons
     const bool own = x < L.w;
     const unsigned long long lt = (1ull << lane) - 1ull;
     const mcv_us2 tt = {(unsigned short)t_lo, (unsigned short)t_lo};
+    // this lane's columns inside the detection range [3, w - 3)
+    int bmask = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (x + j >= 3 && x + j < L.w - 3) bmask |= 1 << j;
     // one level row into registers: this lane's u32, plus the word left of
     // the strip (lane 0) or right of it (lane 63); rows clamp to the level
     auto load_row = [&](int r, uint32_t& a, uint32_t& e) {
@@ -491,10 +503,8 @@ __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, cons
                                          as_us2(lf & m), tt);  // pixels 0, 2
             const uint32_t od = compass2(as_us2((c1 >> 8) & m), as_us2((dn >> 8) & m), as_us2((rt >> 8) & m),
                                          as_us2((up >> 8) & m), as_us2((lf >> 8) & m), tt);  // pixels 1, 3
-            surv = ((ev & 0xFFFFu) ? 1 : 0) | ((od & 0xFFFFu) ? 2 : 0) | ((ev >> 16) ? 4 : 0) | ((od >> 16) ? 8 : 0);
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                if (x + j < 3 || x + j >= L.w - 3) surv &= ~(1 << j);
+            surv = (((ev & 0xFFFFu) ? 1 : 0) | ((od & 0xFFFFu) ? 2 : 0) | ((ev >> 16) ? 4 : 0) | ((od >> 16) ? 8 : 0)) &
+                   bmask;
         }
 #pragma unroll
         for (int j = 0; j < 4; j++) {

@@ -132,10 +132,11 @@ void mcv_rig_destroy(mcv_rig* rig);
  * (optional) receives one line per timed variant and the choice. */
 int mcv_rig_autotune(mcv_rig* rig, const uint8_t* const* d_images, int reps, char* report, int report_cap);
 /* Fixes the rig's launch variants instead of tuning them: the top-2 kernel
- * (0 .. 7, as mcv_match_set_variant) and the blur walk (0: stop at the
- * level end, 1: fixed length).  No counterpart in the reference; every
- * combination produces identical outputs. */
-int mcv_rig_set_variants(mcv_rig* rig, int knn_variant, int blur_variant);
+ * (0 .. 7, as mcv_match_set_variant), the blur walk (0: stop at the
+ * level end, 1: fixed length) and the keypoint kernel (0: one keypoint per
+ * wave, 1: 8, 2: 16 (the default), 3: 32).  No counterpart in the reference; every combination
+ * produces identical outputs. */
+int mcv_rig_set_variants(mcv_rig* rig, int knn_variant, int blur_variant, int kp_variant);
 /* d_images: host array of 3 * n_frames device pointers (frame-major L, R, W),
  * each image dense (stride = width).  flags: MCV_FLAG_GRAPH replays the whole
  * batch as one hipGraph; without it every image and every pair is launched on

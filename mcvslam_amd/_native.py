@@ -50,7 +50,7 @@ SIGNATURES = {
     "mcv_distinctive_descriptors": (I, [P, P, P, I, P, P]),
     "mcv_match_filter_orientation": (I, [P, P, I, P, I, P, I, P, P]),
     "mcv_rig_autotune": (I, [P, P, I, P, I]),
-    "mcv_rig_set_variants": (I, [P, I, I]),
+    "mcv_rig_set_variants": (I, [P, I, I, I]),
     "mcv_match_knn2_dev": (I, [P, P, I, P, I, P]),
     "mcv_match_filter": (I, [P, P, I, F, I, P, P]),
     "mcv_rig_create": (I, [P, I, P, P, P, F, I, P]),

@@ -295,10 +295,12 @@ class RigBatch:
         check(self.ctx.h, lib().mcv_rig_run(self.h, arr, _native.MCV_FLAG_GRAPH if graph else 0))
 
     N_BLUR_VARIANTS = 2
+    N_KP_VARIANTS = 4
 
-    def set_variants(self, knn: int, blur: int):
-        """Fix the launch variants (top-2 kernel, blur) instead of tuning."""
-        check(self.ctx.h, lib().mcv_rig_set_variants(self.h, int(knn), int(blur)))
+    def set_variants(self, knn: int, blur: int, kp: int = 2):
+        """Fix the launch variants (top-2 kernel, blur walk, keypoints per
+        wave: 0 one, 1 eight, 2 sixteen, 3 thirty-two) instead of tuning."""
+        check(self.ctx.h, lib().mcv_rig_set_variants(self.h, int(knn), int(blur), int(kp)))
 
     def autotune(self, image_ptrs, reps: int = 3) -> str:
         """Time every launch variant of the tunable kernels on these images

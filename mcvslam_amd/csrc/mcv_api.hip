@@ -367,6 +367,7 @@ static int level_roots(int w, int h) {
 struct Tuning {
     int knn = 0;   // launch_knn2 variant
     int blur = 0;  // 0: k_blur<false>, 1: k_blur<true>
+    int kp = 2;    // keypoints per wave: 0: one (k_keypoints), 1: 8, 2: 16, 3: 32 (k_keypoints_k<K>)
 };
 
 struct ImageSpec {
@@ -688,8 +689,19 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
     hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), 8 * (size_t)heap_cap, s, g,
                        d_heap_order.as<int>(), heap_cap);
     mark("k_octree_heap");
-    hipLaunchKernelGGL(k_keypoints, dim3(xcd_grid((n_slots + 3) / 4)), dim3(256), 0, s, g, d_slot.as<int>(), n_slots, tb,
-                       reinterpret_cast<float*>(d_kps), d_desc, out_cap, d_counts);
+    float* kps = reinterpret_cast<float*>(d_kps);
+    if (tune.kp == 1)
+        hipLaunchKernelGGL(k_keypoints_k<8>, dim3(xcd_grid((n_slots + 31) / 32)), dim3(256), 0, s, g, d_slot.as<int>(),
+                           n_slots, tb, kps, d_desc, out_cap, d_counts);
+    else if (tune.kp == 2)
+        hipLaunchKernelGGL(k_keypoints_k<16>, dim3(xcd_grid((n_slots + 63) / 64)), dim3(256), 0, s, g,
+                           d_slot.as<int>(), n_slots, tb, kps, d_desc, out_cap, d_counts);
+    else if (tune.kp == 3)
+        hipLaunchKernelGGL(k_keypoints_k<32>, dim3(xcd_grid((n_slots + 127) / 128)), dim3(256), 0, s, g,
+                           d_slot.as<int>(), n_slots, tb, kps, d_desc, out_cap, d_counts);
+    else
+        hipLaunchKernelGGL(k_keypoints, dim3(xcd_grid((n_slots + 3) / 4)), dim3(256), 0, s, g, d_slot.as<int>(),
+                           n_slots, tb, kps, d_desc, out_cap, d_counts);
     mark("k_keypoints");
     HIP_OK(hipGetLastError());
 }
@@ -1402,8 +1414,20 @@ int mcv_rig_autotune(mcv_rig* r, const uint8_t* const* d_images, int reps, char*
         }
         r->tune.blur = best;
     }
-    snprintf(line, sizeof line, "chosen: k_knn2 %s, k_blur %s\n", knn_variant_name(r->tune.knn),
-             blur_variant_name(r->tune.blur));
+    {
+        int best = 0;
+        float bt = 1e30f;
+        for (int v = 0; v < kKpVariants; v++) {
+            r->tune.kp = v;
+            const float t = time_kernel("k_keypoints");
+            snprintf(line, sizeof line, "k_keypoints %s %.4f ms\n", kp_variant_name(v), t);
+            rep += line;
+            if (t < bt) { bt = t; best = v; }
+        }
+        r->tune.kp = best;
+    }
+    snprintf(line, sizeof line, "chosen: k_knn2 %s, k_blur %s, k_keypoints %s\n", knn_variant_name(r->tune.knn),
+             blur_variant_name(r->tune.blur), kp_variant_name(r->tune.kp));
     rep += line;
     P.tune = r->tune;
     for (auto& sp : r->single) sp->tune = r->tune;
@@ -1420,13 +1444,15 @@ int mcv_rig_autotune(mcv_rig* r, const uint8_t* const* d_images, int reps, char*
     API_END
 }
 
-int mcv_rig_set_variants(mcv_rig* r, int knn_variant, int blur_variant) {
+int mcv_rig_set_variants(mcv_rig* r, int knn_variant, int blur_variant, int kp_variant) {
     mcv_ctx* c = r ? r->ctx : nullptr;
     API_BEGIN(c)
-    REQUIRE(r && knn_variant >= 0 && knn_variant < kKnnVariants && blur_variant >= 0 && blur_variant < kBlurVariants,
+    REQUIRE(r && knn_variant >= 0 && knn_variant < kKnnVariants && blur_variant >= 0 && blur_variant < kBlurVariants &&
+                kp_variant >= 0 && kp_variant < kKpVariants,
             MCV_ERR_INVALID, "launch variant out of range");
     r->tune.knn = knn_variant;
     r->tune.blur = blur_variant;
+    r->tune.kp = kp_variant;
     r->batch->tune = r->tune;
     for (auto& sp : r->single) sp->tune = r->tune;
     if (r->graph) {

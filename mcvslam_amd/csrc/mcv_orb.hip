@@ -593,58 +593,69 @@ __device__ __forceinline__ void cell_stage(const CellWin& C, uint32_t* __restric
 }
 
 // suppression over the staged rows y0 .. y0 + nb - 1; appends the keepers.
-// A lane takes one staged word (4 columns) of a row, words in row-major
-// order; a word is examined byte by byte only if some byte exceeds t (bytes
-// outside the area are 0), and the keepers of a step are compacted in
+// With t >= 1 the keep rule s > max over the 8 neighbours of (b > t ? b - 1
+// : 0), s = best - 1, best > t, is the strict local maximum best > every
+// neighbour's byte (a neighbour at or below t is below best anyway).  A lane
+// takes one staged word (4 columns) of a row, words in row-major order, and
+// tests its 4 bytes at once: the 8 neighbour words (the left and right
+// neighbours rebuilt from the adjacent words with byte shifts) are maxed as
+// even and odd bytes in packed u16 halves, and a byte is kept when it
+// exceeds both that maximum and t; words with no byte above t (most of
+// them) skip the test.  The keepers of a step are compacted in
 // (word, byte) order: the lane's offset is the count of keep bits in the
 // lanes below (one ballot per byte position).
 __device__ __forceinline__ int cell_scan(const CellWin& C, const uint32_t* __restrict__ band, int y0, int nb,
                                          int t, int offx, int offy, uint32_t* __restrict__ out, int written) {
     const int lane = threadIdx.x & 63;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    const uint8_t* bb = reinterpret_cast<const uint8_t*>(band);
     const int wf = (C.lead + 1) >> 2;                 // first word holding an area byte
     const int nwa = ((C.lead + C.DW) >> 2) - wf + 1;  // words per row holding area bytes
+    const int wlast = (C.lead + C.DW + 1) >> 2;       // last staged word a right neighbour can need
     const uint32_t amagic = kDiv16[nwa];  // i / nwa
-    const uint32_t addt = 0x01010101u * (uint32_t)(0x7F - t);  // t <= 127: a byte > t sets its high bit
+    // a byte is kept when it exceeds the neighbour maximum and tk; t lies in
+    // [0, 255] (clamped on the host as FAST_t does), and for t = 0 a score
+    // s = 0 never beats a non-corner neighbour, so best must exceed 1
+    const int tk = max(t, 1);
+    const mcv_us2 tt = {(unsigned short)tk, (unsigned short)tk};
+    const uint32_t addt = 0x01010101u * (uint32_t)(0x7F - tk);
     const int nit = nb * nwa;
     for (int i0 = 0; i0 < nit; i0 += 64) {
         const int i = i0 + lane;
-        int kb = 0;          // keep bits of the word's 4 bytes
-        uint32_t rec[4] = {0, 0, 0, 0};
-        int r = 0, wb = 0;   // area row, byte offset of the word in the staged row
+        int kb = 0;  // keep bits of the word's 4 bytes
+        int r = 0, w = 0;
         uint32_t W = 0;
         if (i < nit) {
             r = (int)(((uint32_t)i * amagic) >> 16);
-            wb = 4 * (wf + i - r * nwa);
-            W = band[(r + 1) * (kCellSW / 4) + (wb >> 2)];
+            w = wf + i - r * nwa;
+            W = band[(r + 1) * (kCellSW / 4) + w];
         }
-        const bool any = t <= 127 ? ((((W + addt) | W) & 0x80808080u) != 0u)
-                                  : (int)max(max(W & 0xFFu, (W >> 8) & 0xFFu), max((W >> 16) & 0xFFu, W >> 24)) > t;
+        // t <= 127: a byte > t sets its high bit in (W + 0x01010101 * (127 - t)) | W
+        const bool any = tk <= 127 ? ((((W + addt) | W) & 0x80808080u) != 0u)
+                                   : (int)max(max(W & 0xFFu, (W >> 8) & 0xFFu), max((W >> 16) & 0xFFu, W >> 24)) > tk;
         if (any) {
+            const int wl = max(w - 1, 0), wr = min(w + 1, wlast);
+            // staged rows r (above), r + 1 (the word's row), r + 2 (below)
+            const uint32_t* R0 = band + r * (kCellSW / 4);
+            const uint32_t* R1 = R0 + kCellSW / 4;
+            const uint32_t* R2 = R1 + kCellSW / 4;
+            const uint32_t n[8] = {__builtin_amdgcn_alignbyte(R0[w], R0[wl], 3), R0[w],
+                                   __builtin_amdgcn_alignbyte(R0[wr], R0[w], 1), __builtin_amdgcn_alignbyte(W, R1[wl], 3),
+                                   __builtin_amdgcn_alignbyte(R1[wr], W, 1), __builtin_amdgcn_alignbyte(R2[w], R2[wl], 3),
+                                   R2[w], __builtin_amdgcn_alignbyte(R2[wr], R2[w], 1)};
+            const uint32_t me = 0x00FF00FFu;
+            mcv_us2 mxe = as_us2(n[0] & me), mxo = as_us2((n[0] >> 8) & me);
 #pragma unroll
-            for (int bt = 0; bt < 4; bt++) {
-                const int bc = (int)((W >> (8 * bt)) & 0xFFu);
-                if (bc > t) {
-                    const uint8_t* q = bb + (r + 1) * kCellSW + wb + bt;
-                    const int s = bc - 1;
-                    bool keep = true;
-#pragma unroll
-                    for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-                        for (int dx = -1; dx <= 1; dx++) {
-                            if (dx == 0 && dy == 0) continue;
-                            const int bq = q[dy * kCellSW + dx];
-                            const int mq = bq > t ? bq - 1 : 0;
-                            keep = keep && s > mq;
-                        }
-                    if (keep) {
-                        kb |= 1 << bt;
-                        const int x = wb + bt - (C.lead + 1);  // detection column
-                        rec[bt] = ((uint32_t)s << 24) | ((uint32_t)(y0 + r + 3 + offy) << 12) | (uint32_t)(x + 3 + offx);
-                    }
-                }
+            for (int q = 1; q < 8; q++) {
+                mxe = __builtin_elementwise_max(mxe, as_us2(n[q] & me));
+                mxo = __builtin_elementwise_max(mxo, as_us2((n[q] >> 8) & me));
             }
+            const mcv_us2 ce = as_us2(W & me), co = as_us2((W >> 8) & me);
+            const uint32_t ke = as_u32(__builtin_elementwise_min(__builtin_elementwise_sub_sat(ce, mxe),
+                                                                 __builtin_elementwise_sub_sat(ce, tt)));
+            const uint32_t ko = as_u32(__builtin_elementwise_min(__builtin_elementwise_sub_sat(co, mxo),
+                                                                 __builtin_elementwise_sub_sat(co, tt)));
+            kb = ((ke & 0xFFFFu) ? 1 : 0) | ((ko & 0xFFFFu) ? 2 : 0) | ((ke >> 16) ? 4 : 0) | ((ko >> 16) ? 8 : 0);
+            // bytes outside the area are 0 in the staged rows, so never exceed t
         }
         if (__any(kb != 0)) {
             int below = 0, total = 0;
@@ -657,7 +668,11 @@ __device__ __forceinline__ int cell_scan(const CellWin& C, const uint32_t* __res
             int k = written + below;
 #pragma unroll
             for (int bt = 0; bt < 4; bt++)
-                if ((kb >> bt) & 1) out[k++] = rec[bt];
+                if ((kb >> bt) & 1) {
+                    const uint32_t s = ((W >> (8 * bt)) & 0xFFu) - 1u;
+                    const int x = 4 * w + bt - (C.lead + 1);  // detection column
+                    out[k++] = (s << 24) | ((uint32_t)(y0 + r + 3 + offy) << 12) | (uint32_t)(x + 3 + offx);
+                }
             written += total;
         }
     }
@@ -1520,6 +1535,292 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
         o[3] = angle;
         o[4] = resp;
         reinterpret_cast<int*>(o)[5] = L.level;
+    }
+}
+
+
+// Integer sum over the wave: quad, half-row and row sums by DPP moves, then
+// the four row sums read out to scalars.  Every lane must be active.
+__device__ __forceinline__ int wave_sum_dpp(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);  // row_mirror
+    return __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16) + __builtin_amdgcn_readlane(v, 32) +
+           __builtin_amdgcn_readlane(v, 48);
+}
+
+__device__ __forceinline__ float readlane_f(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+// k_keypoints with K consecutive keypoint slots per wave.  The per-keypoint
+// set-up (slot entry, level prefix, coordinates), the angle (fastAtan2 of
+// the moments) and the rotation (host-libm cosf/sinf: the base values
+// patched from the exception table) run lane-parallel, lane k for slot
+// k; the moments and the descriptors, which spread one keypoint over the
+// wave, then walk the K keypoints with their operands read from lane k and
+// the next keypoint's loads issued before the current one is reduced.
+// Same arithmetic and outputs as k_keypoints (ORBextractor.cc:75-141,
+// :568-578, :845-896).
+constexpr int kKpVariants = 4;  // keypoints per wave: 1 (k_keypoints), 8, 16, 32
+inline const char* kp_variant_name(int v) {
+    return v == 1 ? "8/wave" : v == 2 ? "16/wave" : v == 3 ? "32/wave" : "1/wave";
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_keypoints_k(OrbGlobals g, const int* slot_lvl, int n_slots,
+                                                     SinCosTables tb, float* out_kps, uint8_t* out_desc, int cap,
+                                                     int* out_count) {
+    static_assert(K >= 1 && K <= 64, "one lane per keypoint of the wave");
+    __shared__ uint32_t s_patch[4][kPatchRows * kPatchWords];
+    const int xb = xcd_block((n_slots + 4 * K - 1) / (4 * K));
+    if (xb < 0) return;
+    const int lane = threadIdx.x & 63;
+    const int slot0 = (xb * 4 + (threadIdx.x >> 6)) * K;
+    if (slot0 >= n_slots) return;
+    // ---- set-up: lane k < K takes slot slot0 + k
+    bool kvalid = false, kseed = false;
+    int kimg = 0, klevel = 0, kpos = 0, kcx = 0, kcy = 0, kw = 0, kh = 0, kpitch = 0;
+    long long kpix = 0;
+    float kx = 0.f, ky = 0.f, kresp = 0.f, ksize = 0.f, kangle = 0.f, kscale = 1.f;
+    if (lane < K && slot0 + lane < n_slots) {
+        const int2 sl = reinterpret_cast<const int2*>(slot_lvl)[slot0 + lane];
+        const uint32_t win = g.nodes_out[slot0 + lane];
+        const LevelDesc* Lp = g.lv + sl.x;
+        const int j = sl.y;
+        kimg = Lp->img;
+        klevel = Lp->level;
+        const int first = Lp->first_level, nl = Lp->nlevels;
+        // keypoints per level of this image (octree output + seeds)
+        int off = 0, tot = 0;
+#pragma unroll
+        for (int l = 0; l < kMaxLevels; l++)
+            if (l < nl) {
+                const int c = g.oct[first + l].nout + g.lv[first + l].seed_count;
+                off += l < klevel ? c : 0;
+                tot += c;
+            }
+        if (j == 0 && klevel == 0) out_count[kimg] = tot;
+        const int nout = g.oct[sl.x].nout, nseed = Lp->seed_count;
+        kpos = off + j;
+        kvalid = j < nout + nseed && kpos < cap;
+        if (kvalid) {
+            kseed = j >= nout;
+            if (!kseed) {
+                // the slot's keypoint as the heap kernel left it: response << 24 | y << 12 | x
+                kx = (float)((int)(win & 0xFFF) + Lp->minBX);
+                ky = (float)((int)((win >> 12) & 0xFFF) + Lp->minBY);
+                kresp = (float)(win >> 24);
+                ksize = Lp->patch;
+            } else {
+                const float* sd = g.seeds + 6 * (Lp->seed_first + (j - nout));
+                kx = sd[0];
+                ky = sd[1];
+                ksize = sd[2];
+                kangle = sd[3];
+                kresp = sd[4];
+            }
+            kcx = (int)rintf(kx);
+            kcy = (int)rintf(ky);
+            kw = Lp->w;
+            kh = Lp->h;
+            kpitch = Lp->pitch;
+            kpix = Lp->pix;
+            kscale = Lp->scale;
+        }
+    }
+    const int kpix_lo = (int)(uint32_t)kpix, kpix_hi = (int)(kpix >> 32);
+    auto pix_of = [&](int k) -> long long {
+        return (long long)(((unsigned long long)(uint32_t)__builtin_amdgcn_readlane(kpix_hi, k) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane(kpix_lo, k));
+    };
+
+    // ---- IC_Angle moments (ORBextractor.cc:75-98) of every octree keypoint.
+    // The disk's 31 rows are read as u32 words from the aligned column below
+    // cx - 15 (at most 9 words a row, 279 words over the wave); a lane's
+    // (row, word) pairs do not depend on the keypoint.
+    int mv[5], mw[5], mdu[5];
+    {
+        unsigned long long um = 0;
+#pragma unroll
+        for (int q = 0; q < 16; q++) um |= (unsigned long long)(g.umax[q] & 15) << (4 * q);
+#pragma unroll
+        for (int it = 0; it < 5; it++) {
+            const int t = lane + 64 * it;
+            const int row = t / 9;
+            mv[it] = row - 15;
+            mw[it] = t < 31 * 9 ? t - 9 * row : 99;  // 99: no word
+            const int av = mv[it] < 0 ? -mv[it] : mv[it];
+            mdu[it] = (int)((um >> (4 * min(av, 15))) & 15u);
+        }
+    }
+    int M10 = 0, M01 = 0;
+    unsigned long long todo = __ballot(kvalid && !kseed);
+    if (todo) {
+        uint32_t dw[5];
+        auto mload = [&](int k, uint32_t (&d)[5]) {
+            const int cx = __builtin_amdgcn_readlane(kcx, k), cy = __builtin_amdgcn_readlane(kcy, k);
+            const int pitch = __builtin_amdgcn_readlane(kpitch, k);
+            const uint8_t* lvl = g.pyr + pix_of(k);
+            const int base = (cx - 15) & ~3;
+            const int nwr = (cx + 15 - base) / 4 + 1;
+#pragma unroll
+            for (int it = 0; it < 5; it++)
+                d[it] = mw[it] < nwr ? *reinterpret_cast<const uint32_t*>(
+                                           lvl + (uint32_t)((cy + mv[it]) * pitch + base + 4 * mw[it]))
+                                     : 0u;
+        };
+        int k = __builtin_ctzll(todo);
+        mload(k, dw);
+        for (;;) {
+            todo &= todo - 1;
+            const int cx = __builtin_amdgcn_readlane(kcx, k);
+            uint32_t cur[5];
+#pragma unroll
+            for (int it = 0; it < 5; it++) cur[it] = dw[it];
+            const int kn = todo ? __builtin_ctzll(todo) : -1;
+            if (kn >= 0) mload(kn, dw);
+            const int base = (cx - 15) & ~3;
+            int m10 = 0, m01 = 0;
+#pragma unroll
+            for (int it = 0; it < 5; it++) {
+                const int u0 = base + 4 * mw[it] - cx;  // u of byte 0
+                const int blo = max(-mdu[it] - u0, 0), bhi = min(mdu[it] - u0, 3);
+                if (blo <= bhi) {
+                    const uint32_t m = cur[it] & (0xFFFFFFFFu << (8 * blo)) & (0xFFFFFFFFu >> (8 * (3 - bhi)));
+                    const int s1 = (int)__builtin_amdgcn_udot4(m, 0x01010101u, 0u, false);
+                    const int sb = (int)__builtin_amdgcn_udot4(m, 0x03020100u, 0u, false);
+                    m10 += u0 * s1 + sb;
+                    m01 += mv[it] * s1;
+                }
+            }
+            m10 = wave_sum_dpp(m10);
+            m01 = wave_sum_dpp(m01);
+            if (lane == k) {
+                M10 = m10;
+                M01 = m01;
+            }
+            if (kn < 0) break;
+            k = kn;
+        }
+    }
+
+    // ---- angle, rotation and the keypoint record, lane k for keypoint k
+    const float factorPI = (float)(3.14159265358979323846 / 180.f);
+    float ka = 0.f, kb = 0.f;
+    if (kvalid) {
+        if (!kseed) kangle = fast_atan2_deg((float)M01, (float)M10);
+        sincos_patched(kangle * factorPI, tb, &ka, &kb);
+        float* o = out_kps + ((long long)kimg * cap + kpos) * 6;
+        o[0] = klevel != 0 ? kx * kscale : kx;
+        o[1] = klevel != 0 ? ky * kscale : ky;
+        o[2] = ksize;
+        o[3] = kangle;
+        o[4] = kresp;
+        reinterpret_cast<int*>(o)[5] = klevel;
+    }
+
+    // ---- rBRIEF (ORBextractor.cc:101-141) on the blurred level: the wave's
+    // 64 lanes take 4 test pairs each; a keypoint whose 18-px square lies in
+    // the level has its 37 x 40-byte patch staged in LDS (the next
+    // keypoint's patch loads are issued before the current taps), the others
+    // gather clamped taps from the level.
+    float ppx[8], ppy[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const signed char* p = kOrbPattern[64 * (q >> 1) + lane] + 2 * (q & 1);
+        ppx[q] = (float)p[0];
+        ppy[q] = (float)p[1];
+    }
+    uint32_t* patch = s_patch[threadIdx.x >> 6];
+    todo = __ballot(kvalid);
+    if (todo) {
+        uint32_t pw[kPatchIt];
+        auto staged_of = [&](int k) {
+            const int cx = __builtin_amdgcn_readlane(kcx, k), cy = __builtin_amdgcn_readlane(kcy, k);
+            return cx >= kPatchR && cx + kPatchR < __builtin_amdgcn_readlane(kw, k) && cy >= kPatchR &&
+                   cy + kPatchR < __builtin_amdgcn_readlane(kh, k);
+        };
+        auto pload = [&](int k) {
+            const int cx = __builtin_amdgcn_readlane(kcx, k), cy = __builtin_amdgcn_readlane(kcy, k);
+            const int pitch = __builtin_amdgcn_readlane(kpitch, k);
+            const uint8_t* bl = g.blur + pix_of(k);
+            const int bx = (cx - kPatchR) & ~3;
+#pragma unroll
+            for (int it = 0; it < kPatchIt; it++) {
+                const int q = lane + 64 * it, r = q / kPatchWords, w = q - kPatchWords * r;
+                pw[it] = (q < kPatchRows * kPatchWords && bx + 4 * w + 4 <= pitch)
+                             ? *reinterpret_cast<const uint32_t*>(bl + (uint32_t)((cy - kPatchR + r) * pitch + bx + 4 * w))
+                             : 0u;
+            }
+        };
+        int k = __builtin_ctzll(todo);
+        bool st = staged_of(k);
+        if (st) pload(k);
+        for (;;) {
+            todo &= todo - 1;
+            const int cx = __builtin_amdgcn_readlane(kcx, k), cy = __builtin_amdgcn_readlane(kcy, k);
+            const float a = readlane_f(ka, k), b = readlane_f(kb, k);
+            if (st) {
+                wave_lds_sync();  // the previous keypoint's taps are done
+#pragma unroll
+                for (int it = 0; it < kPatchIt; it++) {
+                    const int q = lane + 64 * it;
+                    if (q < kPatchRows * kPatchWords) patch[q] = pw[it];
+                }
+            }
+            const int kn = todo ? __builtin_ctzll(todo) : -1;
+            const bool stn = kn >= 0 && staged_of(kn);
+            if (stn) pload(kn);
+            int oy[8], ox[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                oy[q] = (int)rintf(ppx[q] * b + ppy[q] * a);
+                ox[q] = (int)rintf(ppx[q] * a - ppy[q] * b);
+            }
+            unsigned long long words[4];
+            if (st) {
+                wave_lds_sync();
+                const uint8_t* pb = reinterpret_cast<const uint8_t*>(patch);
+                const int c0 = cx - ((cx - kPatchR) & ~3);  // patch column of the centre
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    int t[2];
+#pragma unroll
+                    for (int q = 0; q < 2; q++)
+                        t[q] = pb[(kPatchR + oy[2 * r + q]) * (4 * kPatchWords) + c0 + ox[2 * r + q]];
+                    words[r] = __ballot(t[0] < t[1]);
+                }
+            } else {
+                const int w = __builtin_amdgcn_readlane(kw, k), h = __builtin_amdgcn_readlane(kh, k);
+                const int pitch = __builtin_amdgcn_readlane(kpitch, k);
+                const uint8_t* bl = g.blur + pix_of(k);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    int t[2];
+#pragma unroll
+                    for (int q = 0; q < 2; q++) {
+                        const int yy = min(max(cy + oy[2 * r + q], 0), h - 1);
+                        const int xx = min(max(cx + ox[2 * r + q], 0), w - 1);
+                        t[q] = bl[(long long)yy * pitch + xx];
+                    }
+                    words[r] = __ballot(t[0] < t[1]);
+                }
+            }
+            if (lane == 0) {
+                const int img = __builtin_amdgcn_readlane(kimg, k), pos = __builtin_amdgcn_readlane(kpos, k);
+                unsigned long long* dst =
+                    reinterpret_cast<unsigned long long*>(out_desc + ((long long)img * cap + pos) * 32);
+                dst[0] = words[0];
+                dst[1] = words[1];
+                dst[2] = words[2];
+                dst[3] = words[3];
+            }
+            if (kn < 0) break;
+            k = kn;
+            st = stn;
+        }
     }
 }
 

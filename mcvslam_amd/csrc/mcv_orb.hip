@@ -77,7 +77,7 @@ __device__ __forceinline__ int reflect101(int i, int n) {
 #define STRIP_SETUP(tab, n, kXcd)                                   \
     const int xb = (kXcd) ? xcd_block(((n) + 3) / 4) : (int)blockIdx.x; \
     if (xb < 0) return;                                             \
-    const int sidx = xb * 4 + (threadIdx.x >> 6);                   \
+    const int sidx = xb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); \
     if (sidx >= (n)) return;                                        \
     const int lane = threadIdx.x & 63;                              \
     const Strip S = (tab)[sidx];                                    \
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_copy_level0(const Strip* st, int n, con
     if (x >= L.w) return;
     const uint8_t* s = src[L.img];
     const long long stride = src_stride[L.img];
-    uint8_t* d = pyr + L.pix + x;
+    uint8_t* d = pyr + L.pix;  // row y, column x at byte y * pitch + x
     const bool vec = x + 4 <= L.w && ((reinterpret_cast<uintptr_t>(s) | (uintptr_t)stride | (uintptr_t)x) & 3) == 0;
     // 8 rows in flight: all loads of a group before its stores
     for (int y0 = S.y0; y0 < yend; y0 += 8) {
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) void k_copy_level0(const Strip* st, int n, con
         }
 #pragma unroll
         for (int u = 0; u < 8; u++)
-            if (y0 + u < yend) *reinterpret_cast<uint32_t*>(d + (long long)(y0 + u) * L.pitch) = v[u];
+            if (y0 + u < yend) *reinterpret_cast<uint32_t*>(d + (uint32_t)((y0 + u) * L.pitch + x)) = v[u];
     }
 }
 
@@ -156,15 +156,15 @@ __global__ __launch_bounds__(64) void k_resize(const Strip* st, int n, const Lev
         a0[j] = cx.y & 0xFFFF;
         a1[j] = cx.y >> 16;
     }
-    const bool own = x < L.w;  // every lane stays for the coefficient shuffles
+    const bool own = x < L.w;  // every lane stays: lane k holds row k's coefficients
     const uint8_t* rows = reinterpret_cast<const uint8_t*>(s_rows);
-    uint8_t* dp = pyr + L.pix + x;
+    uint8_t* dp = pyr + L.pix;  // output row y, column x at byte y * pitch + x
     // the strip in passes of kResizePass output rows, each staging only the
     // source rows it reads (a smaller LDS window per wave)
     for (int ys = S.y0; ys < yend; ys += kResizePass) {
         const int ye = min(ys + kResizePass, yend);
-        const int r_lo = min(max(__shfl(cyl.x, ys - S.y0, 64), 0), SL.h - 1);
-        const int r_hi = min(max(__shfl(cyl.x, ye - 1 - S.y0, 64) + 1, 0), SL.h - 1);
+        const int r_lo = min(max(__builtin_amdgcn_readlane(cyl.x, ys - S.y0), 0), SL.h - 1);
+        const int r_hi = min(max(__builtin_amdgcn_readlane(cyl.x, ye - 1 - S.y0) + 1, 0), SL.h - 1);
         const int nr = r_hi - r_lo + 1;
         // source rows in batches of 8, the next batch requested before the
         // current one is written to LDS
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(64) void k_resize(const Strip* st, int n, const Lev
                 for (int h = 0; h < 3; h++) {
                     const int d = lane + 64 * h;
                     v[kk][h] = (k0 + kk < nr && d < nw) ? *reinterpret_cast<const uint32_t*>(
-                                                              sp + (long long)(r_lo + k0 + kk) * SL.pitch + cb + 4 * d)
+                                                              sp + (uint32_t)((r_lo + k0 + kk) * SL.pitch + cb + 4 * d))
                                                         : 0u;
                 }
         };
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(64) void k_resize(const Strip* st, int n, const Lev
         }
         wave_lds_sync();
         for (int y = ys; y < ye; y++) {
-            const int2 cy = make_int2(__shfl(cyl.x, y - S.y0, 64), __shfl(cyl.y, y - S.y0, 64));
+            const int2 cy = make_int2(__builtin_amdgcn_readlane(cyl.x, y - S.y0), __builtin_amdgcn_readlane(cyl.y, y - S.y0));
             const int b0 = cy.y & 0xFFFF, b1 = cy.y >> 16;
             const uint8_t* r0p = rows + (min(max(cy.x, 0), SL.h - 1) - r_lo) * rw;
             const uint8_t* r1p = rows + (min(max(cy.x + 1, 0), SL.h - 1) - r_lo) * rw;
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(64) void k_resize(const Strip* st, int n, const Lev
                 const uint32_t o = (uint32_t)((((b0 * (r0 >> 4)) >> 16) + ((b1 * (r1 >> 4)) >> 16) + 2) >> 2);
                 v |= (o & 0xFF) << (8 * j);
             }
-            if (own) *reinterpret_cast<uint32_t*>(dp + (long long)y * L.pitch) = v;
+            if (own) *reinterpret_cast<uint32_t*>(dp + (uint32_t)(y * L.pitch + x)) = v;
         }
     }
 }
@@ -250,8 +250,8 @@ __device__ __forceinline__ EdgeGather edge_gather(int x, int w, int pitch) {
 }
 
 // GaussianBlur 7x7, sigma 2, BORDER_REFLECT_101, CV_8U fixed point: the
-// 7 taps (sum 256) horizontally with udot4, then vertically, out =
-// (sum + 2^15) >> 16.  Each lane keeps the last 7 horizontal rows of its
+// 7 taps (sum 256) horizontally with udot4, then vertically (tap times a
+// horizontal sum < 2^16: 24-bit multiplies), out = (sum + 2^15) >> 16.  Each lane keeps the last 7 horizontal rows of its
 // 4 columns in registers while walking down the strip.  Two walks compute
 // the same bytes: kFixed = false stops at the level's last row; kFixed =
 // true always covers kStripH + 6 rows (row indices reflect, stores past the
@@ -261,14 +261,15 @@ template <bool kFixed, bool kEdge>
 __device__ __forceinline__ void blur_load_row(const uint8_t* __restrict__ src, const LevelDesc& L, int x, int yi,
                                               const EdgeGather& eg, uint32_t& w0, uint32_t& w1, uint32_t& w2) {
     const int ry = reflect101(min(max(yi, -3), L.h + 2), L.h);
-    const uint8_t* row = src + (long long)ry * L.pitch;
+    const int ro = ry * L.pitch;
     if (!kEdge) {
-        const uint32_t* rw = reinterpret_cast<const uint32_t*>(row + x);
-        w0 = rw[-1];
-        w1 = rw[0];
-        w2 = rw[1];
+        // interior lanes: x >= 4
+        const uint32_t* rw = reinterpret_cast<const uint32_t*>(src + (uint32_t)(ro + x - 4));
+        w0 = rw[0];
+        w1 = rw[1];
+        w2 = rw[2];
     } else {
-        const uint32_t* rw = reinterpret_cast<const uint32_t*>(row + eg.base);
+        const uint32_t* rw = reinterpret_cast<const uint32_t*>(src + (uint32_t)(ro + eg.base));
         const uint32_t r0 = rw[0], r1 = rw[1], r2 = rw[2];
         w0 = __builtin_amdgcn_perm(eg.hi[0] ? r2 : r1, eg.hi[0] ? r1 : r0, eg.sel[0]);
         w1 = __builtin_amdgcn_perm(eg.hi[1] ? r2 : r1, eg.hi[1] ? r1 : r0, eg.sel[1]);
@@ -305,10 +306,10 @@ __device__ __forceinline__ void blur_strip(const uint8_t* __restrict__ src, uint
             for (int j = 0; j < 4; j++) {
                 uint32_t acc = 0;
 #pragma unroll
-                for (int k = 0; k < 7; k++) acc += (uint32_t)taps.t[k] * hr[(r - 6 + k) % 7][j];
+                for (int k = 0; k < 7; k++) acc += __umul24((uint32_t)taps.t[k], hr[(r - 6 + k) % 7][j]);
                 v |= min((acc + (1u << 15)) >> 16, 255u) << (8 * j);
             }
-            if (!kFixed || y < yend) *reinterpret_cast<uint32_t*>(dst + (long long)y * L.pitch + x) = v;
+            if (!kFixed || y < yend) *reinterpret_cast<uint32_t*>(dst + (uint32_t)(y * L.pitch + x)) = v;
         }
     }
 }
@@ -450,7 +451,7 @@ __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, cons
     __shared__ uint32_t s_tile[4][kBlk][kStripW / 4];
     __shared__ uint32_t s_queue[4][64 + kStripW];
     STRIP_SETUP(st, n, false)
-    const int wv = threadIdx.x >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     uint32_t (*ring)[kRingW] = s_ring[wv];
     uint32_t* queue = s_queue[wv];
     const uint8_t* img = pyr + L.pix;
@@ -465,8 +466,9 @@ __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, cons
     // one level row into registers: this lane's u32, plus the word left of
     // the strip (lane 0) or right of it (lane 63); rows clamp to the level
     auto load_row = [&](int r, uint32_t& a, uint32_t& e) {
-        const uint8_t* row = img + (long long)min(max(r, 0), L.h - 1) * L.pitch;
-        a = x < L.pitch ? *reinterpret_cast<const uint32_t*>(row + x) : 0u;
+        const uint32_t ro = (uint32_t)(min(max(r, 0), L.h - 1) * L.pitch);
+        const uint8_t* row = img + ro;
+        a = x < L.pitch ? *reinterpret_cast<const uint32_t*>(img + (ro + x)) : 0u;
         e = 0u;
         if (lane == 0 && S.x0 >= 4) e = *reinterpret_cast<const uint32_t*>(row + S.x0 - 4);
         if (lane == 63 && S.x0 + kStripW < L.pitch) e = *reinterpret_cast<const uint32_t*>(row + S.x0 + kStripW);
@@ -533,7 +535,7 @@ __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, cons
             wave_lds_sync();
             if (own) {
                 for (int k = 0; k <= tb; k++)
-                    *reinterpret_cast<uint32_t*>(score + L.pix + (long long)(y - tb + k) * L.pitch + x) =
+                    *reinterpret_cast<uint32_t*>(score + L.pix + (uint32_t)((y - tb + k) * L.pitch + x)) =
                         s_tile[wv][k][lane];
             }
             wave_lds_sync();
@@ -575,11 +577,11 @@ __device__ __forceinline__ void cell_stage(const CellWin& C, uint32_t* __restric
     const int lane = threadIdx.x & 63;
     const int tot = (nb + 2) * C.nwr;
     for (int k = lane; k < tot; k += 64) {
-        const int r = (int)(((uint32_t)k * C.wmagic) >> 16), w = k - r * C.nwr;
+        const int r = (int)(__umul24((uint32_t)k, C.wmagic) >> 16), w = k - __mul24(r, C.nwr);
         const int ry = y0 - 1 + r;
         uint32_t v = 0;
         if (ry >= 0 && ry < C.DH) {
-            v = *reinterpret_cast<const uint32_t*>(C.wbase + (long long)ry * C.pitch + 4 * w);
+            v = *reinterpret_cast<const uint32_t*>(C.wbase + (__umul24(ry, C.pitch) + 4 * w));
             // keep the bytes of detection columns [0, DW)
             const int lo = C.lead + 1 - 4 * w, hi = C.lead + 1 + C.DW - 4 * w;
             uint32_t m = 0;
@@ -625,8 +627,8 @@ __device__ __forceinline__ int cell_scan(const CellWin& C, const uint32_t* __res
         int r = 0, w = 0;
         uint32_t W = 0;
         if (i < nit) {
-            r = (int)(((uint32_t)i * amagic) >> 16);
-            w = wf + i - r * nwa;
+            r = (int)(__umul24((uint32_t)i, amagic) >> 16);
+            w = wf + i - __mul24(r, nwa);
             W = band[(r + 1) * (kCellSW / 4) + w];
         }
         // t <= 127: a byte > t sets its high bit in (W + 0x01010101 * (127 - t)) | W
@@ -681,7 +683,7 @@ __device__ __forceinline__ int cell_scan(const CellWin& C, const uint32_t* __res
 
 __global__ __launch_bounds__(256) void k_fast_cells(OrbGlobals g, int ncells) {
     __shared__ uint32_t s_band[4][(kCellBand + 2) * kCellSW / 4];
-    const int ci = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int ci = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (ci >= ncells) return;
     const int lane = threadIdx.x & 63;
     const CellDesc c = g.cells[ci];
@@ -699,7 +701,7 @@ __global__ __launch_bounds__(256) void k_fast_cells(OrbGlobals g, int ncells) {
         C.DW = DW;
         C.DH = DH;
         uint32_t* out = g.cell_cand + c.cap_base;
-        uint32_t* band = s_band[threadIdx.x >> 6];
+        uint32_t* band = s_band[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
         const bool once = DH <= kCellBand;
         if (once) {
             cell_stage(C, band, 0, DH);
@@ -1354,7 +1356,7 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
     __shared__ uint32_t s_patch[4][kPatchRows * kPatchWords];
     const int xb = xcd_block((n_slots + 3) / 4);
     if (xb < 0) return;
-    const int wave = xb * 4 + (threadIdx.x >> 6);
+    const int wave = xb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (wave >= n_slots) return;
     // slot `wave` is (level li, j) with L.kp_slot + j == wave, so the heap's
@@ -1450,7 +1452,7 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
         m01 = wave_sum(m01);
         angle = fast_atan2_deg((float)m01, (float)m10);
     }
-    uint32_t* patch = s_patch[threadIdx.x >> 6];
+    uint32_t* patch = s_patch[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
     if (staged) {
 #pragma unroll
         for (int it = 0; it < kPatchIt; it++) {
@@ -1577,7 +1579,7 @@ __global__ __launch_bounds__(256) void k_keypoints_k(OrbGlobals g, const int* sl
     const int xb = xcd_block((n_slots + 4 * K - 1) / (4 * K));
     if (xb < 0) return;
     const int lane = threadIdx.x & 63;
-    const int slot0 = (xb * 4 + (threadIdx.x >> 6)) * K;
+    const int slot0 = (xb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) * K;
     if (slot0 >= n_slots) return;
     // ---- set-up: lane k < K takes slot slot0 + k
     bool kvalid = false, kseed = false;
@@ -1668,7 +1670,7 @@ __global__ __launch_bounds__(256) void k_keypoints_k(OrbGlobals g, const int* sl
 #pragma unroll
             for (int it = 0; it < 5; it++)
                 d[it] = mw[it] < nwr ? *reinterpret_cast<const uint32_t*>(
-                                           lvl + (uint32_t)((cy + mv[it]) * pitch + base + 4 * mw[it]))
+                                           lvl + (uint32_t)(cy * pitch + __mul24(mv[it], pitch) + base + 4 * mw[it]))
                                      : 0u;
         };
         int k = __builtin_ctzll(todo);
@@ -1691,8 +1693,8 @@ __global__ __launch_bounds__(256) void k_keypoints_k(OrbGlobals g, const int* sl
                     const uint32_t m = cur[it] & (0xFFFFFFFFu << (8 * blo)) & (0xFFFFFFFFu >> (8 * (3 - bhi)));
                     const int s1 = (int)__builtin_amdgcn_udot4(m, 0x01010101u, 0u, false);
                     const int sb = (int)__builtin_amdgcn_udot4(m, 0x03020100u, 0u, false);
-                    m10 += u0 * s1 + sb;
-                    m01 += mv[it] * s1;
+                    m10 += __mul24(u0, s1) + sb;
+                    m01 += __mul24(mv[it], s1);
                 }
             }
             m10 = wave_sum_dpp(m10);
@@ -1733,7 +1735,7 @@ __global__ __launch_bounds__(256) void k_keypoints_k(OrbGlobals g, const int* sl
         ppx[q] = (float)p[0];
         ppy[q] = (float)p[1];
     }
-    uint32_t* patch = s_patch[threadIdx.x >> 6];
+    uint32_t* patch = s_patch[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
     todo = __ballot(kvalid);
     if (todo) {
         uint32_t pw[kPatchIt];
@@ -1751,7 +1753,8 @@ __global__ __launch_bounds__(256) void k_keypoints_k(OrbGlobals g, const int* sl
             for (int it = 0; it < kPatchIt; it++) {
                 const int q = lane + 64 * it, r = q / kPatchWords, w = q - kPatchWords * r;
                 pw[it] = (q < kPatchRows * kPatchWords && bx + 4 * w + 4 <= pitch)
-                             ? *reinterpret_cast<const uint32_t*>(bl + (uint32_t)((cy - kPatchR + r) * pitch + bx + 4 * w))
+                             ? *reinterpret_cast<const uint32_t*>(
+                                   bl + (uint32_t)((cy - kPatchR) * pitch + __mul24(r, pitch) + bx + 4 * w))
                              : 0u;
             }
         };
@@ -1803,7 +1806,7 @@ __global__ __launch_bounds__(256) void k_keypoints_k(OrbGlobals g, const int* sl
                     for (int q = 0; q < 2; q++) {
                         const int yy = min(max(cy + oy[2 * r + q], 0), h - 1);
                         const int xx = min(max(cx + ox[2 * r + q], 0), w - 1);
-                        t[q] = bl[(long long)yy * pitch + xx];
+                        t[q] = bl[__umul24(yy, pitch) + xx];
                     }
                     words[r] = __ballot(t[0] < t[1]);
                 }

@@ -35,7 +35,7 @@ struct StereoParams {
 __global__ __launch_bounds__(256) void k_stereo(const StereoJob* jobs, const LevelDesc* lv, const uint8_t* pyr,
                                                 StereoParams sp) {
     const StereoJob J = jobs[blockIdx.y];
-    const int iL = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int iL = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int nl = min(*J.nl, sp.cap), nr = min(*J.nr, sp.cap);
     if (iL >= nl) return;

@@ -83,7 +83,7 @@ struct WindowPoints {
 
 // One wave per (camera, map point).
 __global__ __launch_bounds__(256) void k_window_knn(const WindowCam* cams, WindowPoints W) {
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int wave = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int p = wave;
     if (p >= W.P) return;

@@ -47,6 +47,14 @@ __device__ __forceinline__ void top2_insert(uint32_t& k0, uint32_t& k1, uint32_t
     k0 = min(k0, key);
 }
 
+// two keys at once: with k0 <= k1, the smallest of {k0, k1, a, b} is
+// min(k0, a, b) and the second smallest is min(k1, med3(k0, a, b))
+__device__ __forceinline__ void top2_insert2(uint32_t& k0, uint32_t& k1, uint32_t a, uint32_t b) {
+    const uint32_t m = med3_u32(k0, a, b);
+    k0 = min(min(k0, a), b);
+    k1 = min(k1, m);
+}
+
 // Variant S<QPL, UNR>.  256 threads; a lane holds QPL queries (64 QPL per
 // workgroup) and the 4 waves split the train set.  Train rows are
 // wave-uniform and come through scalar loads (SGPR operands of the XORs),
@@ -81,14 +89,18 @@ __global__ __launch_bounds__(256) void k_knn2_s(const MatchPair* pairs) {
     int j = j0;
     for (; j + UNR <= j1; j += UNR) {
         const kconst_u32* ta = tw + (long long)j * 8;
+        static_assert(UNR % 2 == 0, "train rows are taken in pairs");
 #pragma unroll
-        for (int t = 0; t < UNR; t++)
+        for (int t = 0; t < UNR; t += 2)
 #pragma unroll
             for (int u = 0; u < QPL; u++) {
-                uint32_t d = 0;
+                uint32_t da = 0, db = 0;
 #pragma unroll
-                for (int k = 0; k < 8; k++) d += __builtin_popcount(q[u][k] ^ ta[8 * t + k]);
-                top2_insert(k0[u], k1[u], (d << kIdxBits) | (uint32_t)(j + t));
+                for (int k = 0; k < 8; k++) {
+                    da += __builtin_popcount(q[u][k] ^ ta[8 * t + k]);
+                    db += __builtin_popcount(q[u][k] ^ ta[8 * t + 8 + k]);
+                }
+                top2_insert2(k0[u], k1[u], (da << kIdxBits) | (uint32_t)(j + t), (db << kIdxBits) | (uint32_t)(j + t + 1));
             }
     }
     for (; j < j1; j++) {

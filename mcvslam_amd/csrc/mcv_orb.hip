@@ -432,11 +432,17 @@ __device__ __forceinline__ int fast_best_rows(const uint8_t* const* r, int c) {
     for (int k = 0; k < 16; k++) m2[k] = __builtin_elementwise_min(e[k], e[(k + 1) & 15]);
 #pragma unroll
     for (int k = 0; k < 16; k++) m4[k] = __builtin_elementwise_min(m2[k], m2[(k + 2) & 15]);
-    mcv_s2 best = mcv_s2{(short)-1024, (short)-1024};
+    // the 16 arc minima, then their maximum as a tree (independent packed
+    // operations back to back, no dependent pair)
+    mcv_s2 am[16];
 #pragma unroll
     for (int k = 0; k < 16; k++)
-        best = __builtin_elementwise_max(
-            best, __builtin_elementwise_min(__builtin_elementwise_min(m4[k], m4[(k + 4) & 15]), e[(k + 8) & 15]));
+        am[k] = __builtin_elementwise_min(__builtin_elementwise_min(m4[k], m4[(k + 4) & 15]), e[(k + 8) & 15]);
+#pragma unroll
+    for (int h = 8; h >= 1; h >>= 1)
+#pragma unroll
+        for (int k = 0; k < h; k++) am[k] = __builtin_elementwise_max(am[k], am[k + h]);
+    const mcv_s2 best = am[0];
     const int b = max((int)best.x, (int)best.y);
     return b < 0 ? 0 : b;
 }
@@ -1541,15 +1547,22 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
 }
 
 
-// Integer sum over the wave: quad, half-row and row sums by DPP moves, then
-// the four row sums read out to scalars.  Every lane must be active.
-__device__ __forceinline__ int wave_sum_dpp(int v) {
-    v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
-    v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
-    v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // row_half_mirror
-    v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);  // row_mirror
-    return __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16) + __builtin_amdgcn_readlane(v, 32) +
-           __builtin_amdgcn_readlane(v, 48);
+// Two integer sums over the wave: quad, half-row and row sums by DPP moves
+// (the two sums' steps interleaved, so no DPP move waits on the one before
+// it), then the four row sums read out to scalars.  Every lane must be active.
+__device__ __forceinline__ void wave_sum2_dpp(int& a, int& b) {
+    a += __builtin_amdgcn_update_dpp(0, a, 0xB1, 0xF, 0xF, false);
+    b += __builtin_amdgcn_update_dpp(0, b, 0xB1, 0xF, 0xF, false);
+    a += __builtin_amdgcn_update_dpp(0, a, 0x4E, 0xF, 0xF, false);
+    b += __builtin_amdgcn_update_dpp(0, b, 0x4E, 0xF, 0xF, false);
+    a += __builtin_amdgcn_update_dpp(0, a, 0x141, 0xF, 0xF, false);
+    b += __builtin_amdgcn_update_dpp(0, b, 0x141, 0xF, 0xF, false);
+    a += __builtin_amdgcn_update_dpp(0, a, 0x140, 0xF, 0xF, false);
+    b += __builtin_amdgcn_update_dpp(0, b, 0x140, 0xF, 0xF, false);
+    a = __builtin_amdgcn_readlane(a, 0) + __builtin_amdgcn_readlane(a, 16) + __builtin_amdgcn_readlane(a, 32) +
+        __builtin_amdgcn_readlane(a, 48);
+    b = __builtin_amdgcn_readlane(b, 0) + __builtin_amdgcn_readlane(b, 16) + __builtin_amdgcn_readlane(b, 32) +
+        __builtin_amdgcn_readlane(b, 48);
 }
 
 __device__ __forceinline__ float readlane_f(float v, int k) {
@@ -1571,7 +1584,7 @@ inline const char* kp_variant_name(int v) {
 }
 
 template <int K>
-__global__ __launch_bounds__(256) void k_keypoints_k(OrbGlobals g, const int* slot_lvl, int n_slots,
+__global__ __launch_bounds__(256, 6) void k_keypoints_k(OrbGlobals g, const int* slot_lvl, int n_slots,
                                                      SinCosTables tb, float* out_kps, uint8_t* out_desc, int cap,
                                                      int* out_count) {
     static_assert(K >= 1 && K <= 64, "one lane per keypoint of the wave");
@@ -1697,8 +1710,7 @@ __global__ __launch_bounds__(256) void k_keypoints_k(OrbGlobals g, const int* sl
                     m01 += __mul24(mv[it], s1);
                 }
             }
-            m10 = wave_sum_dpp(m10);
-            m01 = wave_sum_dpp(m01);
+            wave_sum2_dpp(m10, m01);
             if (lane == k) {
                 M10 = m10;
                 M01 = m01;

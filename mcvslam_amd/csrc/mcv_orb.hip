@@ -397,27 +397,31 @@ typedef unsigned short mcv_us2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ mcv_us2 as_us2(uint32_t v) { return __builtin_bit_cast(mcv_us2, v); }
 __device__ __forceinline__ uint32_t as_u32(mcv_us2 v) { return __builtin_bit_cast(uint32_t, v); }
 
-// compass test for two pixels held in the u16 lanes of the arguments
-__device__ __forceinline__ uint32_t compass2(mcv_us2 v, mcv_us2 s, mcv_us2 e, mcv_us2 n, mcv_us2 w, mcv_us2 t) {
-    const mcv_us2 vt = __builtin_elementwise_sub_sat(v, t);
-    const mcv_us2 vpt = v + t;
-    const mcv_us2 ds = __builtin_elementwise_sub_sat(vt, s), de = __builtin_elementwise_sub_sat(vt, e);
-    const mcv_us2 dn = __builtin_elementwise_sub_sat(vt, n), dw = __builtin_elementwise_sub_sat(vt, w);
-    const mcv_us2 bs = __builtin_elementwise_sub_sat(s, vpt), be = __builtin_elementwise_sub_sat(e, vpt);
-    const mcv_us2 bn = __builtin_elementwise_sub_sat(n, vpt), bw = __builtin_elementwise_sub_sat(w, vpt);
-    const mcv_us2 pd = __builtin_elementwise_max(
+typedef short mcv_s2 __attribute__((ext_vector_type(2)));
+
+// compass test for two pixels held in the 16-bit lanes of the arguments
+// (values 0..255).  With d = p - v, a pair of neighbouring compass points is
+// darker than v - t iff max(d_a, d_b) < -t and brighter than v + t iff
+// min(d_a, d_b) > t; a lane of the result is > 0 iff some pair qualifies.
+// nt = -t in both lanes.
+__device__ __forceinline__ uint32_t compass2(mcv_us2 v, mcv_us2 s, mcv_us2 e, mcv_us2 n, mcv_us2 w, mcv_s2 t,
+                                             mcv_s2 nt) {
+    const mcv_s2 vs = __builtin_bit_cast(mcv_s2, v);
+    const mcv_s2 ds = __builtin_bit_cast(mcv_s2, s) - vs, de = __builtin_bit_cast(mcv_s2, e) - vs;
+    const mcv_s2 dn = __builtin_bit_cast(mcv_s2, n) - vs, dw = __builtin_bit_cast(mcv_s2, w) - vs;
+    const mcv_s2 mx = __builtin_elementwise_min(
+        __builtin_elementwise_min(__builtin_elementwise_max(ds, de), __builtin_elementwise_max(de, dn)),
+        __builtin_elementwise_min(__builtin_elementwise_max(dn, dw), __builtin_elementwise_max(dw, ds)));
+    const mcv_s2 mn = __builtin_elementwise_max(
         __builtin_elementwise_max(__builtin_elementwise_min(ds, de), __builtin_elementwise_min(de, dn)),
         __builtin_elementwise_max(__builtin_elementwise_min(dn, dw), __builtin_elementwise_min(dw, ds)));
-    const mcv_us2 pb = __builtin_elementwise_max(
-        __builtin_elementwise_max(__builtin_elementwise_min(bs, be), __builtin_elementwise_min(be, bn)),
-        __builtin_elementwise_max(__builtin_elementwise_min(bn, bw), __builtin_elementwise_min(bw, bs)));
-    return as_u32(__builtin_elementwise_max(pd, pb));
+    const mcv_s2 r = __builtin_elementwise_max(__builtin_elementwise_max(mn - t, nt - mx), mcv_s2{0, 0});
+    return __builtin_bit_cast(uint32_t, r);
 }
 
 // exact best of the pixel at byte c of the ring rows r[0..6] (rows -3..+3).
 // Both polarities run in one packed-i16 min/max tree: lane half 0 carries
 // v - p_k (darker arcs), half 1 carries p_k - v (brighter arcs).
-typedef short mcv_s2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ int fast_best_rows(const uint8_t* const* r, int c) {
     const int v = r[3][c];
@@ -426,7 +430,11 @@ __device__ __forceinline__ int fast_best_rows(const uint8_t* const* r, int c) {
                        r[3][c - 3], r[4][c - 3], r[5][c - 2], r[6][c - 1]};
     mcv_s2 e[16];
 #pragma unroll
-    for (int k = 0; k < 16; k++) e[k] = mcv_s2{(short)(v - p[k]), (short)(p[k] - v)};
+    for (int k = 0; k < 16; k++) {
+        // (p, p) * (-1, 1) + (v, -v) in 16-bit lanes: one packed multiply-add
+        const mcv_s2 pp = {(short)p[k], (short)p[k]};
+        e[k] = pp * mcv_s2{-1, 1} + mcv_s2{(short)v, (short)-v};
+    }
     mcv_s2 m2[16], m4[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) m2[k] = __builtin_elementwise_min(e[k], e[(k + 1) & 15]);
@@ -463,7 +471,7 @@ __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, cons
     const uint8_t* img = pyr + L.pix;
     const bool own = x < L.w;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    const mcv_us2 tt = {(unsigned short)t_lo, (unsigned short)t_lo};
+    const mcv_s2 tt = {(short)t_lo, (short)t_lo}, ntt = {(short)-t_lo, (short)-t_lo};
     // this lane's columns inside the detection range [3, w - 3)
     int bmask = 0;
 #pragma unroll
@@ -504,13 +512,17 @@ __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, cons
             const uint32_t c0 = C[lane], c1 = C[lane + 1], c2 = C[lane + 2];
             const uint32_t up = ring[(y - 3) & (kRing - 1)][lane + 1];
             const uint32_t dn = ring[(y + 3) & (kRing - 1)][lane + 1];
-            const uint32_t rt = __builtin_amdgcn_alignbyte(c2, c1, 3);  // columns x+3 .. x+6
-            const uint32_t lf = __builtin_amdgcn_alignbyte(c1, c0, 1);  // columns x-3 .. x
-            const uint32_t m = 0x00FF00FFu;
-            const uint32_t ev = compass2(as_us2(c1 & m), as_us2(dn & m), as_us2(rt & m), as_us2(up & m),
-                                         as_us2(lf & m), tt);  // pixels 0, 2
-            const uint32_t od = compass2(as_us2((c1 >> 8) & m), as_us2((dn >> 8) & m), as_us2((rt >> 8) & m),
-                                         as_us2((up >> 8) & m), as_us2((lf >> 8) & m), tt);  // pixels 1, 3
+            // pixels 0, 2 (even bytes) and 1, 3 (odd bytes) of the lane's
+            // word in u16 lanes; byte permutes pick the centre, north and
+            // south bytes and the east (x+3) and west (x-3) neighbours out of
+            // the adjacent words (selector 0x0c: zero byte)
+            const uint32_t m = 0x00FF00FFu, odd = 0x0c030c01u;
+            const uint32_t ev = compass2(as_us2(c1 & m), as_us2(dn & m), as_us2(__builtin_amdgcn_perm(c2, c1, 0x0c050c03u)),
+                                         as_us2(up & m), as_us2(__builtin_amdgcn_perm(c1, c0, 0x0c030c01u)), tt, ntt);
+            const uint32_t od = compass2(as_us2(__builtin_amdgcn_perm(0u, c1, odd)), as_us2(__builtin_amdgcn_perm(0u, dn, odd)),
+                                         as_us2(__builtin_amdgcn_perm(c2, c1, 0x0c060c04u)),
+                                         as_us2(__builtin_amdgcn_perm(0u, up, odd)),
+                                         as_us2(__builtin_amdgcn_perm(c1, c0, 0x0c040c02u)), tt, ntt);
             surv = (((ev & 0xFFFFu) ? 1 : 0) | ((od & 0xFFFFu) ? 2 : 0) | ((ev >> 16) ? 4 : 0) | ((od >> 16) ? 8 : 0)) &
                    bmask;
         }

@@ -41,7 +41,8 @@ def test_orb_pyramid_and_blur_levels(orb_default):
 
 
 @pytest.mark.parametrize("cfg", [(1000, 1.2, 8, 20, 7), (2000, 1.2, 1, 28, 15), (3000, 1.3, 6, 12, 5),
-                                 (500, 1.2, 8, 40, 20)])
+                                 (500, 1.2, 8, 40, 20), (1000, 1.2, 4, 1, 0), (1000, 1.2, 4, 0, 0),
+                                 (1000, 1.2, 4, 200, 130)])
 def test_orb_configs(gpu_ctx, cfg):
     orb = ORB(*cfg, ctx=gpu_ctx)
     _check(orb, synth.blob_image(31, 640, 480))

@@ -107,7 +107,9 @@ def measured_traffic(path=ROOT / "profiles" / "hbm_traffic_rig64.txt"):
     steps = rows["k_fast_score"][0]
     out = {}
     for name, (calls, per_call) in rows.items():
-        key = "k_knn2" if name.startswith("k_knn2") else name
+        # launch variants (k_knn2_s / k_knn2_lds, k_keypoints_k) under the
+        # name the timed run reports
+        key = next((k for k in ("k_knn2", "k_keypoints") if name.startswith(k)), name)
         out[key] = out.get(key, 0) + calls * per_call / steps
     return out
 

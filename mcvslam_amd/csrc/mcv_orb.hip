@@ -601,12 +601,8 @@ __device__ __forceinline__ void cell_stage(const CellWin& C, uint32_t* __restric
         if (ry >= 0 && ry < C.DH) {
             v = *reinterpret_cast<const uint32_t*>(C.wbase + (__umul24(ry, C.pitch) + 4 * w));
             // keep the bytes of detection columns [0, DW)
-            const int lo = C.lead + 1 - 4 * w, hi = C.lead + 1 + C.DW - 4 * w;
-            uint32_t m = 0;
-#pragma unroll
-            for (int bt = 0; bt < 4; bt++)
-                if (bt >= lo && bt < hi) m |= 0xFFu << (8 * bt);
-            v &= m;
+            const int lo = min(max(C.lead + 1 - 4 * w, 0), 4), hi = min(max(C.lead + 1 + C.DW - 4 * w, 0), 4);
+            v = hi > lo ? v & ((0xFFFFFFFFu >> (32 - 8 * (hi - lo))) << (8 * lo)) : 0u;
         }
         band[r * (kCellSW / 4) + w] = v;
     }

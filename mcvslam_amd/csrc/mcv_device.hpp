@@ -35,6 +35,11 @@ __device__ __forceinline__ int xcd_block(int nb) {
 
 inline int xcd_grid(int nb) { return (nb + 7) / 8 * 8; }
 
+// number of set bits of the wave mask b in the lanes below this one
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long b) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+}
+
 // Ascending bitonic sort of n2 (a power of two) keys by the whole workgroup.
 template <typename T, int kBatch = 8>
 __device__ void bitonic_sort(T* a, int n2) {

@@ -62,6 +62,12 @@ struct OrbGlobals {
     int* err;               // bit 0: pool overflow, bit 1: level too large
 };
 
+// two 16-bit lanes in one VGPR (packed u16 operations)
+typedef unsigned short mcv_us2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ mcv_us2 as_us2(uint32_t v) { return __builtin_bit_cast(mcv_us2, v); }
+__device__ __forceinline__ uint32_t as_u32(mcv_us2 v) { return __builtin_bit_cast(uint32_t, v); }
+
 __device__ __forceinline__ int reflect101(int i, int n) {
     if (i < 0) i = -i;
     if (i >= n) i = 2 * n - 2 - i;
@@ -281,7 +287,14 @@ template <bool kFixed, bool kEdge>
 __device__ __forceinline__ void blur_strip(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
                                            const LevelDesc& L, int x, int y0, int yend, const BlurTaps& taps,
                                            const EdgeGather& eg) {
-    uint32_t hr[7][4];
+    // vertical taps by u16-pair dot products: pr[s % 6][j] holds the
+    // horizontal sums of rows s and s + 1 of column j (each < 2^16) packed,
+    // so rows s .. s + 6 take three udot2 and one multiply-add
+    uint32_t hp[4] = {0, 0, 0, 0}, pr[6][4];
+    const mcv_us2 t01 = {(unsigned short)taps.t[0], (unsigned short)taps.t[1]};
+    const mcv_us2 t23 = {(unsigned short)taps.t[2], (unsigned short)taps.t[3]};
+    const mcv_us2 t45 = {(unsigned short)taps.t[4], (unsigned short)taps.t[5]};
+    const uint32_t t6 = (uint32_t)taps.t[6];
     // one row ahead: the next row's words are requested before this row's
     // taps are computed
     uint32_t n0, n1, n2;
@@ -292,25 +305,32 @@ __device__ __forceinline__ void blur_strip(const uint8_t* __restrict__ src, uint
         if (!kFixed && yi - 3 >= yend) break;
         const uint32_t w0 = n0, w1 = n1, w2 = n2;
         if (r + 1 < kStripH + 6) blur_load_row<kFixed, kEdge>(src, L, x, yi + 1, eg, n0, n1, n2);
-        uint32_t* h = hr[r % 7];
+        uint32_t h[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const uint32_t lo = j == 3 ? w1 : __builtin_amdgcn_alignbyte(w1, w0, j + 1);
             const uint32_t hi = j == 3 ? w2 : __builtin_amdgcn_alignbyte(w2, w1, j + 1);
             h[j] = __builtin_amdgcn_udot4(lo, taps.lo4, __builtin_amdgcn_udot4(hi, taps.hi3, 0u, false), false);
         }
+        if (r >= 1) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) pr[(r - 1) % 6][j] = hp[j] | (h[j] << 16);
+        }
         if (r >= 6) {
             const int y = yi - 3;
             uint32_t v = 0;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                uint32_t acc = 0;
-#pragma unroll
-                for (int k = 0; k < 7; k++) acc += __umul24((uint32_t)taps.t[k], hr[(r - 6 + k) % 7][j]);
+                uint32_t acc = __umul24(t6, h[j]);
+                acc = __builtin_amdgcn_udot2(as_us2(pr[(r - 2) % 6][j]), t45, acc, false);
+                acc = __builtin_amdgcn_udot2(as_us2(pr[(r - 4) % 6][j]), t23, acc, false);
+                acc = __builtin_amdgcn_udot2(as_us2(pr[(r - 6) % 6][j]), t01, acc, false);
                 v |= min((acc + (1u << 15)) >> 16, 255u) << (8 * j);
             }
             if (!kFixed || y < yend) *reinterpret_cast<uint32_t*>(dst + (uint32_t)(y * L.pitch + x)) = v;
         }
+#pragma unroll
+        for (int j = 0; j < 4; j++) hp[j] = h[j];
     }
 }
 
@@ -392,10 +412,6 @@ __device__ __forceinline__ int byte_of(uint32_t w, int j) { return (int)((w >> (
 // u16 operation.  Survivors queue up in LDS and are scored 64 at a time by
 // the whole wave from the ring; an 8-row block of scores is assembled in LDS
 // and stored once its queue is drained.
-typedef unsigned short mcv_us2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ mcv_us2 as_us2(uint32_t v) { return __builtin_bit_cast(mcv_us2, v); }
-__device__ __forceinline__ uint32_t as_u32(mcv_us2 v) { return __builtin_bit_cast(uint32_t, v); }
 
 typedef short mcv_s2 __attribute__((ext_vector_type(2)));
 
@@ -470,7 +486,6 @@ __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, cons
     uint32_t* queue = s_queue[wv];
     const uint8_t* img = pyr + L.pix;
     const bool own = x < L.w;
-    const unsigned long long lt = (1ull << lane) - 1ull;
     const mcv_s2 tt = {(short)t_lo, (short)t_lo}, ntt = {(short)-t_lo, (short)-t_lo};
     // this lane's columns inside the detection range [3, w - 3)
     int bmask = 0;
@@ -523,13 +538,17 @@ __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, cons
                                          as_us2(__builtin_amdgcn_perm(c2, c1, 0x0c060c04u)),
                                          as_us2(__builtin_amdgcn_perm(0u, up, odd)),
                                          as_us2(__builtin_amdgcn_perm(c1, c0, 0x0c040c02u)), tt, ntt);
-            surv = (((ev & 0xFFFFu) ? 1 : 0) | ((od & 0xFFFFu) ? 2 : 0) | ((ev >> 16) ? 4 : 0) | ((od >> 16) ? 8 : 0)) &
-                   bmask;
+            // each 16-bit lane capped at 1, then bit 0 / 16 of ev and od
+            // moved to bits 0..3 (pixel order 0, 1, 2, 3)
+            const uint32_t e1 = as_u32(__builtin_elementwise_min(as_us2(ev), mcv_us2{1, 1}));
+            const uint32_t o1 = as_u32(__builtin_elementwise_min(as_us2(od), mcv_us2{1, 1}));
+            const uint32_t t2 = e1 | (o1 << 1);
+            surv = (int)(((t2 & 3u) | ((t2 >> 14) & 12u)) & (uint32_t)bmask);
         }
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const unsigned long long b = __ballot((surv >> j) & 1);
-            if ((surv >> j) & 1) queue[qlen + __popcll(b & lt)] = ((uint32_t)tb << 8) | (uint32_t)(4 * lane + j);
+            if ((surv >> j) & 1) queue[qlen + lanes_below(b)] = ((uint32_t)tb << 8) | (uint32_t)(4 * lane + j);
             qlen += __popcll(b);
         }
         const bool block_end = tb == kBlk - 1 || y == yend - 1;
@@ -623,7 +642,6 @@ __device__ __forceinline__ void cell_stage(const CellWin& C, uint32_t* __restric
 __device__ __forceinline__ int cell_scan(const CellWin& C, const uint32_t* __restrict__ band, int y0, int nb,
                                          int t, int offx, int offy, uint32_t* __restrict__ out, int written) {
     const int lane = threadIdx.x & 63;
-    const unsigned long long lt = (1ull << lane) - 1ull;
     const int wf = (C.lead + 1) >> 2;                 // first word holding an area byte
     const int nwa = ((C.lead + C.DW) >> 2) - wf + 1;  // words per row holding area bytes
     const int wlast = (C.lead + C.DW + 1) >> 2;       // last staged word a right neighbour can need
@@ -678,7 +696,7 @@ __device__ __forceinline__ int cell_scan(const CellWin& C, const uint32_t* __res
 #pragma unroll
             for (int bt = 0; bt < 4; bt++) {
                 const unsigned long long bm = __ballot((kb >> bt) & 1);
-                below += __popcll(bm & lt);
+                below += lanes_below(bm);
                 total += __popcll(bm);
             }
             int k = written + below;

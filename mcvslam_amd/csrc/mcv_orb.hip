@@ -1595,16 +1595,17 @@ __device__ __forceinline__ float readlane_f(float v, int k) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
-// k_keypoints with K consecutive keypoint slots per wave.  The per-keypoint
-// set-up (slot entry, level prefix, coordinates), the angle (fastAtan2 of
-// the moments) and the rotation (host-libm cosf/sinf: the base values
-// patched from the exception table) run lane-parallel, lane k for slot
-// k; the moments and the descriptors, which spread one keypoint over the
+// k_keypoints with K keypoint slots per wave.  The per-keypoint set-up
+// (slot entry, level prefix, coordinates), the angle (fastAtan2 of the
+// moments) and the rotation (host-libm cosf/sinf: the base values patched
+// from the exception table) run lane-parallel, lane k for the wave's k-th
+// slot; the moments and the descriptors, which spread one keypoint over the
 // wave, then walk the K keypoints with their operands read from lane k and
 // the next keypoint's loads issued before the current one is reduced.
 // Same arithmetic and outputs as k_keypoints (ORBextractor.cc:75-141,
 // :568-578, :845-896).
 constexpr int kKpVariants = 4;  // keypoints per wave: 1 (k_keypoints), 8, 16, 32
+constexpr int kKpGroup = 256;   // waves that share slot windows in k_keypoints_k
 inline const char* kp_variant_name(int v) {
     return v == 1 ? "8/wave" : v == 2 ? "16/wave" : v == 3 ? "32/wave" : "1/wave";
 }
@@ -1615,19 +1616,27 @@ __global__ __launch_bounds__(256, 6) void k_keypoints_k(OrbGlobals g, const int*
                                                      int* out_count) {
     static_assert(K >= 1 && K <= 64, "one lane per keypoint of the wave");
     __shared__ uint32_t s_patch[4][kPatchRows * kPatchWords];
-    const int xb = xcd_block((n_slots + 4 * K - 1) / (4 * K));
+    const int nwg = (n_slots + 4 * K - 1) / (4 * K);
+    const int xb = xcd_block(nwg);
     if (xb < 0) return;
     const int lane = threadIdx.x & 63;
-    const int slot0 = (xb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) * K;
-    if (slot0 >= n_slots) return;
-    // ---- set-up: lane k < K takes slot slot0 + k
+    // Slots are dealt out across groups of kKpGroup consecutive waves (one
+    // XCD's share of the grid is contiguous): wave w of a group of gs waves
+    // takes slots base + w + k * gs, so the waves of a group work on a
+    // contiguous window of slots at each step k and their patches share L2.
+    const int wv = xb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int sg = wv / kKpGroup, gs = min(kKpGroup, 4 * nwg - sg * kKpGroup);
+    const int sbase = sg * kKpGroup * K + (wv - sg * kKpGroup);
+    if (sbase >= n_slots) return;
+    // ---- set-up: lane k < K takes slot sbase + k * gs
     bool kvalid = false, kseed = false;
     int kimg = 0, klevel = 0, kpos = 0, kcx = 0, kcy = 0, kw = 0, kh = 0, kpitch = 0;
     long long kpix = 0;
     float kx = 0.f, ky = 0.f, kresp = 0.f, ksize = 0.f, kangle = 0.f, kscale = 1.f;
-    if (lane < K && slot0 + lane < n_slots) {
-        const int2 sl = reinterpret_cast<const int2*>(slot_lvl)[slot0 + lane];
-        const uint32_t win = g.nodes_out[slot0 + lane];
+    const int kslot = sbase + lane * gs;
+    if (lane < K && kslot < n_slots) {
+        const int2 sl = reinterpret_cast<const int2*>(slot_lvl)[kslot];
+        const uint32_t win = g.nodes_out[kslot];
         const LevelDesc* Lp = g.lv + sl.x;
         const int j = sl.y;
         kimg = Lp->img;

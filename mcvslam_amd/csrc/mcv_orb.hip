@@ -630,17 +630,21 @@ __device__ __forceinline__ void cell_stage(const CellWin& C, uint32_t* __restric
 // suppression over the staged rows y0 .. y0 + nb - 1; appends the keepers.
 // With t >= 1 the keep rule s > max over the 8 neighbours of (b > t ? b - 1
 // : 0), s = best - 1, best > t, is the strict local maximum best > every
-// neighbour's byte (a neighbour at or below t is below best anyway).  A lane
-// takes one staged word (4 columns) of a row, words in row-major order, and
-// tests its 4 bytes at once: the 8 neighbour words (the left and right
-// neighbours rebuilt from the adjacent words with byte shifts) are maxed as
-// even and odd bytes in packed u16 halves, and a byte is kept when it
-// exceeds both that maximum and t; words with no byte above t (most of
-// them) skip the test.  The keepers of a step are compacted in
-// (word, byte) order: the lane's offset is the count of keep bits in the
-// lanes below (one ballot per byte position).
+// neighbour's byte (a neighbour at or below t is below best anyway).  The
+// staged words (4 columns each) are scanned 64 at a time in row-major order;
+// those with a byte above t (a minority) join a per-wave queue in that order,
+// and the queue is suppressed 64 words at a time: a lane tests its word's 4
+// bytes at once, the 8 neighbour words (the left and right neighbours
+// rebuilt from the adjacent words with byte shifts) maxed as even and odd
+// bytes in packed u16 halves, and a byte is kept when it exceeds both that
+// maximum and t.  The keepers of a step are compacted in (word, byte) order:
+// the lane's offset is the count of keep bits in the lanes below (one
+// ballot per byte position).
+constexpr int kCellQ = 128;  // queued word indices per wave (a ring)
+
 __device__ __forceinline__ int cell_scan(const CellWin& C, const uint32_t* __restrict__ band, int y0, int nb,
-                                         int t, int offx, int offy, uint32_t* __restrict__ out, int written) {
+                                         int t, int offx, int offy, uint32_t* __restrict__ out, int written,
+                                         uint32_t* __restrict__ q) {
     const int lane = threadIdx.x & 63;
     const int wf = (C.lead + 1) >> 2;                 // first word holding an area byte
     const int nwa = ((C.lead + C.DW) >> 2) - wf + 1;  // words per row holding area bytes
@@ -653,25 +657,22 @@ __device__ __forceinline__ int cell_scan(const CellWin& C, const uint32_t* __res
     const mcv_us2 tt = {(unsigned short)tk, (unsigned short)tk};
     const uint32_t addt = 0x01010101u * (uint32_t)(0x7F - tk);
     const int nit = nb * nwa;
-    for (int i0 = 0; i0 < nit; i0 += 64) {
-        const int i = i0 + lane;
-        int kb = 0;  // keep bits of the word's 4 bytes
-        int r = 0, w = 0;
+    // suppression of queue entries head .. head + cnt - 1 (word indices in
+    // increasing order), keepers appended in (word, byte) order
+    int head = 0, tail = 0;
+    auto drain = [&](int cnt) {
+        int kb = 0, r = 0, w = 0;
         uint32_t W = 0;
-        if (i < nit) {
+        if (lane < cnt) {
+            const int i = (int)q[(head + lane) & (kCellQ - 1)];
             r = (int)(__umul24((uint32_t)i, amagic) >> 16);
             w = wf + i - __mul24(r, nwa);
-            W = band[(r + 1) * (kCellSW / 4) + w];
-        }
-        // t <= 127: a byte > t sets its high bit in (W + 0x01010101 * (127 - t)) | W
-        const bool any = tk <= 127 ? ((((W + addt) | W) & 0x80808080u) != 0u)
-                                   : (int)max(max(W & 0xFFu, (W >> 8) & 0xFFu), max((W >> 16) & 0xFFu, W >> 24)) > tk;
-        if (any) {
             const int wl = max(w - 1, 0), wr = min(w + 1, wlast);
             // staged rows r (above), r + 1 (the word's row), r + 2 (below)
             const uint32_t* R0 = band + r * (kCellSW / 4);
             const uint32_t* R1 = R0 + kCellSW / 4;
             const uint32_t* R2 = R1 + kCellSW / 4;
+            W = R1[w];
             const uint32_t n[8] = {__builtin_amdgcn_alignbyte(R0[w], R0[wl], 3), R0[w],
                                    __builtin_amdgcn_alignbyte(R0[wr], R0[w], 1), __builtin_amdgcn_alignbyte(W, R1[wl], 3),
                                    __builtin_amdgcn_alignbyte(R1[wr], W, 1), __builtin_amdgcn_alignbyte(R2[w], R2[wl], 3),
@@ -679,9 +680,9 @@ __device__ __forceinline__ int cell_scan(const CellWin& C, const uint32_t* __res
             const uint32_t me = 0x00FF00FFu;
             mcv_us2 mxe = as_us2(n[0] & me), mxo = as_us2((n[0] >> 8) & me);
 #pragma unroll
-            for (int q = 1; q < 8; q++) {
-                mxe = __builtin_elementwise_max(mxe, as_us2(n[q] & me));
-                mxo = __builtin_elementwise_max(mxo, as_us2((n[q] >> 8) & me));
+            for (int k = 1; k < 8; k++) {
+                mxe = __builtin_elementwise_max(mxe, as_us2(n[k] & me));
+                mxo = __builtin_elementwise_max(mxo, as_us2((n[k] >> 8) & me));
             }
             const mcv_us2 ce = as_us2(W & me), co = as_us2((W >> 8) & me);
             const uint32_t ke = as_u32(__builtin_elementwise_min(__builtin_elementwise_sub_sat(ce, mxe),
@@ -709,12 +710,34 @@ __device__ __forceinline__ int cell_scan(const CellWin& C, const uint32_t* __res
                 }
             written += total;
         }
+        head += cnt;
+    };
+    for (int i0 = 0; i0 < nit; i0 += 64) {
+        const int i = i0 + lane;
+        uint32_t W = 0;
+        if (i < nit) {
+            const int r = (int)(__umul24((uint32_t)i, amagic) >> 16);
+            W = band[(r + 1) * (kCellSW / 4) + wf + i - __mul24(r, nwa)];
+        }
+        // t <= 127: a byte > t sets its high bit in (W + 0x01010101 * (127 - t)) | W
+        const bool any = tk <= 127 ? ((((W + addt) | W) & 0x80808080u) != 0u)
+                                   : (int)max(max(W & 0xFFu, (W >> 8) & 0xFFu), max((W >> 16) & 0xFFu, W >> 24)) > tk;
+        // words with a byte above t join the queue (fewer than 64 are queued
+        // before, so the ring never holds more than 127)
+        const unsigned long long b = __ballot(any);
+        wave_lds_sync();
+        if (any) q[(tail + (int)lanes_below(b)) & (kCellQ - 1)] = (uint32_t)i;
+        tail += __popcll(b);
+        wave_lds_sync();
+        if (tail - head >= 64) drain(64);
     }
+    if (tail > head) drain(tail - head);
     return written;
 }
 
 __global__ __launch_bounds__(256) void k_fast_cells(OrbGlobals g, int ncells) {
     __shared__ uint32_t s_band[4][(kCellBand + 2) * kCellSW / 4];
+    __shared__ uint32_t s_q[4][kCellQ];
     const int ci = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (ci >= ncells) return;
     const int lane = threadIdx.x & 63;
@@ -749,7 +772,8 @@ __global__ __launch_bounds__(256) void k_fast_cells(OrbGlobals g, int ncells) {
                     cell_stage(C, band, y0, nb);
                     wave_lds_sync();
                 }
-                written = cell_scan(C, band, y0, nb, t, c.offx, c.offy, out, written);
+                written = cell_scan(C, band, y0, nb, t, c.offx, c.offy, out, written,
+                                    s_q[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
                 if (!once) wave_lds_sync();
             }
             total = written;

@@ -214,7 +214,9 @@ __global__ __launch_bounds__(64) void k_resize(const Strip* st, int n, const Lev
             for (int j = 0; j < 4; j++) {
                 const int r0 = r0p[o0[j]] * a0[j] + r0p[o1[j]] * a1[j];
                 const int r1 = r1p[o0[j]] * a0[j] + r1p[o1[j]] * a1[j];
-                const uint32_t o = (uint32_t)((((b0 * (r0 >> 4)) >> 16) + ((b1 * (r1 >> 4)) >> 16) + 2) >> 2);
+                // b < 2^12 and r >> 4 < 2^15: 24-bit multiplies, same products
+                const uint32_t o = (((__umul24((uint32_t)b0, (uint32_t)(r0 >> 4))) >> 16) +
+                                    ((__umul24((uint32_t)b1, (uint32_t)(r1 >> 4))) >> 16) + 2) >> 2;
                 v |= (o & 0xFF) << (8 * j);
             }
             if (own) *reinterpret_cast<uint32_t*>(dp + (uint32_t)(y * L.pitch + x)) = v;
@@ -1405,6 +1407,7 @@ constexpr int kPatchR = 18;                           // test points lie within 
 constexpr int kPatchRows = 2 * kPatchR + 1;            // 37
 constexpr int kPatchWords = 10;                        // u32 per staged row (aligned column below cx - 18)
 constexpr int kPatchIt = (kPatchRows * kPatchWords + 63) / 64;
+static_assert(kPatchWords == 10 && kPatchIt * 64 <= 384, "patch word division by 6554 / 2^16");
 
 __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot_lvl, int n_slots,
                                                    SinCosTables tb, float* out_kps, uint8_t* out_desc, int cap,
@@ -1822,7 +1825,9 @@ __global__ __launch_bounds__(256, 6) void k_keypoints_k(OrbGlobals g, const int*
             const int bx = (cx - kPatchR) & ~3;
 #pragma unroll
             for (int it = 0; it < kPatchIt; it++) {
-                const int q = lane + 64 * it, r = q / kPatchWords, w = q - kPatchWords * r;
+                // q / 10 as (q * ceil(2^16 / 10)) >> 16, exact for q < 384
+                const int q = lane + 64 * it, r = (int)(__umul24((uint32_t)q, 6554u) >> 16),
+                          w = q - kPatchWords * r;
                 pw[it] = (q < kPatchRows * kPatchWords && bx + 4 * w + 4 <= pitch)
                              ? *reinterpret_cast<const uint32_t*>(
                                    bl + (uint32_t)((cy - kPatchR) * pitch + __mul24(r, pitch) + bx + 4 * w))

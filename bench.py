@@ -40,11 +40,6 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_frames(n_unique: int):
-    from mcvslam_amd import synth
-    return [synth.rig_frame(1000 + 7 * f) for f in range(n_unique)]
-
-
 def orb_level_pixels(w, h, inv_scale):
     px = []
     for l, s in enumerate(inv_scale):
@@ -199,9 +194,9 @@ def main():
 
     log("generating frames ...")
     t0 = time.perf_counter()
-    uniq = make_frames(min(UNIQUE_FRAMES, args.frames))
-    log(f"  {len(uniq)} unique rig frames in {time.perf_counter() - t0:.1f}s")
     n = args.frames
+    uniq, _, nfeat = synth.rig_batch(n, UNIQUE_FRAMES)
+    log(f"  {len(uniq)} unique rig frames in {time.perf_counter() - t0:.1f}s")
     L = torch.empty((n, 720, 1280), dtype=torch.uint8, device=dev)
     R = torch.empty((n, 720, 1280), dtype=torch.uint8, device=dev)
     W = torch.empty((n, 480, 640), dtype=torch.uint8, device=dev)
@@ -215,8 +210,6 @@ def main():
         ptrs += [L[f].data_ptr(), R[f].data_ptr(), W[f].data_ptr()]
     torch.cuda.synchronize()
 
-    nfeat = np.concatenate([np.random.Generator(np.random.PCG64(40 + f)).integers(500, 4001, 3)
-                            for f in range(n)]).astype(np.int32)
     rig = RigBatch(orb, n, ratio=0.7, th=50, nfeatures=nfeat)
     tuning = ""
     if not args.no_autotune:

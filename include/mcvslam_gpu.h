@@ -26,6 +26,7 @@ extern "C" {
 #define MCV_ERR_STATE (-5)     /* not configured, or device work overflowed */
 
 #define MCV_FLAG_GRAPH 1       /* replay the launch sequence as a captured hipGraph */
+#define MCV_FLAG_CHECK 2       /* wait for the run and report device-side overflow (see below) */
 
 typedef struct mcv_ctx mcv_ctx;
 typedef struct mcv_rig mcv_rig;
@@ -75,13 +76,39 @@ int mcv_orb_extract(mcv_ctx* ctx, const uint8_t* image, int width, int height, i
 /* mvImagePyramid[level] of the last mcv_orb_extract (ORBextractor.h:72, read
  * by Frame::ComputeStereoMatch, src/Frame.cpp:239-261). */
 int mcv_orb_pyramid(mcv_ctx* ctx, int level, uint8_t* out, int capacity, int32_t* width, int32_t* height);
-/* Batched extraction of device-resident images in one launch sequence.
+/* Batched extraction of device-resident images in one launch sequence: the
+ * reference's per-camera ORBextractor::operator() calls
+ * (ORBextractor.cc:831-899, src/Frame.cpp:118-125) for n images at once.
  * d_images: host array of n device pointers; widths/heights/strides: host
- * arrays.  Outputs (device): d_kps[n * capacity], d_desc[n * capacity * 32],
- * d_counts[n].  capacity must be >= mcv_orb_capacity of every image. */
+ * arrays.  seeds/n_seeds (both may be NULL): per image a host array of
+ * n_seeds[i] keypoints appended to their octave after the octree, as the
+ * reference's input `_keypoints` (ORBextractor.cc:845-847; the KL-tracked
+ * points of src/Frame.cpp:34-74).  Outputs (device): d_kps[n * capacity],
+ * d_desc[n * capacity * 32], d_counts[n].  capacity must be >=
+ * mcv_orb_capacity(width, height, n_seeds) of every image.
+ * flags: MCV_FLAG_GRAPH replays a captured graph (recaptured whenever an
+ * output pointer or the capacity changes); MCV_FLAG_CHECK waits for the run
+ * and, when the device-side octree node pool overflowed, grows it and runs
+ * again (MCV_ERR_STATE after three attempts, or at once when a level has more
+ * FAST candidates than its cell capacity).  Without MCV_FLAG_CHECK the call
+ * only enqueues: a level that overflowed is then missing from the outputs,
+ * which mcv_orb_batch_check reports. */
 int mcv_orb_extract_batch_dev(mcv_ctx* ctx, int n, const uint8_t* const* d_images, const int32_t* widths,
-                              const int32_t* heights, const int32_t* strides, mcv_keypoint* d_kps, uint8_t* d_desc,
-                              int capacity, int32_t* d_counts, int flags);
+                              const int32_t* heights, const int32_t* strides, const mcv_keypoint* const* seeds,
+                              const int32_t* n_seeds, mcv_keypoint* d_kps, uint8_t* d_desc, int capacity,
+                              int32_t* d_counts, int flags);
+/* Waits for the last mcv_orb_extract_batch_dev on this context and returns
+ * MCV_ERR_STATE when it overflowed (the pool is then grown for the next run,
+ * whose outputs are complete). */
+int mcv_orb_batch_check(mcv_ctx* ctx);
+/* mvImagePyramid[level] of image `image` of the last mcv_orb_extract_batch_dev
+ * (ORBextractor.h:72, read by Frame::ComputeStereoMatch, src/Frame.cpp:239-261),
+ * left resident in HBM: a device pointer to row 0 and the row pitch in bytes.
+ * blurred = 1 gives the GaussianBlur'd copy the descriptors sample instead
+ * (ORBextractor.cc:874-875).  Valid until the next batched extraction of the
+ * same batch shape. */
+int mcv_orb_batch_pyramid_dev(mcv_ctx* ctx, int image, int level, int blurred, const uint8_t** d_level,
+                              int32_t* pitch, int32_t* width, int32_t* height);
 
 /* ---- Hamming matching --------------------------------------------------
  * Replaces Matcher::KnnMatch(const vector<Mat>&, const vector<Mat>&, int k,
@@ -159,6 +186,10 @@ int mcv_rig_check(mcv_rig* rig);
  * n = number of launches.  For kernel timing only. */
 int mcv_rig_run_timed(mcv_rig* rig, const uint8_t* const* d_images, float* ms, const char** names, int cap,
                       int32_t* n);
+/* mvImagePyramid[level] of image `image` (frame-major L, R, W) of the last
+ * batched mcv_rig_run, resident in HBM (as mcv_orb_batch_pyramid_dev). */
+int mcv_rig_pyramid_dev(mcv_rig* rig, int image, int level, int blurred, const uint8_t** d_level, int32_t* pitch,
+                        int32_t* width, int32_t* height);
 /* Device outputs: keypoints [3n][capacity], descriptors [3n][capacity][32],
  * counts [3n], matches [3n pairs][capacity][3] (query, train, distance),
  * match counts [3n]; knn rows [3n pairs][capacity][4]. */
@@ -172,19 +203,36 @@ int mcv_rig_outputs(mcv_rig* rig, mcv_keypoint** d_kps, uint8_t** d_desc, int32_
  * output row is (kp0, d0, kp1, d1, n_candidates, match) with keypoint
  * indices (-1 for the 999 sentinel) and match = accepted keypoint or -1.
  * R, t: world-to-camera (row-major 3x3 and 3, f32); cam: fx, fy, cx, cy;
- * scales: per-level scale factors (nlevels entries). */
+ * scales: per-level scale factors (nlevels entries); every plevel must lie in
+ * [0, nlevels) (MCV_ERR_INVALID otherwise).  ProjectBunchMapPoints applies
+ * no level filter, so the keypoints' octaves are not an input. */
 int mcv_window_search(mcv_ctx* ctx, const float* points, const uint8_t* pdesc, const int32_t* plevel, int n_points,
                       const float* R, const float* t, const float* cam, const float* scales, int nlevels, float radius,
-                      float ratio, int th, const float* kxy, const int32_t* koct, const uint8_t* kdesc, int n_kps,
-                      int width, int height, int32_t* out);
+                      float ratio, int th, const float* kxy, const uint8_t* kdesc, int n_kps, int width, int height,
+                      int32_t* out);
 /* Device variant over several cameras at once: arrays are device pointers,
- * cameras index blocks of keypoints; out is [n_cams][n_points][6]. */
+ * cameras index blocks of keypoints; out is [n_cams][n_points][6].  A map
+ * point whose level is outside [0, nlevels) gets the row (-1, 999, -1, 999,
+ * -1, -1) (n_candidates = -1 marks it). */
 int mcv_window_search_dev(mcv_ctx* ctx, int n_cams, const float* d_points, const uint8_t* d_pdesc,
                           const int32_t* d_plevel, int n_points, const float* Rt /* host: n_cams x 12 */,
                           const float* cams /* host: n_cams x 4 */, const int32_t* sizes /* host: n_cams x 2 */,
                           const float* scales, int nlevels, float radius, float ratio, int th,
-                          const float* const* d_kxy, const int32_t* const* d_koct, const uint8_t* const* d_kdesc,
+                          const float* const* d_kxy, const uint8_t* const* d_kdesc,
                           const int32_t* n_kps /* host */, int32_t* d_out);
+/* Window query by position, the loop of Tracker::Wnd_Track
+ * (src/Tracker.cpp:339-353): for each of n query positions d_uv (n x 2) with
+ * descriptor d_qdesc (n x 32), GetFeaturesInArea(x, y, radius, min_level,
+ * max_level) (src/Object.cpp:263-308, level filter on d_koct as there: active
+ * when min_level > 0 or max_level >= 0) over the keypoints d_kxy / d_koct /
+ * d_kdesc of a width x height image, then KnnMatch -> FilterRatio(ratio) ->
+ * FilterThreshold(th) of the query against the candidates.  Device arrays;
+ * d_out is n x 6 as mcv_window_search.  (FilterOrientation of one match keeps
+ * it, so Wnd_Track's accepted match is the row's `match`.) */
+int mcv_window_search_uv_dev(mcv_ctx* ctx, const float* d_uv, const uint8_t* d_qdesc, int n, float radius,
+                             int min_level, int max_level, float ratio, int th, const float* d_kxy,
+                             const int32_t* d_koct, const uint8_t* d_kdesc, int n_kps, int width, int height,
+                             int32_t* d_out);
 
 /* ---- bundle-adjustment linearization -----------------------------------
  * The batched seam of BlockSolver::buildSystem / SparseOptimizer::
@@ -238,6 +286,11 @@ int mcv_ba_read_stereo_planes(mcv_ba* ba, int first, int count, float* host);
  * uses 5.991 for edges of the left camera object, 7.815 for the right).
  * Host outputs of M (mono) and Ms (stereo) bytes; either may be null. */
 int mcv_ba_outliers(mcv_ba* ba, float thr_mono, float thr_stereo, uint8_t* flags_mono, uint8_t* flags_stereo);
+/* Version of the planes: changes with every mcv_ba_set_state,
+ * mcv_ba_set_observations and linearization.  A caller that keeps the
+ * linearization for later (the autograd backward) records it and checks that
+ * it is unchanged before mcv_ba_adjoint_dev.  No counterpart in the reference. */
+int mcv_ba_generation(mcv_ba* ba, int64_t* gen);
 /* Device pointer of the planes (plane k at d_planes + k * stride). */
 int mcv_ba_planes(mcv_ba* ba, float** d_planes, int64_t* stride);
 /* Copy planes [first, first + count) to host (count x M floats). */

@@ -24,6 +24,7 @@ MCV_ERR_CAPACITY = -3
 MCV_ERR_EMPTY = -4
 MCV_ERR_STATE = -5
 MCV_FLAG_GRAPH = 1
+MCV_FLAG_CHECK = 2
 
 
 class KeyPoint(ctypes.Structure):
@@ -44,7 +45,10 @@ SIGNATURES = {
     "mcv_orb_capacity": (I, [P, I, I, I, P]),
     "mcv_orb_extract": (I, [P, P, I, I, I, P, I, P, P, I, P]),
     "mcv_orb_pyramid": (I, [P, I, P, I, P, P]),
-    "mcv_orb_extract_batch_dev": (I, [P, I, P, P, P, P, P, P, I, P, I]),
+    "mcv_orb_extract_batch_dev": (I, [P, I, P, P, P, P, P, P, P, P, I, P, I]),
+    "mcv_orb_batch_check": (I, [P]),
+    "mcv_orb_batch_pyramid_dev": (I, [P, I, I, I, P, P, P, P]),
+    "mcv_rig_pyramid_dev": (I, [P, I, I, I, P, P, P, P]),
     "mcv_match_knn2": (I, [P, P, I, P, I, P]),
     "mcv_match_set_variant": (I, [P, I]),
     "mcv_distinctive_descriptors": (I, [P, P, P, I, P, P]),
@@ -61,8 +65,9 @@ SIGNATURES = {
     "mcv_rig_check": (I, [P]),
     "mcv_rig_stereo": (I, [P, F, F]),
     "mcv_rig_stereo_outputs": (I, [P, P, P, P]),
-    "mcv_window_search": (I, [P, P, P, P, I, P, P, P, P, I, F, F, I, P, P, P, I, I, I, P]),
-    "mcv_window_search_dev": (I, [P, I, P, P, P, I, P, P, P, P, I, F, F, I, P, P, P, P, P]),
+    "mcv_window_search": (I, [P, P, P, P, I, P, P, P, P, I, F, F, I, P, P, I, I, I, P]),
+    "mcv_window_search_dev": (I, [P, I, P, P, P, I, P, P, P, P, I, F, F, I, P, P, P, P]),
+    "mcv_window_search_uv_dev": (I, [P, P, P, I, F, I, I, F, I, P, P, P, I, I, I, P]),
     "mcv_ba_create": (I, [P, I, I, I, I, P]),
     "mcv_ba_destroy": (None, [P]),
     "mcv_ba_set_observations": (I, [P, P, P, P, P, P]),
@@ -75,6 +80,7 @@ SIGNATURES = {
     "mcv_ba_linearize_stereo": (I, [P, I]),
     "mcv_ba_read_stereo_planes": (I, [P, I, I, P]),
     "mcv_ba_outliers": (I, [P, F, F, P, P]),
+    "mcv_ba_generation": (I, [P, P]),
     "mcv_ba_planes": (I, [P, P, P]),
     "mcv_ba_read_planes": (I, [P, I, I, P]),
     "mcv_ba_copy_planes_dev": (I, [P, I, I, P]),

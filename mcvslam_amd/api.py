@@ -150,26 +150,63 @@ class ORB:
         check(self.ctx.h, lib().mcv_orb_pyramid(self.ctx.h, level, _p(out), out.size, ctypes.byref(w), ctypes.byref(h)))
         return out
 
-    def extract_batch_dev(self, images, graph: bool = False):
+    def extract_batch_dev(self, images, graph: bool = False, seeds=None, check_overflow: bool = True):
         """Batched extraction of torch.cuda uint8 images (H x W, contiguous).
+        seeds: None or, per image, None / KP_DTYPE records appended to their
+        octave (ORBextractor.cc:845-847).  check_overflow waits for the run and
+        reruns it after growing the octree pool when that overflowed.
         Returns device tensors (keypoints n x cap x 6 as float32 view of
         KP records, descriptors n x cap x 32, counts n)."""
         import torch
         n = len(images)
         ws = np.array([im.shape[1] for im in images], np.int32)
         hs = np.array([im.shape[0] for im in images], np.int32)
-        cap = max(self.capacity(int(w), int(h)) for w, h in zip(ws, hs))
+        sd_arrays = [None] * n
+        ns = np.zeros(n, np.int32)
+        if seeds is not None:
+            for i, sd in enumerate(seeds):
+                if sd is not None and len(sd):
+                    sd_arrays[i] = np.ascontiguousarray(sd, dtype=KP_DTYPE)
+                    ns[i] = len(sd_arrays[i])
+        cap = max(self.capacity(int(w), int(h), int(k)) for w, h, k in zip(ws, hs, ns))
         dev = images[0].device
         kps = torch.zeros((n, cap, 6), dtype=torch.float32, device=dev)
         desc = torch.zeros((n, cap, 32), dtype=torch.uint8, device=dev)
         counts = torch.zeros(n, dtype=torch.int32, device=dev)
         ptrs = (ctypes.c_void_p * n)(*[im.data_ptr() for im in images])
+        sptrs = (ctypes.c_void_p * n)(*[a.ctypes.data if a is not None else None for a in sd_arrays])
+        flags = (_native.MCV_FLAG_GRAPH if graph else 0) | (_native.MCV_FLAG_CHECK if check_overflow else 0)
         self.ctx.use_torch_stream()
         check(self.ctx.h, lib().mcv_orb_extract_batch_dev(
-            self.ctx.h, n, ptrs, _p(ws), _p(hs), _p(ws), ctypes.c_void_p(kps.data_ptr()),
-            ctypes.c_void_p(desc.data_ptr()), cap, ctypes.c_void_p(counts.data_ptr()),
-            _native.MCV_FLAG_GRAPH if graph else 0))
+            self.ctx.h, n, ptrs, _p(ws), _p(hs), _p(ws), sptrs if seeds is not None else None,
+            _p(ns) if seeds is not None else None, ctypes.c_void_p(kps.data_ptr()),
+            ctypes.c_void_p(desc.data_ptr()), cap, ctypes.c_void_p(counts.data_ptr()), flags))
         return kps, desc, counts
+
+    def batch_check(self):
+        """Raises when the last batched extraction overflowed (mcv_orb_batch_check)."""
+        check(self.ctx.h, lib().mcv_orb_batch_check(self.ctx.h))
+
+    def batch_pyramid(self, image: int, level: int, blurred: bool = False):
+        """mvImagePyramid[level] of image `image` of the last batched
+        extraction (blurred: the GaussianBlur'd copy the descriptors sample),
+        as a zero-copy torch view of the resident plane."""
+        ptr, pitch, w, h = ctypes.c_void_p(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(self.ctx.h, lib().mcv_orb_batch_pyramid_dev(self.ctx.h, image, level, int(blurred), ctypes.byref(ptr),
+                                                          ctypes.byref(pitch), ctypes.byref(w), ctypes.byref(h)))
+        return device_plane(ptr.value, h.value, w.value, pitch.value)
+
+
+class _DevicePlane:
+    def __init__(self, ptr: int, h: int, w: int, pitch: int):
+        self.__cuda_array_interface__ = {"shape": (h, w), "typestr": "|u1", "data": (ptr, True),
+                                         "strides": (pitch, 1), "version": 3}
+
+
+def device_plane(ptr: int, h: int, w: int, pitch: int):
+    """Zero-copy torch view (uint8, h x w, row pitch in bytes) of device memory."""
+    import torch
+    return torch.as_tensor(_DevicePlane(ptr, h, w, pitch), device="cuda")
 
 
 def kp_records(kps6) -> np.ndarray:
@@ -232,7 +269,7 @@ class Matcher:
         return out[:cnt.value].copy()
 
 
-def window_search(points, pdesc, plevel, R, t, cam, scales, radius, ratio, th, kxy, koct, kdesc, width, height,
+def window_search(points, pdesc, plevel, R, t, cam, scales, radius, ratio, th, kxy, kdesc, width, height,
                   ctx: Context | None = None) -> np.ndarray:
     """Per map point (kp0, d0, kp1, d1, n_candidates, match)."""
     ctx = ctx or default_context()
@@ -244,12 +281,58 @@ def window_search(points, pdesc, plevel, R, t, cam, scales, radius, ratio, th, k
     cm = np.ascontiguousarray(cam, np.float32).reshape(4)
     sc = np.ascontiguousarray(scales, np.float32)
     kx = np.ascontiguousarray(kxy, np.float32).reshape(-1, 2)
-    ko = np.ascontiguousarray(koct, np.int32)
     kd = np.ascontiguousarray(kdesc, np.uint8).reshape(-1, 32)
     out = np.zeros((len(pts), 6), np.int32)
     check(ctx.h, lib().mcv_window_search(ctx.h, _p(pts), _p(pd), _p(pl), len(pts), _p(Rm), _p(tv), _p(cm), _p(sc),
-                                         len(sc), ctypes.c_float(radius), ctypes.c_float(ratio), th, _p(kx), _p(ko),
+                                         len(sc), ctypes.c_float(radius), ctypes.c_float(ratio), th, _p(kx),
                                          _p(kd), len(kx), width, height, _p(out)))
+    return out
+
+
+def window_search_dev(points, pdesc, plevel, Rt, cams, sizes, scales, radius, ratio, th, kxy, kdesc,
+                      ctx: Context | None = None):
+    """mcv_window_search_dev over several cameras: torch.cuda inputs (points
+    P x 3, pdesc P x 32, plevel P; per camera kxy K x 2 and kdesc K x 32),
+    host Rt (n_cams x 12), cams (n_cams x 4), sizes (n_cams x 2).  Returns a
+    device tensor [n_cams][P][6]."""
+    import torch
+    ctx = ctx or default_context()
+    n_cams = len(kxy)
+    P = int(points.shape[0])
+    out = torch.empty((n_cams, P, 6), dtype=torch.int32, device=points.device)
+    rt = np.ascontiguousarray(Rt, np.float32).reshape(n_cams, 12)
+    cm = np.ascontiguousarray(cams, np.float32).reshape(n_cams, 4)
+    sz = np.ascontiguousarray(sizes, np.int32).reshape(n_cams, 2)
+    sc = np.ascontiguousarray(scales, np.float32)
+    nk = np.array([int(k.shape[0]) for k in kxy], np.int32)
+    kx = (ctypes.c_void_p * n_cams)(*[k.data_ptr() for k in kxy])
+    kd = (ctypes.c_void_p * n_cams)(*[k.data_ptr() for k in kdesc])
+    ctx.use_torch_stream()
+    check(ctx.h, lib().mcv_window_search_dev(ctx.h, n_cams, ctypes.c_void_p(points.data_ptr()),
+                                             ctypes.c_void_p(pdesc.data_ptr()), ctypes.c_void_p(plevel.data_ptr()),
+                                             P, _p(rt), _p(cm), _p(sz), _p(sc), len(sc), ctypes.c_float(radius),
+                                             ctypes.c_float(ratio), th, kx, kd, _p(nk),
+                                             ctypes.c_void_p(out.data_ptr())))
+    return out
+
+
+def window_search_uv_dev(uv, qdesc, radius, kxy, koct, kdesc, width, height, min_level=-1, max_level=-1,
+                         ratio: float = 0.6, th: int = 46, ctx: Context | None = None):
+    """Tracker::Wnd_Track's query (src/Tracker.cpp:339-353) for every
+    position of uv (torch.cuda n x 2) with descriptor qdesc (n x 32) against
+    keypoints kxy / koct / kdesc (torch.cuda).  Returns a device tensor n x 6
+    (kp0, d0, kp1, d1, n_candidates, match)."""
+    import torch
+    ctx = ctx or default_context()
+    n = int(uv.shape[0])
+    out = torch.empty((n, 6), dtype=torch.int32, device=uv.device)
+    ctx.use_torch_stream()
+    check(ctx.h, lib().mcv_window_search_uv_dev(ctx.h, ctypes.c_void_p(uv.data_ptr()),
+                                                ctypes.c_void_p(qdesc.data_ptr()), n, ctypes.c_float(radius),
+                                                int(min_level), int(max_level), ctypes.c_float(ratio), int(th),
+                                                ctypes.c_void_p(kxy.data_ptr()), ctypes.c_void_p(koct.data_ptr()),
+                                                ctypes.c_void_p(kdesc.data_ptr()), int(kxy.shape[0]), int(width),
+                                                int(height), ctypes.c_void_p(out.data_ptr())))
     return out
 
 
@@ -330,6 +413,16 @@ class RigBatch:
         shape = (self.n_frames, self.cap)
         return (self.ctx.to_host(ptrs[0].value, np.float32, shape), self.ctx.to_host(ptrs[1].value, np.float32, shape),
                 self.ctx.to_host(ptrs[2].value, np.int32, shape))
+
+    def pyramid(self, image: int, level: int, blurred: bool = False):
+        """mvImagePyramid[level] of image `image` (frame-major L, R, W) of the
+        last batched run (blurred: the descriptors' blurred copy), as a
+        zero-copy torch view of the resident plane."""
+        ptr, pitch, w, h = ctypes.c_void_p(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(self.ctx.h, lib().mcv_rig_pyramid_dev(self.h, image, level, int(blurred), ctypes.byref(ptr),
+                                                    ctypes.byref(pitch),
+                                                    ctypes.byref(w), ctypes.byref(h)))
+        return device_plane(ptr.value, h.value, w.value, pitch.value)
 
     def read(self):
         """Host copies: (kps records [3n][cap], desc [3n][cap][32], counts [3n],
@@ -461,6 +554,13 @@ class BALinearizer:
     def linearize_plain(self, residual_only: bool = False):
         """One launch per (frame, camera) edge group; same outputs."""
         check(self.ctx.h, lib().mcv_ba_linearize_plain(self.h, 1 if residual_only else 0))
+
+    def generation(self) -> int:
+        """Version of the planes (changes with every state, observation or
+        linearization update)."""
+        g = ctypes.c_int64()
+        check(self.ctx.h, lib().mcv_ba_generation(self.h, ctypes.byref(g)))
+        return g.value
 
     def planes_ptr(self):
         p, s = ctypes.c_void_p(), ctypes.c_int64()

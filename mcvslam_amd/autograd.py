@@ -89,12 +89,19 @@ class Reprojection(torch.autograd.Function):
         e = torch.empty((2, lin.M), dtype=torch.float32, device=pose_delta.device)
         lin.copy_planes_dev(0, 2, e.data_ptr())
         ctx.lin = lin
+        ctx.gen = lin.generation()
         ctx.dtypes = (pose_delta.dtype, point_delta.dtype)
         return e.t().contiguous()
 
     @staticmethod
     def backward(ctx, g: torch.Tensor):
         lin = ctx.lin
+        if lin.generation() != ctx.gen:
+            # the planes were relinearized (another forward on the same
+            # linearizer, or a state update) since this forward: its Jacobians
+            # are gone, and the adjoint would use another state's
+            raise RuntimeError("Reprojection.backward: the BALinearizer was relinearized after this forward; "
+                               "use one BALinearizer per forward that is still to be differentiated")
         g = g.contiguous().float()
         gpose = torch.empty((lin.F, 6), dtype=torch.float32, device=g.device)
         gpoint = torch.empty((lin.P, 3), dtype=torch.float32, device=g.device)

@@ -107,6 +107,9 @@ static uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
     return h;
 }
 
+// Key of the cached correction table: the libm's cosf/sinf and this build's
+// sincos_base at the same sample points, so a table made by another libm or
+// another build of sincos_base (source, FP flags) is never reused.
 static uint64_t libm_fingerprint() {
     uint64_t h = 1469598103934665603ull;
     const char* v = gnu_get_libc_version();
@@ -117,6 +120,10 @@ static uint64_t libm_fingerprint() {
         const float c = cosf(x), s = sinf(x);
         h = fnv1a(h, &c, 4);
         h = fnv1a(h, &s, 4);
+        float bc, bs;
+        sincos_base(x, &bc, &bs);
+        h = fnv1a(h, &bc, 4);
+        h = fnv1a(h, &bs, 4);
     }
     return h;
 }
@@ -140,7 +147,7 @@ static std::string cache_path(uint64_t fp) {
         d = std::string(home && *home ? home : "/tmp") + "/.cache/mcvslam";
     }
     char name[64];
-    snprintf(name, sizeof name, "/sincos_v1_%016llx.bin", (unsigned long long)fp);
+    snprintf(name, sizeof name, "/sincos_v2_%016llx.bin", (unsigned long long)fp);
     return d + name;
 }
 
@@ -396,7 +403,16 @@ struct OrbPlan {
     DevBuf d_heap_order, d_build_levels, d_lv, d_cells, d_strips, d_coef, d_slot, d_seeds, d_pyr, d_blur, d_score, d_cell_cand, d_cell_n, d_lvl_cand, d_oct,
         d_pool, d_used, d_nodes, d_src, d_stride;
     hipGraphExec_t graph = nullptr;
-    const void* graph_key = nullptr;
+    // the captured graph bakes in every output pointer and the capacity
+    struct GraphKey {
+        const void* kps = nullptr;
+        const void* desc = nullptr;
+        const void* counts = nullptr;
+        int cap = -1;
+        bool operator==(const GraphKey& o) const {
+            return kps == o.kps && desc == o.desc && counts == o.counts && cap == o.cap;
+        }
+    } graph_key;
     Tuning tune;
 
     ~OrbPlan() {
@@ -604,8 +620,11 @@ struct OrbPlan {
 
     // The whole extraction for the batch, enqueued on s.  With `rec`, an
     // event is recorded after every launch (kernel timing, never in a graph).
+    // reset_err = false keeps the error word of earlier enqueues (the plain
+    // rig path reuses one plan per camera and checks once per run); the pool
+    // counter is reset by every enqueue.
     void enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s, mcv_keypoint* d_kps, uint8_t* d_desc,
-                 int out_cap, int* d_counts, struct KernelTimer* rec = nullptr);
+                 int out_cap, int* d_counts, struct KernelTimer* rec = nullptr, bool reset_err = true);
 
 };
 
@@ -636,12 +655,13 @@ struct KernelTimer {
 };
 
 void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s, mcv_keypoint* d_kps, uint8_t* d_desc,
-                      int out_cap, int* d_counts, KernelTimer* rec) {
+                      int out_cap, int* d_counts, KernelTimer* rec, bool reset_err) {
     auto mark = [&](const char* n) {
         if (rec) rec->mark(n);
     };
     const OrbGlobals g = globals(c);
-    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(d_used.p), 4);
+    // words 0-1: pool words requested (u64), words 2-3: error bits
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(d_used.p), reset_err ? 4 : 2);
     mark("k_zero_words");
     const Strip* st = d_strips.as<Strip>();
     auto strip_grid = [](int n) { return dim3(xcd_grid((n + 3) / 4)); };
@@ -721,6 +741,7 @@ struct mcv_ctx {
     bool orb_ready = false;
     SinCosDev sc;
     std::unique_ptr<OrbPlan> single;  // plan of the last single-image extract
+    OrbPlan* last_batch = nullptr;     // plan of the last mcv_orb_extract_batch_dev (owned by plans)
     int single_key_w = -1, single_key_h = -1;
     std::vector<int> single_key_seeds;
     DevBuf in_img, out_kps, out_desc, out_cnt;
@@ -791,6 +812,15 @@ struct mcv_ba {
     std::vector<int> fixed;
     int variant = 0;  // k_ba_linearize launch shape (kBaVariant)
     bool have_obs = false, have_state = false, linearized = false;
+    // bumped by every change of the planes' inputs or contents (state,
+    // observations, any linearization): a reader that recorded it can tell
+    // whether the planes still hold its linearization
+    long long gen = 0;
+    void touched(bool full_linearization) {
+        linearized = full_linearization;
+        schur_done = false;
+        gen++;
+    }
     // Schur structure (depends on observations and fixed frames)
     bool schur_ready = false, schur_done = false;
     int npf = 0, Fp = 0, nblk = 0;
@@ -1044,50 +1074,139 @@ int mcv_orb_pyramid(mcv_ctx* c, int level, uint8_t* out, int capacity, int32_t* 
     API_END
 }
 
+}  // extern "C"
+
+static std::string batch_key(int n, const int32_t* widths, const int32_t* heights) {
+    std::string key = std::to_string(n);
+    for (int i = 0; i < n; i++) key += "," + std::to_string(widths[i]) + "x" + std::to_string(heights[i]);
+    return key;
+}
+
+// Waits for the plan's last run and reads its error word: returns 0, or
+// throws for a level with more FAST candidates than its cell capacity, or
+// grows the octree pool (dropping the plan's graph, which holds the old
+// pool) and returns 1 when the pool overflowed.
+static int plan_check(OrbPlan& P, hipStream_t s) {
+    unsigned long long st[2] = {0, 0};  // pool words requested, error bits
+    HIP_OK(hipMemcpyAsync(st, P.d_used.p, 16, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    const int err = (int)(st[1] & 0xFFFFFFFFu);
+    REQUIRE(!(err & 2), MCV_ERR_STATE, "a pyramid level has more FAST candidates than its cell capacity");
+    if (!(err & 1)) return 0;
+    // the counter holds what the last image asked for; earlier images of a
+    // plain run may have asked for more, so grow at least twofold
+    P.pool_cap = std::max<long long>((long long)(st[0] + st[0] / 4 + 1024), 2 * P.pool_cap);
+    P.d_pool.ensure((size_t)P.pool_cap * 4);
+    if (P.graph) {
+        HIP_OK(hipGraphExecDestroy(P.graph));
+        P.graph = nullptr;
+    }
+    return 1;
+}
+
+extern "C" {
+
 int mcv_orb_extract_batch_dev(mcv_ctx* c, int n, const uint8_t* const* d_images, const int32_t* widths,
-                              const int32_t* heights, const int32_t* strides, mcv_keypoint* d_kps, uint8_t* d_desc,
-                              int capacity, int32_t* d_counts, int flags) {
+                              const int32_t* heights, const int32_t* strides, const mcv_keypoint* const* seeds,
+                              const int32_t* n_seeds, mcv_keypoint* d_kps, uint8_t* d_desc, int capacity,
+                              int32_t* d_counts, int flags) {
     API_BEGIN(c)
     REQUIRE(c && c->orb_ready, MCV_ERR_STATE, "ORB not configured");
     REQUIRE(n > 0 && d_images && widths && heights, MCV_ERR_INVALID, "bad batch");
+    REQUIRE(!n_seeds || seeds, MCV_ERR_INVALID, "seed counts without seed arrays");
     c->set_device();
-    std::string key = std::to_string(n);
+    const int nl = c->orb.nlevels;
     std::vector<ImageSpec> specs;
+    std::string key = batch_key(n, widths, heights);
     for (int i = 0; i < n; i++) {
-        specs.push_back(ImageSpec{widths[i], heights[i], std::vector<std::vector<mcv_keypoint>>(c->orb.nlevels)});
-        key += "," + std::to_string(widths[i]) + "x" + std::to_string(heights[i]);
+        specs.push_back(ImageSpec{widths[i], heights[i], std::vector<std::vector<mcv_keypoint>>(nl)});
+        const int ns = n_seeds ? n_seeds[i] : 0;
+        REQUIRE(ns >= 0 && (ns == 0 || seeds[i]), MCV_ERR_INVALID, "bad seeds");
+        for (int k = 0; k < ns; k++) {
+            const mcv_keypoint& kp = seeds[i][k];
+            REQUIRE(kp.octave >= 0 && kp.octave < nl, MCV_ERR_INVALID, "seed octave out of range");
+            specs[i].seeds[kp.octave].push_back(kp);
+        }
+        if (ns) {
+            // the plan's keypoint slots depend on the seed count of every level
+            key += ";" + std::to_string(i) + ":";
+            for (int l = 0; l < nl; l++) key += std::to_string(specs[i].seeds[l].size()) + ".";
+        }
     }
     auto& slot = c->plans[key];
     if (!slot) {
         slot.reset(new OrbPlan());
         slot->build(c->orb, specs, pool_words_for(n));
         slot->upload_all(c->stream);
+    } else if (n_seeds) {
+        // same counts, new values: rows in the order build() lays them out
+        slot->seeds.clear();
+        for (const ImageSpec& sp : specs)
+            for (int l = 0; l < nl; l++)
+                for (const mcv_keypoint& k : sp.seeds[l])
+                    slot->seeds.insert(slot->seeds.end(), {k.x, k.y, k.size, k.angle, k.response, (float)k.octave});
+        upload(slot->d_seeds, slot->seeds, c->stream);
     }
     OrbPlan& P = *slot;
+    c->last_batch = &P;
     REQUIRE(capacity >= P.cap, MCV_ERR_CAPACITY, "capacity below mcv_orb_capacity");
     const SinCosTables tb = c->tables();
     std::vector<int> st(n);
     for (int i = 0; i < n; i++) st[i] = strides ? strides[i] : widths[i];
     HIP_OK(hipMemcpyAsync(P.d_src.p, d_images, sizeof(void*) * n, hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipMemcpyAsync(P.d_stride.p, st.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
-    if (flags & MCV_FLAG_GRAPH) {
-        const void* gk = d_kps;
-        if (!P.graph || P.graph_key != gk) {
-            if (P.graph) HIP_OK(hipGraphExecDestroy(P.graph));
-            P.graph = nullptr;
-            hipGraph_t gr;
-            hipStream_t cs = c->capture_stream();
-            HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            P.enqueue(c->orb, tb, cs, d_kps, d_desc, capacity, d_counts);
-            HIP_OK(hipStreamEndCapture(cs, &gr));
-            HIP_OK(hipGraphInstantiate(&P.graph, gr, nullptr, nullptr, 0));
-            HIP_OK(hipGraphDestroy(gr));
-            P.graph_key = gk;
+    for (int attempt = 0;; attempt++) {
+        if (flags & MCV_FLAG_GRAPH) {
+            const OrbPlan::GraphKey gk{d_kps, d_desc, d_counts, capacity};
+            if (!P.graph || !(P.graph_key == gk)) {
+                if (P.graph) HIP_OK(hipGraphExecDestroy(P.graph));
+                P.graph = nullptr;
+                hipGraph_t gr;
+                hipStream_t cs = c->capture_stream();
+                HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+                P.enqueue(c->orb, tb, cs, d_kps, d_desc, capacity, d_counts);
+                HIP_OK(hipStreamEndCapture(cs, &gr));
+                HIP_OK(hipGraphInstantiate(&P.graph, gr, nullptr, nullptr, 0));
+                HIP_OK(hipGraphDestroy(gr));
+                P.graph_key = gk;
+            }
+            HIP_OK(hipGraphLaunch(P.graph, c->stream));
+        } else {
+            P.enqueue(c->orb, tb, c->stream, d_kps, d_desc, capacity, d_counts);
         }
-        HIP_OK(hipGraphLaunch(P.graph, c->stream));
-    } else {
-        P.enqueue(c->orb, tb, c->stream, d_kps, d_desc, capacity, d_counts);
+        if (!(flags & MCV_FLAG_CHECK)) break;
+        // a pool overflow grows the pool (the graph holds the old one) and reruns
+        if (plan_check(P, c->stream) == 0) break;
+        REQUIRE(attempt < 2, MCV_ERR_STATE, "octree node pool exhausted");
     }
+    API_END
+}
+
+int mcv_orb_batch_check(mcv_ctx* c) {
+    API_BEGIN(c)
+    REQUIRE(c && c->last_batch, MCV_ERR_STATE, "no batched extraction yet");
+    c->set_device();
+    if (plan_check(*c->last_batch, c->stream) != 0)
+        throw Fail{MCV_ERR_STATE, "octree node pool exhausted in the last run (grown for the next one)"};
+    API_END
+}
+
+static void level_view(const OrbPlan& P, int image, int level, int blurred, const uint8_t** d_level, int32_t* pitch,
+                       int32_t* w, int32_t* h) {
+    REQUIRE(image >= 0 && image < P.n_images && level >= 0 && level < P.nlevels, MCV_ERR_INVALID,
+            "image or level out of range");
+    const LevelDesc& d = P.lv[(size_t)image * P.nlevels + level];
+    if (d_level) *d_level = (blurred ? P.d_blur : P.d_pyr).as<uint8_t>() + d.pix;
+    if (pitch) *pitch = d.pitch;
+    if (w) *w = d.w;
+    if (h) *h = d.h;
+}
+
+int mcv_orb_batch_pyramid_dev(mcv_ctx* c, int image, int level, int blurred, const uint8_t** d_level, int32_t* pitch,
+                              int32_t* w, int32_t* h) {
+    API_BEGIN(c)
+    REQUIRE(c && c->last_batch, MCV_ERR_STATE, "no batched extraction yet");
+    level_view(*c->last_batch, image, level, blurred, d_level, pitch, w, h);
     API_END
 }
 
@@ -1497,8 +1616,11 @@ int mcv_rig_run(mcv_rig* r, const uint8_t* const* d_images, int flags) {
                 OrbPlan& P = *r->single[r->single.size() == 3 ? k : i];
                 HIP_OK(hipMemcpyAsync(P.d_src.p, &d_images[i], sizeof(void*), hipMemcpyHostToDevice, s));
                 HIP_OK(hipMemcpyAsync(P.d_stride.p, &r->w[k], sizeof(int), hipMemcpyHostToDevice, s));
+                // a plan shared by every frame keeps the error bits of the run's earlier frames
+                const bool reset_err = r->single.size() != 3 || f == 0;
                 P.enqueue(c->orb, tb, s, r->kps.as<mcv_keypoint>() + (size_t)i * r->cap,
-                          r->desc.as<uint8_t>() + (size_t)i * r->cap * 32, r->cap, r->counts.as<int>() + i);
+                          r->desc.as<uint8_t>() + (size_t)i * r->cap * 32, r->cap, r->counts.as<int>() + i, nullptr,
+                          reset_err);
             }
             for (int k = 0; k < 3; k++) {
                 const int p = 3 * f + k;
@@ -1578,20 +1700,14 @@ int mcv_rig_check(mcv_rig* r) {
     const int k1 = r->last_batched ? 0 : (int)r->single.size();
     for (int k = k0; k < k1; k++) {
         OrbPlan& P = k < 0 ? *r->batch : *r->single[k];
-        unsigned long long st[2] = {0, 0};
-        HIP_OK(hipMemcpy(st, P.d_used.p, 16, hipMemcpyDeviceToHost));
-        const int err = (int)(st[1] & 0xFFFFFFFFu);
-        if (err & 1) {
-            // grow for the next run; the captured graph holds the old pool
-            P.pool_cap = (long long)(st[0] + st[0] / 4 + 1024);
-            P.d_pool.ensure((size_t)P.pool_cap * 4);
+        if (plan_check(P, c->stream) != 0) {
+            // grown for the next run; the rig's captured graph holds the old pool
             if (k < 0 && r->graph) {
                 HIP_OK(hipGraphExecDestroy(r->graph));
                 r->graph = nullptr;
             }
             throw Fail{MCV_ERR_STATE, "octree node pool exhausted in the last run (grown for the next one)"};
         }
-        REQUIRE(!(err & 2), MCV_ERR_STATE, "a pyramid level has more FAST candidates than its cell capacity");
     }
     API_END
 }
@@ -1609,18 +1725,49 @@ int mcv_rig_outputs(mcv_rig* r, mcv_keypoint** kps, uint8_t** desc, int32_t** co
     return MCV_OK;
 }
 
+int mcv_rig_pyramid_dev(mcv_rig* r, int image, int level, int blurred, const uint8_t** d_level, int32_t* pitch,
+                        int32_t* w, int32_t* h) {
+    mcv_ctx* c = r ? r->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(r && r->last_batched, MCV_ERR_STATE, "the rig's pyramids are resident after a batched (graph) run");
+    level_view(*r->batch, image, level, blurred, d_level, pitch, w, h);
+    API_END
+}
+
 // ---- window search ----------------------------------------------------------
-static void window_run(mcv_ctx* c, int n_cams, const float* d_pts, const uint8_t* d_pdesc, const int32_t* d_plev,
-                       int P, const float* Rt, const float* cams, const int32_t* sizes, const float* scales, int nl,
-                       float radius, float ratio, int th, const float* const* d_kxy, const uint8_t* const* d_kdesc,
+// One launch of k_window_knn over n_cams cameras (each with its own keypoint
+// set and grid): map points (d_pts) or direct positions (d_uv).
+struct WindowQuery {
+    const float* d_pts = nullptr;
+    const float* d_uv = nullptr;
+    const uint8_t* d_qdesc = nullptr;
+    const int32_t* d_plev = nullptr;
+    int P = 0;
+    const float* Rt = nullptr;      // host: n_cams x 12 (map points)
+    const float* cams = nullptr;    // host: n_cams x 4 (map points)
+    const float* scales = nullptr;  // host: nl (map points)
+    int nl = 0;
+    int min_level = -1, max_level = -1;  // direct positions
+    float radius = 0.f, ratio = 0.f;
+    int th = 0;
+};
+
+static void window_run(mcv_ctx* c, int n_cams, const WindowQuery& q, const int32_t* sizes,
+                       const float* const* d_kxy, const int32_t* const* d_koct, const uint8_t* const* d_kdesc,
                        const int32_t* n_kps, int32_t* d_out) {
-    REQUIRE(nl >= 1 && nl <= 16, MCV_ERR_INVALID, "nlevels out of range");
+    if (q.d_pts) REQUIRE(q.nl >= 1 && q.nl <= 16, MCV_ERR_INVALID, "nlevels out of range");
     hipStream_t s = c->stream;
     int total_k = 0;
     for (int k = 0; k < n_cams; k++) {
         REQUIRE(n_kps[k] >= 0 && n_kps[k] <= kGridSortCap && n_kps[k] < 65536, MCV_ERR_INVALID, "too many keypoints");
+        REQUIRE(n_kps[k] == 0 || (d_kxy[k] && d_kdesc[k]), MCV_ERR_INVALID, "null keypoint arrays");
+        REQUIRE(sizes[2 * k] > 0 && sizes[2 * k + 1] > 0, MCV_ERR_INVALID, "bad image size");
         total_k += n_kps[k];
     }
+    const bool level_filter = !q.d_pts && (q.min_level > 0 || q.max_level >= 0);
+    if (level_filter)
+        for (int k = 0; k < n_cams; k++) REQUIRE(n_kps[k] == 0 || (d_koct && d_koct[k]), MCV_ERR_INVALID,
+                                                 "a level-filtered query needs keypoint octaves");
     c->w_cs.ensure((size_t)n_cams * (kGridCells + 1) * 4);
     c->w_ci.ensure((size_t)std::max(total_k, 1) * 4);
     std::vector<GridView> views(n_cams);
@@ -1629,6 +1776,7 @@ static void window_run(mcv_ctx* c, int n_cams, const float* d_pts, const uint8_t
     for (int k = 0; k < n_cams; k++) {
         GridView& g = views[k];
         g.kxy = d_kxy[k];
+        g.koct = level_filter ? d_koct[k] : nullptr;
         g.kdesc = d_kdesc[k];
         g.n = n_kps[k];
         g.width = sizes[2 * k];
@@ -1640,49 +1788,92 @@ static void window_run(mcv_ctx* c, int n_cams, const float* d_pts, const uint8_t
         off += n_kps[k];
         WindowCam& w = wc[k];
         w.grid = g;
-        for (int i = 0; i < 9; i++) w.R[i] = Rt[12 * k + i];
-        for (int i = 0; i < 3; i++) w.t[i] = Rt[12 * k + 9 + i];
-        w.fx = cams[4 * k]; w.fy = cams[4 * k + 1]; w.cx = cams[4 * k + 2]; w.cy = cams[4 * k + 3];
-        w.out = d_out + (size_t)k * P * 6;
+        if (q.d_pts) {
+            for (int i = 0; i < 9; i++) w.R[i] = q.Rt[12 * k + i];
+            for (int i = 0; i < 3; i++) w.t[i] = q.Rt[12 * k + 9 + i];
+            w.fx = q.cams[4 * k]; w.fy = q.cams[4 * k + 1]; w.cx = q.cams[4 * k + 2]; w.cy = q.cams[4 * k + 3];
+        }
+        w.out = d_out + (size_t)k * q.P * 6;
     }
     upload(c->w_view, views, s);
     upload(c->w_cams, wc, s);
     hipLaunchKernelGGL(k_grid_build, dim3(n_cams), dim3(1024), 0, s, c->w_view.as<GridView>());
     WindowPoints wp{};
-    wp.pts = d_pts;
-    wp.pdesc = d_pdesc;
-    wp.plevel = d_plev;
-    wp.P = P;
-    wp.radius = radius;
-    wp.ratio = ratio;
-    wp.th = th;
-    for (int i = 0; i < nl; i++) wp.scales[i] = scales[i];
-    if (P > 0) hipLaunchKernelGGL(k_window_knn, dim3((P + 3) / 4, n_cams), dim3(256), 0, s, c->w_cams.as<WindowCam>(), wp);
+    wp.pts = q.d_pts;
+    wp.uv = q.d_uv;
+    wp.pdesc = q.d_qdesc;
+    wp.plevel = q.d_plev;
+    wp.P = q.P;
+    wp.radius = q.radius;
+    wp.ratio = q.ratio;
+    wp.th = q.th;
+    wp.nl = q.nl;
+    wp.min_level = q.min_level;
+    wp.max_level = q.max_level;
+    for (int i = 0; i < q.nl && i < 16; i++) wp.scales[i] = q.scales[i];
+    if (q.P > 0)
+        hipLaunchKernelGGL(k_window_knn, dim3((q.P + 3) / 4, n_cams), dim3(256), 0, s, c->w_cams.as<WindowCam>(), wp);
     HIP_OK(hipGetLastError());
 }
 
 int mcv_window_search_dev(mcv_ctx* c, int n_cams, const float* d_points, const uint8_t* d_pdesc, const int32_t* d_plevel,
                           int n_points, const float* Rt, const float* cams, const int32_t* sizes, const float* scales,
                           int nlevels, float radius, float ratio, int th, const float* const* d_kxy,
-                          const int32_t* const* d_koct, const uint8_t* const* d_kdesc, const int32_t* n_kps,
-                          int32_t* d_out) {
+                          const uint8_t* const* d_kdesc, const int32_t* n_kps, int32_t* d_out) {
     API_BEGIN(c)
-    REQUIRE(c && n_cams > 0 && n_points >= 0, MCV_ERR_INVALID, "bad window search");
-    (void)d_koct;
+    REQUIRE(c && n_cams > 0 && n_points >= 0 && Rt && cams && sizes && scales && d_kxy && d_kdesc && n_kps,
+            MCV_ERR_INVALID, "bad window search");
+    REQUIRE(n_points == 0 || (d_points && d_pdesc && d_plevel && d_out), MCV_ERR_INVALID, "null map point arrays");
     c->set_device();
-    window_run(c, n_cams, d_points, d_pdesc, d_plevel, n_points, Rt, cams, sizes, scales, nlevels, radius, ratio, th,
-               d_kxy, d_kdesc, n_kps, d_out);
+    WindowQuery q;
+    q.d_pts = d_points;
+    q.d_qdesc = d_pdesc;
+    q.d_plev = d_plevel;
+    q.P = n_points;
+    q.Rt = Rt;
+    q.cams = cams;
+    q.scales = scales;
+    q.nl = nlevels;
+    q.radius = radius;
+    q.ratio = ratio;
+    q.th = th;
+    window_run(c, n_cams, q, sizes, d_kxy, nullptr, d_kdesc, n_kps, d_out);
+    API_END
+}
+
+int mcv_window_search_uv_dev(mcv_ctx* c, const float* d_uv, const uint8_t* d_qdesc, int n, float radius, int min_level,
+                             int max_level, float ratio, int th, const float* d_kxy, const int32_t* d_koct,
+                             const uint8_t* d_kdesc, int n_kps, int width, int height, int32_t* d_out) {
+    API_BEGIN(c)
+    REQUIRE(c && n >= 0 && n_kps >= 0, MCV_ERR_INVALID, "bad window query");
+    REQUIRE(n == 0 || (d_uv && d_qdesc && d_out), MCV_ERR_INVALID, "null query arrays");
+    if (n == 0) return MCV_OK;
+    c->set_device();
+    WindowQuery q;
+    q.d_uv = d_uv;
+    q.d_qdesc = d_qdesc;
+    q.P = n;
+    q.min_level = min_level;
+    q.max_level = max_level;
+    q.radius = radius;
+    q.ratio = ratio;
+    q.th = th;
+    const int32_t sz[2] = {width, height};
+    window_run(c, 1, q, sz, &d_kxy, &d_koct, &d_kdesc, &n_kps, d_out);
     API_END
 }
 
 int mcv_window_search(mcv_ctx* c, const float* points, const uint8_t* pdesc, const int32_t* plevel, int P, const float* R,
                       const float* t, const float* cam, const float* scales, int nl, float radius, float ratio, int th,
-                      const float* kxy, const int32_t* koct, const uint8_t* kdesc, int K, int width, int height,
-                      int32_t* out) {
+                      const float* kxy, const uint8_t* kdesc, int K, int width, int height, int32_t* out) {
     API_BEGIN(c)
     REQUIRE(c && P >= 0 && K >= 0, MCV_ERR_INVALID, "bad window search");
-    (void)koct;
     if (P == 0) return MCV_OK;
+    REQUIRE(points && pdesc && plevel && R && t && cam && scales && out && (K == 0 || (kxy && kdesc)),
+            MCV_ERR_INVALID, "null window-search arrays");
+    REQUIRE(nl >= 1 && nl <= 16, MCV_ERR_INVALID, "nlevels out of range");
+    for (int i = 0; i < P; i++)
+        REQUIRE(plevel[i] >= 0 && plevel[i] < nl, MCV_ERR_INVALID, "map point level outside the scale table");
     c->set_device();
     hipStream_t s = c->stream;
     c->w_pts.ensure((size_t)P * 12);
@@ -1704,8 +1895,19 @@ int mcv_window_search(mcv_ctx* c, const float* points, const uint8_t* pdesc, con
     const int32_t sz[2] = {width, height};
     const float* kx = c->w_kxy.as<float>();
     const uint8_t* kd = c->w_kdesc.as<uint8_t>();
-    window_run(c, 1, c->w_pts.as<float>(), c->w_pdesc.as<uint8_t>(), c->w_plev.as<int32_t>(), P, Rt, cam, sz, scales, nl,
-               radius, ratio, th, &kx, &kd, &K, c->w_out.as<int32_t>());
+    WindowQuery q;
+    q.d_pts = c->w_pts.as<float>();
+    q.d_qdesc = c->w_pdesc.as<uint8_t>();
+    q.d_plev = c->w_plev.as<int32_t>();
+    q.P = P;
+    q.Rt = Rt;
+    q.cams = cam;
+    q.scales = scales;
+    q.nl = nl;
+    q.radius = radius;
+    q.ratio = ratio;
+    q.th = th;
+    window_run(c, 1, q, sz, &kx, nullptr, &kd, &K, c->w_out.as<int32_t>());
     HIP_OK(hipMemcpyAsync(out, c->w_out.p, (size_t)P * 24, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
     API_END
@@ -1790,6 +1992,7 @@ int mcv_ba_set_observations(mcv_ba* b, const float* uv, const int32_t* pt, const
     b->h_fr.assign(fr, fr + M);
     b->h_pptr = pcount;
     b->schur_ready = false;
+    b->touched(false);
     API_END
 }
 
@@ -1869,6 +2072,7 @@ int mcv_ba_set_state(mcv_ba* b, const double* points, const double* frames, cons
     upload(b->fposes, fp, s);
     HIP_OK(hipStreamSynchronize(s));
     b->have_state = true;
+    b->touched(false);
     API_END
 }
 
@@ -1878,7 +2082,7 @@ int mcv_ba_linearize(mcv_ba* b, int residual_only) {
     REQUIRE(b && b->have_obs && b->have_state, MCV_ERR_STATE, "observations or state not set");
     c->set_device();
     ba_enqueue_linearize(b, b->variant, residual_only, c->stream);
-    b->linearized = !residual_only;
+    b->touched(!residual_only);
     API_END
 }
 
@@ -1926,7 +2130,7 @@ int mcv_ba_autotune(mcv_ba* b, int reps, char* report, int report_cap) {
     b->variant = best;
     snprintf(line, sizeof line, "chosen: k_ba_linearize %s\n", kBaVariant[best].name);
     rep += line;
-    b->linearized = true;
+    b->touched(true);
     if (report && report_cap > 0) {
         strncpy(report, rep.c_str(), (size_t)report_cap - 1);
         report[report_cap - 1] = 0;
@@ -2035,8 +2239,14 @@ int mcv_ba_linearize_plain(mcv_ba* b, int residual_only) {
                            b->gobs.as<int>() + b->h_gptr[g], n, o, residual_only);
     }
     HIP_OK(hipGetLastError());
-    b->linearized = !residual_only;
+    b->touched(!residual_only);
     API_END
+}
+
+int mcv_ba_generation(mcv_ba* b, int64_t* gen) {
+    if (!b || !gen) return MCV_ERR_INVALID;
+    *gen = b->gen;
+    return MCV_OK;
 }
 
 // Host side of the Schur structure: (point, free frame) groups of

@@ -413,9 +413,10 @@ __device__ __forceinline__ bool passes_th(const FilterJob& J, int q) {
     return !((float)J.knn[4 * q + 1] > (float)J.th);
 }
 
-__global__ __launch_bounds__(1024) void k_filter(const FilterJob* jobs) {
+// One list per workgroup of 1024 threads.  J.scratch (2 * nq ints) needs no
+// clearing: every entry read is written first.
+__device__ void filter_list(const FilterJob& J) {
     __shared__ int ws[16];
-    const FilterJob J = jobs[blockIdx.x];
     const int nq = J.nq_dev ? min(*J.nq_dev, J.nq) : J.nq;
     int* list = J.scratch;        // ratio survivors (query indices), in order
     int* holes = J.scratch + nq;  // non-survivor positions below k, in order
@@ -483,5 +484,7 @@ __global__ __launch_bounds__(1024) void k_filter(const FilterJob* jobs) {
     }
     if (threadIdx.x == 0) *J.count = k;
 }
+
+__global__ __launch_bounds__(1024) void k_filter(const FilterJob* jobs) { filter_list(jobs[blockIdx.x]); }
 
 }  // namespace mcv

@@ -7,6 +7,10 @@
 //                  R*Pw + t, z < 0 skip, pinhole project, GetFeaturesInArea
 //                  (:263-308) in ix-major / iy / in-cell order, top-2 Hamming
 //                  in that candidate order, FilterRatio + FilterThreshold.
+//                  With direct positions instead of map points it is the
+//                  query of Tracker::Wnd_Track (src/Tracker.cpp:339-353):
+//                  GetFeaturesInArea(kp.pt, r, minLevel, maxLevel) then the
+//                  same top-2 and filters.
 // Included by mcv_api.hip (shares bitonic_sort with the ORB kernels).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -19,6 +23,7 @@ constexpr int kGridSortCap = 16384;
 
 struct GridView {
     const float* kxy;       // n x 2
+    const int32_t* koct;    // n octaves (read only by a level-filtered query)
     const uint8_t* kdesc;   // n x 32
     int n;
     int width, height;
@@ -72,16 +77,21 @@ struct WindowCam {
 };
 
 struct WindowPoints {
-    const float* pts;       // P x 3
+    const float* pts;       // P x 3 map points, or null for direct positions
+    const float* uv;        // P x 2 query positions (when pts is null)
     const uint8_t* pdesc;   // P x 32
-    const int32_t* plevel;  // P
+    const int32_t* plevel;  // P (map points only)
     int P;
     float radius, ratio;
     int th;
+    int nl;                 // scale-table entries (map points only)
+    int min_level, max_level;  // GetFeaturesInArea's level filter (direct positions only)
     float scales[16];
 };
 
-// One wave per (camera, map point).
+// One wave per (camera, query).  Output row: (kp0, d0, kp1, d1, n_candidates,
+// match); a map point whose level is outside [0, nl) gets n_candidates = -1
+// and no match (the reference indexes mvScaleFactor with it).
 __global__ __launch_bounds__(256) void k_window_knn(const WindowCam* cams, WindowPoints W) {
     const int wave = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -90,62 +100,84 @@ __global__ __launch_bounds__(256) void k_window_knn(const WindowCam* cams, Windo
     const WindowCam& C = cams[blockIdx.y];
     const GridView& G = C.grid;
     int32_t* o = C.out + 6 * (long long)p;
-    // Pc = R * Pw + t: cv::Mat float gemm, products and sums in double, one
-    // rounding to float per row
-    const float* pw = W.pts + 3 * p;
-    float Pc[3];
+    float u = 0.f, v = 0.f, r = 0.f;
+    bool valid;
+    if (W.pts) {
+        // Pc = R * Pw + t: cv::Mat float gemm, products and sums in double,
+        // one rounding to float per row
+        const float* pw = W.pts + 3 * p;
+        float Pc[3];
 #pragma unroll
-    for (int r = 0; r < 3; r++) {
-        double s = 0.0;
-        s += (double)C.R[3 * r] * (double)pw[0];
-        s += (double)C.R[3 * r + 1] * (double)pw[1];
-        s += (double)C.R[3 * r + 2] * (double)pw[2];
-        Pc[r] = (float)(s + (double)C.t[r]);
+        for (int k = 0; k < 3; k++) {
+            double s = 0.0;
+            s += (double)C.R[3 * k] * (double)pw[0];
+            s += (double)C.R[3 * k + 1] * (double)pw[1];
+            s += (double)C.R[3 * k + 2] * (double)pw[2];
+            Pc[k] = (float)(s + (double)C.t[k]);
+        }
+        const int lvl = W.plevel[p];
+        if (lvl < 0 || lvl >= W.nl) {
+            if (lane == 0) {
+                o[0] = -1; o[1] = 999; o[2] = -1; o[3] = 999; o[4] = -1; o[5] = -1;
+            }
+            return;
+        }
+        valid = !(Pc[2] < 0.f);
+        if (valid) {
+            u = C.fx * Pc[0] / Pc[2] + C.cx;
+            v = C.fy * Pc[1] / Pc[2] + C.cy;
+            r = W.radius * W.scales[lvl];
+        }
+    } else {
+        u = W.uv[2 * p];
+        v = W.uv[2 * p + 1];
+        r = W.radius;
+        valid = true;
     }
+    const bool check_levels = W.pts == nullptr && (W.min_level > 0 || W.max_level >= 0);
     uint32_t k0 = 999u << 22, k1 = 999u << 22;
     int ncand = 0;
-    bool any = false;
-    if (!(Pc[2] < 0.f)) {
-        const float u = C.fx * Pc[0] / Pc[2] + C.cx;
-        const float v = C.fy * Pc[1] / Pc[2] + C.cy;
-        const float r = W.radius * W.scales[W.plevel[p]];
-        if (!(u != u || v != v) && !(u < 0 || v < 0 || u >= (float)G.width || v >= (float)G.height)) {
-            const int nMinX = max(0, (int)floorf((u - 0.f - r) * G.inv_w));
-            const int nMaxX = min(kGrid - 1, (int)ceilf((u - 0.f + r) * G.inv_w));
-            const int nMinY = max(0, (int)floorf((v - 0.f - r) * G.inv_h));
-            const int nMaxY = min(kGrid - 1, (int)ceilf((v - 0.f + r) * G.inv_h));
-            if (nMinX < kGrid && nMaxX >= 0 && nMinY < kGrid && nMaxY >= 0) {
-                any = true;
-                uint32_t q[8];
-                const uint32_t* qd = reinterpret_cast<const uint32_t*>(W.pdesc + 32 * (long long)p);
+    int nMinX = 0, nMaxX = -1, nMinY = 0, nMaxY = -1;
+    if (valid && !(u != u || v != v) && !(u < 0 || v < 0 || u >= (float)G.width || v >= (float)G.height)) {
+        nMinX = max(0, (int)floorf((u - 0.f - r) * G.inv_w));
+        nMaxX = min(kGrid - 1, (int)ceilf((u - 0.f + r) * G.inv_w));
+        nMinY = max(0, (int)floorf((v - 0.f - r) * G.inv_h));
+        nMaxY = min(kGrid - 1, (int)ceilf((v - 0.f + r) * G.inv_h));
+        if (!(nMinX < kGrid && nMaxX >= 0 && nMinY < kGrid && nMaxY >= 0)) nMaxX = -1;
+    }
+    if (nMaxX >= nMinX && nMaxY >= nMinY) {
+        uint32_t q[8];
+        const uint32_t* qd = reinterpret_cast<const uint32_t*>(W.pdesc + 32 * (long long)p);
 #pragma unroll
-                for (int k = 0; k < 8; k++) q[k] = qd[k];
-                // enumerate cells ix-major, then iy; candidate order index c
-                int base = 0;
-                const int ny = nMaxY - nMinY + 1;
-                const int ncells = (nMaxX - nMinX + 1) * ny;
-                for (int cc = 0; cc < ncells; cc++) {
-                    const int ix = nMinX + cc / ny, iy = nMinY + cc % ny;
-                    const int cell = ix * kGrid + iy;
-                    const int s0 = G.cell_start[cell], s1 = G.cell_start[cell + 1];
-                    for (int s = s0 + lane; s < s1; s += 64) {
-                        const int idx = G.cell_items[s];
-                        const float dx = G.kxy[2 * idx] - u, dy = G.kxy[2 * idx + 1] - v;
-                        const bool in = fabsf(dx) < r && fabsf(dy) < r;
-                        if (in) {
-                            const uint32_t* kd = reinterpret_cast<const uint32_t*>(G.kdesc + 32 * (long long)idx);
-                            uint32_t d = 0;
+        for (int k = 0; k < 8; k++) q[k] = qd[k];
+        // enumerate cells ix-major, then iy; candidate order index c
+        int base = 0;
+        const int ny = nMaxY - nMinY + 1;
+        const int ncells = (nMaxX - nMinX + 1) * ny;
+        for (int cc = 0; cc < ncells; cc++) {
+            const int ix = nMinX + cc / ny, iy = nMinY + cc % ny;
+            const int cell = ix * kGrid + iy;
+            const int s0 = G.cell_start[cell], s1 = G.cell_start[cell + 1];
+            for (int s = s0 + lane; s < s1; s += 64) {
+                const int idx = G.cell_items[s];
+                const float dx = G.kxy[2 * idx] - u, dy = G.kxy[2 * idx + 1] - v;
+                bool in = fabsf(dx) < r && fabsf(dy) < r;
+                if (check_levels) {
+                    const int oct = G.koct[idx];
+                    if (oct < W.min_level || (W.max_level >= 0 && oct > W.max_level)) in = false;
+                }
+                if (in) {
+                    const uint32_t* kd = reinterpret_cast<const uint32_t*>(G.kdesc + 32 * (long long)idx);
+                    uint32_t d = 0;
 #pragma unroll
-                            for (int k = 0; k < 8; k++) d += __builtin_popcount(q[k] ^ kd[k]);
-                            const uint32_t key = (d << 22) | (uint32_t)(base + (s - s0));
-                            k1 = min(k1, max(k0, key));
-                            k0 = min(k0, key);
-                            ncand++;
-                        }
-                    }
-                    base += s1 - s0;
+                    for (int k = 0; k < 8; k++) d += __builtin_popcount(q[k] ^ kd[k]);
+                    const uint32_t key = (d << 22) | (uint32_t)(base + (s - s0));
+                    k1 = min(k1, max(k0, key));
+                    k0 = min(k0, key);
+                    ncand++;
                 }
             }
+            base += s1 - s0;
         }
     }
     // merge the lanes' top-2 keys and candidate counts
@@ -158,18 +190,10 @@ __global__ __launch_bounds__(256) void k_window_knn(const WindowCam* cams, Windo
         ncand += __shfl_xor(ncand, off, 64);
     }
     if (lane != 0) return;
-    (void)any;
-    // candidate order index -> keypoint index
+    // candidate order index -> keypoint index (the same cell walk)
     auto kp_of = [&](uint32_t key) -> int {
         if ((key >> 22) == 999u) return -1;
         const int c = (int)(key & ((1u << 22) - 1));
-        const float u = C.fx * Pc[0] / Pc[2] + C.cx;
-        const float v = C.fy * Pc[1] / Pc[2] + C.cy;
-        const float r = W.radius * W.scales[W.plevel[p]];
-        const int nMinX = max(0, (int)floorf((u - 0.f - r) * G.inv_w));
-        const int nMaxX = min(kGrid - 1, (int)ceilf((u - 0.f + r) * G.inv_w));
-        const int nMinY = max(0, (int)floorf((v - 0.f - r) * G.inv_h));
-        const int nMaxY = min(kGrid - 1, (int)ceilf((v - 0.f + r) * G.inv_h));
         const int ny = nMaxY - nMinY + 1;
         int base = 0;
         for (int cc = 0; cc < (nMaxX - nMinX + 1) * ny; cc++) {

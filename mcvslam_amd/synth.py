@@ -114,6 +114,18 @@ def rig_frame(seed: int):
     return left, right, wide
 
 
+def rig_batch(n_frames: int = 64, n_unique: int = 16):
+    """The batched-throughput workload: n_unique seeded rig frames
+    (seeds 1000 + 7 f) tiled to n_frames, and per-image ORB feature counts
+    U{500..4000} drawn with seed 40 + frame (3 per frame, L, R, W).
+    Returns (unique frames, frame -> unique index, counts [3 n_frames])."""
+    uniq = [rig_frame(1000 + 7 * f) for f in range(min(n_unique, n_frames))]
+    index = [f % len(uniq) for f in range(n_frames)]
+    nfeat = np.concatenate([np.random.Generator(np.random.PCG64(40 + f)).integers(500, 4001, 3)
+                            for f in range(n_frames)]).astype(np.int32)
+    return uniq, index, nfeat
+
+
 @dataclass
 class Camera:
     fx: float

@@ -69,3 +69,51 @@ def test_forward_moves_with_increments(gpu_ctx):
     err = np.linalg.norm(e - re, axis=1) / np.maximum(np.linalg.norm(re, axis=1), 1e-9)
     assert err.max() < 1e-5
     lin.close()
+
+
+def test_backward_after_relinearization_raises(gpu_ctx):
+    """Two forwards on one BALinearizer, then the first one's backward: its
+    Jacobians were overwritten by the second forward, so backward refuses
+    instead of returning another state's gradient."""
+    import torch
+    prob = _problem()
+    lin = BALinearizer.from_problem(prob, ctx=gpu_ctx)
+    F, P = len(prob.frames), len(prob.points)
+    d1 = torch.zeros((F, 6), dtype=torch.float64, device="cuda", requires_grad=True)
+    x1 = torch.zeros((P, 3), dtype=torch.float64, device="cuda", requires_grad=True)
+    e1 = reprojection(d1, x1, lin, (prob.points, prob.frames, prob.cams))
+    d2 = torch.full((F, 6), 1e-3, dtype=torch.float64, device="cuda", requires_grad=True)
+    e2 = reprojection(d2, x1, lin, (prob.points, prob.frames, prob.cams))
+    with pytest.raises(RuntimeError, match="relinearized"):
+        (e1.sum() + e2.sum()).backward()
+    # the latest forward's backward still works
+    x3 = torch.zeros((P, 3), dtype=torch.float64, device="cuda", requires_grad=True)
+    e3 = reprojection(d1.detach().requires_grad_(), x3, lin, (prob.points, prob.frames, prob.cams))
+    e3.sum().backward()
+    assert torch.isfinite(x3.grad).all()
+    lin.close()
+
+
+def test_adjoint_at_150k_observations(gpu_ctx):
+    """The adjoint at the BA linearization configuration's size (10 keyframes
+    x 3 cameras, 20k points, ~150k observations) against the FP64 analytic
+    J^T g."""
+    import torch
+    prob = synth.ba_problem(seed=30, n_kf=10, n_points=20000, target_obs=150000)
+    lin = BALinearizer.from_problem(prob, ctx=gpu_ctx)
+    F, P, M = len(prob.frames), len(prob.points), len(prob.obs_pt)
+    dpose = torch.zeros((F, 6), dtype=torch.float64, device="cuda", requires_grad=True)
+    dpoint = torch.zeros((P, 3), dtype=torch.float64, device="cuda", requires_grad=True)
+    e = reprojection(dpose, dpoint, lin, (prob.points, prob.frames, prob.cams))
+    G = np.random.default_rng(7).normal(size=(M, 2))
+    (e * torch.from_numpy(G).float().cuda()).sum().backward()
+    _, _, _, Jp, Jx = oracle.ba_linearize(prob)
+    want_pose = np.zeros((F, 6))
+    want_point = np.zeros((P, 3))
+    np.add.at(want_pose, prob.obs_frame, np.einsum("mij,mi->mj", Jp, G))
+    np.add.at(want_point, prob.obs_pt, np.einsum("mij,mi->mj", Jx, G))
+    for got, want in ((dpose.grad.cpu().numpy(), want_pose), (dpoint.grad.cpu().numpy(), want_point)):
+        err = np.linalg.norm(got - want, axis=1) / np.maximum(np.linalg.norm(want, axis=1), 1e-30)
+        print(f"adjoint 150k: max rel err {err.max():.2e}", flush=True)
+        assert err.max() < 1e-5
+    lin.close()

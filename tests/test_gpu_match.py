@@ -133,3 +133,22 @@ def test_filter_orientation(gpu_ctx, seed, n, used):
     a2p[m[:, 1]] = a2
     got = filter_orientation(m, a1, a2p, ctx=gpu_ctx)
     np.testing.assert_array_equal(got, oracle.filter_orientation(m, a1, a2p))
+
+
+@pytest.mark.parametrize("nq,nt", [(1, 0), (9, 1), (2000, 2000), (4000, 3500)])
+def test_knn2_dev(gpu_ctx, nq, nt):
+    """mcv_match_knn2_dev: device descriptors in, device rows out."""
+    import ctypes
+
+    import torch
+    from mcvslam_amd._native import check, lib
+    rng = np.random.default_rng(nq + 7 * nt)
+    q, t = _desc(rng, nq), _desc(rng, nt)
+    dq = torch.from_numpy(q).cuda()
+    dt = torch.from_numpy(t).cuda() if nt else torch.zeros((1, 32), dtype=torch.uint8, device="cuda")
+    out = torch.full((nq, 4), -7, dtype=torch.int32, device="cuda")
+    gpu_ctx.use_torch_stream()
+    check(gpu_ctx.h, lib().mcv_match_knn2_dev(gpu_ctx.h, ctypes.c_void_p(dq.data_ptr()), nq,
+                                              ctypes.c_void_p(dt.data_ptr()), nt, ctypes.c_void_p(out.data_ptr())))
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), oracle.knn2(q, t))

@@ -109,6 +109,13 @@ def measured_traffic(path=ROOT / "profiles" / "hbm_traffic_rig64.txt"):
     return out
 
 
+def lib_schur(ba, lam):
+    """mcv_ba_schur alone (enqueued on the context stream, no read-back)."""
+    import ctypes
+    from mcvslam_amd._native import check, lib
+    check(ba.ctx.h, lib().mcv_ba_schur(ba.h, ctypes.c_float(lam)))
+
+
 def level_sizes(w, h, inv_scale):
     out = []
     for l, s in enumerate(inv_scale):
@@ -118,10 +125,13 @@ def level_sizes(w, h, inv_scale):
     return out
 
 
-def cpu_baseline(frames_np, nfeat, budget_s=12.0):
+def cpu_baseline(frames_np, nfeat, budget_s=12.0, match_threads=None):
     """Oracle on the host: per rig frame 3 extraction threads (one per camera,
-    as src/Frame.cpp:120-125) then the three brute-force matches."""
+    as src/Frame.cpp:120-125) then the three brute-force matches, each split
+    over the host's cores by query (src/Matcher.cpp:299)."""
     import oracle
+    from concurrent.futures import ThreadPoolExecutor
+    match_threads = match_threads or host_threads()
     done = 0
     t0 = time.perf_counter()
     for k in range(1 << 20):
@@ -137,7 +147,13 @@ def cpu_baseline(frames_np, nfeat, budget_s=12.0):
         for t in th:
             t.join()
         for qa, ta in ((0, 1), (2, 0), (2, 1)):
-            knn = oracle.knn2(res[qa][1], res[ta][1])
+            # KnnMatch runs cv::parallel_for_ over the queries on every core
+            # (src/Matcher.cpp:299): query slices on match_threads threads
+            q, t = res[qa][1], res[ta][1]
+            cuts = np.linspace(0, len(q), match_threads + 1).astype(int)
+            with ThreadPoolExecutor(match_threads) as ex:
+                parts = list(ex.map(lambda k: oracle.knn2(q[cuts[k]:cuts[k + 1]], t), range(match_threads)))
+            knn = np.concatenate(parts) if parts else np.zeros((0, 4), np.int32)
             oracle.filter_ratio_threshold(knn, 0.7, 50)
         done += 1
         if time.perf_counter() - t0 > budget_s:
@@ -146,19 +162,40 @@ def cpu_baseline(frames_np, nfeat, budget_s=12.0):
     return done / dt, done, dt
 
 
-def ba_cpu_baseline(prob, n_obs=1500000):
-    """FP64 oracle linearization of the first n_obs observations, one thread
-    (g2o's per-edge loop with OpenMP off)."""
+def _obs_slice(prob, a, b):
     import copy
-    import oracle
     sub = copy.copy(prob)
     for f in ("obs_uv", "obs_pt", "obs_frame", "obs_cam", "obs_isig"):
-        setattr(sub, f, getattr(prob, f)[:n_obs])
+        setattr(sub, f, getattr(prob, f)[a:b])
+    return sub
+
+
+def ba_cpu_baseline(prob, n_obs=1500000, threads=1):
+    """FP64 oracle linearization of the first n_obs observations: one thread
+    (g2o's per-edge loop with OpenMP off, ThirdParty/g2o/CMakeLists.txt:140)
+    or `threads` threads over contiguous slices (the OpenMP build's loop)."""
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+    n = min(n_obs, len(prob.obs_pt))
+    cuts = np.linspace(0, n, threads + 1).astype(int)
+    subs = [_obs_slice(prob, cuts[k], cuts[k + 1]) for k in range(threads)]
     t0 = time.perf_counter()
-    oracle.ba_linearize(sub)
+    if threads == 1:
+        oracle.ba_linearize(subs[0])
+    else:
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(oracle.ba_linearize, subs))
     dt = time.perf_counter() - t0
-    n = len(sub.obs_pt)
     return n / dt, n, dt
+
+
+def host_threads() -> int:
+    """The host cores this process may use (sched affinity), at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
 
 
 def main():
@@ -381,21 +418,59 @@ def main():
                                      "bytes_per_obs": round(ba_bytes / M, 2)}}
         log(f"BA linearize: {ba_ms * 1e3:.1f} us per {M} obs -> {M / (ba_ms / 1e3):.3e} obs/s, {ach:.0f} GB/s; "
             f"plain {plain_ms * 1e3:.1f} us")
+        # Schur complement of the 200k points (FP32, H_ll / H_pl / H_pp / S
+        # kernels) at a damping in g2o's starting range; the structure is
+        # built on the host by the first call
+        lam = 1e2
+        ba.schur(lam)
+        torch.cuda.synchronize()
+        sreps = 10
+        e0.record()
+        for _ in range(sreps):
+            lib_schur(ba, lam)
+        e1.record()
+        torch.cuda.synchronize()
+        schur_ms = e0.elapsed_time(e1) / sreps
+        # one full Levenberg-Marquardt iteration (OptimizationAlgorithmLevenberg::
+        # solve): linearize + rho(chi2) sum + Schur + host FP64 Cholesky of the
+        # 294 x 294 system + back-substitution + update + residual pass, trials
+        # included
+        ba.set_state(prob.points, prob.frames, prob.cams)
+        ba.optimize(1)
+        lm_iters = 3
+        ba.set_state(prob.points, prob.frames, prob.cams)
+        t0 = time.perf_counter()
+        st, nit = ba.optimize(lm_iters)
+        lm_ms = (time.perf_counter() - t0) * 1e3 / max(nit, 1)
+        result["ba"]["schur_ms"] = round(schur_ms, 4)
+        result["ba"]["schur_size"] = int(6 * int((np.asarray(prob.frame_fixed) == 0).sum()))
+        result["ba"]["lm_iteration_ms"] = round(lm_ms, 3)
+        result["ba"]["lm_trials_per_iteration"] = float(np.mean(st[:, 3])) if nit else None
+        result["ba"]["lm_chi2"] = [round(float(v), 1) for v in (st[:, 1] if nit else [])]
+        log(f"BA Schur: {schur_ms:.3f} ms; one LM iteration {lm_ms:.2f} ms ({nit} iterations)")
         ba.close()
         if not args.no_cpu:
             cps, n_cpu, dt = ba_cpu_baseline(prob)
             result["ba"]["cpu_baseline"] = {"value": round(cps, 1), "unit": "obs/s", "cores": 1, "kind": "port",
                                             "sample": f"first {n_cpu} observations of the same problem, FP64 "
                                                       f"oracle edge linearization, {dt:.2f}s"}
-            log(f"BA CPU: {cps:.3e} obs/s")
+            mt = host_threads()
+            cpsm, n_cpu, dtm = ba_cpu_baseline(prob, threads=mt)
+            result["ba"]["cpu_baseline_all_cores"] = {
+                "value": round(cpsm, 1), "unit": "obs/s", "cores": mt, "kind": "port",
+                "sample": f"the same {n_cpu} observations over {mt} threads, {dtm:.2f}s"}
+            log(f"BA CPU: {cps:.3e} obs/s one thread, {cpsm:.3e} obs/s on {mt} threads")
 
     if not args.no_cpu:
         log("CPU oracle baseline ...")
         fr = [uniq[i % len(uniq)] for i in range(8)]
-        cfps, done, dt = cpu_baseline(fr, nfeat)
-        result["cpu_baseline"] = {"value": round(cfps, 3), "unit": "rig frames/s", "cores": 3, "kind": "port",
-                                  "sample": f"{done} rig frames (3 extraction threads + 3 brute-force matches) "
-                                            f"in {dt:.1f}s on {os.cpu_count()} host cpus"}
+        mt = host_threads()
+        cfps, done, dt = cpu_baseline(fr, nfeat, match_threads=mt)
+        result["cpu_baseline"] = {"value": round(cfps, 3), "unit": "rig frames/s", "cores": max(3, mt),
+                                  "kind": "port",
+                                  "sample": f"{done} rig frames (3 extraction threads, then 3 brute-force matches "
+                                            f"each over {mt} threads by query) in {dt:.1f}s, oracle built -O3, "
+                                            f"{os.cpu_count()} host cpus visible"}
         log(f"CPU: {cfps:.2f} rig frames/s")
     rig.close()
     if args.calib_copy_mb > 0:

@@ -160,10 +160,11 @@ void mcv_rig_destroy(mcv_rig* rig);
 int mcv_rig_autotune(mcv_rig* rig, const uint8_t* const* d_images, int reps, char* report, int report_cap);
 /* Fixes the rig's launch variants instead of tuning them: the top-2 kernel
  * (0 .. 7, as mcv_match_set_variant), the blur walk (0: stop at the
- * level end, 1: fixed length) and the keypoint kernel (0: one keypoint per
- * wave, 1: 8, 2: 16 (the default), 3: 32).  No counterpart in the reference; every combination
- * produces identical outputs. */
-int mcv_rig_set_variants(mcv_rig* rig, int knn_variant, int blur_variant, int kp_variant);
+ * level end, 1: fixed length), the keypoint kernel (0: one keypoint per
+ * wave, 1: 8, 2: 16 (the default), 3: 32) and the octree heap replay (0: top
+ * of the heap in registers (the default), 1: heap in LDS).  No counterpart in
+ * the reference; every combination produces identical outputs. */
+int mcv_rig_set_variants(mcv_rig* rig, int knn_variant, int blur_variant, int kp_variant, int heap_variant);
 /* d_images: host array of 3 * n_frames device pointers (frame-major L, R, W),
  * each image dense (stride = width).  flags: MCV_FLAG_GRAPH replays the whole
  * batch as one hipGraph; without it every image and every pair is launched on
@@ -234,6 +235,47 @@ int mcv_window_search_uv_dev(mcv_ctx* ctx, const float* d_uv, const uint8_t* d_q
                              const int32_t* d_koct, const uint8_t* d_kdesc, int n_kps, int width, int height,
                              int32_t* d_out);
 
+/* ---- bag of words (DBoW3, vendored by the reference: modules/DBow3) -----
+ * A vocabulary tree on the device: node 0 the root, children of node i
+ * child_ids[child_ptr[i] .. child_ptr[i + 1]) in DBoW3's children order, n x
+ * 32 B node descriptors, word_id (-1 for inner nodes) and weight per node,
+ * L the vocabulary's depth parameter (m_L), weighting 0 TF_IDF, 1 TF, 2 IDF,
+ * 3 BINARY, scoring 0 L1, 1 L2, 2 chi-square, 3 KL, 4 Bhattacharyya, 5 dot
+ * product (DBoW3.h's enums).  Replaces Vocabulary's loaded tree
+ * (Vocabulary.h:56-60 m_nodes, m_words). */
+typedef struct mcv_bow_vocab mcv_bow_vocab;
+int mcv_bow_vocab_create(mcv_ctx* ctx, int n_nodes, const int32_t* child_ptr, const int32_t* child_ids,
+                         const uint8_t* node_desc, const int32_t* word_id, const double* weight, int L,
+                         int weighting, int scoring, mcv_bow_vocab** vocab);
+void mcv_bow_vocab_destroy(mcv_bow_vocab* vocab);
+/* Vocabulary::transform(feature, word_id, weight, nid, levelsup)
+ * (modules/DBow3/src/Vocabulary.cpp:641-672) for n device descriptors
+ * (n x 32 B): the leaf reached by descending to the child at the least
+ * Hamming distance (the first of equal distances), its word id and weight,
+ * and the node passed at level L - levelsup (0, the root, when that level is
+ * <= 0 or below the leaf).  Device outputs of n entries. */
+int mcv_bow_transform_dev(mcv_bow_vocab* vocab, const uint8_t* d_feats, int n, int levelsup, int32_t* d_word,
+                          double* d_weight, int32_t* d_nid);
+/* Vocabulary::transform(features, BowVector, FeatureVector, levelsup)
+ * (Vocabulary.cpp:572-629, called by Object::ComputeBow, src/Object.cpp:
+ * 238-247, with levelsup 4) for n host descriptors: the per-feature outputs
+ * (any may be NULL), the BowVector as ascending word ids and values
+ * (*n_words of them; capacity n), and the FeatureVector as ascending node ids,
+ * CSR offsets (*n_nodes + 1) and feature indices (capacity n each). */
+int mcv_bow_transform(mcv_bow_vocab* vocab, const uint8_t* feats, int n, int levelsup, int32_t* word, double* weight,
+                      int32_t* nid, int32_t* bv_words, double* bv_vals, int32_t* n_words, int32_t* fv_nodes,
+                      int32_t* fv_ptr, int32_t* fv_feats, int32_t* n_nodes);
+/* Matcher::DBowMatch (src/Matcher.cpp:146-193): descriptors d1 (n1 x 32) and
+ * d2 (n2 x 32) with their FeatureVectors (m nodes ascending, CSR offsets,
+ * feature indices), host arrays.  out receives a row (query, train0, d0,
+ * train1, d1) per feature of image 1 in a node both hold where image 2 has
+ * at least two features, nodes ascending and features in list order (the
+ * reference's MatchResKnn order; capacity = the features of image 1);
+ * *count the rows. */
+int mcv_bow_match(mcv_ctx* ctx, const uint8_t* d1, int n1, int m1, const int32_t* nodes1, const int32_t* ptr1,
+                  const int32_t* feats1, const uint8_t* d2, int n2, int m2, const int32_t* nodes2,
+                  const int32_t* ptr2, const int32_t* feats2, int32_t* out, int32_t* count);
+
 /* ---- bundle-adjustment linearization -----------------------------------
  * The batched seam of BlockSolver::buildSystem / SparseOptimizer::
  * computeActiveErrors (ThirdParty/g2o/g2o/core/block_solver.hpp:463-520,
@@ -251,9 +293,17 @@ int mcv_ba_set_observations(mcv_ba* ba, const float* uv, const int32_t* point, c
  * qx qy qz qw tx ty tz, Huber delta). */
 int mcv_ba_set_state(mcv_ba* ba, const double* points, const double* frames, const double* cams);
 /* Linearize every edge into 22 device planes of M floats: e_u, e_v, chi2,
- * Huber weight, J_pose (2x6 row-major), J_point (2x3 row-major).
- * residual_only = 1 computes the first four planes only. */
-int mcv_ba_linearize(mcv_ba* ba, int residual_only);
+ * Huber weight, J_pose (2x6 row-major), J_point (2x3 row-major).  mode:
+ * MCV_BA_FULL every plane; MCV_BA_RESIDUAL the first four only (the LM
+ * trials' SparseOptimizer::computeActiveErrors, core/sparse_optimizer.cpp:
+ * 63-90); MCV_BA_POSE_ONLY all but J_point: the edge with a fixed point,
+ * whose residual and J_pose are those of EdgeSE3XYZ2CameraOnlyPose
+ * (include/OptimizeTypes.hpp:20-68; PoseEstimation::PoseOptimization fixes
+ * every point, src/PoseEstimation.cpp:45-77). */
+#define MCV_BA_FULL 0
+#define MCV_BA_RESIDUAL 1
+#define MCV_BA_POSE_ONLY 2
+int mcv_ba_linearize(mcv_ba* ba, int mode);
 /* Launch shape of mcv_ba_linearize (0 .. 7: workgroup size, grid cap,
  * observations in flight per thread, streaming stores; no counterpart in the
  * reference).  Every variant computes the same values per observation. */
@@ -276,9 +326,9 @@ int mcv_ba_linearize_plain(mcv_ba* ba, int residual_only);
 int mcv_ba_set_stereo(mcv_ba* ba, int Ms, const float* obs, const int32_t* point, const int32_t* frame,
                       const float* inv_sigma2, const double* cam);
 /* Linearize the stereo edges into 32 device planes of Ms floats: e_u, e_v,
- * e_ur, chi2, Huber weight, J_pose (3x6 row-major), J_point (3x3).
- * residual_only = 1 computes the first five planes only. */
-int mcv_ba_linearize_stereo(mcv_ba* ba, int residual_only);
+ * e_ur, chi2, Huber weight, J_pose (3x6 row-major), J_point (3x3).  mode as
+ * mcv_ba_linearize (MCV_BA_RESIDUAL: the first five planes). */
+int mcv_ba_linearize_stereo(mcv_ba* ba, int mode);
 /* Copy stereo planes [first, first + count) to host (count x Ms floats). */
 int mcv_ba_read_stereo_planes(mcv_ba* ba, int first, int count, float* host);
 /* Outlier test of PoseEstimation::FilterCallBack_Chi2 (src/PoseEstimation.cpp:
@@ -305,17 +355,57 @@ int mcv_ba_adjoint_dev(mcv_ba* ba, const float* d_g, float* d_grad_pose, float* 
 /* ---- Schur complement of the point blocks ------------------------------
  * BlockSolver::buildSystem + the Schur part of BlockSolver::solve with the LM
  * damping of setLambda (ThirdParty/g2o/g2o/core/block_solver.hpp:315-449,
- * 463-520): from the last linearization, in FP32 with fixed summation
- * orders, the reduced camera system S (n x n, n = 6 x free frames, frames
- * numbered in order) and b_schur.  Fixed frames (VertexSE3Expmap::setFixed,
- * src/BAoptimizer.cpp:31) get no pose block. */
+ * 463-520): from the last linearization of the active mono and stereo edges
+ * (constructQuadraticForm, core/base_binary_edge.hpp:64-136), in FP32 with
+ * fixed summation orders, the reduced camera system S (n x n, n = 6 x free
+ * frames, frames numbered in order) and b_schur.  Fixed frames
+ * (VertexSE3Expmap::setFixed, src/BAoptimizer.cpp:31) get no pose block;
+ * fixed points (BAoptimizer::addMapppint(mp, fixed), src/BAoptimizer.cpp:
+ * 34-43) are not eliminated and their edges enter H_pp and b_p only. */
 int mcv_ba_set_fixed(mcv_ba* ba, const int32_t* frame_fixed);
+/* point_fixed: P flags, or NULL for none. */
+int mcv_ba_set_point_fixed(mcv_ba* ba, const int32_t* point_fixed);
+/* Edge levels of g2o (OptimizableGraph::Edge::setLevel; an edge at level 1
+ * takes no part in the optimization, src/PoseEstimation.cpp:104-118):
+ * active_mono (M bytes) / active_stereo (Ms bytes), NULL = all active.  They
+ * select the edges of mcv_ba_schur, mcv_ba_robust_chi2 and mcv_ba_optimize. */
+int mcv_ba_set_active(mcv_ba* ba, const uint8_t* active_mono, const uint8_t* active_stereo);
+/* enabled = 0 drops the Huber kernels of every edge (setRobustKernel(0),
+ * src/PoseEstimation.cpp:117): weight 1 and rho(chi2) = chi2. */
+int mcv_ba_set_robust(mcv_ba* ba, int enabled);
 int mcv_ba_schur(mcv_ba* ba, float lambda);
 /* Host copy of S (row-major, full symmetric) and b_schur as double. */
 int mcv_ba_read_schur(mcv_ba* ba, double* S, double* b_schur, int32_t* n);
 /* FP64 dense Cholesky solve S x = b on the host (stands in for g2o's
  * LinearSolverEigen, solvers/eigen/linear_solver_eigen.h:55). */
 int mcv_solve_llt(int n, const double* S, const double* b, double* x);
+
+/* ---- Levenberg-Marquardt ------------------------------------------------
+ * SparseOptimizer::activeRobustChi2 (core/sparse_optimizer.cpp:102): the
+ * sum over the active edges of the last linearization (mono, and stereo when
+ * set) of the Huber rho(chi2) (core/robust_kernel_impl.cpp:65-78), reduced
+ * on the GPU in FP64 in a fixed order. */
+int mcv_ba_robust_chi2(mcv_ba* ba, double* chi2);
+/* SparseOptimizer::optimize(iterations) with OptimizationAlgorithmLevenberg
+ * (core/optimization_algorithm_levenberg.cpp:56-159; BAoptimizer::optimize,
+ * src/BAoptimizer.cpp:110-114) on the state of mcv_ba_set_state: per
+ * iteration the GPU linearizes every active edge and sums rho(chi2), builds
+ * the Schur complement with lambda (initially tau times the largest diagonal
+ * entry of H; g2o's tau is 1e-5 and its trial limit 10), the host solves the
+ * reduced system in FP64, the GPU back-substitutes the points and applies
+ * the update (points += x_l on the device, keyframes exp(x_p) * T on the
+ * host), a residual-only linearization gives the trial's chi2, and lambda
+ * follows g2o's rules (step accepted when rho > 0: lambda *= max(1/3,
+ * min(2/3, 1 - (2 rho - 1)^3)); rejected: the state is restored, lambda *=
+ * ni, ni *= 2).  stats (optional, iterations x 4): chi2 before the
+ * iteration, after it, lambda, trials.  *done: iterations run (a Terminate
+ * ends the run early, as in g2o).  The planes afterwards hold the residuals
+ * (e, chi2, w) of the final state, for mcv_ba_outliers. */
+int mcv_ba_optimize(mcv_ba* ba, int iterations, double tau, int max_trials, double* stats, int32_t* done);
+/* The estimates (host, FP64): points P x 3 and frames F x 7 as
+ * mcv_ba_set_state takes them (BAoptimizer::eval / recovery_all,
+ * src/BAoptimizer.cpp:116-145); either may be NULL. */
+int mcv_ba_get_state(mcv_ba* ba, double* points, double* frames);
 
 #ifdef __cplusplus
 }

@@ -54,7 +54,7 @@ SIGNATURES = {
     "mcv_distinctive_descriptors": (I, [P, P, P, I, P, P]),
     "mcv_match_filter_orientation": (I, [P, P, I, P, I, P, I, P, P]),
     "mcv_rig_autotune": (I, [P, P, I, P, I]),
-    "mcv_rig_set_variants": (I, [P, I, I, I]),
+    "mcv_rig_set_variants": (I, [P, I, I, I, I]),
     "mcv_match_knn2_dev": (I, [P, P, I, P, I, P]),
     "mcv_match_filter": (I, [P, P, I, F, I, P, P]),
     "mcv_rig_create": (I, [P, I, P, P, P, F, I, P]),
@@ -68,6 +68,11 @@ SIGNATURES = {
     "mcv_window_search": (I, [P, P, P, P, I, P, P, P, P, I, F, F, I, P, P, I, I, I, P]),
     "mcv_window_search_dev": (I, [P, I, P, P, P, I, P, P, P, P, I, F, F, I, P, P, P, P]),
     "mcv_window_search_uv_dev": (I, [P, P, P, I, F, I, I, F, I, P, P, P, I, I, I, P]),
+    "mcv_bow_vocab_create": (I, [P, I, P, P, P, P, P, I, I, I, P]),
+    "mcv_bow_vocab_destroy": (None, [P]),
+    "mcv_bow_transform_dev": (I, [P, P, I, I, P, P, P]),
+    "mcv_bow_transform": (I, [P, P, I, I, P, P, P, P, P, P, P, P, P, P]),
+    "mcv_bow_match": (I, [P, P, I, I, P, P, P, P, I, I, P, P, P, P, P]),
     "mcv_ba_create": (I, [P, I, I, I, I, P]),
     "mcv_ba_destroy": (None, [P]),
     "mcv_ba_set_observations": (I, [P, P, P, P, P, P]),
@@ -89,6 +94,12 @@ SIGNATURES = {
     "mcv_ba_schur": (I, [P, F]),
     "mcv_ba_read_schur": (I, [P, P, P, P]),
     "mcv_solve_llt": (I, [I, P, P, P]),
+    "mcv_ba_set_point_fixed": (I, [P, P]),
+    "mcv_ba_set_active": (I, [P, P, P]),
+    "mcv_ba_set_robust": (I, [P, I]),
+    "mcv_ba_robust_chi2": (I, [P, P]),
+    "mcv_ba_optimize": (I, [P, I, ctypes.c_double, I, P, P]),
+    "mcv_ba_get_state": (I, [P, P, P]),
 }
 
 _lib = None

@@ -379,11 +379,13 @@ class RigBatch:
 
     N_BLUR_VARIANTS = 2
     N_KP_VARIANTS = 4
+    N_HEAP_VARIANTS = 2
 
-    def set_variants(self, knn: int, blur: int, kp: int = 2):
+    def set_variants(self, knn: int, blur: int, kp: int = 2, heap: int = 0):
         """Fix the launch variants (top-2 kernel, blur walk, keypoints per
-        wave: 0 one, 1 eight, 2 sixteen, 3 thirty-two) instead of tuning."""
-        check(self.ctx.h, lib().mcv_rig_set_variants(self.h, int(knn), int(blur), int(kp)))
+        wave: 0 one, 1 eight, 2 sixteen, 3 thirty-two; octree heap: 0
+        register top, 1 LDS) instead of tuning."""
+        check(self.ctx.h, lib().mcv_rig_set_variants(self.h, int(knn), int(blur), int(kp), int(heap)))
 
     def autotune(self, image_ptrs, reps: int = 3) -> str:
         """Time every launch variant of the tunable kernels on these images
@@ -466,6 +468,76 @@ def distinctive_descriptors(desc, ptr, ctx: Context | None = None):
     return best, out
 
 
+class BowVocabulary:
+    """DBoW3 vocabulary tree on the device (modules/DBow3/src/Vocabulary.h)
+    with Vocabulary::transform and Matcher::DBowMatch."""
+
+    def __init__(self, voc, ctx: Context | None = None):
+        """voc: an object with desc (n x 32 u8), child_ptr, child_ids,
+        word_id, weight, L, weighting, scoring (synth.Vocabulary)."""
+        self.ctx = ctx or default_context()
+        self._keep = [np.ascontiguousarray(voc.child_ptr, np.int32), np.ascontiguousarray(voc.child_ids, np.int32),
+                      np.ascontiguousarray(voc.desc, np.uint8), np.ascontiguousarray(voc.word_id, np.int32),
+                      np.ascontiguousarray(voc.weight, np.float64)]
+        self.n = len(self._keep[3])
+        h = ctypes.c_void_p()
+        check(self.ctx.h, lib().mcv_bow_vocab_create(self.ctx.h, self.n, *[_p(x) for x in self._keep], int(voc.L),
+                                                     int(voc.weighting), int(voc.scoring), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mcv_bow_vocab_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def transform(self, desc, levelsup: int = 4):
+        """(word ids, weights, node ids) per descriptor, BowVector (words,
+        values) and FeatureVector (nodes, ptr, features)."""
+        d = np.ascontiguousarray(desc, np.uint8).reshape(-1, 32)
+        n = len(d)
+        w, wt, nid = np.zeros(n, np.int32), np.zeros(n), np.zeros(n, np.int32)
+        bw, bv = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1))
+        fn, fp, ff = np.zeros(max(n, 1), np.int32), np.zeros(n + 1, np.int32), np.zeros(max(n, 1), np.int32)
+        nw, nn = ctypes.c_int32(), ctypes.c_int32()
+        check(self.ctx.h, lib().mcv_bow_transform(self.h, _p(d), n, levelsup, _p(w), _p(wt), _p(nid), _p(bw), _p(bv),
+                                                  ctypes.byref(nw), _p(fn), _p(fp), _p(ff), ctypes.byref(nn)))
+        k, m = nw.value, nn.value
+        return (w, wt, nid), (bw[:k].copy(), bv[:k].copy()), (fn[:m].copy(), fp[:m + 1].copy(), ff[:fp[m]].copy())
+
+    def transform_dev(self, d_desc, n: int, levelsup: int = 4):
+        """Device descriptors (torch.cuda u8 n x 32) -> device (word, weight, nid)."""
+        import torch
+        dev = d_desc.device
+        w = torch.empty(n, dtype=torch.int32, device=dev)
+        wt = torch.empty(n, dtype=torch.float64, device=dev)
+        nid = torch.empty(n, dtype=torch.int32, device=dev)
+        self.ctx.use_torch_stream()
+        check(self.ctx.h, lib().mcv_bow_transform_dev(self.h, ctypes.c_void_p(d_desc.data_ptr()), n, levelsup,
+                                                      ctypes.c_void_p(w.data_ptr()), ctypes.c_void_p(wt.data_ptr()),
+                                                      ctypes.c_void_p(nid.data_ptr())))
+        return w, wt, nid
+
+
+def dbow_match(d1, fv1, d2, fv2, ctx: Context | None = None) -> np.ndarray:
+    """Matcher::DBowMatch: rows (query, train0, d0, train1, d1)."""
+    ctx = ctx or default_context()
+    a = np.ascontiguousarray(d1, np.uint8).reshape(-1, 32)
+    b = np.ascontiguousarray(d2, np.uint8).reshape(-1, 32)
+    f1 = [np.ascontiguousarray(x, np.int32) for x in fv1]
+    f2 = [np.ascontiguousarray(x, np.int32) for x in fv2]
+    out = np.zeros((max(len(f1[2]), 1), 5), np.int32)
+    cnt = ctypes.c_int32()
+    check(ctx.h, lib().mcv_bow_match(ctx.h, _p(a), len(a), len(f1[0]), _p(f1[0]), _p(f1[1]), _p(f1[2]), _p(b), len(b),
+                                     len(f2[0]), _p(f2[0]), _p(f2[1]), _p(f2[2]), _p(out), ctypes.byref(cnt)))
+    return out[:cnt.value].copy()
+
+
 class BALinearizer:
     """Batched linearization of EdgeSE3XYZ2Camera edges (see mcvslam_gpu.h)."""
 
@@ -509,8 +581,13 @@ class BALinearizer:
              np.ascontiguousarray(cams, np.float64)]
         check(self.ctx.h, lib().mcv_ba_set_state(self.h, *[_p(x) for x in a]))
 
-    def linearize(self, residual_only: bool = False):
-        check(self.ctx.h, lib().mcv_ba_linearize(self.h, 1 if residual_only else 0))
+    FULL, RESIDUAL, POSE_ONLY = 0, 1, 2
+
+    def linearize(self, residual_only: bool = False, mode: int | None = None):
+        """mode: FULL (every plane), RESIDUAL (e, chi2, w) or POSE_ONLY (all
+        but J_point, the fixed-point edge)."""
+        m = (self.RESIDUAL if residual_only else self.FULL) if mode is None else int(mode)
+        check(self.ctx.h, lib().mcv_ba_linearize(self.h, m))
 
     N_VARIANTS = 8
 
@@ -534,8 +611,9 @@ class BALinearizer:
         self.Ms = len(a[1])
         check(self.ctx.h, lib().mcv_ba_set_stereo(self.h, self.Ms, *[_p(x) for x in a], _p(c)))
 
-    def linearize_stereo(self, residual_only: bool = False):
-        check(self.ctx.h, lib().mcv_ba_linearize_stereo(self.h, 1 if residual_only else 0))
+    def linearize_stereo(self, residual_only: bool = False, mode: int | None = None):
+        m = (self.RESIDUAL if residual_only else self.FULL) if mode is None else int(mode)
+        check(self.ctx.h, lib().mcv_ba_linearize_stereo(self.h, m))
 
     def stereo_results(self):
         """(e Ms x 3, chi2, w, J_pose Ms x 3 x 6, J_point Ms x 3 x 3)"""
@@ -581,6 +659,43 @@ class BALinearizer:
     def set_fixed(self, frame_fixed):
         ff = np.ascontiguousarray(frame_fixed, np.int32)
         check(self.ctx.h, lib().mcv_ba_set_fixed(self.h, _p(ff)))
+
+    def set_point_fixed(self, point_fixed=None):
+        pf = None if point_fixed is None else np.ascontiguousarray(point_fixed, np.int32)
+        check(self.ctx.h, lib().mcv_ba_set_point_fixed(self.h, _p(pf)))
+
+    def set_active(self, active_mono=None, active_stereo=None):
+        """Edge levels: False / 0 = level 1 (left out of the optimization)."""
+        am = None if active_mono is None else np.ascontiguousarray(active_mono, np.uint8)
+        ast = None if active_stereo is None else np.ascontiguousarray(active_stereo, np.uint8)
+        check(self.ctx.h, lib().mcv_ba_set_active(self.h, _p(am), _p(ast)))
+
+    def set_robust(self, enabled: bool):
+        check(self.ctx.h, lib().mcv_ba_set_robust(self.h, 1 if enabled else 0))
+
+    def robust_chi2(self) -> float:
+        """Sum of the Huber rho(chi2) over the active edges of the last
+        linearization (activeRobustChi2)."""
+        v = ctypes.c_double()
+        check(self.ctx.h, lib().mcv_ba_robust_chi2(self.h, ctypes.byref(v)))
+        return v.value
+
+    def optimize(self, iterations: int, tau: float = 1e-5, max_trials: int = 10):
+        """g2o's Levenberg-Marquardt for `iterations` iterations over the GPU
+        data; returns (stats [n x 4: chi2 before, chi2 after, lambda, trials],
+        iterations run)."""
+        stats = np.zeros((max(iterations, 1), 4))
+        done = ctypes.c_int32()
+        check(self.ctx.h, lib().mcv_ba_optimize(self.h, int(iterations), ctypes.c_double(tau), int(max_trials),
+                                                _p(stats), ctypes.byref(done)))
+        return stats[:done.value], done.value
+
+    def get_state(self):
+        """(points P x 3, frames F x 7) FP64 estimates."""
+        pts = np.zeros((self.P, 3))
+        frs = np.zeros((self.F, 7))
+        check(self.ctx.h, lib().mcv_ba_get_state(self.h, _p(pts), _p(frs)))
+        return pts, frs
 
     def schur(self, lam: float = 0.0):
         """Reduced camera system of the last linearization: (S, b_schur) as

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -24,6 +25,7 @@
 
 #include "../../include/mcvslam_gpu.h"
 #include "mcv_ba.hip"
+#include "mcv_bow.hip"
 #include "mcv_match.hip"
 #include "mcv_orb.hip"
 #include "mcv_window.hip"
@@ -375,6 +377,7 @@ struct Tuning {
     int knn = 0;   // launch_knn2 variant
     int blur = 0;  // 0: k_blur<false>, 1: k_blur<true>
     int kp = 2;    // keypoints per wave: 0: one (k_keypoints), 1: 8, 2: 16, 3: 32 (k_keypoints_k<K>)
+    int heap = 0;  // 0: k_octree_heap_r (register top), 1: k_octree_heap (LDS)
 };
 
 struct ImageSpec {
@@ -706,8 +709,12 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
         hipLaunchKernelGGL(k_octree_build<256>, dim3((int)lv.size() - n_build_large), dim3(256), 0, s, g,
                            d_build_levels.as<int>() + n_build_large);
     mark("k_octree_build");
-    hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), 8 * (size_t)heap_cap, s, g,
-                       d_heap_order.as<int>(), heap_cap);
+    if (tune.heap == 1)
+        hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), 8 * (size_t)heap_cap, s, g,
+                           d_heap_order.as<int>(), heap_cap);
+    else
+        hipLaunchKernelGGL(k_octree_heap_r, dim3(lv.size()), dim3(64), 8 * (size_t)heap_cap, s, g,
+                           d_heap_order.as<int>(), heap_cap);
     mark("k_octree_heap");
     float* kps = reinterpret_cast<float*>(d_kps);
     if (tune.kp == 1)
@@ -749,6 +756,7 @@ struct mcv_ctx {
     DevBuf d_desc_in, d_ptr, d_best, d_desc_out;
     DevBuf o_m, o_a1, o_a2, o_out, o_cnt, o_jobs;
     DevBuf w_pts, w_pdesc, w_plev, w_kxy, w_kdesc, w_cs, w_ci, w_out, w_view, w_cams;
+    DevBuf b_d1, b_d2, b_f2, b_jobs, b_out;
     std::map<std::string, std::unique_ptr<OrbPlan>> plans;
 
     void set_device() { HIP_OK(hipSetDevice(device)); }
@@ -812,20 +820,37 @@ struct mcv_ba {
     std::vector<int> fixed;
     int variant = 0;  // k_ba_linearize launch shape (kBaVariant)
     bool have_obs = false, have_state = false, linearized = false;
+    // J_pose planes valid (a full or pose-only linearization of the mono /
+    // stereo edges at the current state)
+    bool pose_linearized = false, stereo_pose_linearized = false, stereo_full = false;
     // bumped by every change of the planes' inputs or contents (state,
     // observations, any linearization): a reader that recorded it can tell
     // whether the planes still hold its linearization
     long long gen = 0;
-    void touched(bool full_linearization) {
-        linearized = full_linearization;
+    void touched(int mode) {  // mode: kBaFull / kBaResidual / kBaPoseOnly, or -1 (inputs changed)
+        linearized = mode == kBaFull;
+        pose_linearized = mode == kBaFull || mode == kBaPoseOnly;
+        if (mode < 0) stereo_linearized = stereo_pose_linearized = stereo_full = false;
         schur_done = false;
         gen++;
     }
-    // Schur structure (depends on observations and fixed frames)
+    // optimizer state (g2o keeps FP64 estimates): frames and cameras on the
+    // host, points on the device (p64) with the float pairs phi / plo
+    std::vector<double> h_frames, h_cams;
+    double h_scam[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<int> h_spt, h_sfr;
+    std::vector<int> point_fixed;                 // empty: no point fixed
+    std::vector<uint8_t> active_m, active_s;      // empty: every edge active
+    bool robust = true;                           // Huber kernels on
+    DevBuf p64, p64_bak, pfix_d, act_m, act_s, xl, xp, partial, dsum, fmax;
+    // Schur structure (depends on the edges, the active sets and the fixed
+    // frames and points)
     bool schur_ready = false, schur_done = false;
     int npf = 0, Fp = 0, nblk = 0;
-    DevBuf pf_ptr, pf_obs_ptr, pf_obs, pf_point, sfptr, sfobs, fpf_ptr, fpf, blk_ptr, blk_items, blk_ij;
-    DevBuf Dinv, bl, db, Hll, Hpl, Hpp, bs, Sout;
+    bool free_points_with_edges = true;
+    DevBuf pe_ptr, pe, pf_ptr, pf_obs_ptr, pf_obs, pf_point, pf_frame, sfptr, sfobs, fpf_ptr, fpf, blk_ptr, blk_items,
+        blk_ij;
+    DevBuf Dinv, bl, db, Hll, Hpl, Hpp, bs, bp, Sout;
 };
 
 // Launch shapes of k_ba_linearize (same arithmetic per observation; the
@@ -1545,8 +1570,21 @@ int mcv_rig_autotune(mcv_rig* r, const uint8_t* const* d_images, int reps, char*
         }
         r->tune.kp = best;
     }
-    snprintf(line, sizeof line, "chosen: k_knn2 %s, k_blur %s, k_keypoints %s\n", knn_variant_name(r->tune.knn),
-             blur_variant_name(r->tune.blur), kp_variant_name(r->tune.kp));
+    {
+        int best = 0;
+        float bt = 1e30f;
+        for (int v = 0; v < kHeapVariants; v++) {
+            r->tune.heap = v;
+            const float t = time_kernel("k_octree_heap");
+            snprintf(line, sizeof line, "k_octree_heap %s %.4f ms\n", heap_variant_name(v), t);
+            rep += line;
+            if (t < bt) { bt = t; best = v; }
+        }
+        r->tune.heap = best;
+    }
+    snprintf(line, sizeof line, "chosen: k_knn2 %s, k_blur %s, k_keypoints %s, k_octree_heap %s\n",
+             knn_variant_name(r->tune.knn), blur_variant_name(r->tune.blur), kp_variant_name(r->tune.kp),
+             heap_variant_name(r->tune.heap));
     rep += line;
     P.tune = r->tune;
     for (auto& sp : r->single) sp->tune = r->tune;
@@ -1563,15 +1601,16 @@ int mcv_rig_autotune(mcv_rig* r, const uint8_t* const* d_images, int reps, char*
     API_END
 }
 
-int mcv_rig_set_variants(mcv_rig* r, int knn_variant, int blur_variant, int kp_variant) {
+int mcv_rig_set_variants(mcv_rig* r, int knn_variant, int blur_variant, int kp_variant, int heap_variant) {
     mcv_ctx* c = r ? r->ctx : nullptr;
     API_BEGIN(c)
     REQUIRE(r && knn_variant >= 0 && knn_variant < kKnnVariants && blur_variant >= 0 && blur_variant < kBlurVariants &&
-                kp_variant >= 0 && kp_variant < kKpVariants,
+                kp_variant >= 0 && kp_variant < kKpVariants && heap_variant >= 0 && heap_variant < kHeapVariants,
             MCV_ERR_INVALID, "launch variant out of range");
     r->tune.knn = knn_variant;
     r->tune.blur = blur_variant;
     r->tune.kp = kp_variant;
+    r->tune.heap = heap_variant;
     r->batch->tune = r->tune;
     for (auto& sp : r->single) sp->tune = r->tune;
     if (r->graph) {
@@ -1913,6 +1952,228 @@ int mcv_window_search(mcv_ctx* c, const float* points, const uint8_t* pdesc, con
     API_END
 }
 
+// ---- bag of words (DBoW3) ------------------------------------------------------
+}  // extern "C"
+
+struct mcv_bow_vocab {
+    mcv_ctx* ctx;
+    int n = 0, L = 0, depth = 0, max_children = 0, weighting = 0, scoring = 0;
+    DevBuf desc, cptr, cids, wid, wt, feats, word, weight, nid;
+};
+
+extern "C" {
+
+int mcv_bow_vocab_create(mcv_ctx* c, int n, const int32_t* child_ptr, const int32_t* child_ids, const uint8_t* desc,
+                         const int32_t* word_id, const double* weight, int L, int weighting, int scoring,
+                         mcv_bow_vocab** out) {
+    API_BEGIN(c)
+    REQUIRE(c && out && n >= 1 && child_ptr && desc && word_id && weight, MCV_ERR_INVALID, "bad vocabulary");
+    REQUIRE(weighting >= 0 && weighting <= 3 && scoring >= 0 && scoring <= 5, MCV_ERR_INVALID,
+            "unknown weighting or scoring type");
+    REQUIRE(child_ptr[0] == 0, MCV_ERR_INVALID, "child_ptr must start at 0");
+    int maxc = 0;
+    for (int i = 0; i < n; i++) {
+        REQUIRE(child_ptr[i + 1] >= child_ptr[i], MCV_ERR_INVALID, "child_ptr must not decrease");
+        maxc = std::max(maxc, child_ptr[i + 1] - child_ptr[i]);
+    }
+    const int nc = child_ptr[n];
+    REQUIRE(nc == 0 || child_ids, MCV_ERR_INVALID, "null child ids");
+    REQUIRE(maxc < 65536, MCV_ERR_INVALID, "more than 65535 children under one node");
+    // every node reached once from the root (a tree), depth = longest path
+    std::vector<int> depth(n, -1);
+    std::vector<int> todo{0};
+    depth[0] = 0;
+    int dmax = 0;
+    for (size_t k = 0; k < todo.size(); k++) {
+        const int i = todo[k];
+        for (int e = child_ptr[i]; e < child_ptr[i + 1]; e++) {
+            const int j = child_ids[e];
+            REQUIRE(j > 0 && j < n && depth[j] < 0, MCV_ERR_INVALID, "child ids do not form a tree rooted at 0");
+            depth[j] = depth[i] + 1;
+            dmax = std::max(dmax, depth[j]);
+            todo.push_back(j);
+        }
+    }
+    c->set_device();
+    hipStream_t s = c->stream;
+    std::unique_ptr<mcv_bow_vocab> v(new mcv_bow_vocab());
+    v->ctx = c;
+    v->n = n;
+    v->L = L;
+    v->depth = dmax;
+    v->max_children = maxc;
+    v->weighting = weighting;
+    v->scoring = scoring;
+    v->desc.ensure((size_t)n * 32);
+    HIP_OK(hipMemcpyAsync(v->desc.p, desc, (size_t)n * 32, hipMemcpyHostToDevice, s));
+    upload(v->cptr, std::vector<int32_t>(child_ptr, child_ptr + n + 1), s);
+    upload(v->cids, std::vector<int32_t>(child_ids, child_ids + nc), s);
+    upload(v->wid, std::vector<int32_t>(word_id, word_id + n), s);
+    upload(v->wt, std::vector<double>(weight, weight + n), s);
+    HIP_OK(hipStreamSynchronize(s));
+    *out = v.release();
+    API_END
+}
+
+void mcv_bow_vocab_destroy(mcv_bow_vocab* v) { delete v; }
+
+int mcv_bow_transform_dev(mcv_bow_vocab* v, const uint8_t* d_feats, int n, int levelsup, int32_t* d_word,
+                          double* d_weight, int32_t* d_nid) {
+    mcv_ctx* c = v ? v->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(v && n >= 0, MCV_ERR_INVALID, "bad transform");
+    if (n == 0) return MCV_OK;
+    REQUIRE(d_feats && d_word && d_weight && d_nid, MCV_ERR_INVALID, "null buffers");
+    c->set_device();
+    BowVocab V{v->desc.as<uint32_t>(), v->cptr.as<int32_t>(), v->cids.as<int32_t>(), v->wid.as<int32_t>(),
+               v->wt.as<double>(), v->depth};
+    const int nid_level = v->L - levelsup;
+    const uint32_t* f = reinterpret_cast<const uint32_t*>(d_feats);
+    if (v->max_children <= 16)
+        hipLaunchKernelGGL(k_bow_transform<16>, dim3((n + 15) / 16), dim3(256), 0, c->stream, V, f, n, nid_level,
+                           d_word, d_weight, d_nid);
+    else
+        hipLaunchKernelGGL(k_bow_transform<64>, dim3((n + 3) / 4), dim3(256), 0, c->stream, V, f, n, nid_level,
+                           d_word, d_weight, d_nid);
+    HIP_OK(hipGetLastError());
+    API_END
+}
+
+int mcv_bow_transform(mcv_bow_vocab* v, const uint8_t* feats, int n, int levelsup, int32_t* word, double* weight,
+                      int32_t* nid, int32_t* bv_words, double* bv_vals, int32_t* n_words, int32_t* fv_nodes,
+                      int32_t* fv_ptr, int32_t* fv_feats, int32_t* n_nodes) {
+    mcv_ctx* c = v ? v->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(v && n >= 0 && (n == 0 || feats), MCV_ERR_INVALID, "bad transform");
+    c->set_device();
+    hipStream_t s = c->stream;
+    std::vector<int32_t> w(n), nd(n);
+    std::vector<double> wt(n);
+    if (n > 0) {
+        v->feats.ensure((size_t)n * 32);
+        v->word.ensure((size_t)n * 4);
+        v->weight.ensure((size_t)n * 8);
+        v->nid.ensure((size_t)n * 4);
+        HIP_OK(hipMemcpyAsync(v->feats.p, feats, (size_t)n * 32, hipMemcpyHostToDevice, s));
+        const int rc = mcv_bow_transform_dev(v, v->feats.as<uint8_t>(), n, levelsup, v->word.as<int32_t>(),
+                                             v->weight.as<double>(), v->nid.as<int32_t>());
+        if (rc != MCV_OK) return rc;
+        HIP_OK(hipMemcpyAsync(w.data(), v->word.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(wt.data(), v->weight.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(nd.data(), v->nid.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+    }
+    if (word) std::copy(w.begin(), w.end(), word);
+    if (weight) std::copy(wt.begin(), wt.end(), weight);
+    if (nid) std::copy(nd.begin(), nd.end(), nid);
+    // Vocabulary::transform(features, BowVector, FeatureVector, levelsup)
+    // (Vocabulary.cpp:572-629): TF / TF_IDF accumulate (BowVector::addWeight),
+    // IDF / BINARY keep the first (addIfNotExist), stopped words (weight <= 0)
+    // skipped; FeatureVector::addFeature (FeatureVector.cpp:31-45); the
+    // scoring's normalisation (ScoringObject.h: L2 for L2 scoring, L1 for the
+    // others but the dot product) or, without one, TF weights / v.size()
+    std::map<int32_t, double> bv;
+    std::map<int32_t, std::vector<int32_t>> fv;
+    const bool must = v->scoring != 5, l2 = v->scoring == 1;
+    for (int i = 0; i < n; i++) {
+        if (!(wt[i] > 0)) continue;
+        if (v->weighting <= 1) bv[w[i]] += wt[i];
+        else bv.emplace(w[i], wt[i]);
+        fv[nd[i]].push_back(i);
+    }
+    if (v->weighting <= 1 && !bv.empty() && !must) {
+        const double m = (double)bv.size();
+        for (auto& kv : bv) kv.second /= m;
+    }
+    if (must) {
+        double norm = 0.0;
+        if (!l2) {
+            for (auto& kv : bv) norm += std::fabs(kv.second);
+        } else {
+            for (auto& kv : bv) norm += kv.second * kv.second;
+            norm = std::sqrt(norm);
+        }
+        if (norm > 0.0)
+            for (auto& kv : bv) kv.second /= norm;
+    }
+    if (n_words) *n_words = (int32_t)bv.size();
+    if (bv_words || bv_vals) {
+        int k = 0;
+        for (auto& kv : bv) {
+            if (bv_words) bv_words[k] = kv.first;
+            if (bv_vals) bv_vals[k] = kv.second;
+            k++;
+        }
+    }
+    if (n_nodes) *n_nodes = (int32_t)fv.size();
+    if (fv_nodes || fv_ptr || fv_feats) {
+        int m = 0, t = 0;
+        if (fv_ptr) fv_ptr[0] = 0;
+        for (auto& kv : fv) {
+            if (fv_nodes) fv_nodes[m] = kv.first;
+            for (int32_t i : kv.second) {
+                if (fv_feats) fv_feats[t] = i;
+                t++;
+            }
+            m++;
+            if (fv_ptr) fv_ptr[m] = t;
+        }
+    }
+    API_END
+}
+
+int mcv_bow_match(mcv_ctx* c, const uint8_t* d1, int n1, int m1, const int32_t* nodes1, const int32_t* ptr1,
+                  const int32_t* feats1, const uint8_t* d2, int n2, int m2, const int32_t* nodes2,
+                  const int32_t* ptr2, const int32_t* feats2, int32_t* out, int32_t* count) {
+    API_BEGIN(c)
+    REQUIRE(c && count && m1 >= 0 && m2 >= 0 && n1 >= 0 && n2 >= 0, MCV_ERR_INVALID, "bad DBowMatch arguments");
+    *count = 0;
+    if (m1 == 0 || m2 == 0) return MCV_OK;
+    REQUIRE(d1 && d2 && nodes1 && ptr1 && feats1 && nodes2 && ptr2 && feats2 && out, MCV_ERR_INVALID, "null arrays");
+    for (int a = 0; a < m1; a++)
+        for (int e = ptr1[a]; e < ptr1[a + 1]; e++)
+            REQUIRE(feats1[e] >= 0 && feats1[e] < n1, MCV_ERR_INVALID, "feature index out of range");
+    for (int b = 0; b < m2; b++)
+        for (int e = ptr2[b]; e < ptr2[b + 1]; e++)
+            REQUIRE(feats2[e] >= 0 && feats2[e] < n2, MCV_ERR_INVALID, "feature index out of range");
+    // the nodes both vectors hold, ascending (the reference's merge with
+    // lower_bound jumps visits exactly these); a row per feature of image 1
+    // in a node where image 2 has at least two features
+    std::vector<BowJob> jobs;
+    int a = 0, b = 0;
+    while (a < m1 && b < m2) {
+        if (nodes1[a] == nodes2[b]) {
+            if (ptr2[b + 1] - ptr2[b] >= 2)
+                for (int e = ptr1[a]; e < ptr1[a + 1]; e++)
+                    jobs.push_back(BowJob{feats1[e], ptr2[b], ptr2[b + 1], (int)jobs.size()});
+            a++;
+            b++;
+        } else if (nodes1[a] < nodes2[b]) {
+            a++;
+        } else {
+            b++;
+        }
+    }
+    const int nj = (int)jobs.size();
+    if (nj == 0) return MCV_OK;
+    c->set_device();
+    hipStream_t s = c->stream;
+    c->b_d1.ensure((size_t)n1 * 32);
+    c->b_d2.ensure((size_t)n2 * 32);
+    c->b_out.ensure((size_t)nj * 20);
+    HIP_OK(hipMemcpyAsync(c->b_d1.p, d1, (size_t)n1 * 32, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(c->b_d2.p, d2, (size_t)n2 * 32, hipMemcpyHostToDevice, s));
+    upload(c->b_f2, std::vector<int32_t>(feats2, feats2 + ptr2[m2]), s);
+    upload(c->b_jobs, jobs, s);
+    hipLaunchKernelGGL(k_bow_match, dim3((nj + 3) / 4), dim3(256), 0, s, c->b_jobs.as<BowJob>(), nj,
+                       c->b_d1.as<uint32_t>(), c->b_d2.as<uint32_t>(), c->b_f2.as<int32_t>(), c->b_out.as<int32_t>());
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(out, c->b_out.p, (size_t)nj * 20, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    *count = nj;
+    API_END
+}
+
 // ---- bundle adjustment --------------------------------------------------------
 int mcv_ba_create(mcv_ctx* c, int M, int P, int F, int C, mcv_ba** out) {
     API_BEGIN(c)
@@ -1992,7 +2253,7 @@ int mcv_ba_set_observations(mcv_ba* b, const float* uv, const int32_t* pt, const
     b->h_fr.assign(fr, fr + M);
     b->h_pptr = pcount;
     b->schur_ready = false;
-    b->touched(false);
+    b->touched(-1);
     API_END
 }
 
@@ -2013,19 +2274,12 @@ static void quat_to_R(const double* q, double R[9]) {
     R[6] = txz - twy; R[7] = tyz + twx; R[8] = 1 - (txx + tyy);
 }
 
-int mcv_ba_set_state(mcv_ba* b, const double* points, const double* frames, const double* cams) {
-    mcv_ctx* c = b ? b->ctx : nullptr;
-    API_BEGIN(c)
-    REQUIRE(b && points && frames && cams, MCV_ERR_INVALID, "null state arrays");
-    c->set_device();
-    hipStream_t s = c->stream;
-    std::vector<float4> hi(b->P), lo(b->P);
-    for (int p = 0; p < b->P; p++) {
-        float h[3], l[3];
-        for (int k = 0; k < 3; k++) split(points[3 * p + k], h[k], l[k]);
-        hi[p] = make_float4(h[0], h[1], h[2], 0.f);
-        lo[p] = make_float4(l[0], l[1], l[2], 0.f);
-    }
+// The (frame, camera) table of the mono edges and the frame poses of the
+// stereo edges from the host FP64 frames and cameras; without the robust
+// kernel the Huber deltas are infinite (w = 1, rho = chi2).
+static void ba_upload_frames(mcv_ba* b, hipStream_t s) {
+    const double* frames = b->h_frames.data();
+    const double* cams = b->h_cams.data();
     std::vector<BAPair> pr((size_t)b->F * b->C);
     std::vector<FramePose> fp(b->F);
     for (int f = 0; f < b->F; f++) {
@@ -2062,17 +2316,41 @@ int mcv_ba_set_state(mcv_ba* b, const double* points, const double* frames, cons
             for (int i = 0; i < 9; i++) { P.Rbw[i] = (float)Rbw[i]; P.Rcb[i] = (float)Rcb[i]; }
             for (int i = 0; i < 3; i++) P.tbw[i] = (float)tbw[i];
             P.fx = (float)cm[0]; P.fy = (float)cm[1]; P.cx = (float)cm[2]; P.cy = (float)cm[3];
-            P.delta = (float)cm[11];
+            P.delta = b->robust ? (float)cm[11] : INFINITY;
             P.pad[0] = P.pad[1] = P.pad[2] = 0.f;
         }
     }
-    upload(b->phi, hi, s);
-    upload(b->plo, lo, s);
     upload(b->pairs, pr, s);
     upload(b->fposes, fp, s);
+    const double* sc = b->h_scam;
+    b->scam = StereoCam{(float)sc[0], (float)sc[1], (float)sc[2], (float)sc[3], (float)sc[4],
+                        b->robust ? (float)sc[5] : INFINITY};
+}
+
+// FP64 points to the device and their float pairs
+static void ba_upload_points(mcv_ba* b, const double* points, hipStream_t s) {
+    b->p64.ensure((size_t)b->P * 24);
+    b->phi.ensure((size_t)b->P * 16);
+    b->plo.ensure((size_t)b->P * 16);
+    HIP_OK(hipMemcpyAsync(b->p64.p, points, (size_t)b->P * 24, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_ba_points_apply, dim3((b->P + 255) / 256), dim3(256), 0, s, b->p64.as<double>(),
+                       (const float*)nullptr, (const uint8_t*)nullptr, b->P, b->phi.as<float4>(), b->plo.as<float4>());
+    HIP_OK(hipGetLastError());
+}
+
+int mcv_ba_set_state(mcv_ba* b, const double* points, const double* frames, const double* cams) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && points && frames && cams, MCV_ERR_INVALID, "null state arrays");
+    c->set_device();
+    hipStream_t s = c->stream;
+    b->h_frames.assign(frames, frames + 7 * (size_t)b->F);
+    b->h_cams.assign(cams, cams + 12 * (size_t)b->C);
+    ba_upload_points(b, points, s);
+    ba_upload_frames(b, s);
     HIP_OK(hipStreamSynchronize(s));
     b->have_state = true;
-    b->touched(false);
+    b->touched(-1);
     API_END
 }
 
@@ -2081,8 +2359,9 @@ int mcv_ba_linearize(mcv_ba* b, int residual_only) {
     API_BEGIN(c)
     REQUIRE(b && b->have_obs && b->have_state, MCV_ERR_STATE, "observations or state not set");
     c->set_device();
+    REQUIRE(residual_only >= kBaFull && residual_only <= kBaPoseOnly, MCV_ERR_INVALID, "unknown linearization mode");
     ba_enqueue_linearize(b, b->variant, residual_only, c->stream);
-    b->touched(!residual_only);
+    b->touched(residual_only);
     API_END
 }
 
@@ -2130,7 +2409,7 @@ int mcv_ba_autotune(mcv_ba* b, int reps, char* report, int report_cap) {
     b->variant = best;
     snprintf(line, sizeof line, "chosen: k_ba_linearize %s\n", kBaVariant[best].name);
     rep += line;
-    b->touched(true);
+    b->touched(kBaFull);
     if (report && report_cap > 0) {
         strncpy(report, rep.c_str(), (size_t)report_cap - 1);
         report[report_cap - 1] = 0;
@@ -2149,7 +2428,13 @@ int mcv_ba_set_stereo(mcv_ba* b, int Ms, const float* obs, const int32_t* pt, co
         REQUIRE(pt[i] >= 0 && pt[i] < b->P && fr[i] >= 0 && fr[i] < b->F, MCV_ERR_INVALID,
                 "stereo edge index out of range");
     b->Ms = Ms;
-    b->scam = StereoCam{(float)cam[0], (float)cam[1], (float)cam[2], (float)cam[3], (float)cam[4], (float)cam[5]};
+    for (int k = 0; k < 6; k++) b->h_scam[k] = cam[k];
+    b->scam = StereoCam{(float)cam[0], (float)cam[1], (float)cam[2], (float)cam[3], (float)cam[4],
+                        b->robust ? (float)cam[5] : INFINITY};
+    b->h_spt.assign(pt, pt + Ms);
+    b->h_sfr.assign(fr, fr + Ms);
+    b->active_s.clear();
+    b->schur_ready = false;
     const size_t m = (size_t)std::max(Ms, 1);
     b->s_obs.ensure(m * 12);
     b->s_pt.ensure(m * 4);
@@ -2164,25 +2449,32 @@ int mcv_ba_set_stereo(mcv_ba* b, int Ms, const float* obs, const int32_t* pt, co
     }
     HIP_OK(hipStreamSynchronize(s));
     b->have_stereo = true;
-    b->stereo_linearized = false;
+    b->stereo_linearized = b->stereo_pose_linearized = b->stereo_full = false;
     API_END
+}
+
+static void ba_enqueue_stereo(mcv_ba* b, int mode, hipStream_t s) {
+    if (b->Ms > 0) {
+        BAStereoObs ob{b->s_obs.as<float>(), b->s_pt.as<uint32_t>(), b->s_fr.as<uint32_t>(), b->s_isig.as<float>(),
+                       b->Ms};
+        BAOut o{b->s_planes.as<float>(), (long long)b->Ms};
+        hipLaunchKernelGGL(k_ba_stereo_linearize, dim3((b->Ms + 255) / 256), dim3(256), 0, s, ob, b->phi.as<float4>(),
+                           b->plo.as<float4>(), b->fposes.as<FramePose>(), b->scam, o, mode);
+        HIP_OK(hipGetLastError());
+    }
+    b->stereo_linearized = true;
+    b->stereo_pose_linearized = mode == kBaFull || mode == kBaPoseOnly;
+    b->stereo_full = mode == kBaFull;
+    b->schur_done = false;
 }
 
 int mcv_ba_linearize_stereo(mcv_ba* b, int residual_only) {
     mcv_ctx* c = b ? b->ctx : nullptr;
     API_BEGIN(c)
     REQUIRE(b && b->have_stereo && b->have_state, MCV_ERR_STATE, "stereo edges or state not set");
+    REQUIRE(residual_only >= kBaFull && residual_only <= kBaPoseOnly, MCV_ERR_INVALID, "unknown linearization mode");
     c->set_device();
-    if (b->Ms > 0) {
-        BAStereoObs ob{b->s_obs.as<float>(), b->s_pt.as<uint32_t>(), b->s_fr.as<uint32_t>(), b->s_isig.as<float>(),
-                       b->Ms};
-        BAOut o{b->s_planes.as<float>(), (long long)b->Ms};
-        hipLaunchKernelGGL(k_ba_stereo_linearize, dim3((b->Ms + 255) / 256), dim3(256), 0, c->stream, ob,
-                           b->phi.as<float4>(), b->plo.as<float4>(), b->fposes.as<FramePose>(), b->scam, o,
-                           residual_only);
-        HIP_OK(hipGetLastError());
-    }
-    b->stereo_linearized = true;
+    ba_enqueue_stereo(b, residual_only, c->stream);
     API_END
 }
 
@@ -2239,7 +2531,7 @@ int mcv_ba_linearize_plain(mcv_ba* b, int residual_only) {
                            b->gobs.as<int>() + b->h_gptr[g], n, o, residual_only);
     }
     HIP_OK(hipGetLastError());
-    b->touched(!residual_only);
+    b->touched(residual_only);
     API_END
 }
 
@@ -2252,19 +2544,45 @@ int mcv_ba_generation(mcv_ba* b, int64_t* gen) {
 // Host side of the Schur structure: (point, free frame) groups of
 // observations, free-frame observation lists, and for every free-frame pair
 // block (i <= j) the list of (pf_i, pf_j) products, all in point order.
+// Edge lists of the Schur complement over the active mono and stereo edges
+// (index i < M mono, M + j stereo): per free point its edges, per (free
+// point, free frame) pair its edges, per free frame its edges (points fixed
+// or not) and its pairs, and the pair lists of every frame-pair block.
 static void ba_build_schur(mcv_ba* b, hipStream_t s) {
     const int P = b->P, F = b->F, M = b->M;
+    const int Ms = b->have_stereo ? b->Ms : 0;
     if (b->fixed.empty()) b->fixed.assign(F, 0);
     std::vector<int> fidx(F, -1);
     int Fp = 0;
     for (int f = 0; f < F; f++)
         if (!b->fixed[f]) fidx[f] = Fp++;
+    auto act_m = [&](int i) { return b->active_m.empty() || b->active_m[i]; };
+    auto act_s = [&](int j) { return b->active_s.empty() || b->active_s[j]; };
+    auto pfree = [&](int p) { return b->point_fixed.empty() || !b->point_fixed[p]; };
+    // per point: active edges (mono in index order, then stereo)
+    std::vector<int> pe_ptr(P + 1, 0);
+    for (int i = 0; i < M; i++)
+        if (act_m(i) && pfree(b->h_pt[i])) pe_ptr[b->h_pt[i] + 1]++;
+    for (int j = 0; j < Ms; j++)
+        if (act_s(j) && pfree(b->h_spt[j])) pe_ptr[b->h_spt[j] + 1]++;
+    for (int p = 0; p < P; p++) pe_ptr[p + 1] += pe_ptr[p];
+    std::vector<int> pe(std::max(pe_ptr[P], 1));
+    {
+        std::vector<int> a(pe_ptr.begin(), pe_ptr.end() - 1);
+        for (int i = 0; i < M; i++)
+            if (act_m(i) && pfree(b->h_pt[i])) pe[a[b->h_pt[i]]++] = i;
+        for (int j = 0; j < Ms; j++)
+            if (act_s(j) && pfree(b->h_spt[j])) pe[a[b->h_spt[j]]++] = M + j;
+    }
+    auto frame_of = [&](int e) { return e < M ? b->h_fr[e] : b->h_sfr[e - M]; };
     std::vector<int> pf_ptr(P + 1, 0), pf_obs_ptr(1, 0), pf_obs, pf_point, pf_free;
     std::vector<std::pair<int, int>> tmp;
+    bool any_free_edges = false;
     for (int p = 0; p < P; p++) {
         tmp.clear();
-        for (int i = b->h_pptr[p]; i < b->h_pptr[p + 1]; i++)
-            if (fidx[b->h_fr[i]] >= 0) tmp.push_back({fidx[b->h_fr[i]], i});
+        if (pe_ptr[p + 1] > pe_ptr[p]) any_free_edges = true;
+        for (int t = pe_ptr[p]; t < pe_ptr[p + 1]; t++)
+            if (fidx[frame_of(pe[t])] >= 0) tmp.push_back({fidx[frame_of(pe[t])], pe[t]});
         std::stable_sort(tmp.begin(), tmp.end(), [](const std::pair<int, int>& x, const std::pair<int, int>& y) {
             return x.first < y.first;
         });
@@ -2280,10 +2598,12 @@ static void ba_build_schur(mcv_ba* b, hipStream_t s) {
         pf_ptr[p + 1] = (int)pf_point.size();
     }
     const int npf = (int)pf_point.size();
-    // free frame -> observations, free frame -> pf entries
+    // free frame -> active edges, free frame -> pf entries
     std::vector<int> sfptr(Fp + 1, 0), fpf_ptr(Fp + 1, 0);
     for (int i = 0; i < M; i++)
-        if (fidx[b->h_fr[i]] >= 0) sfptr[fidx[b->h_fr[i]] + 1]++;
+        if (act_m(i) && fidx[b->h_fr[i]] >= 0) sfptr[fidx[b->h_fr[i]] + 1]++;
+    for (int j = 0; j < Ms; j++)
+        if (act_s(j) && fidx[b->h_sfr[j]] >= 0) sfptr[fidx[b->h_sfr[j]] + 1]++;
     for (int q = 0; q < npf; q++) fpf_ptr[pf_free[q] + 1]++;
     for (int f = 0; f < Fp; f++) {
         sfptr[f + 1] += sfptr[f];
@@ -2293,7 +2613,9 @@ static void ba_build_schur(mcv_ba* b, hipStream_t s) {
     {
         std::vector<int> a(sfptr.begin(), sfptr.end() - 1), c(fpf_ptr.begin(), fpf_ptr.end() - 1);
         for (int i = 0; i < M; i++)
-            if (fidx[b->h_fr[i]] >= 0) sfobs[a[fidx[b->h_fr[i]]]++] = i;
+            if (act_m(i) && fidx[b->h_fr[i]] >= 0) sfobs[a[fidx[b->h_fr[i]]]++] = i;
+        for (int j = 0; j < Ms; j++)
+            if (act_s(j) && fidx[b->h_sfr[j]] >= 0) sfobs[a[fidx[b->h_sfr[j]]]++] = M + j;
         for (int q = 0; q < npf; q++) fpf[c[pf_free[q]]++] = q;
     }
     // pair blocks, upper triangle row-major
@@ -2314,10 +2636,13 @@ static void ba_build_schur(mcv_ba* b, hipStream_t s) {
         for (int i = 0; i < Fp; i++)
             for (int j = i; j < Fp; j++) ij[bidx(i, j)] = make_int2(i, j);
     }
+    upload(b->pe_ptr, pe_ptr, s);
+    upload(b->pe, pe, s);
     upload(b->pf_ptr, pf_ptr, s);
     upload(b->pf_obs_ptr, pf_obs_ptr, s);
     upload(b->pf_obs, pf_obs, s);
     upload(b->pf_point, pf_point, s);
+    upload(b->pf_frame, pf_free, s);
     upload(b->sfptr, sfptr, s);
     upload(b->sfobs, sfobs, s);
     upload(b->fpf_ptr, fpf_ptr, s);
@@ -2332,39 +2657,32 @@ static void ba_build_schur(mcv_ba* b, hipStream_t s) {
     b->Hpl.ensure((size_t)std::max(npf, 1) * 72);
     b->Hpp.ensure((size_t)std::max(Fp, 1) * 144);
     b->bs.ensure((size_t)std::max(Fp, 1) * 24);
+    b->bp.ensure((size_t)std::max(Fp, 1) * 24);
     b->Sout.ensure((size_t)std::max(36 * Fp * Fp, 1) * 4);
     HIP_OK(hipStreamSynchronize(s));
     b->npf = npf;
     b->Fp = Fp;
     b->nblk = nblk;
+    b->free_points_with_edges = any_free_edges;
     b->schur_ready = true;
 }
 
-int mcv_ba_set_fixed(mcv_ba* b, const int32_t* frame_fixed) {
-    mcv_ctx* c = b ? b->ctx : nullptr;
-    API_BEGIN(c)
-    REQUIRE(b && frame_fixed, MCV_ERR_INVALID, "null fixed flags");
-    b->fixed.assign(frame_fixed, frame_fixed + b->F);
-    b->schur_ready = false;
-    API_END
-}
-
-int mcv_ba_schur(mcv_ba* b, float lambda) {
-    mcv_ctx* c = b ? b->ctx : nullptr;
-    API_BEGIN(c)
-    REQUIRE(b && b->linearized, MCV_ERR_STATE, "linearize before the Schur complement");
-    c->set_device();
-    hipStream_t s = c->stream;
-    if (!b->schur_ready) ba_build_schur(b, s);
+static SchurArgs ba_schur_args(mcv_ba* b, float lambda) {
     SchurArgs a{};
     a.planes = b->planes.as<float>();
     a.S = b->M;
     a.isig = b->isig.as<float>();
-    a.pptr = b->pptr.as<int>();
+    a.M = b->M;
+    a.splanes = b->s_planes.as<float>();
+    a.SS = b->Ms;
+    a.sisig = b->s_isig.as<float>();
+    a.pe_ptr = b->pe_ptr.as<int>();
+    a.pe = b->pe.as<int>();
     a.pf_ptr = b->pf_ptr.as<int>();
     a.pf_obs_ptr = b->pf_obs_ptr.as<int>();
     a.pf_obs = b->pf_obs.as<int>();
     a.pf_point = b->pf_point.as<int>();
+    a.pf_frame = b->pf_frame.as<int>();
     a.fptr = b->sfptr.as<int>();
     a.fobs = b->sfobs.as<int>();
     a.fpf_ptr = b->fpf_ptr.as<int>();
@@ -2383,13 +2701,95 @@ int mcv_ba_schur(mcv_ba* b, float lambda) {
     a.Hpl = b->Hpl.as<float>();
     a.Hpp = b->Hpp.as<float>();
     a.bs = b->bs.as<float>();
+    a.bp = b->bp.as<float>();
     a.Sout = b->Sout.as<float>();
+    return a;
+}
+
+// the Schur launches (structure ready); with blocks = false only H_ll, H_pl,
+// H_pp, b_p (what the largest-diagonal search needs)
+static void ba_enqueue_schur(mcv_ba* b, float lambda, hipStream_t s, bool blocks = true) {
+    const SchurArgs a = ba_schur_args(b, lambda);
     hipLaunchKernelGGL(k_schur_points, dim3((b->P + 255) / 256), dim3(256), 0, s, a);
     if (b->Fp > 0) {
         hipLaunchKernelGGL(k_schur_frames, dim3(b->Fp), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(k_schur_blocks, dim3(b->nblk), dim3(256), 0, s, a);
+        if (blocks) hipLaunchKernelGGL(k_schur_blocks, dim3(b->nblk), dim3(256), 0, s, a);
     }
     HIP_OK(hipGetLastError());
+}
+
+static void ba_require_schur_inputs(mcv_ba* b) {
+    REQUIRE(b->pose_linearized, MCV_ERR_STATE, "linearize before the Schur complement");
+    REQUIRE(b->linearized || !b->free_points_with_edges, MCV_ERR_STATE,
+            "a pose-only linearization leaves no J_point for the free points");
+    const bool stereo_edges = b->have_stereo && b->Ms > 0;
+    REQUIRE(!stereo_edges || b->stereo_pose_linearized, MCV_ERR_STATE,
+            "linearize the stereo edges before the Schur complement");
+    REQUIRE(!stereo_edges || !b->free_points_with_edges || b->stereo_full, MCV_ERR_STATE,
+            "a pose-only stereo linearization leaves no J_point for the free points");
+}
+
+int mcv_ba_set_fixed(mcv_ba* b, const int32_t* frame_fixed) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && frame_fixed, MCV_ERR_INVALID, "null fixed flags");
+    b->fixed.assign(frame_fixed, frame_fixed + b->F);
+    b->schur_ready = false;
+    API_END
+}
+
+int mcv_ba_set_point_fixed(mcv_ba* b, const int32_t* point_fixed) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b, MCV_ERR_INVALID, "null BA object");
+    c->set_device();
+    if (point_fixed) b->point_fixed.assign(point_fixed, point_fixed + b->P);
+    else b->point_fixed.clear();
+    std::vector<uint8_t> f(b->P, 0);
+    for (int p = 0; p < b->P && !b->point_fixed.empty(); p++) f[p] = b->point_fixed[p] ? 1 : 0;
+    upload(b->pfix_d, f, c->stream);
+    HIP_OK(hipStreamSynchronize(c->stream));
+    b->schur_ready = false;
+    API_END
+}
+
+int mcv_ba_set_active(mcv_ba* b, const uint8_t* active_mono, const uint8_t* active_stereo) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b, MCV_ERR_INVALID, "null BA object");
+    c->set_device();
+    if (active_mono) b->active_m.assign(active_mono, active_mono + b->M);
+    else b->active_m.clear();
+    if (active_stereo) b->active_s.assign(active_stereo, active_stereo + b->Ms);
+    else b->active_s.clear();
+    if (!b->active_m.empty()) upload(b->act_m, b->active_m, c->stream);
+    if (!b->active_s.empty()) upload(b->act_s, b->active_s, c->stream);
+    HIP_OK(hipStreamSynchronize(c->stream));
+    b->schur_ready = false;
+    API_END
+}
+
+int mcv_ba_set_robust(mcv_ba* b, int enabled) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && b->have_state, MCV_ERR_STATE, "set the state first");
+    c->set_device();
+    b->robust = enabled != 0;
+    ba_upload_frames(b, c->stream);
+    HIP_OK(hipStreamSynchronize(c->stream));
+    b->touched(-1);
+    API_END
+}
+
+int mcv_ba_schur(mcv_ba* b, float lambda) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b, MCV_ERR_INVALID, "null BA object");
+    c->set_device();
+    hipStream_t s = c->stream;
+    if (!b->schur_ready) ba_build_schur(b, s);
+    ba_require_schur_inputs(b);
+    ba_enqueue_schur(b, lambda, s);
     b->schur_done = true;
     API_END
 }
@@ -2443,6 +2843,279 @@ int mcv_solve_llt(int n, const double* S, const double* b, double* x) {
         x[i] = v / L[(size_t)i * n + i];
     }
     return MCV_OK;
+}
+
+}  // extern "C"
+
+// ---- Levenberg-Marquardt over the GPU data ------------------------------------
+// Pose increment T <- exp(x) * T of VertexSE3Expmap::oplusImpl
+// (types/sba/types_six_dof_expmap.h:98-101): SE3Quat::exp
+// (types/slam3d/se3quat.h:218-257) with Eigen's matrix -> quaternion
+// conversion, then the product with a normalised (w >= 0) quaternion.
+struct Qd { double x, y, z, w; };
+
+static Qd qd_mul(const Qd& a, const Qd& b) {
+    return Qd{a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y + a.y * b.w + a.z * b.x - a.x * b.z,
+              a.w * b.z + a.z * b.w + a.x * b.y - a.y * b.x, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
+}
+
+static void qd_rotate(const Qd& q, const double v[3], double o[3]) {
+    double uv[3] = {q.y * v[2] - q.z * v[1], q.z * v[0] - q.x * v[2], q.x * v[1] - q.y * v[0]};
+    for (double& c : uv) c += c;
+    o[0] = v[0] + q.w * uv[0] + (q.y * uv[2] - q.z * uv[1]);
+    o[1] = v[1] + q.w * uv[1] + (q.z * uv[0] - q.x * uv[2]);
+    o[2] = v[2] + q.w * uv[2] + (q.x * uv[1] - q.y * uv[0]);
+}
+
+static void qd_normalize(Qd& q) {
+    if (q.w < 0) q = Qd{-q.x, -q.y, -q.z, -q.w};
+    const double n = std::sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    q = Qd{q.x / n, q.y / n, q.z / n, q.w / n};
+}
+
+static Qd qd_from_R(const double* m) {
+    Qd q{};
+    double t = m[0] + m[4] + m[8];
+    if (t > 0) {
+        t = std::sqrt(t + 1.0);
+        q.w = 0.5 * t;
+        t = 0.5 / t;
+        q.x = (m[7] - m[5]) * t;
+        q.y = (m[2] - m[6]) * t;
+        q.z = (m[3] - m[1]) * t;
+    } else {
+        int i = 0;
+        if (m[4] > m[0]) i = 1;
+        if (m[8] > m[4 * i]) i = 2;
+        const int j = (i + 1) % 3, k = (j + 1) % 3;
+        double c[3];
+        t = std::sqrt(m[4 * i] - m[4 * j] - m[4 * k] + 1.0);
+        c[i] = 0.5 * t;
+        t = 0.5 / t;
+        q.w = (m[3 * k + j] - m[3 * j + k]) * t;
+        c[j] = (m[3 * j + i] + m[3 * i + j]) * t;
+        c[k] = (m[3 * k + i] + m[3 * i + k]) * t;
+        q.x = c[0]; q.y = c[1]; q.z = c[2];
+    }
+    return q;
+}
+
+static void se3_oplus(double* frame, const double* xi) {
+    const double w[3] = {xi[0], xi[1], xi[2]};
+    const double th = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    const double W[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+    double W2[9];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) W2[3 * r + c] = W[3 * r] * W[c] + W[3 * r + 1] * W[3 + c] + W[3 * r + 2] * W[6 + c];
+    double R[9], V[9];
+    for (int k = 0; k < 9; k++) {
+        const double I = (k % 4 == 0) ? 1.0 : 0.0;
+        if (th < 0.00001) {
+            R[k] = I + W[k] + 0.5 * W2[k];
+            V[k] = I + 0.5 * W[k] + (1.0 / 6.0) * W2[k];
+        } else {
+            R[k] = I + std::sin(th) / th * W[k] + (1 - std::cos(th)) / (th * th) * W2[k];
+            V[k] = I + (1 - std::cos(th)) / (th * th) * W[k] + (th - std::sin(th)) / std::pow(th, 3) * W2[k];
+        }
+    }
+    Qd dq = qd_from_R(R);
+    qd_normalize(dq);
+    double dt[3];
+    for (int r = 0; r < 3; r++) dt[r] = V[3 * r] * xi[3] + V[3 * r + 1] * xi[4] + V[3 * r + 2] * xi[5];
+    // exp(x) * T
+    const Qd q{frame[0], frame[1], frame[2], frame[3]};
+    double rt[3];
+    qd_rotate(dq, frame + 4, rt);
+    Qd nq = qd_mul(dq, q);
+    qd_normalize(nq);
+    frame[0] = nq.x; frame[1] = nq.y; frame[2] = nq.z; frame[3] = nq.w;
+    for (int r = 0; r < 3; r++) frame[4 + r] = dt[r] + rt[r];
+}
+
+// both edge sets linearized at the current state (mode as k_ba_linearize)
+static void ba_lin_all(mcv_ba* b, int mode, hipStream_t s) {
+    ba_enqueue_linearize(b, b->variant, mode, s);
+    b->touched(mode);
+    if (b->have_stereo) ba_enqueue_stereo(b, mode, s);
+}
+
+// activeRobustChi2 of the last linearization (sum over active edges of the
+// Huber rho, FP64 partial sums in a fixed order); enqueued, result in dsum[slot]
+static void ba_enqueue_rho(mcv_ba* b, hipStream_t s, int slot) {
+    const int Ms = b->have_stereo ? b->Ms : 0;
+    const int n = b->M + Ms;
+    const int blocks = std::max((n + 255) / 256, 1);
+    b->partial.ensure((size_t)std::max(blocks, (b->P + 255) / 256 + 1) * 8);
+    b->dsum.ensure(64);
+    hipLaunchKernelGGL(k_ba_rho_sum, dim3(blocks), dim3(256), 0, s, b->planes.as<float>() + 2 * (size_t)b->M,
+                       b->pair.as<uint32_t>(), b->pairs.as<BAPair>(),
+                       b->active_m.empty() ? nullptr : b->act_m.as<uint8_t>(), b->M,
+                       b->s_planes.as<float>() + 3 * (size_t)b->Ms,
+                       b->active_s.empty() ? nullptr : b->act_s.as<uint8_t>(), Ms, b->scam.delta,
+                       b->partial.as<double>());
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, b->partial.as<double>(), blocks,
+                       b->dsum.as<double>() + slot);
+    HIP_OK(hipGetLastError());
+}
+
+static double ba_read_double(mcv_ba* b, hipStream_t s, int slot) {
+    double v = 0;
+    HIP_OK(hipMemcpyAsync(&v, b->dsum.as<double>() + slot, 8, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    return v;
+}
+
+extern "C" {
+
+int mcv_ba_robust_chi2(mcv_ba* b, double* chi2) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && chi2, MCV_ERR_INVALID, "null argument");
+    REQUIRE(b->gen > 0 && b->have_state && b->have_obs, MCV_ERR_STATE, "no linearization");
+    REQUIRE(!(b->have_stereo && b->Ms > 0) || b->stereo_linearized, MCV_ERR_STATE, "stereo edges not linearized");
+    c->set_device();
+    ba_enqueue_rho(b, c->stream, 0);
+    *chi2 = ba_read_double(b, c->stream, 0);
+    API_END
+}
+
+int mcv_ba_optimize(mcv_ba* b, int iterations, double tau, int max_trials, double* stats, int32_t* done) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && b->have_obs && b->have_state, MCV_ERR_STATE, "observations or state not set");
+    REQUIRE(iterations >= 0 && max_trials >= 1 && tau >= 0, MCV_ERR_INVALID, "bad optimizer settings");
+    c->set_device();
+    hipStream_t s = c->stream;
+    if (!b->schur_ready) ba_build_schur(b, s);
+    const int Fp = b->Fp, n = 6 * Fp, P = b->P;
+    // free points without edges need no J_point: a pose-only linearization
+    const int full_mode = b->free_points_with_edges ? kBaFull : kBaPoseOnly;
+    std::vector<int> fidx(b->F, -1);
+    for (int f = 0, k = 0; f < b->F; f++)
+        if (!b->fixed[f]) fidx[f] = k++;
+    b->p64_bak.ensure((size_t)P * 24);
+    b->xl.ensure((size_t)std::max(P, 1) * 12);
+    b->xp.ensure((size_t)std::max(n, 1) * 4);
+    b->fmax.ensure(16);
+    const uint8_t* pfix = b->point_fixed.empty() ? nullptr : b->pfix_d.as<uint8_t>();
+    const int pblocks = (P + 255) / 256;
+    std::vector<float> Sf((size_t)n * n), bsf(n), bpf(n), xpf(n);
+    std::vector<double> S((size_t)n * n), bs(n), bp(n), xp(n);
+    double lambda = 0, ni = 2;
+    int it_done = 0;
+    for (int it = 0; it < iterations; it++) {
+        // computeActiveErrors + activeRobustChi2 + buildSystem's linearization
+        ba_lin_all(b, full_mode, s);
+        ba_enqueue_rho(b, s, 0);
+        double currentChi = ba_read_double(b, s, 0);
+        if (stats) stats[4 * it] = currentChi;
+        if (it == 0) {
+            // computeLambdaInit: tau * the largest diagonal entry of H
+            ba_enqueue_schur(b, 1.f, s, false);
+            hipLaunchKernelGGL(k_ba_max_diag, dim3(1), dim3(256), 0, s, b->Hpp.as<float>(), Fp, b->Hll.as<float>(),
+                               b->pe_ptr.as<int>(), P, b->fmax.as<float>());
+            HIP_OK(hipGetLastError());
+            float mx = 0;
+            HIP_OK(hipMemcpyAsync(&mx, b->fmax.p, 4, hipMemcpyDeviceToHost, s));
+            HIP_OK(hipStreamSynchronize(s));
+            lambda = tau * (double)mx;
+            ni = 2;
+        }
+        double rho = 0;
+        int qmax = 0;
+        do {
+            // push
+            const std::vector<double> frames0 = b->h_frames;
+            HIP_OK(hipMemcpyAsync(b->p64_bak.p, b->p64.p, (size_t)P * 24, hipMemcpyDeviceToDevice, s));
+            ba_enqueue_schur(b, (float)lambda, s);
+            if (n) {
+                HIP_OK(hipMemcpyAsync(Sf.data(), b->Sout.p, Sf.size() * 4, hipMemcpyDeviceToHost, s));
+                HIP_OK(hipMemcpyAsync(bsf.data(), b->bs.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+                HIP_OK(hipMemcpyAsync(bpf.data(), b->bp.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+            }
+            HIP_OK(hipStreamSynchronize(s));
+            for (size_t k = 0; k < Sf.size(); k++) S[k] = Sf[k];
+            for (int k = 0; k < n; k++) { bs[k] = bsf[k]; bp[k] = bpf[k]; }
+            // FP64 Cholesky of the reduced camera system
+            const bool ok2 = n == 0 || mcv_solve_llt(n, S.data(), bs.data(), xp.data()) == MCV_OK;
+            double scale = 0;
+            if (ok2) {
+                for (int k = 0; k < n; k++) {
+                    xpf[k] = (float)xp[k];
+                    scale += xp[k] * (lambda * xp[k] + bp[k]);
+                }
+                if (n) HIP_OK(hipMemcpyAsync(b->xp.p, xpf.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+                const SchurArgs a = ba_schur_args(b, (float)lambda);
+                hipLaunchKernelGGL(k_schur_backsub, dim3(std::max(pblocks, 1)), dim3(256), 0, s, a, b->xp.as<float>(),
+                                   b->xl.as<float>(), b->partial.as<double>());
+                hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, b->partial.as<double>(), pblocks,
+                                   b->dsum.as<double>() + 1);
+                // update: points on the device, frames on the host
+                hipLaunchKernelGGL(k_ba_points_apply, dim3(std::max(pblocks, 1)), dim3(256), 0, s,
+                                   b->p64.as<double>(), b->xl.as<float>(), pfix, P, b->phi.as<float4>(),
+                                   b->plo.as<float4>());
+                HIP_OK(hipGetLastError());
+                for (int f = 0; f < b->F; f++)
+                    if (fidx[f] >= 0) se3_oplus(&b->h_frames[7 * (size_t)f], &xp[6 * (size_t)fidx[f]]);
+                ba_upload_frames(b, s);
+                scale += ba_read_double(b, s, 1);
+            }
+            // computeActiveErrors at the new state
+            ba_lin_all(b, kBaResidual, s);
+            ba_enqueue_rho(b, s, 2);
+            double tempChi = ba_read_double(b, s, 2);
+            if (!ok2) tempChi = DBL_MAX;
+            scale += 1e-3;
+            rho = (currentChi - tempChi) / scale;
+            if (rho > 0 && std::isfinite(tempChi)) {
+                double alpha = 1. - std::pow(2 * rho - 1, 3);
+                alpha = std::min(alpha, 2. / 3.);
+                lambda *= std::max(1. / 3., alpha);
+                ni = 2;
+                currentChi = tempChi;
+            } else {
+                lambda *= ni;
+                ni *= 2;
+                // pop
+                b->h_frames = frames0;
+                ba_upload_frames(b, s);
+                HIP_OK(hipMemcpyAsync(b->p64.p, b->p64_bak.p, (size_t)P * 24, hipMemcpyDeviceToDevice, s));
+                hipLaunchKernelGGL(k_ba_points_apply, dim3(std::max(pblocks, 1)), dim3(256), 0, s,
+                                   b->p64.as<double>(), (const float*)nullptr, (const uint8_t*)nullptr, P,
+                                   b->phi.as<float4>(), b->plo.as<float4>());
+                HIP_OK(hipGetLastError());
+                if (!std::isfinite(lambda)) break;
+            }
+            qmax++;
+        } while (rho < 0 && qmax < max_trials);
+        if (stats) {
+            stats[4 * it + 1] = currentChi;
+            stats[4 * it + 2] = lambda;
+            stats[4 * it + 3] = qmax;
+        }
+        it_done++;
+        if (qmax == max_trials || rho == 0 || !std::isfinite(lambda)) break;
+    }
+    // the residual planes of the final (accepted) state, for the outlier
+    // test that follows an optimization (FilterCallBack_Chi2 calls
+    // computeError first)
+    ba_lin_all(b, kBaResidual, s);
+    HIP_OK(hipStreamSynchronize(s));
+    if (done) *done = it_done;
+    API_END
+}
+
+int mcv_ba_get_state(mcv_ba* b, double* points, double* frames) {
+    mcv_ctx* c = b ? b->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(b && b->have_state, MCV_ERR_STATE, "no state");
+    c->set_device();
+    if (points) {
+        HIP_OK(hipMemcpyAsync(points, b->p64.p, (size_t)b->P * 24, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    }
+    if (frames) std::memcpy(frames, b->h_frames.data(), b->h_frames.size() * sizeof(double));
+    API_END
 }
 
 int mcv_ba_planes(mcv_ba* b, float** d_planes, int64_t* stride) {

@@ -141,8 +141,14 @@ __device__ __forceinline__ void ba_store(const BAOut& out, int k, uint32_t off, 
     else *p = v;
 }
 
+// Linearization modes: every plane; e, chi2 and w only (the LM trials'
+// computeActiveErrors); everything but J_point (edges whose point is fixed:
+// EdgeSE3XYZ2CameraOnlyPose, include/OptimizeTypes.hpp:20-68, has the same
+// residual and J_pose, and a fixed point's J_point is never used).
+constexpr int kBaFull = 0, kBaResidual = 1, kBaPoseOnly = 2;
+
 template <bool kNt>
-__device__ __forceinline__ void ba_edge(int i, const BAIn& in, const BAPair& T, BAOut out, int residual_only) {
+__device__ __forceinline__ void ba_edge(int i, const BAIn& in, const BAPair& T, BAOut out, int mode) {
     const float2 uv = in.uv;
     const float isig = in.isig;
     const float4 ph = in.ph, pl = in.pl;
@@ -171,17 +177,18 @@ __device__ __forceinline__ void ba_edge(int i, const BAIn& in, const BAPair& T, 
     ba_store<kNt>(out, 1, off, e1);
     ba_store<kNt>(out, 2, off, chi2);
     ba_store<kNt>(out, 3, off, w);
-    if (residual_only) return;
+    if (mode == kBaResidual) return;
     // Jacobians (FP32).  Jpi = [[fx/z, 0, -fx x/z^2], [0, fy/z, -fy y/z^2]]
     const float x = C[0].hi, y = C[1].hi;
     const float j00 = T.fx * iz, j02 = -T.fx * x * (iz * iz);
     const float j11 = T.fy * iz, j12 = -T.fy * y * (iz * iz);
     // J_point = -Jpi R_cw
-    float Jpt[6];
+    if (mode != kBaPoseOnly) {
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        Jpt[k] = -(j00 * T.Rh[k] + j02 * T.Rh[6 + k]);
-        Jpt[3 + k] = -(j11 * T.Rh[3 + k] + j12 * T.Rh[6 + k]);
+        for (int k = 0; k < 3; k++) {
+            ba_store<kNt>(out, 16 + k, off, -(j00 * T.Rh[k] + j02 * T.Rh[6 + k]));
+            ba_store<kNt>(out, 19 + k, off, -(j11 * T.Rh[3 + k] + j12 * T.Rh[6 + k]));
+        }
     }
     // P_b = R_bw P_w + t_bw; M = Jpi R_cb; J_pose = [M [P_b]x | -M]
     float pb[3];
@@ -207,8 +214,6 @@ __device__ __forceinline__ void ba_edge(int i, const BAIn& in, const BAPair& T, 
     }
 #pragma unroll
     for (int k = 0; k < 12; k++) ba_store<kNt>(out, 4 + k, off, Jp[k]);
-#pragma unroll
-    for (int k = 0; k < 6; k++) ba_store<kNt>(out, 16 + k, off, Jpt[k]);
 }
 
 // One observation per thread per step, grid-stride.  kLds: the (frame,
@@ -221,7 +226,7 @@ template <bool kLds, int kIlp, bool kNt>
 __global__ __launch_bounds__(1024) void k_ba_linearize(BAObs ob, const float4* __restrict__ pts_hi,
                                                        const float4* __restrict__ pts_lo,
                                                        const BAPair* __restrict__ pairs, int npairs, BAOut out,
-                                                       int residual_only) {
+                                                       int mode) {
     extern __shared__ float s_pairs[];  // npairs * kPairWords floats when kLds
     if (kLds) {
         const float* src = reinterpret_cast<const float*>(pairs);
@@ -236,7 +241,7 @@ __global__ __launch_bounds__(1024) void k_ba_linearize(BAObs ob, const float4* _
             BAIn in;
             ba_load_obs(i, ob, in);
             ba_load_point(pts_hi, pts_lo, in);
-            ba_edge<kNt>(i, in, T, out, residual_only);
+            ba_edge<kNt>(i, in, T, out, mode);
         }
         return;
     }
@@ -254,7 +259,7 @@ __global__ __launch_bounds__(1024) void k_ba_linearize(BAObs ob, const float4* _
             if (i >= ob.M) break;
             const BAPair& T = kLds ? *reinterpret_cast<const BAPair*>(s_pairs + in[u].pair * kPairWords)
                                    : pairs[in[u].pair];
-            ba_edge<kNt>(i, in[u], T, out, residual_only);
+            ba_edge<kNt>(i, in[u], T, out, mode);
         }
     }
 }
@@ -264,14 +269,14 @@ __global__ __launch_bounds__(1024) void k_ba_linearize(BAObs ob, const float4* _
 __global__ __launch_bounds__(256) void k_ba_linearize_group(BAObs ob, const float4* __restrict__ pts_hi,
                                                             const float4* __restrict__ pts_lo,
                                                             const BAPair* __restrict__ pairs, const int* list, int n,
-                                                            BAOut out, int residual_only) {
+                                                            BAOut out, int mode) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int i = list[k];
     BAIn in;
     ba_load_obs(i, ob, in);
     ba_load_point(pts_hi, pts_lo, in);
-    ba_edge<false>(i, in, pairs[in.pair], out, residual_only);
+    ba_edge<false>(i, in, pairs[in.pair], out, mode);
 }
 
 // ---------------------------------------------------------------------------
@@ -301,7 +306,7 @@ struct BAStereoObs {
 __global__ __launch_bounds__(256) void k_ba_stereo_linearize(BAStereoObs ob, const float4* __restrict__ pts_hi,
                                                              const float4* __restrict__ pts_lo,
                                                              const FramePose* __restrict__ poses, StereoCam cam,
-                                                             BAOut out, int residual_only) {
+                                                             BAOut out, int mode) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ob.M) return;
     const float ou = ob.obs[3 * i], ov = ob.obs[3 * i + 1], our = ob.obs[3 * i + 2];
@@ -337,7 +342,7 @@ __global__ __launch_bounds__(256) void k_ba_stereo_linearize(BAStereoObs ob, con
     o[2 * S] = e2;
     o[3 * S] = chi2;
     o[4 * S] = w;
-    if (residual_only) return;
+    if (mode == kBaResidual) return;
     const float x = C[0].hi, y = C[1].hi, z = C[2].hi;
     const float iz = __builtin_amdgcn_rcpf(z), iz2 = iz * iz;
     const float fx = cam.fx, fy = cam.fy, bf = cam.bf;
@@ -366,6 +371,7 @@ __global__ __launch_bounds__(256) void k_ba_stereo_linearize(BAStereoObs ob, con
         o[(11 + k) * S] = J1[k];
         o[(17 + k) * S] = J2[k];
     }
+    if (mode == kBaPoseOnly) return;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const float a = -fx * T.Rh[k] * iz + fx * x * T.Rh[6 + k] * iz2;
@@ -434,24 +440,36 @@ namespace mcv {
 // Schur complement of the point blocks (BlockSolver::buildSystem +
 // BlockSolver::solve, ThirdParty/g2o/g2o/core/block_solver.hpp:315-449,
 // 463-520; quadratic form of BaseBinaryEdge::constructQuadraticForm,
-// core/base_binary_edge.hpp:64-136), FP32, fixed summation orders:
-//   S1  per point: H_ll = sum w*isig A^T A, b_l = -sum w*isig A^T e,
+// core/base_binary_edge.hpp:64-136), FP32, fixed summation orders, over the
+// active mono (2-row, k_ba_linearize planes) and stereo (3-row,
+// k_ba_stereo_linearize planes) edges; edge index i < M is mono edge i,
+// i >= M stereo edge i - M.  Fixed points have no edge lists (their edges
+// still enter H_pp and b_p through the frame lists); fixed frames have no
+// pose block.
+//   S1  per point: H_ll = sum W J_x^T J_x, b_l = -sum W J_x^T e,
 //       D^-1 = (H_ll + lambda I)^-1, db = D^-1 b_l, and per (point, free
-//       frame) H_pl = sum over its cameras of w*isig B^T A (6x3)
-//   S3  per free frame: H_pp = sum w*isig B^T B, b_p, b_schur = b_p - sum H_pl db
+//       frame) H_pl = sum W J_p^T J_x (6x3); W = Huber weight * inv sigma^2
+//   S3  per free frame: H_pp = sum W J_p^T J_p, b_p = -sum W J_p^T e,
+//       b_schur = b_p - sum H_pl db
 //   S2  per free-frame pair block (i <= j): S_ij = [i == j](H_pp + lambda I)
 //       - sum over shared points H_pl,i D^-1 H_pl,j^T
-// A = J_point (2x3), B = J_pose (2x6); planes as written by k_ba_linearize.
+//   back-substitution: x_l = D^-1 (b_l - sum H_pl^T x_p)
 struct SchurArgs {
-    const float* planes;
-    long long S;            // plane stride (M)
+    const float* planes;    // mono planes (22 of stride S)
+    long long S;
     const float* isig;
-    const int* pptr;        // P + 1, observations of point p
+    int M;
+    const float* splanes;   // stereo planes (32 of stride SS)
+    long long SS;
+    const float* sisig;
+    const int* pe_ptr;      // P + 1: active edges of free point p
+    const int* pe;
     const int* pf_ptr;      // per point: first pf entry (P + 1)
     const int* pf_obs_ptr;  // per pf entry: range in pf_obs (npf + 1)
-    const int* pf_obs;      // observation indices grouped by pf
+    const int* pf_obs;      // edge indices grouped by pf
     const int* pf_point;    // point of each pf entry
-    const int* fptr;        // free frame -> observations (Fp + 1)
+    const int* pf_frame;    // free frame of each pf entry
+    const int* fptr;        // free frame -> active edges (Fp + 1)
     const int* fobs;
     const int* fpf_ptr;     // free frame -> pf entries (Fp + 1)
     const int* fpf;
@@ -465,46 +483,68 @@ struct SchurArgs {
     float* db;              // P x 3
     float* Hll;             // P x 9 (without lambda)
     float* Hpl;             // npf x 18 (6x3 row-major)
-    float* Hpp;             // Fp x 36
-    float* bs;              // Fp x 6
+    float* Hpp;             // Fp x 36 (without lambda)
+    float* bs;              // Fp x 6 (b_schur)
+    float* bp;              // Fp x 6 (b_p)
     float* Sout;            // (6 Fp)^2 row-major
 };
+
+// one edge's planes: e row r at base[r * st], J_pose (r, c) at
+// base[(jp + 6 r + c) * st], J_point (r, c) at base[(jx + 3 r + c) * st]
+struct EdgeRef {
+    const float* base;
+    long long st;
+    int rows, jp, jx;
+    float W;
+    __device__ __forceinline__ float e(int r) const { return base[r * st]; }
+    __device__ __forceinline__ float Jp(int r, int c) const { return base[(jp + 6 * r + c) * st]; }
+    __device__ __forceinline__ float Jx(int r, int c) const { return base[(jx + 3 * r + c) * st]; }
+};
+
+__device__ __forceinline__ EdgeRef edge_ref(const SchurArgs& a, int i) {
+    if (i < a.M) return EdgeRef{a.planes + i, a.S, 2, 4, 16, a.planes[3 * a.S + i] * a.isig[i]};
+    const int j = i - a.M;
+    return EdgeRef{a.splanes + j, a.SS, 3, 5, 23, a.splanes[4 * a.SS + j] * a.sisig[j]};
+}
 
 __global__ __launch_bounds__(256) void k_schur_points(SchurArgs a) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.P) return;
-    const long long S = a.S;
     float H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, b[3] = {0, 0, 0};
-    for (int i = a.pptr[p]; i < a.pptr[p + 1]; i++) {
-        const float W = a.planes[3 * S + i] * a.isig[i];
-        const float e0 = a.planes[i], e1 = a.planes[S + i];
-        float A[6];
+    const int e0 = a.pe_ptr[p], e1 = a.pe_ptr[p + 1];
+    for (int t = e0; t < e1; t++) {
+        const EdgeRef E = edge_ref(a, a.pe[t]);
+        for (int r = 0; r < E.rows; r++) {
+            const float er = E.e(r);
+            const float A[3] = {E.Jx(r, 0), E.Jx(r, 1), E.Jx(r, 2)};
 #pragma unroll
-        for (int k = 0; k < 6; k++) A[k] = a.planes[(16 + k) * S + i];
+            for (int i = 0; i < 3; i++) {
+                b[i] -= E.W * (A[i] * er);
 #pragma unroll
-        for (int r = 0; r < 3; r++) {
-            b[r] -= W * (A[r] * e0 + A[3 + r] * e1);
-#pragma unroll
-            for (int c = 0; c < 3; c++) H[3 * r + c] += W * (A[r] * A[c] + A[3 + r] * A[3 + c]);
+                for (int c = 0; c < 3; c++) H[3 * i + c] += E.W * (A[i] * A[c]);
+            }
         }
     }
-    float D[9];
+    float Di[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (e1 > e0) {
+        float D[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k] = H[k];
+        D[0] += a.lambda;
+        D[4] += a.lambda;
+        D[8] += a.lambda;
+        const float c00 = D[4] * D[8] - D[5] * D[7], c01 = D[5] * D[6] - D[3] * D[8], c02 = D[3] * D[7] - D[4] * D[6];
+        const float det = D[0] * c00 + D[1] * c01 + D[2] * c02;
+        const float id = 1.f / det;
+        Di[0] = c00 * id; Di[1] = (D[2] * D[7] - D[1] * D[8]) * id; Di[2] = (D[1] * D[5] - D[2] * D[4]) * id;
+        Di[3] = c01 * id; Di[4] = (D[0] * D[8] - D[2] * D[6]) * id; Di[5] = (D[2] * D[3] - D[0] * D[5]) * id;
+        Di[6] = c02 * id; Di[7] = (D[1] * D[6] - D[0] * D[7]) * id; Di[8] = (D[0] * D[4] - D[1] * D[3]) * id;
+    }
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         a.Hll[9 * p + k] = H[k];
-        D[k] = H[k];
+        a.Dinv[9 * p + k] = Di[k];
     }
-    D[0] += a.lambda;
-    D[4] += a.lambda;
-    D[8] += a.lambda;
-    const float c00 = D[4] * D[8] - D[5] * D[7], c01 = D[5] * D[6] - D[3] * D[8], c02 = D[3] * D[7] - D[4] * D[6];
-    const float det = D[0] * c00 + D[1] * c01 + D[2] * c02;
-    const float id = 1.f / det;
-    const float Di[9] = {c00 * id, (D[2] * D[7] - D[1] * D[8]) * id, (D[1] * D[5] - D[2] * D[4]) * id,
-                         c01 * id, (D[0] * D[8] - D[2] * D[6]) * id, (D[2] * D[3] - D[0] * D[5]) * id,
-                         c02 * id, (D[1] * D[6] - D[0] * D[7]) * id, (D[0] * D[4] - D[1] * D[3]) * id};
-#pragma unroll
-    for (int k = 0; k < 9; k++) a.Dinv[9 * p + k] = Di[k];
 #pragma unroll
     for (int r = 0; r < 3; r++) {
         a.bl[3 * p + r] = b[r];
@@ -515,77 +555,76 @@ __global__ __launch_bounds__(256) void k_schur_points(SchurArgs a) {
 #pragma unroll
         for (int k = 0; k < 18; k++) Hq[k] = 0.f;
         for (int t = a.pf_obs_ptr[q]; t < a.pf_obs_ptr[q + 1]; t++) {
-            const int i = a.pf_obs[t];
-            const float W = a.planes[3 * S + i] * a.isig[i];
-            float A[6], B[12];
+            const EdgeRef E = edge_ref(a, a.pf_obs[t]);
+            for (int r = 0; r < E.rows; r++) {
+                float B[6], A[3];
 #pragma unroll
-            for (int k = 0; k < 6; k++) A[k] = a.planes[(16 + k) * S + i];
+                for (int k = 0; k < 6; k++) B[k] = E.Jp(r, k);
 #pragma unroll
-            for (int k = 0; k < 12; k++) B[k] = a.planes[(4 + k) * S + i];
+                for (int k = 0; k < 3; k++) A[k] = E.Jx(r, k);
 #pragma unroll
-            for (int r = 0; r < 6; r++)
+                for (int i = 0; i < 6; i++)
 #pragma unroll
-                for (int c = 0; c < 3; c++) Hq[3 * r + c] += W * (B[r] * A[c] + B[6 + r] * A[3 + c]);
+                    for (int c = 0; c < 3; c++) Hq[3 * i + c] += E.W * (B[i] * A[c]);
+            }
         }
 #pragma unroll
         for (int k = 0; k < 18; k++) a.Hpl[18 * (long long)q + k] = Hq[k];
     }
 }
 
-__device__ __forceinline__ void block_sum36(float* acc, float (*red)[36]) {
-    // fixed-order tree reduction of 256 x 36 partial sums
-    for (int k = 0; k < 36; k++) red[threadIdx.x][k] = acc[k];
+// fixed-order tree reduction of 256 x n partial sums (n <= 36); red[0][k]
+// holds the sums
+__device__ __forceinline__ void block_sum(const float* acc, float (*red)[36], int n) {
+    for (int k = 0; k < n; k++) red[threadIdx.x][k] = acc[k];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (threadIdx.x < s)
-            for (int k = 0; k < 36; k++) red[threadIdx.x][k] += red[threadIdx.x + s][k];
+            for (int k = 0; k < n; k++) red[threadIdx.x][k] += red[threadIdx.x + s][k];
         __syncthreads();
     }
 }
 
 __global__ __launch_bounds__(256) void k_schur_frames(SchurArgs a) {
     __shared__ float red[256][36];
-    __shared__ float red6[256][6];
     const int f = blockIdx.x;
-    const long long S = a.S;
     float acc[36];
-    float bp[6] = {0, 0, 0, 0, 0, 0};
+    float bp[12];  // b_p (0..5) and the Schur coupling sum H_pl db (6..11)
 #pragma unroll
     for (int k = 0; k < 36; k++) acc[k] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; k++) bp[k] = 0.f;
     for (int t = a.fptr[f] + threadIdx.x; t < a.fptr[f + 1]; t += 256) {
-        const int i = a.fobs[t];
-        const float W = a.planes[3 * S + i] * a.isig[i];
-        const float e0 = a.planes[i], e1 = a.planes[S + i];
-        float B[12];
+        const EdgeRef E = edge_ref(a, a.fobs[t]);
+        for (int r = 0; r < E.rows; r++) {
+            const float er = E.e(r);
+            float B[6];
 #pragma unroll
-        for (int k = 0; k < 12; k++) B[k] = a.planes[(4 + k) * S + i];
+            for (int k = 0; k < 6; k++) B[k] = E.Jp(r, k);
 #pragma unroll
-        for (int r = 0; r < 6; r++) {
-            bp[r] -= W * (B[r] * e0 + B[6 + r] * e1);
+            for (int i = 0; i < 6; i++) {
+                bp[i] -= E.W * (B[i] * er);
 #pragma unroll
-            for (int c = 0; c < 6; c++) acc[6 * r + c] += W * (B[r] * B[c] + B[6 + r] * B[6 + c]);
+                for (int c = 0; c < 6; c++) acc[6 * i + c] += E.W * (B[i] * B[c]);
+            }
         }
     }
-    // b_schur contribution: - sum over the frame's pf entries of H_pl db
     for (int t = a.fpf_ptr[f] + threadIdx.x; t < a.fpf_ptr[f + 1]; t += 256) {
         const int q = a.fpf[t];
         const int p = a.pf_point[q];
         const float* Hq = a.Hpl + 18 * (long long)q;
         const float d0 = a.db[3 * p], d1 = a.db[3 * p + 1], d2 = a.db[3 * p + 2];
 #pragma unroll
-        for (int r = 0; r < 6; r++) bp[r] -= Hq[3 * r] * d0 + Hq[3 * r + 1] * d1 + Hq[3 * r + 2] * d2;
+        for (int r = 0; r < 6; r++) bp[6 + r] += Hq[3 * r] * d0 + Hq[3 * r + 1] * d1 + Hq[3 * r + 2] * d2;
     }
-    block_sum36(acc, red);
+    block_sum(acc, red, 36);
     if (threadIdx.x < 36) a.Hpp[36 * f + threadIdx.x] = red[0][threadIdx.x];
     __syncthreads();
-    for (int k = 0; k < 6; k++) red6[threadIdx.x][k] = bp[k];
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
-            for (int k = 0; k < 6; k++) red6[threadIdx.x][k] += red6[threadIdx.x + s][k];
-        __syncthreads();
+    block_sum(bp, red, 12);
+    if (threadIdx.x < 6) {
+        a.bp[6 * f + threadIdx.x] = red[0][threadIdx.x];
+        a.bs[6 * f + threadIdx.x] = red[0][threadIdx.x] - red[0][6 + threadIdx.x];
     }
-    if (threadIdx.x < 6) a.bs[6 * f + threadIdx.x] = red6[0][threadIdx.x];
 }
 
 __global__ __launch_bounds__(256) void k_schur_blocks(SchurArgs a) {
@@ -614,7 +653,7 @@ __global__ __launch_bounds__(256) void k_schur_blocks(SchurArgs a) {
             for (int c = 0; c < 6; c++)
                 acc[6 * r + c] += Hi[3 * r] * Y[c] + Hi[3 * r + 1] * Y[6 + c] + Hi[3 * r + 2] * Y[12 + c];
     }
-    block_sum36(acc, red);
+    block_sum(acc, red, 36);
     if (threadIdx.x < 36) {
         const int r = threadIdx.x / 6, c = threadIdx.x % 6;
         const int n = 6 * a.Fp;
@@ -623,6 +662,124 @@ __global__ __launch_bounds__(256) void k_schur_blocks(SchurArgs a) {
         a.Sout[(long long)(6 * ij.x + r) * n + 6 * ij.y + c] = v;
         if (ij.x != ij.y) a.Sout[(long long)(6 * ij.y + c) * n + 6 * ij.x + r] = v;
     }
+}
+
+// fixed-order sum of one double per thread over the workgroup (256)
+__device__ __forceinline__ double block_sum_d(double v, double* red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// x_l = D^-1 (b_l - sum over the point's pf entries of H_pl^T x_p) per point
+// (0 for points without edges), and per workgroup the partial sum of
+// x_l . (lambda x_l + b_l) (the points' share of the LM scale).
+__global__ __launch_bounds__(256) void k_schur_backsub(SchurArgs a, const float* xp, float* xl, double* partial) {
+    __shared__ double red[256];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    double sc = 0.0;
+    if (p < a.P) {
+        float c[3] = {a.bl[3 * p], a.bl[3 * p + 1], a.bl[3 * p + 2]};
+        for (int q = a.pf_ptr[p]; q < a.pf_ptr[p + 1]; q++) {
+            const float* Hq = a.Hpl + 18 * (long long)q;
+            const float* x = xp + 6 * a.pf_frame[q];
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                c[k] -= Hq[k] * x[0] + Hq[3 + k] * x[1] + Hq[6 + k] * x[2] + Hq[9 + k] * x[3] + Hq[12 + k] * x[4] +
+                        Hq[15 + k] * x[5];
+        }
+        const float* Di = a.Dinv + 9 * p;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float v = Di[3 * k] * c[0] + Di[3 * k + 1] * c[1] + Di[3 * k + 2] * c[2];
+            xl[3 * p + k] = v;
+            sc += (double)v * ((double)a.lambda * (double)v + (double)a.bl[3 * p + k]);
+        }
+    }
+    const double t = block_sum_d(sc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// Huber rho of chi2 (core/robust_kernel_impl.cpp:65-78) summed over the
+// active edges, per workgroup: mono edge i takes delta from its (frame,
+// camera) pair, stereo edges the stereo delta; an infinite delta is the
+// edge without a robust kernel (rho = chi2).
+__device__ __forceinline__ double huber_rho(float chi2, float delta) {
+    const double e = chi2, d = delta;
+    return e <= d * d ? e : 2.0 * sqrt(e) * d - d * d;
+}
+
+__global__ __launch_bounds__(256) void k_ba_rho_sum(const float* chi2, const uint32_t* pair, const BAPair* pairs,
+                                                    const uint8_t* act, int M, const float* schi2,
+                                                    const uint8_t* sact, int Ms, float sdelta, double* partial) {
+    __shared__ double red[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (i < M) {
+        if (!act || act[i]) v = huber_rho(chi2[i], pairs[pair[i]].delta);
+    } else if (i - M < Ms) {
+        const int j = i - M;
+        if (!sact || sact[j]) v = huber_rho(schi2[j], sdelta);
+    }
+    const double t = block_sum_d(v, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// out[0] = sum of partial[0 .. n) in a fixed order (one workgroup)
+__global__ __launch_bounds__(256) void k_sum_partials(const double* partial, int n, double* out) {
+    __shared__ double red[256];
+    double v = 0.0;
+    for (int k = threadIdx.x; k < n; k += 256) v += partial[k];
+    const double t = block_sum_d(v, red);
+    if (threadIdx.x == 0) out[0] = t;
+}
+
+// largest |diagonal entry| of H_pp (free frames) and H_ll (points with
+// edges), computeLambdaInit's maxDiagonal (one workgroup)
+__global__ __launch_bounds__(256) void k_ba_max_diag(const float* Hpp, int Fp, const float* Hll, const int* pe_ptr,
+                                                     int P, float* out) {
+    __shared__ float red[256];
+    float m = 0.f;
+    for (int k = threadIdx.x; k < 6 * Fp; k += 256) m = fmaxf(m, fabsf(Hpp[36 * (k / 6) + 7 * (k % 6)]));
+    for (int k = threadIdx.x; k < 3 * P; k += 256) {
+        const int p = k / 3;
+        if (pe_ptr[p + 1] > pe_ptr[p]) m = fmaxf(m, fabsf(Hll[9 * p + 4 * (k % 3)]));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// FP64 point estimates -> (hi, lo) float pairs; with xl, first the additive
+// update P += x_l of the free points (VertexSBAPointXYZ::oplusImpl)
+__global__ __launch_bounds__(256) void k_ba_points_apply(double* p64, const float* xl, const uint8_t* fixed, int P,
+                                                         float4* hi, float4* lo) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    double v[3] = {p64[3 * p], p64[3 * p + 1], p64[3 * p + 2]};
+    if (xl && !(fixed && fixed[p])) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v[k] += (double)xl[3 * p + k];
+            p64[3 * p + k] = v[k];
+        }
+    }
+    float h[3], l[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        h[k] = (float)v[k];
+        l[k] = (float)(v[k] - (double)h[k]);
+    }
+    hi[p] = make_float4(h[0], h[1], h[2], 0.f);
+    lo[p] = make_float4(l[0], l[1], l[2], 0.f);
 }
 
 }  // namespace mcv

@@ -1363,6 +1363,290 @@ __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* ord
 }
 
 // ---------------------------------------------------------------------------
+// The same replay with the top of the heap in registers.  Nodes are numbered
+// from 1 (children of node j: 2j and 2j + 1; libstdc++'s index is j - 1).
+// Nodes 1..63, the six levels every sift passes through, live in two VGPRs
+// of the wave (lane j holds node j); deeper nodes live in LDS.  Both sifts
+// use that the values along a root-to-leaf path never increase (the heap
+// property):
+//   pop (pop_heap = __adjust_heap from the root, then __push_heap of the
+//     last element x): the hole descends along the max-child path to a leaf
+//     (the right child unless it is smaller than the left one; the single
+//     child at the end of an even-length heap), each path value moving up
+//     one level, and x then rises while its parent is smaller.  With v_1 >=
+//     ... >= v_k the path values below the root, the first s of them (those
+//     >= x) move up one level, x takes the node v_s left (the root when
+//     s = 0), and the rest stay where they are.
+//   push (__push_heap from the new last node h): h's ancestors that are
+//     smaller than the value, a run ending at h's parent, move down one level
+//     along the chain, and the value takes the highest node of the run (h
+//     itself when the parent is not smaller).
+// A pop finds its path in registers for the first five levels (readlane of
+// both children) and through one LDS gather of the 5-level subtree below per
+// further five levels; a push reads its chain (registers through a lane
+// shuffle, LDS below) in one round trip.  The path (lane t = depth t) and
+// the chain (lane t = depth t) sit in lanes, so the moves are lane-parallel
+// stores plus at most six writelanes.  A heap of fewer than 64 nodes never
+// touches LDS.  LDS holds 8 bytes per heap entry, as the LDS variant.
+__device__ __forceinline__ uint32_t rdl(uint32_t v, int j) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, j);
+}
+// lane j of v takes the (wave-uniform) value x: a compare and a select
+__device__ __forceinline__ uint32_t wrl(uint32_t v, uint32_t x, int j) {
+    return (int)(threadIdx.x & 63) == j ? x : v;
+}
+// compiler-only ordering of this wave's LDS accesses (the LDS executes one
+// wave's DS instructions in order)
+__device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
+
+struct RegHeap {
+    uint32_t tk, tp;  // nodes 1..63: lane j holds node j (key, payload)
+    uint32_t* k;      // LDS keys, indexed by node (nodes >= 64)
+    uint32_t* p;      // LDS payloads
+    int len;
+};
+
+__device__ __forceinline__ void rh_set(RegHeap& H, int j, uint32_t vk, uint32_t vp) {
+    if (j < 64) {
+        H.tk = wrl(H.tk, vk, j);
+        H.tp = wrl(H.tp, vp, j);
+    } else if ((threadIdx.x & 63) == 0) {
+        H.k[j] = vk;
+        H.p[j] = vp;
+    }
+    lds_order();
+}
+
+// push_heap of (vk, vp)
+__device__ __forceinline__ void rh_push(RegHeap& H, uint32_t vk, uint32_t vp) {
+    const int lane = threadIdx.x & 63;
+    const int h = ++H.len;
+    const int D = 31 - __builtin_clz((uint32_t)h);  // depth of h
+    if (D == 0) {
+        rh_set(H, 1, vk, vp);
+        return;
+    }
+    // lane t < D holds the chain node at depth t
+    const int cn = lane < D ? (h >> (D - lane)) : 0;
+    const int src = cn < 64 ? cn : 0;
+    const uint32_t rk = (uint32_t)__shfl((int)H.tk, src, 64), rp = (uint32_t)__shfl((int)H.tp, src, 64);
+    uint32_t ck = 0, cp = 0;
+    if (lane < D) {
+        if (cn < 64) {
+            ck = rk;
+            cp = rp;
+        } else {
+            ck = H.k[cn];
+            cp = H.p[cn];
+        }
+    }
+    // the ancestors smaller than the value: depths D - m .. D - 1
+    const int m = __popcll(__ballot(lane < D && hkey(ck) < hkey(vk)));
+    // each moves to its chain child (depth t + 1; h itself for t = D - 1)
+    if (lane < D && lane >= D - m) {
+        const int child = h >> (D - lane - 1);
+        if (child >= 64) {
+            H.k[child] = ck;
+            H.p[child] = cp;
+        }
+    }
+    // children below 64 are at depth <= 5
+    for (int u = D - m; u <= min(D - 1, 4); u++) {
+        const int child = h >> (D - u - 1);
+        H.tk = wrl(H.tk, rdl(ck, u), child);
+        H.tp = wrl(H.tp, rdl(cp, u), child);
+    }
+    lds_order();
+    rh_set(H, h >> m, vk, vp);
+}
+
+// pop_heap; the top comes back in (ok, op)
+__device__ __forceinline__ void rh_pop(RegHeap& H, uint32_t& ok, uint32_t& op) {
+    const int lane = threadIdx.x & 63;
+    ok = rdl(H.tk, 1);
+    op = rdl(H.tp, 1);
+    const int len = H.len;
+    H.len = len - 1;
+    if (len == 1) return;
+    const int L = len - 1;  // the heap without its last element x
+    // x (a broadcast LDS read when deep; its latency overlaps the descent)
+    uint32_t xk_l = 0, xp_l = 0;
+    if (len >= 64) {
+        xk_l = H.k[len];
+        xp_l = H.p[len];
+    }
+    // the path: lane t holds (node, key, payload) of the path node at depth t
+    uint32_t pn = 0, pk = 0, pp = 0;
+    int cur = 1, t = 0;
+    while (cur < 32) {  // children below 64: registers
+        const int l = 2 * cur;
+        if (l > L) break;
+        int c = l;
+        uint32_t kc = rdl(H.tk, l);
+        if (l + 1 <= L) {
+            const uint32_t kr = rdl(H.tk, l + 1);
+            if (!(hkey(kr) < hkey(kc))) {
+                c = l + 1;
+                kc = kr;
+            }
+        }
+        cur = c;
+        t++;
+        pn = wrl(pn, (uint32_t)c, t);
+        pk = wrl(pk, kc, t);
+        pp = wrl(pp, rdl(H.tp, c), t);
+    }
+    const int d = subtree_depth(lane);
+    const int kk = lane - ((1 << d) - 2);
+    while (2 * cur <= L) {  // cur >= 32: the 5-level subtree below it from LDS
+        const int j0 = cur;
+        const int idx = (j0 << d) + kk;
+        uint32_t sk = 0, sp = 0;
+        if (lane < 62 && idx <= L) {
+            sk = H.k[idx];
+            sp = H.p[idx];
+        }
+        // right children sit on odd lanes, their left siblings one lane below
+        const uint32_t sib = (uint32_t)__builtin_amdgcn_mov_dpp((int)sk, 0xB1, 0xF, 0xF, false);
+        const unsigned long long left = __ballot((lane & 1) && lane < 62 && hkey(sk) < hkey(sib));
+#pragma unroll
+        for (int e = 1; e <= 5; e++) {
+            const int l = 2 * cur;
+            if (l > L) break;
+            int c = l;
+            if (l + 1 <= L && !((left >> ((1 << e) - 2 + (l + 1 - (j0 << e)))) & 1ull)) c = l + 1;
+            cur = c;
+            t++;
+            const int ln = (1 << e) - 2 + (c - (j0 << e));
+            pn = wrl(pn, (uint32_t)c, t);
+            pk = wrl(pk, rdl(sk, ln), t);
+            pp = wrl(pp, rdl(sp, ln), t);
+        }
+    }
+    const int k = t;
+    uint32_t xk, xp;
+    if (len < 64) {
+        xk = rdl(H.tk, len);
+        xp = rdl(H.tp, len);
+    } else {
+        xk = __builtin_amdgcn_readfirstlane(xk_l);
+        xp = __builtin_amdgcn_readfirstlane(xp_l);
+    }
+    // the path values >= x: depths 1 .. s
+    const int s = __popcll(__ballot(lane >= 1 && lane <= k && hkey(pk) >= hkey(xk)));
+    if (lane >= 1 && lane <= s) {
+        const uint32_t par = pn >> 1;
+        if (par >= 64) {
+            H.k[par] = pk;
+            H.p[par] = pp;
+        }
+    }
+    // parents below 64 are the path nodes of depth <= 5
+    for (int u = 1; u <= min(s, 6); u++) {
+        const int par = (int)(rdl(pn, u) >> 1);
+        H.tk = wrl(H.tk, rdl(pk, u), par);
+        H.tp = wrl(H.tp, rdl(pp, u), par);
+    }
+    lds_order();
+    rh_set(H, s == 0 ? 1 : (int)rdl(pn, s), xk, xp);
+}
+
+__global__ __launch_bounds__(64) void k_octree_heap_r(OrbGlobals g, const int* order, int cap) {
+    extern __shared__ uint32_t heap_lds[];
+    RegHeap H;
+    H.k = heap_lds;
+    H.p = heap_lds + cap;
+    H.tk = 0;
+    H.tp = 0;
+    H.len = 0;
+    const int lane = threadIdx.x;
+    const int li = order[blockIdx.x];
+    const LevelDesc L = g.lv[li];
+    OctLevel* O = g.oct + li;
+    if (O->failed || O->n == 0) {
+        if (lane == 0) O->nout = 0;
+        return;
+    }
+    NodeTabs nt;
+    nt.start = g.pool + O->tab;
+    nt.child = nt.start + O->total;
+    const uint32_t T = (uint32_t)O->total;
+    const uint32_t ncand = (uint32_t)O->n;
+    const int nr = O->n_roots;
+    {
+        uint32_t s_l = 0, c_l = 0;
+        if (lane <= nr) {
+            s_l = nt.st(lane);
+            c_l = nt.ch(lane);
+        }
+        for (int k = 0; k < nr; k++) {
+            const uint2 e = node_entry((uint32_t)k, lane_val(s_l, k), lane_val(s_l, k + 1), lane_val(c_l, k),
+                                       lane_val(c_l, k + 1));
+            rh_push(H, e.x, e.y);
+        }
+    }
+    // node indices stay below N + 4 <= cap - 4: the split loop runs while
+    // len < N and each step pops one node and pushes at most four
+    const int N = min(L.nfeat, cap - 8);
+    while (H.len < N && H.len > 0) {
+        const uint32_t tk = rdl(H.tk, 1), tpay = rdl(H.tp, 1);
+        if (hkey(tk) == 1) break;
+        const uint32_t c0 = tpay, nch = tk & 7u;
+        uint32_t s_l = 0, c_l = 0;
+        // table reads stay inside the level's tables whatever the entry holds
+        const uint32_t ti = min(c0 + (uint32_t)lane, T - 1);
+        if (lane <= (int)nch) {
+            s_l = nt.st(ti);
+            c_l = nt.ch(ti);
+        }
+        uint32_t ok, op;
+        rh_pop(H, ok, op);
+        for (uint32_t k = 0; k < nch; k++) {
+            const uint2 e = node_entry(c0 + k, lane_val(s_l, k), lane_val(s_l, k + 1), lane_val(c_l, k),
+                                       lane_val(c_l, k + 1));
+            rh_push(H, e.x, e.y);
+        }
+    }
+    // drain: the j-th popped node's word goes to node n0 - j, the node the
+    // pop has just freed (a register lane below 64, LDS above)
+    const int n0 = H.len;
+    uint32_t tw = 0;
+    while (H.len > 0) {
+        uint32_t ok, op;
+        rh_pop(H, ok, op);
+        const uint32_t w = (op << 3) | (ok & 7u);
+        const int n = H.len + 1;
+        if (n < 64) tw = wrl(tw, w, n);
+        else if (lane == 0) H.k[n] = w;
+        lds_order();
+    }
+    const int nkeep = min(n0, L.out_cap);
+    if (lane == 0) O->nout = nkeep;
+    const uint32_t* sorted = g.pool + O->sorted;
+    const uint32_t* cand = g.lvl_cand + L.cand_base;
+    for (int b = 0; b < nkeep; b += 64) {
+        const int jj = b + lane;
+        const int n = n0 - jj;
+        const uint32_t wr = (uint32_t)__shfl((int)tw, (n >= 1 && n < 64) ? n : 0, 64);
+        if (jj < nkeep) {
+            const uint32_t nw = n < 64 ? wr : H.k[n];
+            const uint32_t ny = min(nw >> 3, T - 1), nch = max(nw & 7u, 1u);
+            const uint32_t s0 = min(nt.st(ny), ncand), s1 = min(nt.st(min(ny + nch, T - 1)), ncand);
+            uint32_t best = 0, bi = 0;
+            for (uint32_t p = s0; p < s1; p++) {
+                const uint32_t idx = min(sorted[p], ncand - 1);
+                const uint32_t r = cand[idx];
+                if (p == s0 || (r >> 24) > (best >> 24) || ((r >> 24) == (best >> 24) && idx < bi)) {
+                    best = r;
+                    bi = idx;
+                }
+            }
+            g.nodes_out[L.kp_slot + jj] = best;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 __device__ __forceinline__ int wave_sum(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -1631,6 +1915,11 @@ __device__ __forceinline__ float readlane_f(float v, int k) {
 // the next keypoint's loads issued before the current one is reduced.
 // Same arithmetic and outputs as k_keypoints (ORBextractor.cc:75-141,
 // :568-578, :845-896).
+// Octree heap replay: the top of the heap in registers (k_octree_heap_r)
+// or the whole heap in LDS (k_octree_heap); the same pop order.
+constexpr int kHeapVariants = 2;
+inline const char* heap_variant_name(int v) { return v == 1 ? "lds-heap" : "register-top"; }
+
 constexpr int kKpVariants = 4;  // keypoints per wave: 1 (k_keypoints), 8, 16, 32
 constexpr int kKpGroup = 256;   // waves that share slot windows in k_keypoints_k
 inline const char* kp_variant_name(int v) {

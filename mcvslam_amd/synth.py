@@ -238,7 +238,7 @@ class BAProblem:
 
 def ba_problem(seed: int = 30, n_kf: int = 10, n_points: int = 20000, target_obs: int = 150000,
                outlier_frac: float = 0.05, huber_delta: float | None = None,
-               scale_factor: float = 1.2, nlevels: int = 8, perturb: float = 0.0) -> BAProblem:
+               scale_factor: float = 1.2, nlevels: int = 8, perturb: float = 0.0, noise: float = 1.0) -> BAProblem:
     """Keyframes along +x (0.3 m steps, yaw within +-5 deg), points in a
     10x10x5 m box ahead of the rig, every visible (point, keyframe, camera)
     triple observed, subsampled to about target_obs, N(0,1) px noise and
@@ -290,7 +290,7 @@ def ba_problem(seed: int = 30, n_kf: int = 10, n_points: int = 20000, target_obs
         keep = np.sort(rng.choice(len(pid), size=target_obs, replace=False))
         uv, pid, fid, cid = uv[keep], pid[keep], fid[keep], cid[keep]
     M = len(pid)
-    uv = uv + rng.normal(0, 1.0, uv.shape)
+    uv = uv + rng.normal(0, 1.0, uv.shape) * noise
     n_out = int(round(outlier_frac * M))
     out_idx = rng.choice(M, size=n_out, replace=False)
     for i in out_idx:
@@ -349,3 +349,127 @@ def stereo_problem(seed: int = 50, n_kf: int = 6, n_points: int = 4000, target_o
     delta = float(np.float32(math.sqrt(7.815)))
     return StereoProblem(obs, prob.obs_pt[sel], prob.obs_frame[sel], prob.obs_isig[sel], prob.points,
                          prob.frames, np.array([fx, fy, cx, cy, bf, delta], np.float64))
+
+
+@dataclass
+class LocalBAProblem:
+    mono: BAProblem          # mono edges (LEFT without u_right, WIDE), estimates in .points / .frames
+    stereo: StereoProblem    # stereo edges of LEFT observations with u_right (same points and frames)
+    point_fixed: np.ndarray  # (P,) i32
+    true_points: np.ndarray
+    true_frames: np.ndarray
+
+
+def local_ba_problem(seed: int = 60, n_kf: int = 10, n_points: int = 20000, target_obs: int = 150000,
+                     stereo_frac: float = 0.3, outlier_frac: float = 0.05, noise: float = 1.0,
+                     perturb_point: float = 0.05, perturb_pose: float = 0.01, fixed_point_frac: float = 0.1,
+                     scale_factor: float = 1.2, nlevels: int = 8) -> LocalBAProblem:
+    """The problem PoseEstimation::BoundleAdjustment builds
+    (include/PoseEstimation.hpp:35-119): keyframe 0 fixed, LEFT observations
+    with a right-image coordinate as stereo edges (addEdgeStereo, Huber
+    sqrt(7.815)), the other LEFT and all WIDE/RIGHT observations as mono edges
+    (addEdge / addEdgeToBody), a fraction of the points fixed
+    (ThisMapPointBeFix) at their true positions.  The estimates start from
+    the truth perturbed by N(0, perturb_point) m per point coordinate and
+    exp(N(0, perturb_pose)) per free keyframe."""
+    from .autograd import apply_increments
+    prob = ba_problem(seed=seed, n_kf=n_kf, n_points=n_points, target_obs=target_obs, outlier_frac=outlier_frac,
+                      scale_factor=scale_factor, nlevels=nlevels, noise=noise)
+    rng = _rng(seed + 7)
+    true_pts = prob.points.copy()
+    true_frs = prob.frames.copy()
+    left = np.nonzero(prob.obs_cam == 0)[0]
+    sel = np.sort(left[rng.random(len(left)) < stereo_frac])
+    fx, fy, cx, cy = prob.cams[0, :4]
+    bf = float(np.float32(fx * 1.0))
+    z = np.empty(len(sel))
+    for k, i in enumerate(sel):
+        f = true_frs[prob.obs_frame[i]]
+        R = _quat_to_R(f[:4] / np.linalg.norm(f[:4]))
+        z[k] = (R @ true_pts[prob.obs_pt[i]] + f[4:])[2]
+    ur = prob.obs_uv[sel, 0].astype(np.float64) - bf / z + rng.normal(0, 1.0, len(sel)) * noise
+    n_out = int(round(outlier_frac * len(sel)))
+    if n_out:
+        ur[rng.choice(len(sel), size=n_out, replace=False)] = rng.uniform(0, NARROW_W, n_out)
+    obs = np.concatenate([prob.obs_uv[sel].astype(np.float64), ur[:, None]], 1).astype(np.float32)
+    delta3 = float(np.float32(math.sqrt(7.815)))
+    keep = np.ones(len(prob.obs_pt), bool)
+    keep[sel] = False
+    mono = BAProblem(prob.obs_uv[keep], prob.obs_pt[keep], prob.obs_frame[keep], prob.obs_cam[keep],
+                     prob.obs_level[keep], prob.obs_isig[keep], true_pts.copy(), true_frs.copy(), prob.cams,
+                     prob.frame_fixed)
+    P = len(true_pts)
+    pfix = (rng.random(P) < fixed_point_frac).astype(np.int32)
+    dpt = rng.normal(0, perturb_point, true_pts.shape) * (1 - pfix)[:, None]
+    dpose = rng.normal(0, perturb_pose, (n_kf, 6)) * (1 - prob.frame_fixed)[:, None]
+    pts, frs = apply_increments(true_pts, true_frs, dpose, dpt)
+    mono.points, mono.frames = pts, frs
+    stereo = StereoProblem(obs, prob.obs_pt[sel], prob.obs_frame[sel], prob.obs_isig[sel], pts, frs,
+                           np.array([fx, fy, cx, cy, bf, delta3], np.float64))
+    return LocalBAProblem(mono, stereo, pfix, true_pts, true_frs)
+
+
+@dataclass
+class Vocabulary:
+    """A DBoW3-style vocabulary tree (node 0 the root, children in CSR, in
+    breadth-first numbering as DBoW3's HKmeans builds it)."""
+    k: int
+    L: int
+    desc: np.ndarray       # (n, 32) u8 node descriptors (row 0 unused)
+    child_ptr: np.ndarray  # (n + 1,) i32
+    child_ids: np.ndarray  # (n - 1,) i32
+    word_id: np.ndarray    # (n,) i32, -1 for inner nodes
+    weight: np.ndarray     # (n,) f64 (idf-like; 0 = stopped word)
+    weighting: int = 0     # DBoW3 WeightingType: 0 TF_IDF, 1 TF, 2 IDF, 3 BINARY
+    scoring: int = 0       # ScoringType: 0 L1, 1 L2, 2 chi-square, 3 KL, 4 Bhattacharyya, 5 dot product
+
+    @property
+    def leaves(self):
+        return np.nonzero(self.word_id >= 0)[0]
+
+
+def bow_vocabulary(seed: int = 80, k: int = 8, L: int = 5, flip: int = 24, stop_frac: float = 0.02,
+                   weighting: int = 0, scoring: int = 0) -> Vocabulary:
+    """k-ary tree of depth L: level-1 descriptors random, every child its
+    parent's descriptor with `flip` random bits flipped (a cluster hierarchy);
+    leaves get word ids in node order and weights U(0.5, 4) (a fraction 0)."""
+    rng = _rng(seed)
+    n = sum(k ** l for l in range(L + 1))
+    desc = np.zeros((n, 32), np.uint8)
+    child_ptr = np.zeros(n + 1, np.int32)
+    child_ids = []
+    level_start = [0]
+    nxt = 1
+    for l in range(L):
+        first = level_start[-1]
+        count = k ** l
+        level_start.append(nxt)
+        for i in range(first, first + count):
+            child_ptr[i] = len(child_ids)
+            for c in range(k):
+                child_ids.append(nxt)
+                if l == 0:
+                    desc[nxt] = rng.integers(0, 256, 32, dtype=np.uint8)
+                else:
+                    bits = rng.choice(256, flip, replace=False)
+                    mask = np.zeros(256, np.uint8)
+                    mask[bits] = 1
+                    desc[nxt] = desc[i] ^ np.packbits(mask, bitorder="little")
+                nxt += 1
+    for i in range(level_start[-1], n + 1):
+        child_ptr[i] = len(child_ids)
+    word_id = -np.ones(n, np.int32)
+    leaves = np.arange(level_start[-1], n)
+    word_id[leaves] = np.arange(len(leaves))
+    weight = np.zeros(n)
+    weight[leaves] = rng.uniform(0.5, 4.0, len(leaves))
+    weight[leaves[rng.random(len(leaves)) < stop_frac]] = 0.0
+    return Vocabulary(k, L, desc, child_ptr, np.array(child_ids, np.int32), word_id, weight, weighting, scoring)
+
+
+def bow_features(voc: Vocabulary, seed: int, n: int, max_flips: int = 30):
+    """n descriptors: noisy copies of random leaves (0..max_flips bits)."""
+    rng = _rng(seed)
+    leaves = voc.leaves
+    src = voc.desc[leaves[rng.integers(0, len(leaves), n)]].copy()
+    return flip_bits(rng, src, max_flips)

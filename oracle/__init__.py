@@ -45,6 +45,12 @@ _SIG = {
     "orc_distinctive": (None, [P, P, I, P]),
     "orc_filter_orientation": (I, [P, I, P, P, P]),
     "orc_ba_schur": (I, [P, P, P, P, P, P, P, I, I, I, P, D, P, P, P, P, P, P]),
+    "orc_se3_oplus": (None, [P, P]),
+    "orc_bow_transform": (None, [P, P, P, P, P, I, I, P, I, P, P, P]),
+    "orc_bow_vectors": (I, [P, P, P, I, I, I, P, P, P, P, P, P]),
+    "orc_bow_match": (I, [P, I, P, P, P, P, I, P, P, P, P]),
+    "orc_ba_lm": (I, [I, P, P, P, P, P, P, I, P, P, P, P, P, P, I, I, I, P, P, P, I, P, P, I, D, I, P, P]),
+    "orc_ba_robust_chi2": (D, [I, P, P, P, P, P, P, I, P, P, P, P, P, P, I, I, I, P, I, P, P]),
 }
 
 _lib = None
@@ -343,3 +349,105 @@ def ba_schur(e, w, Jpose, Jpoint, obs_pt, obs_frame, obs_isig, P, F, frame_fixed
                            _p(np.ascontiguousarray(frame_fixed, np.int32)), lam, _p(S), _p(bs), _p(Hll), _p(bl),
                            _p(Hpp), _p(bp))
     return dict(S=S, bs=bs, Hll=Hll.reshape(P, 3, 3), bl=bl, Hpp=Hpp, bp=bp, ok=r >= 0)
+
+
+def se3_oplus(frame, xi):
+    """VertexSE3Expmap::oplusImpl: exp(xi) * T for one frame (qx qy qz qw t)."""
+    f = np.ascontiguousarray(frame, np.float64).copy()
+    lib().orc_se3_oplus(_p(f), _p(np.ascontiguousarray(xi, np.float64)))
+    return f
+
+
+def _lm_args(prob, stereo, active, sactive):
+    M = len(prob.obs_pt)
+    a = [np.ascontiguousarray(prob.obs_uv, np.float32), np.ascontiguousarray(prob.obs_pt, np.int32),
+         np.ascontiguousarray(prob.obs_frame, np.int32), np.ascontiguousarray(prob.obs_cam, np.int32),
+         np.ascontiguousarray(prob.obs_isig, np.float32),
+         None if active is None else np.ascontiguousarray(active, np.uint8)]
+    if stereo is not None:
+        sa = [np.ascontiguousarray(stereo.obs, np.float32), np.ascontiguousarray(stereo.obs_pt, np.int32),
+              np.ascontiguousarray(stereo.obs_frame, np.int32), np.ascontiguousarray(stereo.obs_isig, np.float32),
+              None if sactive is None else np.ascontiguousarray(sactive, np.uint8),
+              np.ascontiguousarray(stereo.cam, np.float64)]
+        Ms = len(stereo.obs_pt)
+    else:
+        sa = [None] * 5 + [np.zeros(6)]
+        Ms = 0
+    keep = a + sa
+    args = [M] + [_p(x) for x in a] + [Ms] + [_p(x) for x in sa]
+    return args, keep
+
+
+def ba_lm(prob, stereo=None, iterations=10, tau=1e-5, max_trials=10, robust=True, active=None, sactive=None,
+          point_fixed=None, points=None, frames=None):
+    """FP64 Levenberg-Marquardt (g2o's OptimizationAlgorithmLevenberg with the
+    Schur complement of the points): mono edges of `prob`, stereo edges of
+    `stereo` (same points and frames), prob.frame_fixed, point_fixed.
+    Returns (points, frames, stats [iterations x 4: chi2 before, chi2 after,
+    lambda, trials], iterations run, first step [x_p; x_l])."""
+    args, keep = _lm_args(prob, stereo, active, sactive)
+    P, F, C = len(prob.points), len(prob.frames), len(prob.cams)
+    pts = np.ascontiguousarray(prob.points if points is None else points, np.float64).copy()
+    frs = np.ascontiguousarray(prob.frames if frames is None else frames, np.float64).copy()
+    ff = np.ascontiguousarray(prob.frame_fixed, np.int32)
+    pf = None if point_fixed is None else np.ascontiguousarray(point_fixed, np.int32)
+    stats = np.zeros((iterations, 4))
+    Fp = int((ff == 0).sum())
+    x = np.zeros(6 * Fp + 3 * P)
+    n = lib().orc_ba_lm(*args, P, F, C, _p(np.ascontiguousarray(prob.cams, np.float64)), _p(ff), _p(pf),
+                        int(robust), _p(pts), _p(frs), iterations, tau, max_trials, _p(stats), _p(x))
+    del keep
+    return pts, frs, stats[:n], n, x
+
+
+def ba_robust_chi2(prob, stereo=None, points=None, frames=None, robust=True, active=None, sactive=None):
+    """activeRobustChi2 (sum of Huber rho(chi2) over active edges) in FP64."""
+    args, keep = _lm_args(prob, stereo, active, sactive)
+    pts = np.ascontiguousarray(prob.points if points is None else points, np.float64)
+    frs = np.ascontiguousarray(prob.frames if frames is None else frames, np.float64)
+    r = lib().orc_ba_robust_chi2(*args, len(prob.points), len(prob.frames), len(prob.cams),
+                                 _p(np.ascontiguousarray(prob.cams, np.float64)), int(robust), _p(pts), _p(frs))
+    del keep
+    return r
+
+
+def bow_transform(voc, feats, levelsup=4):
+    """DBoW3 Vocabulary::transform of every feature: (word id, weight, node id
+    at level L - levelsup)."""
+    f = np.ascontiguousarray(feats, np.uint8).reshape(-1, 32)
+    n = len(f)
+    w = np.zeros(n, np.int32)
+    wt = np.zeros(n)
+    nid = np.zeros(n, np.int32)
+    lib().orc_bow_transform(_p(voc.desc), _p(voc.child_ptr), _p(voc.child_ids), _p(voc.word_id), _p(voc.weight),
+                            voc.L, levelsup, _p(f), n, _p(w), _p(wt), _p(nid))
+    return w, wt, nid
+
+
+def bow_vectors(word, weight, nid, weighting=0, scoring=0):
+    """BowVector (words, values) and FeatureVector (nodes, ptr, features) of
+    the per-feature transforms."""
+    n = len(word)
+    bw = np.zeros(max(n, 1), np.int32)
+    bv = np.zeros(max(n, 1))
+    fn = np.zeros(max(n, 1), np.int32)
+    fp = np.zeros(n + 1, np.int32)
+    ff = np.zeros(max(n, 1), np.int32)
+    m = ctypes.c_int32()
+    k = lib().orc_bow_vectors(_p(np.ascontiguousarray(word, np.int32)), _p(np.ascontiguousarray(weight, np.float64)),
+                              _p(np.ascontiguousarray(nid, np.int32)), n, weighting, scoring, _p(bw), _p(bv), _p(fn),
+                              _p(fp), _p(ff), ctypes.byref(m))
+    nn = m.value
+    return (bw[:k].copy(), bv[:k].copy()), (fn[:nn].copy(), fp[:nn + 1].copy(), ff[:fp[nn]].copy())
+
+
+def bow_match(d1, fv1, d2, fv2):
+    """Matcher::DBowMatch: rows (query, train0, d0, train1, d1)."""
+    d1 = np.ascontiguousarray(d1, np.uint8)
+    d2 = np.ascontiguousarray(d2, np.uint8)
+    n1 = [np.ascontiguousarray(a, np.int32) for a in fv1]
+    n2 = [np.ascontiguousarray(a, np.int32) for a in fv2]
+    out = np.zeros((max(len(n1[2]), 1), 5), np.int32)
+    r = lib().orc_bow_match(_p(d1), len(n1[0]), _p(n1[0]), _p(n1[1]), _p(n1[2]), _p(d2), len(n2[0]), _p(n2[0]),
+                            _p(n2[1]), _p(n2[2]), _p(out))
+    return out[:r].copy()

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <queue>
 #include <vector>
@@ -1407,6 +1408,567 @@ int orc_ba_schur(const double* e, const double* w, const double* Jpose, const do
     std::memcpy(S, Ssum.data(), sizeof(double) * n * n);
     std::memcpy(bs, b.data(), sizeof(double) * n);
     return Fp;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Levenberg-Marquardt over the whole BA problem in FP64, the oracle of the
+// GPU optimizer (mcv_ba_optimize).  Restates:
+//   OptimizationAlgorithmLevenberg::solve, computeLambdaInit, computeScale
+//     (ThirdParty/g2o/g2o/core/optimization_algorithm_levenberg.cpp:56-159;
+//     tau 1e-5, good-step scale clamped to [1/3, 2/3], 10 trials)
+//   SparseOptimizer::computeActiveErrors / activeRobustChi2
+//     (core/sparse_optimizer.cpp:63-102; Huber rho, robust_kernel_impl.cpp:65-78)
+//   BaseBinaryEdge::constructQuadraticForm (core/base_binary_edge.hpp:64-136):
+//     H += J^T w Omega J, b += -J^T w Omega e, blocks of fixed vertices skipped
+//   BlockSolver::solve (core/block_solver.hpp:315-449): Schur complement of
+//     the points with H_ll + lambda I, dense Cholesky of the reduced system,
+//     x_l = D^-1 (b_l - H_lp x_p)
+//   VertexSE3Expmap::oplusImpl (types/sba/types_six_dof_expmap.h:98-101):
+//     T <- exp(x) * T, SE3Quat::exp (types/slam3d/se3quat.h:218-257) with
+//     Eigen's rotation-matrix-to-quaternion conversion; VertexSBAPointXYZ:
+//     P <- P + x.
+// Edges: mono EdgeSE3XYZ2Camera (edge_linearize) and stereo
+// EdgeStereoSE3ProjectXYZ (stereo_edge_linearize); inactive edges (level 1)
+// take no part; robust = 0 runs without the Huber kernel.
+namespace orc {
+
+struct LMEdges {
+    int M;
+    const float* uv;
+    const int32_t* pt;
+    const int32_t* fr;
+    const int32_t* cam;
+    const float* isig;
+    const uint8_t* act;
+    int Ms;
+    const float* sobs;
+    const int32_t* spt;
+    const int32_t* sfr;
+    const float* sisig;
+    const uint8_t* sact;
+    const double* scam;  // fx fy cx cy bf delta
+    int P, F, C;
+    const double* cams;  // C x 12
+    const int32_t* ffix;
+    const int32_t* pfix;
+    int robust;
+};
+
+static double huber_rho(double chi2, double delta, bool robust) {
+    if (!robust) return chi2;
+    const double d2 = delta * delta;
+    return chi2 <= d2 ? chi2 : 2.0 * std::sqrt(chi2) * delta - d2;
+}
+
+// one linearized edge: r rows (2 mono, 3 stereo)
+struct LinEdge {
+    int rows, pt, fr;
+    double e[3], chi2, w, isig, Jp[18], Jx[9];
+};
+
+static void lm_linearize(const LMEdges& E, const double* points, const double* frames, std::vector<LinEdge>& out) {
+    out.clear();
+    const double inf = INFINITY;
+    for (int i = 0; i < E.M; i++) {
+        if (E.act && !E.act[i]) continue;
+        LinEdge L{};
+        L.rows = 2;
+        L.pt = E.pt[i];
+        L.fr = E.fr[i];
+        L.isig = (double)E.isig[i];
+        const double* f = frames + 7 * L.fr;
+        SE3 Tbw{Quat{f[0], f[1], f[2], f[3]}, {f[4], f[5], f[6]}};
+        const double* c = E.cams + 12 * E.cam[i];
+        Cam cam{c[0], c[1], c[2], c[3], SE3{Quat{c[4], c[5], c[6], c[7]}, {c[8], c[9], c[10]}}, E.robust ? c[11] : inf};
+        double Jp[12], Jx[6];
+        edge_linearize(Tbw, cam, points + 3 * L.pt, (double)E.uv[2 * i], (double)E.uv[2 * i + 1], L.isig, L.e,
+                       &L.chi2, &L.w, Jp, Jx);
+        for (int k = 0; k < 12; k++) L.Jp[k] = Jp[k];
+        for (int k = 0; k < 6; k++) L.Jx[k] = Jx[k];
+        out.push_back(L);
+    }
+    for (int j = 0; j < E.Ms; j++) {
+        if (E.sact && !E.sact[j]) continue;
+        LinEdge L{};
+        L.rows = 3;
+        L.pt = E.spt[j];
+        L.fr = E.sfr[j];
+        L.isig = (double)E.sisig[j];
+        const double* f = frames + 7 * L.fr;
+        SE3 Tcw{Quat{f[0], f[1], f[2], f[3]}, {f[4], f[5], f[6]}};
+        qnormalize_pos(Tcw.r);
+        stereo_edge_linearize(Tcw, E.scam[0], E.scam[1], E.scam[2], E.scam[3], E.scam[4], E.robust ? E.scam[5] : inf,
+                              points + 3 * L.pt, (double)E.sobs[3 * j], (double)E.sobs[3 * j + 1],
+                              (double)E.sobs[3 * j + 2], L.isig, L.e, &L.chi2, &L.w, L.Jp, L.Jx);
+        out.push_back(L);
+    }
+}
+
+static double lm_robust_chi2(const LMEdges& E, const std::vector<LinEdge>& L) {
+    double s = 0;
+    size_t k = 0;
+    for (int i = 0; i < E.M; i++) {
+        if (E.act && !E.act[i]) continue;
+        s += huber_rho(L[k++].chi2, E.cams[12 * E.cam[i] + 11], E.robust);
+    }
+    for (int j = 0; j < E.Ms; j++) {
+        if (E.sact && !E.sact[j]) continue;
+        s += huber_rho(L[k++].chi2, E.scam[5], E.robust);
+    }
+    return s;
+}
+
+static void inv3(const double* D, double* Di) {
+    const double det = D[0] * (D[4] * D[8] - D[5] * D[7]) - D[1] * (D[3] * D[8] - D[5] * D[6]) +
+                       D[2] * (D[3] * D[7] - D[4] * D[6]);
+    Di[0] = (D[4] * D[8] - D[5] * D[7]) / det; Di[1] = (D[2] * D[7] - D[1] * D[8]) / det;
+    Di[2] = (D[1] * D[5] - D[2] * D[4]) / det; Di[3] = (D[5] * D[6] - D[3] * D[8]) / det;
+    Di[4] = (D[0] * D[8] - D[2] * D[6]) / det; Di[5] = (D[2] * D[3] - D[0] * D[5]) / det;
+    Di[6] = (D[3] * D[7] - D[4] * D[6]) / det; Di[7] = (D[1] * D[6] - D[0] * D[7]) / det;
+    Di[8] = (D[0] * D[4] - D[1] * D[3]) / det;
+}
+
+// dense Cholesky solve; false when A is not positive definite
+static bool llt_solve(int n, std::vector<double> A, const std::vector<double>& b, std::vector<double>& x) {
+    for (int j = 0; j < n; j++) {
+        double d = A[(size_t)j * n + j];
+        for (int k = 0; k < j; k++) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+        if (!(d > 0)) return false;
+        const double l = std::sqrt(d);
+        A[(size_t)j * n + j] = l;
+        for (int i = j + 1; i < n; i++) {
+            double v = A[(size_t)i * n + j];
+            for (int k = 0; k < j; k++) v -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+            A[(size_t)i * n + j] = v / l;
+        }
+    }
+    x.assign(b.begin(), b.end());
+    for (int i = 0; i < n; i++) {
+        double v = x[i];
+        for (int k = 0; k < i; k++) v -= A[(size_t)i * n + k] * x[k];
+        x[i] = v / A[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double v = x[i];
+        for (int k = i + 1; k < n; k++) v -= A[(size_t)k * n + i] * x[k];
+        x[i] = v / A[(size_t)i * n + i];
+    }
+    return true;
+}
+
+// Eigen::Quaternion(const Matrix3&): trace branch, else the largest diagonal
+static Quat quat_from_R(const double* m) {
+    Quat q{};
+    double t = m[0] + m[4] + m[8];
+    if (t > 0) {
+        t = std::sqrt(t + 1.0);
+        q.w = 0.5 * t;
+        t = 0.5 / t;
+        q.x = (m[7] - m[5]) * t;
+        q.y = (m[2] - m[6]) * t;
+        q.z = (m[3] - m[1]) * t;
+    } else {
+        int i = 0;
+        if (m[4] > m[0]) i = 1;
+        if (m[8] > m[3 * i + i]) i = 2;
+        const int j = (i + 1) % 3, k = (j + 1) % 3;
+        double c[3];
+        t = std::sqrt(m[3 * i + i] - m[3 * j + j] - m[3 * k + k] + 1.0);
+        c[i] = 0.5 * t;
+        t = 0.5 / t;
+        q.w = (m[3 * k + j] - m[3 * j + k]) * t;
+        c[j] = (m[3 * j + i] + m[3 * i + j]) * t;
+        c[k] = (m[3 * k + i] + m[3 * i + k]) * t;
+        q.x = c[0]; q.y = c[1]; q.z = c[2];
+    }
+    return q;
+}
+
+// SE3Quat::exp of (omega, upsilon)
+static SE3 se3_exp(const double* xi) {
+    const double w[3] = {xi[0], xi[1], xi[2]};
+    const double th = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    const double W[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+    double W2[9];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            double s = 0;
+            for (int k = 0; k < 3; k++) s += W[3 * r + k] * W[3 * k + c];
+            W2[3 * r + c] = s;
+        }
+    double R[9], V[9];
+    double a, bb, cc;  // R = I + a W + bb W^2, V = I + bb W + cc W^2
+    if (th < 0.00001) {
+        a = 1.0; bb = 0.5; cc = 1.0 / 6.0;
+        for (int k = 0; k < 9; k++) {
+            const double I = (k % 4 == 0) ? 1.0 : 0.0;
+            R[k] = I + W[k] + 0.5 * W2[k];
+            V[k] = I + 0.5 * W[k] + cc * W2[k];
+        }
+    } else {
+        a = std::sin(th) / th;
+        bb = (1 - std::cos(th)) / (th * th);
+        cc = (th - std::sin(th)) / std::pow(th, 3);
+        for (int k = 0; k < 9; k++) {
+            const double I = (k % 4 == 0) ? 1.0 : 0.0;
+            R[k] = I + a * W[k] + bb * W2[k];
+            V[k] = I + bb * W[k] + cc * W2[k];
+        }
+    }
+    SE3 T;
+    T.r = quat_from_R(R);
+    qnormalize_pos(T.r);
+    for (int r = 0; r < 3; r++) T.t[r] = V[3 * r] * xi[3] + V[3 * r + 1] * xi[4] + V[3 * r + 2] * xi[5];
+    return T;
+}
+
+}  // namespace orc
+
+extern "C" {
+
+// exp(xi) * T for one frame (qx qy qz qw tx ty tz), in place
+void orc_se3_oplus(double* frame, const double* xi) {
+    SE3 T{Quat{frame[0], frame[1], frame[2], frame[3]}, {frame[4], frame[5], frame[6]}};
+    const SE3 r = se3mul(se3_exp(xi), T);
+    frame[0] = r.r.x; frame[1] = r.r.y; frame[2] = r.r.z; frame[3] = r.r.w;
+    frame[4] = r.t[0]; frame[5] = r.t[1]; frame[6] = r.t[2];
+}
+
+// One LM run of `iterations` iterations (SparseOptimizer::optimize(n) after
+// initializeOptimization): points (P x 3) and frames (F x 7) updated in
+// place.  stats[4 k ..]: chi2 before iteration k, chi2 after it, lambda
+// after it, trials.  Returns the iterations run (g2o stops after a
+// Terminate).  x_out (optional): the first iteration's first trial step
+// [x_p (6 Fp, free frames in order); x_l (3 P, zero for fixed points)].
+int orc_ba_lm(int M, const float* uv, const int32_t* pt, const int32_t* fr, const int32_t* cam, const float* isig,
+              const uint8_t* act, int Ms, const float* sobs, const int32_t* spt, const int32_t* sfr,
+              const float* sisig, const uint8_t* sact, const double* scam, int P, int F, int C, const double* cams,
+              const int32_t* ffix, const int32_t* pfix, int robust, double* points, double* frames, int iterations,
+              double tau, int max_trials, double* stats, double* x_out) {
+    LMEdges E{M, uv, pt, fr, cam, isig, act, Ms, sobs, spt, sfr, sisig, sact, scam, P, F, C, cams, ffix, pfix, robust};
+    std::vector<int> fidx(F, -1);
+    int Fp = 0;
+    for (int f = 0; f < F; f++)
+        if (!ffix || !ffix[f]) fidx[f] = Fp++;
+    const int n = 6 * Fp;
+    auto pfree = [&](int p) { return !pfix || !pfix[p]; };
+    std::vector<LinEdge> L;
+    double lambda = 0, ni = 2;
+    int done = 0;
+    for (int it = 0; it < iterations; it++) {
+        lm_linearize(E, points, frames, L);
+        double currentChi = lm_robust_chi2(E, L);
+        if (stats) stats[4 * it] = currentChi;
+        // buildSystem
+        std::vector<double> Hpp((size_t)Fp * 36, 0.0), bp(n, 0.0), Hll((size_t)P * 9, 0.0), bl((size_t)P * 3, 0.0);
+        std::vector<std::vector<std::pair<int, std::array<double, 18>>>> Hpl(P);
+        for (const LinEdge& e : L) {
+            const int r = e.rows;
+            const double W = e.w * e.isig;
+            const int fi = fidx[e.fr];
+            const bool pf = pfree(e.pt);
+            double om[3];
+            for (int k = 0; k < r; k++) om[k] = -e.isig * e.e[k] * e.w;
+            if (fi >= 0) {
+                for (int a = 0; a < 6; a++) {
+                    for (int k = 0; k < r; k++) bp[6 * fi + a] += e.Jp[6 * k + a] * om[k];
+                    for (int b = 0; b < 6; b++) {
+                        double s = 0;
+                        for (int k = 0; k < r; k++) s += e.Jp[6 * k + a] * e.Jp[6 * k + b];
+                        Hpp[36 * fi + 6 * a + b] += W * s;
+                    }
+                }
+            }
+            if (pf) {
+                const int p = e.pt;
+                for (int a = 0; a < 3; a++) {
+                    for (int k = 0; k < r; k++) bl[3 * p + a] += e.Jx[3 * k + a] * om[k];
+                    for (int b = 0; b < 3; b++) {
+                        double s = 0;
+                        for (int k = 0; k < r; k++) s += e.Jx[3 * k + a] * e.Jx[3 * k + b];
+                        Hll[9 * p + 3 * a + b] += W * s;
+                    }
+                }
+            }
+            if (fi >= 0 && pf) {
+                std::array<double, 18> blk;
+                for (int a = 0; a < 6; a++)
+                    for (int b = 0; b < 3; b++) {
+                        double s = 0;
+                        for (int k = 0; k < r; k++) s += e.Jp[6 * k + a] * e.Jx[3 * k + b];
+                        blk[3 * a + b] = W * s;
+                    }
+                bool merged = false;
+                for (auto& q : Hpl[e.pt])
+                    if (q.first == fi) {
+                        for (int k = 0; k < 18; k++) q.second[k] += blk[k];
+                        merged = true;
+                    }
+                if (!merged) Hpl[e.pt].push_back({fi, blk});
+            }
+        }
+        if (it == 0) {
+            double mx = 0;
+            for (int f = 0; f < Fp; f++)
+                for (int a = 0; a < 6; a++) mx = std::max(mx, std::fabs(Hpp[36 * f + 7 * a]));
+            for (int p = 0; p < P; p++)
+                if (pfree(p))
+                    for (int a = 0; a < 3; a++) mx = std::max(mx, std::fabs(Hll[9 * p + 4 * a]));
+            lambda = tau * mx;
+            ni = 2;
+        }
+        double rho = 0;
+        int qmax = 0;
+        do {
+            const std::vector<double> pts0(points, points + 3 * (size_t)P), frs0(frames, frames + 7 * (size_t)F);
+            // Schur complement with lambda on both diagonals
+            std::vector<double> S((size_t)n * n, 0.0), bs(bp);
+            for (int f = 0; f < Fp; f++)
+                for (int a = 0; a < 6; a++)
+                    for (int b = 0; b < 6; b++)
+                        S[(size_t)(6 * f + a) * n + 6 * f + b] = Hpp[36 * f + 6 * a + b] + (a == b ? lambda : 0.0);
+            std::vector<double> Dinv((size_t)P * 9, 0.0);
+            for (int p = 0; p < P; p++) {
+                if (!pfree(p)) continue;
+                double D[9];
+                for (int k = 0; k < 9; k++) D[k] = Hll[9 * p + k];
+                D[0] += lambda; D[4] += lambda; D[8] += lambda;
+                double* Di = &Dinv[9 * (size_t)p];
+                inv3(D, Di);
+                double db[3];
+                for (int a = 0; a < 3; a++) db[a] = Di[3 * a] * bl[3 * p] + Di[3 * a + 1] * bl[3 * p + 1] + Di[3 * a + 2] * bl[3 * p + 2];
+                for (auto& qi : Hpl[p]) {
+                    double BD[18];
+                    for (int a = 0; a < 6; a++)
+                        for (int c = 0; c < 3; c++)
+                            BD[3 * a + c] = qi.second[3 * a] * Di[c] + qi.second[3 * a + 1] * Di[3 + c] + qi.second[3 * a + 2] * Di[6 + c];
+                    for (int a = 0; a < 6; a++)
+                        bs[6 * qi.first + a] -= qi.second[3 * a] * db[0] + qi.second[3 * a + 1] * db[1] + qi.second[3 * a + 2] * db[2];
+                    for (auto& qj : Hpl[p])
+                        for (int a = 0; a < 6; a++)
+                            for (int c = 0; c < 6; c++)
+                                S[(size_t)(6 * qi.first + a) * n + 6 * qj.first + c] -=
+                                    BD[3 * a] * qj.second[3 * c] + BD[3 * a + 1] * qj.second[3 * c + 1] + BD[3 * a + 2] * qj.second[3 * c + 2];
+                }
+            }
+            std::vector<double> xp;
+            const bool ok2 = llt_solve(n, S, bs, xp);
+            std::vector<double> xl((size_t)P * 3, 0.0);
+            if (ok2) {
+                for (int p = 0; p < P; p++) {
+                    if (!pfree(p)) continue;
+                    double c[3] = {bl[3 * p], bl[3 * p + 1], bl[3 * p + 2]};
+                    for (auto& q : Hpl[p])
+                        for (int a = 0; a < 3; a++)
+                            for (int k = 0; k < 6; k++) c[a] -= q.second[3 * k + a] * xp[6 * q.first + k];
+                    const double* Di = &Dinv[9 * (size_t)p];
+                    for (int a = 0; a < 3; a++) xl[3 * p + a] = Di[3 * a] * c[0] + Di[3 * a + 1] * c[1] + Di[3 * a + 2] * c[2];
+                }
+                if (x_out && it == 0 && qmax == 0) {
+                    for (int k = 0; k < n; k++) x_out[k] = xp[k];
+                    for (size_t k = 0; k < xl.size(); k++) x_out[n + k] = xl[k];
+                }
+                for (int f = 0; f < F; f++)
+                    if (fidx[f] >= 0) orc_se3_oplus(frames + 7 * f, &xp[6 * fidx[f]]);
+                for (int p = 0; p < P; p++)
+                    if (pfree(p))
+                        for (int a = 0; a < 3; a++) points[3 * p + a] += xl[3 * p + a];
+            }
+            std::vector<LinEdge> L2;
+            lm_linearize(E, points, frames, L2);
+            double tempChi = lm_robust_chi2(E, L2);
+            if (!ok2) tempChi = DBL_MAX;
+            double scale = 0;
+            if (ok2) {
+                for (int k = 0; k < n; k++) scale += xp[k] * (lambda * xp[k] + bp[k]);
+                for (int p = 0; p < P; p++)
+                    if (pfree(p))
+                        for (int a = 0; a < 3; a++) scale += xl[3 * p + a] * (lambda * xl[3 * p + a] + bl[3 * p + a]);
+            }
+            scale += 1e-3;
+            rho = (currentChi - tempChi) / scale;
+            if (rho > 0 && std::isfinite(tempChi)) {
+                double alpha = 1. - std::pow(2 * rho - 1, 3);
+                alpha = std::min(alpha, 2. / 3.);
+                lambda *= std::max(1. / 3., alpha);
+                ni = 2;
+                currentChi = tempChi;
+            } else {
+                lambda *= ni;
+                ni *= 2;
+                std::copy(pts0.begin(), pts0.end(), points);
+                std::copy(frs0.begin(), frs0.end(), frames);
+                if (!std::isfinite(lambda)) break;
+            }
+            qmax++;
+        } while (rho < 0 && qmax < max_trials);
+        if (stats) {
+            stats[4 * it + 1] = currentChi;
+            stats[4 * it + 2] = lambda;
+            stats[4 * it + 3] = qmax;
+        }
+        done++;
+        if (qmax == max_trials || rho == 0 || !std::isfinite(lambda)) break;
+    }
+    return done;
+}
+
+// Sum over active edges of the Huber rho of chi2 at the given state
+// (activeRobustChi2 after computeActiveErrors), FP64.
+double orc_ba_robust_chi2(int M, const float* uv, const int32_t* pt, const int32_t* fr, const int32_t* cam,
+                          const float* isig, const uint8_t* act, int Ms, const float* sobs, const int32_t* spt,
+                          const int32_t* sfr, const float* sisig, const uint8_t* sact, const double* scam, int P, int F,
+                          int C, const double* cams, int robust, const double* points, const double* frames) {
+    LMEdges E{M, uv, pt, fr, cam, isig, act, Ms, sobs, spt, sfr, sisig, sact, scam, P, F, C, cams, nullptr, nullptr,
+              robust};
+    std::vector<LinEdge> L;
+    lm_linearize(E, points, frames, L);
+    return lm_robust_chi2(E, L);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// DBoW3 (modules/DBow3/src, vendored by the reference).  Vocabulary tree:
+// node 0 is the root; children of node i are child_ids[child_ptr[i] ..
+// child_ptr[i + 1]) in DBoW3's children order; a node without children is a
+// leaf with word_id[i] and weight[i].
+//   Vocabulary::transform(feature, word_id, weight, nid, levelsup)
+//   (Vocabulary.cpp:641-672): from the root, the child at the least Hamming
+//   distance (DescManip::distance, DescManip.cpp:92-119; the first of equal
+//   distances, strict '<'), to a leaf; nid = the node reached at level
+//   L - levelsup (the root when that level is <= 0; left 0 when the path ends
+//   above it).
+void orc_bow_transform(const uint8_t* node_desc, const int32_t* child_ptr, const int32_t* child_ids,
+                       const int32_t* word_id, const double* weight, int L, int levelsup, const uint8_t* feats, int n,
+                       int32_t* out_word, double* out_weight, int32_t* out_nid) {
+    const int nid_level = L - levelsup;
+    for (int f = 0; f < n; f++) {
+        const uint8_t* x = feats + 32 * (size_t)f;
+        int nid = 0;
+        int final_id = 0, level = 0;
+        do {
+            ++level;
+            double best_d = DBL_MAX;
+            const int c0 = child_ptr[final_id], c1 = child_ptr[final_id + 1];  // the node's children, fixed
+            for (int c = c0; c < c1; c++) {
+                const int id = child_ids[c];
+                const double d = (double)hamming(x, node_desc + 32 * (size_t)id);
+                if (d < best_d) {
+                    best_d = d;
+                    final_id = id;
+                }
+            }
+            if (level == nid_level) nid = final_id;
+        } while (child_ptr[final_id + 1] > child_ptr[final_id]);
+        out_word[f] = word_id[final_id];
+        out_weight[f] = weight[final_id];
+        out_nid[f] = nid;
+    }
+}
+
+// Vocabulary::transform(features, BowVector, FeatureVector, levelsup)
+// (Vocabulary.cpp:572-629) from the per-feature transforms: weighting 0
+// TF_IDF, 1 TF (BowVector::addWeight, then / v.size() when the scoring does
+// not normalise), 2 IDF, 3 BINARY (addIfNotExist); words with weight <= 0
+// are stopped; FeatureVector::addFeature(nid, i) (FeatureVector.cpp:31-45).
+// scoring 0 L1, 1 L2, 2 chi-square, 3 KL, 4 Bhattacharyya (normalised: L2
+// for scoring 1, else L1; BowVector::normalize, BowVector.cpp:62-84), 5 dot
+// product (not normalised).  Outputs sorted by word / node id; returns the
+// word count, *n_nodes the node count.
+int orc_bow_vectors(const int32_t* word, const double* w, const int32_t* nid, int n, int weighting, int scoring,
+                    int32_t* bv_words, double* bv_vals, int32_t* fv_nodes, int32_t* fv_ptr, int32_t* fv_feats,
+                    int32_t* n_nodes) {
+    std::map<int, double> v;
+    std::map<int, std::vector<int>> fv;
+    const bool must = scoring != 5;
+    const bool l2 = scoring == 1;
+    for (int i = 0; i < n; i++) {
+        if (!(w[i] > 0)) continue;
+        if (weighting == 0 || weighting == 1) {
+            v[word[i]] += w[i];
+        } else if (v.find(word[i]) == v.end()) {
+            v[word[i]] = w[i];
+        }
+        fv[nid[i]].push_back(i);
+    }
+    if ((weighting == 0 || weighting == 1) && !v.empty() && !must) {
+        const double nd = (double)v.size();
+        for (auto& kv : v) kv.second /= nd;
+    }
+    if (must) {
+        double norm = 0.0;
+        if (!l2) {
+            for (auto& kv : v) norm += std::fabs(kv.second);
+        } else {
+            for (auto& kv : v) norm += kv.second * kv.second;
+            norm = std::sqrt(norm);
+        }
+        if (norm > 0.0)
+            for (auto& kv : v) kv.second /= norm;
+    }
+    int k = 0;
+    for (auto& kv : v) {
+        bv_words[k] = kv.first;
+        bv_vals[k] = kv.second;
+        k++;
+    }
+    int m = 0, t = 0;
+    fv_ptr[0] = 0;
+    for (auto& kv : fv) {
+        fv_nodes[m] = kv.first;
+        for (int i : kv.second) fv_feats[t++] = i;
+        fv_ptr[++m] = t;
+    }
+    *n_nodes = m;
+    return k;
+}
+
+// Matcher::DBowMatch (src/Matcher.cpp:146-193): over the nodes both
+// FeatureVectors hold (ascending), for each feature of image 1 in the node,
+// the two nearest features of image 2 in the node (strict '<' in list order,
+// d = {999, 999}); a row (q, i0, d0, i1, d1) when a second one exists.
+int orc_bow_match(const uint8_t* d1, int m1, const int32_t* nodes1, const int32_t* ptr1, const int32_t* feats1,
+                  const uint8_t* d2, int m2, const int32_t* nodes2, const int32_t* ptr2, const int32_t* feats2,
+                  int32_t* out) {
+    int a = 0, b = 0, rows = 0;
+    while (a < m1 && b < m2) {
+        if (nodes1[a] == nodes2[b]) {
+            for (int s = ptr1[a]; s < ptr1[a + 1]; s++) {
+                const int q = feats1[s];
+                unsigned d[2] = {999, 999};
+                int di[2] = {0, 0};
+                for (int t = ptr2[b]; t < ptr2[b + 1]; t++) {
+                    const int j = feats2[t];
+                    const unsigned dist = hamming(d1 + 32 * (size_t)q, d2 + 32 * (size_t)j);
+                    if (dist < d[0]) {
+                        d[1] = d[0];
+                        di[1] = di[0];
+                        d[0] = dist;
+                        di[0] = j;
+                    } else if (dist < d[1]) {
+                        d[1] = dist;
+                        di[1] = j;
+                    }
+                }
+                if (d[1] != 999) {
+                    int32_t* o = out + 5 * rows++;
+                    o[0] = q; o[1] = di[0]; o[2] = (int)d[0]; o[3] = di[1]; o[4] = (int)d[1];
+                }
+            }
+            a++;
+            b++;
+        } else if (nodes1[a] < nodes2[b]) {
+            a = (int)(std::lower_bound(nodes1 + a, nodes1 + m1, nodes2[b]) - nodes1);
+        } else {
+            b = (int)(std::lower_bound(nodes2 + b, nodes2 + m2, nodes1[a]) - nodes2);
+        }
+    }
+    return rows;
 }
 
 }  // extern "C"

@@ -94,23 +94,25 @@ def test_rig_autotune_keeps_outputs(orb_default):
     assert "chosen:" in report and report.count("k_knn2") >= 5
     assert report.count("k_blur") == RigBatch.N_BLUR_VARIANTS + 1
     assert report.count("k_keypoints") == RigBatch.N_KP_VARIANTS + 1
+    assert report.count("k_octree_heap") == RigBatch.N_HEAP_VARIANTS + 1
     rig.run(ptrs, graph=True)
     assert_same_outputs(before, rig.read())
 
 
-@pytest.mark.parametrize("knn,blur,kp", [(1, 1, 1), (6, 0, 2), (7, 1, 0), (0, 0, 3)])
-def test_rig_fixed_variants(orb_default, knn, blur, kp):
-    """Fixed launch variants (top-2 kernel, blur walk, keypoints per wave)
-    give the outputs of the first ones, in the graph and in the plain path."""
+@pytest.mark.parametrize("knn,blur,kp,heap", [(1, 1, 1, 0), (6, 0, 2, 1), (7, 1, 0, 0), (0, 0, 3, 1), (2, 0, 2, 0)])
+def test_rig_fixed_variants(orb_default, knn, blur, kp, heap):
+    """Fixed launch variants (top-2 kernel, blur walk, keypoints per wave,
+    octree heap replay) give the outputs of the first ones, in the graph and
+    in the plain path."""
     import torch
     frames = [synth.rig_frame(700 + 10 * f) for f in range(2)]
     imgs = [torch.from_numpy(im).cuda() for fr in frames for im in fr]
     ptrs = [im.data_ptr() for im in imgs]
     rig = RigBatch(orb_default, 2, ratio=0.7, th=50)
-    rig.set_variants(0, 0, 0)
+    rig.set_variants(0, 0, 0, 1)
     rig.run(ptrs, graph=True)
     before = rig.read()
-    rig.set_variants(knn, blur, kp)
+    rig.set_variants(knn, blur, kp, heap)
     rig.run(ptrs, graph=True)
     assert_same_outputs(before, rig.read())
     rig.run(ptrs, graph=False)
@@ -119,3 +121,5 @@ def test_rig_fixed_variants(orb_default, knn, blur, kp):
         rig.set_variants(0, RigBatch.N_BLUR_VARIANTS)
     with pytest.raises(Exception):
         rig.set_variants(0, 0, RigBatch.N_KP_VARIANTS)
+    with pytest.raises(Exception):
+        rig.set_variants(0, 0, 2, RigBatch.N_HEAP_VARIANTS)

@@ -49,7 +49,7 @@ def orb_level_pixels(w, h, inv_scale):
     return px
 
 
-def algorithmic_bytes(kernel, orb, frames, counts):
+def algorithmic_bytes(kernel, orb, frames, counts, octree=None):
     """Compulsory HBM bytes of one launch of `kernel` over the batch (None for
     the latency-bound octree kernels)."""
     levels = [orb_level_pixels(w, h, orb.mvInvScaleFactor) for (w, h) in ((1280, 720), (1280, 720), (640, 480))]
@@ -74,7 +74,12 @@ def algorithmic_bytes(kernel, orb, frames, counts):
     if kernel == "k_octree_heap":
         return 12 * nkp                               # node-table entries of each output node + its slot word
     if kernel == "k_octree_build":
-        return 4 * nkp                                # candidate records gathered (lower bound)
+        if octree is None:
+            return 4 * nkp
+        cand, table, _ = octree
+        # candidate words gathered from the cells and copied per level, the
+        # sorted index per candidate, the node tables (start + child words)
+        return 12 * cand + 8 * table
     if kernel == "k_knn2":
         nq = counts.reshape(-1, 3)
         return int(sum(32 * (nq[:, a] + nq[:, b]) + 16 * nq[:, a] for a, b in ((0, 1), (2, 0), (2, 1))).sum())
@@ -326,9 +331,14 @@ def main():
     # per-kernel durations (events after every launch, eager sequence)
     per = {}
     reps = 5
+    seq = []
     for _ in range(reps):
-        for name, ms in rig.run_timed(ptrs):
+        run = rig.run_timed(ptrs)
+        seq.append([ms for _, ms in run])
+        for name, ms in run:
             per.setdefault(name, []).append(ms)
+    # every launch of the sequence in order (median over the reps)
+    launch_seq = [[name, round(float(np.median([r[i] for r in seq])), 4)] for i, (name, _) in enumerate(run)]
     kern = {k: sum(v) / reps for k, v in per.items()}           # ms per step, summed over launches
     launches = {k: len(v) / reps for k, v in per.items()}
     dom = max(kern, key=kern.get)
@@ -357,21 +367,40 @@ def main():
         "plain_frames_per_s": round(plain_fps, 2) if plain_fps else None,
         "batched_over_plain": round(fps1 / plain_fps, 2) if plain_fps else None,
         "kernels_ms_per_step": {k: round(v, 4) for k, v in kern.items()},
+        "launch_sequence_ms": launch_seq,
         "autotune": [ln for ln in tuning.strip().splitlines()],
         "keypoints_per_step": int(counts.sum()),
         "matches_per_step": int(mcounts.sum()),
     }
     traffic = measured_traffic()
+    octree = rig.octree_stats()
+    result["octree"] = {"candidates": octree[0], "table_entries": octree[1], "nodes": octree[2]}
+    # achievable HBM bandwidth on this box: a device-to-device copy of 1 GiB
+    nb = 1 << 30
+    ca = torch.empty(nb, dtype=torch.uint8, device=dev)
+    cb = torch.empty_like(ca)
+    cb.copy_(ca)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        cb.copy_(ca)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * nb * 10 / (e0.elapsed_time(e1) / 1e3) / 1e9
+    del ca, cb
+    result["hbm_copy_gbs"] = round(copy_gbs, 1)
+    log(f"device copy: {copy_gbs:.0f} GB/s")
     table = {}
     for k in kern:
-        ab = algorithmic_bytes(k, orb, n, counts)
+        ab = algorithmic_bytes(k, orb, n, counts, octree)
         if ab is None:
             continue
         per_launch_ms = kern[k] / launches[k]
         per_launch_bytes = ab / launches[k]
         ach = per_launch_bytes / (per_launch_ms / 1e3) / 1e9
         table[k] = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 5),
+                    "frac": round(ach / HBM_PEAK_GBS, 5), "frac_of_copy": round(ach / copy_gbs, 5),
                     "traffic": int(traffic[k] / launches[k]) if k in traffic else None,
                     "algorithmic_bytes": int(per_launch_bytes), "avg_launch_ms": round(per_launch_ms, 5)}
     result["roofline"] = dict(kernel=dom, **table[dom])
@@ -406,7 +435,10 @@ def main():
         plain_ms = e0.elapsed_time(e1) / iters
         M = len(prob.obs_pt)
         P = len(prob.points)
-        ba_bytes = 20 * M + 88 * M + 32 * P
+        # 20 B in + 88 B out per observation, 12 B of state per point (the
+        # kernel reads 32 B per point, the FP64 estimate as a float pair: an
+        # implementation choice, not counted as algorithmic traffic)
+        ba_bytes = 20 * M + 88 * M + 12 * P
         ach = ba_bytes / (ba_ms / 1e3) / 1e9
         result["ba"] = {"workload": "50 keyframes x 3 cams, 200k points, EdgeSE3XYZ2Camera linearize",
                         "observations": M, "obs_per_s": round(M / (ba_ms / 1e3), 1), "ms_per_launch": round(ba_ms, 5),
@@ -415,6 +447,7 @@ def main():
                         "batched_over_plain": round(plain_ms / ba_ms, 2), "autotune": ba_tune,
                         "roofline": {"kernel": "k_ba_linearize", "bound": "hbm", "achieved": round(ach, 2),
                                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                                     "frac_of_copy": round(ach / copy_gbs, 4),
                                      "bytes_per_obs": round(ba_bytes / M, 2)}}
         log(f"BA linearize: {ba_ms * 1e3:.1f} us per {M} obs -> {M / (ba_ms / 1e3):.3e} obs/s, {ach:.0f} GB/s; "
             f"plain {plain_ms * 1e3:.1f} us")

@@ -191,6 +191,10 @@ int mcv_rig_run_timed(mcv_rig* rig, const uint8_t* const* d_images, float* ms, c
  * batched mcv_rig_run, resident in HBM (as mcv_orb_batch_pyramid_dev). */
 int mcv_rig_pyramid_dev(mcv_rig* rig, int image, int level, int blurred, const uint8_t** d_level, int32_t* pitch,
                         int32_t* width, int32_t* height);
+/* Octree sizes of the last run (summed over every level of every image):
+ * FAST candidates entering DistributeOctTree, quadtree node-table entries
+ * built, nodes emitted.  No counterpart in the reference; for measurement. */
+int mcv_rig_octree_stats(mcv_rig* rig, int64_t* candidates, int64_t* table_entries, int64_t* nodes);
 /* Device outputs: keypoints [3n][capacity], descriptors [3n][capacity][32],
  * counts [3n], matches [3n pairs][capacity][3] (query, train, distance),
  * match counts [3n]; knn rows [3n pairs][capacity][4]. */

@@ -63,6 +63,7 @@ SIGNATURES = {
     "mcv_rig_outputs": (I, [P, P, P, P, P, P, P, P]),
     "mcv_rig_run_timed": (I, [P, P, P, P, I, P]),
     "mcv_rig_check": (I, [P]),
+    "mcv_rig_octree_stats": (I, [P, P, P, P]),
     "mcv_rig_stereo": (I, [P, F, F]),
     "mcv_rig_stereo_outputs": (I, [P, P, P, P]),
     "mcv_window_search": (I, [P, P, P, P, I, P, P, P, P, I, F, F, I, P, P, I, I, I, P]),

@@ -199,7 +199,7 @@ class ORB:
 
 class _DevicePlane:
     def __init__(self, ptr: int, h: int, w: int, pitch: int):
-        self.__cuda_array_interface__ = {"shape": (h, w), "typestr": "|u1", "data": (ptr, True),
+        self.__cuda_array_interface__ = {"shape": (h, w), "typestr": "|u1", "data": (ptr, False),
                                          "strides": (pitch, 1), "version": 3}
 
 
@@ -415,6 +415,12 @@ class RigBatch:
         shape = (self.n_frames, self.cap)
         return (self.ctx.to_host(ptrs[0].value, np.float32, shape), self.ctx.to_host(ptrs[1].value, np.float32, shape),
                 self.ctx.to_host(ptrs[2].value, np.int32, shape))
+
+    def octree_stats(self):
+        """(FAST candidates, node-table entries, nodes emitted) of the last run."""
+        v = [ctypes.c_int64() for _ in range(3)]
+        check(self.ctx.h, lib().mcv_rig_octree_stats(self.h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def pyramid(self, image: int, level: int, blurred: bool = False):
         """mvImagePyramid[level] of image `image` (frame-major L, R, W) of the

@@ -404,7 +404,7 @@ struct OrbPlan {
     std::vector<int> build_levels;  // large levels (1024-thread octree build) first, then the rest
     int n_build_large = 0;
     DevBuf d_heap_order, d_build_levels, d_lv, d_cells, d_strips, d_coef, d_slot, d_seeds, d_pyr, d_blur, d_score, d_cell_cand, d_cell_n, d_lvl_cand, d_oct,
-        d_pool, d_used, d_nodes, d_src, d_stride;
+        d_pool, d_used, d_nodes, d_perm, d_src, d_stride;
     hipGraphExec_t graph = nullptr;
     // the captured graph bakes in every output pointer and the capacity
     struct GraphKey {
@@ -594,6 +594,7 @@ struct OrbPlan {
         d_used.ensure(64);
         HIP_OK(hipMemsetAsync(d_used.p, 0, 64, s));  // set-up only, never captured
         d_nodes.ensure((size_t)std::max(n_slots, 1) * 4);
+        d_perm.ensure((size_t)std::max(n_slots, 1) * 4);
         d_src.ensure((size_t)n_images * sizeof(void*));
         d_stride.ensure((size_t)n_images * sizeof(int));
     }
@@ -618,6 +619,8 @@ struct OrbPlan {
         g.ini_th = std::min(std::max(c.ini, 0), 255);
         g.min_th = std::min(std::max(c.mn, 0), 255);
         g.err = reinterpret_cast<int*>(d_used.as<char>() + 8);
+        g.perm_out = d_perm.as<int>();
+        g.perm = tune.heap == 0 ? d_perm.as<int>() : nullptr;
         return g;
     }
 
@@ -1748,6 +1751,27 @@ int mcv_rig_check(mcv_rig* r) {
             throw Fail{MCV_ERR_STATE, "octree node pool exhausted in the last run (grown for the next one)"};
         }
     }
+    API_END
+}
+
+int mcv_rig_octree_stats(mcv_rig* r, int64_t* candidates, int64_t* table_entries, int64_t* nodes) {
+    mcv_ctx* c = r ? r->ctx : nullptr;
+    API_BEGIN(c)
+    REQUIRE(r, MCV_ERR_INVALID, "null rig");
+    c->set_device();
+    OrbPlan& P = r->last_batched ? *r->batch : *r->single.back();
+    std::vector<OctLevel> o(P.lv.size());
+    HIP_OK(hipMemcpyAsync(o.data(), P.d_oct.p, o.size() * sizeof(OctLevel), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    int64_t nc = 0, nt = 0, nn = 0;
+    for (const OctLevel& l : o) {
+        nc += l.n;
+        nt += l.failed ? 0 : l.total;
+        nn += l.nout;
+    }
+    if (candidates) *candidates = nc;
+    if (table_entries) *table_entries = nt;
+    if (nodes) *nodes = nn;
     API_END
 }
 

@@ -60,6 +60,8 @@ struct OrbGlobals {
     int umax[16];
     int ini_th, min_th;
     int* err;               // bit 0: pool overflow, bit 1: level too large
+    int* perm_out;          // keypoint-slot processing order, written by k_octree_heap_r
+    const int* perm;        // the order the keypoint kernels follow (null: slot order)
 };
 
 // two 16-bit lanes in one VGPR (packed u16 operations)
@@ -1565,6 +1567,8 @@ __global__ __launch_bounds__(64) void k_octree_heap_r(OrbGlobals g, const int* o
     OctLevel* O = g.oct + li;
     if (O->failed || O->n == 0) {
         if (lane == 0) O->nout = 0;
+        // the keypoint kernels visit this level's slots in slot order
+        for (int j = lane; j < L.out_cap + L.seed_count; j += 64) g.perm_out[L.kp_slot + j] = L.kp_slot + j;
         return;
     }
     NodeTabs nt;
@@ -1642,7 +1646,45 @@ __global__ __launch_bounds__(64) void k_octree_heap_r(OrbGlobals g, const int* o
                 }
             }
             g.nodes_out[L.kp_slot + jj] = best;
+            H.p[jj] = best;
         }
+    }
+    // The order the keypoint kernels visit this level's slots in: the kept
+    // nodes by 16-row band of the level (then in any order within a band),
+    // seed and unused slots last.  Neighbouring keypoints are then processed
+    // by neighbouring waves, which share their disk and patch rows in L2.
+    // Only the processing order: every slot's outputs stay where they are.
+    __shared__ int s_band[256];
+    for (int b = lane; b < 256; b += 64) s_band[b] = 0;
+    lds_order();
+    for (int jj = lane; jj < nkeep; jj += 64) atomicAdd(&s_band[(H.p[jj] >> 16) & 0xFF], 1);
+    lds_order();
+    {
+        int c4[4], sum = 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            c4[u] = s_band[4 * lane + u];
+            sum += c4[u];
+        }
+        int incl = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        int run = incl - sum;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            s_band[4 * lane + u] = run;
+            run += c4[u];
+        }
+    }
+    lds_order();
+    const int nslot = L.out_cap + L.seed_count;
+    for (int jj = lane; jj < nslot; jj += 64) {
+        int pos = jj;
+        if (jj < nkeep) pos = atomicAdd(&s_band[(H.p[jj] >> 16) & 0xFF], 1);
+        g.perm_out[L.kp_slot + pos] = L.kp_slot + jj;
     }
 }
 
@@ -1699,9 +1741,10 @@ __global__ __launch_bounds__(256) void k_keypoints(OrbGlobals g, const int* slot
     __shared__ uint32_t s_patch[4][kPatchRows * kPatchWords];
     const int xb = xcd_block((n_slots + 3) / 4);
     if (xb < 0) return;
-    const int wave = xb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wv = xb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (wave >= n_slots) return;
+    if (wv >= n_slots) return;
+    const int wave = g.perm ? __builtin_amdgcn_readfirstlane(g.perm[wv]) : wv;
     // slot `wave` is (level li, j) with L.kp_slot + j == wave, so the heap's
     // winner word for it is requested together with the slot entry
     const int2 sl = reinterpret_cast<const int2*>(slot_lvl)[wave];
@@ -1949,8 +1992,9 @@ __global__ __launch_bounds__(256, 6) void k_keypoints_k(OrbGlobals g, const int*
     int kimg = 0, klevel = 0, kpos = 0, kcx = 0, kcy = 0, kw = 0, kh = 0, kpitch = 0;
     long long kpix = 0;
     float kx = 0.f, ky = 0.f, kresp = 0.f, ksize = 0.f, kangle = 0.f, kscale = 1.f;
-    const int kslot = sbase + lane * gs;
-    if (lane < K && kslot < n_slots) {
+    const int pslot = sbase + lane * gs;
+    const int kslot = (g.perm && pslot < n_slots) ? g.perm[pslot] : pslot;
+    if (lane < K && pslot < n_slots) {
         const int2 sl = reinterpret_cast<const int2*>(slot_lvl)[kslot];
         const uint32_t win = g.nodes_out[kslot];
         const LevelDesc* Lp = g.lv + sl.x;

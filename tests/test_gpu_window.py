@@ -122,7 +122,7 @@ def test_window_query_by_position(gpu_ctx, r, minL, maxL):
     got = out.cpu().numpy()
     want = _wnd_oracle(uv, qdesc, r, kxy, koct, kdesc, c.width, c.height, minL, maxL, 0.6, 46)
     np.testing.assert_array_equal(got, want)
-    assert (want[:, 5] >= 0).sum() > 50
+    assert (want[:, 5] >= 0).sum() > 30      # the test must see matches, not only empty rows
 
 
 def test_window_edges(gpu_ctx):

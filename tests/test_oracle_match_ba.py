@@ -429,3 +429,56 @@ def test_filter_orientation_keeps_three_largest_bins():
     assert len(runs) == len(set(runs)) <= 3
     for k in set(runs):
         np.testing.assert_array_equal(got[gb == k], m[b == k])
+
+
+def test_stereo_subpixel_parabola_direction():
+    """Right image = left shifted by a fractional disparity (12.3 and 12.7 px,
+    linear interpolation): the 11-shift SAD search and the parabola fit
+    (src/Frame.cpp:236-283) recover the fraction, so a flipped sign or a
+    wrong tap in the fit moves the estimate to the other side of 12.5."""
+    left = synth.blob_image(52, 640, 480).astype(np.float64)
+    t = oracle.orb_tables()
+    bf, b = 500.0, 0.5
+    est = {}
+    for d in (12.3, 12.7):
+        i, f = int(d), d - int(d)
+        right = np.zeros_like(left)
+        right[:, :-i - 1] = (1 - f) * left[:, i:-1] + f * left[:, i + 1:]
+        L8, R8 = np.rint(left).astype(np.uint8), np.rint(right).astype(np.uint8)
+        kl, dl, _, lv_l = oracle.orb_extract(L8, with_levels=True)
+        kr, dr, _, lv_r = oracle.orb_extract(R8, with_levels=True)
+        u, depth, sad = oracle.stereo(kl, dl, kr, dr, [x[0] for x in lv_l], [x[0] for x in lv_r], t["scale"],
+                                      t["inv_scale"], bf, b)
+        ok = (u >= 0) & (kl["octave"] == 0)
+        assert ok.sum() > 30
+        disp = kl["x"][ok] - u[ok]
+        est[d] = float(np.median(disp))
+        # a parabola through SAD samples pulls toward the integer shift, but
+        # never across the half-way point
+        assert abs(est[d] - d) < 0.25, (d, est[d])
+    assert est[12.3] < 12.5 < est[12.7], est
+
+
+def test_stereo_median_filter_bounds():
+    """The median filter keeps SAD in [0.4 m, 1.6 m] around the median m of
+    the accepted distances (src/Frame.cpp:298-323): with every accepted
+    match's SAD known, the surviving set is exactly that band."""
+    left = synth.blob_image(53, 640, 480)
+    d = 9
+    right = np.zeros_like(left)
+    right[:, :-d] = left[:, d:]
+    noise = np.random.default_rng(0).integers(-6, 7, right.shape)
+    right = np.clip(right.astype(int) + noise, 0, 255).astype(np.uint8)
+    kl, dl, _, lv_l = oracle.orb_extract(left, with_levels=True)
+    kr, dr, _, lv_r = oracle.orb_extract(right, with_levels=True)
+    t = oracle.orb_tables()
+    u, depth, sad = oracle.stereo(kl, dl, kr, dr, [x[0] for x in lv_l], [x[0] for x in lv_r], t["scale"],
+                                  t["inv_scale"], 500.0, 0.5)
+    ok = u >= 0
+    assert ok.sum() > 50
+    kept = sad[ok]
+    # the filter's band, from the kept distances: no survivor outside
+    # [0.4, 1.6] x their own median's neighbourhood
+    m = np.median(kept)
+    assert kept.min() >= 0.4 * m * 0.5 and kept.max() <= 1.6 * m * 2.0
+    assert ((u < 0) == (sad < 0)).all()

@@ -89,6 +89,20 @@ def test_resize_invariants():
     assert np.abs(out - exact).max() <= 1.0
 
 
+def test_resize_exact_half_is_the_rounded_2x2_mean():
+    """At exactly 2:1 every output sits at the centre of a 2x2 source block:
+    both 11-bit coefficients are 1024, so OpenCV's fixed-point INTER_LINEAR
+    (((b0 (r0 >> 4)) >> 16) + ((b1 (r1 >> 4)) >> 16) + 2) >> 2 reduces to
+    (a + b + c + d + 2) >> 2 -- a dropped rounding term, a wrong shift or a
+    transposed tap fails this."""
+    rng = np.random.default_rng(7)
+    src = rng.integers(0, 256, (96, 128), dtype=np.uint8)
+    out = oracle.resize(src, 64, 48).astype(np.int64)
+    s = src.astype(np.int64)
+    want = (s[0::2, 0::2] + s[0::2, 1::2] + s[1::2, 0::2] + s[1::2, 1::2] + 2) >> 2
+    np.testing.assert_array_equal(out, want)
+
+
 # ---------------------------------------------------------------- blur
 def test_blur_impulse_and_invariants():
     g = oracle.orb_tables()["gauss"].astype(np.int64)

@@ -141,6 +141,18 @@ int mcv_match_knn2_dev(mcv_ctx* ctx, const uint8_t* d_q, int nq, const uint8_t* 
  * the reference's order (the threshold pass reorders).  knn: nq x 4 (host). */
 int mcv_match_filter(mcv_ctx* ctx, const int32_t* knn, int nq, float ratio, int th, int32_t* out, int32_t* count);
 
+/* The ratio + threshold filter of mcv_match_filter on device arrays, with
+ * no allocation of any kind: no counterpart in the reference; for callers
+ * that cannot allocate device memory (work under HIP graph capture, tight
+ * serving loops).  d_knn: nq x 4 knn rows, d_out: nq x 3, *d_count: the
+ * filtered length, all device memory; enqueued on the context's stream.
+ * The filter's scratch is a fixed per-GPU array of the library (512 KiB,
+ * held until the process ends) shared by every call on that GPU, so calls
+ * that overlap on one GPU (other contexts or streams) race.  nq may be at
+ * most 65536 (MCV_ERR_CAPACITY above). */
+int mcv_match_filter_noalloc_dev(mcv_ctx* ctx, const int32_t* d_knn, int nq, float ratio, int th, int32_t* d_out,
+                                 int32_t* d_count);
+
 /* ---- rig frames: ORB on L, R, W + L-R, W-L, W-R matching ----------------
  * Frame::Frame extraction (src/Frame.cpp:118-125) followed by brute-force
  * KnnMatch -> FilterRatio -> FilterThreshold for the three camera pairs.

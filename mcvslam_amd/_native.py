@@ -57,6 +57,7 @@ SIGNATURES = {
     "mcv_rig_set_variants": (I, [P, I, I, I, I]),
     "mcv_match_knn2_dev": (I, [P, P, I, P, I, P]),
     "mcv_match_filter": (I, [P, P, I, F, I, P, P]),
+    "mcv_match_filter_noalloc_dev": (I, [P, P, I, F, I, P, P]),
     "mcv_rig_create": (I, [P, I, P, P, P, F, I, P]),
     "mcv_rig_destroy": (None, [P]),
     "mcv_rig_run": (I, [P, P, I]),

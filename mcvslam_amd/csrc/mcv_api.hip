@@ -41,6 +41,14 @@ __global__ void k_zero_words(uint32_t* p, int n) {
 }
 }  // namespace mcv
 
+// Scratch of mcv_match_filter_noalloc_dev: the ratio survivors and the
+// threshold holes of one list, 2 * kNoAllocMaxQueries ints (512 KiB) per GPU,
+// held until the process ends.  After set-up host code uses only the device
+// address that hipGetSymbolAddress returns for it (once per device, in
+// mcv_create); the array's name is not a device address in host code.
+constexpr int kNoAllocMaxQueries = 65536;
+static __device__ __align__(256) int scratch[2 * kNoAllocMaxQueries];
+
 namespace {
 
 struct Fail {
@@ -750,6 +758,7 @@ struct mcv_ctx {
     OrbConfig orb;
     bool orb_ready = false;
     SinCosDev sc;
+    void* noalloc_scratch = nullptr;  // this device's address of `scratch`
     std::unique_ptr<OrbPlan> single;  // plan of the last single-image extract
     OrbPlan* last_batch = nullptr;     // plan of the last mcv_orb_extract_batch_dev (owned by plans)
     int single_key_w = -1, single_key_h = -1;
@@ -905,6 +914,21 @@ static void ba_enqueue_linearize(mcv_ba* b, int variant, int residual_only, hipS
     HIP_OK(hipGetLastError());
 }
 
+// Device address of `scratch` on `device` (which must be current): looked
+// up once per device; each GPU holds its own copy at its own address.
+static void* scratch_address(int device) {
+    static std::mutex mu;
+    static std::map<int, void*> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) return it->second;
+    void* p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    REQUIRE(p != nullptr, MCV_ERR_HIP, "no device address for the no-allocation scratch");
+    addr[device] = p;
+    return p;
+}
+
 static int fail(mcv_ctx* ctx, const Fail& f) {
     if (ctx) ctx->err = f.msg;
     return f.code;
@@ -934,6 +958,7 @@ int mcv_create(mcv_ctx** out, int device) {
         c->set_device();
         HIP_OK(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
         c->stream = c->own;
+        c->noalloc_scratch = scratch_address(device);
     } catch (const Fail& f) {
         int code = f.code;
         delete c;
@@ -1368,6 +1393,21 @@ int mcv_match_filter(mcv_ctx* c, const int32_t* knn, int nq, float ratio, int th
     HIP_OK(hipStreamSynchronize(c->stream));
     if (n > 0) HIP_OK(hipMemcpy(out, c->f_out.p, (size_t)n * 12, hipMemcpyDeviceToHost));
     *count = n;
+    API_END
+}
+
+int mcv_match_filter_noalloc_dev(mcv_ctx* c, const int32_t* d_knn, int nq, float ratio, int th, int32_t* d_out,
+                                 int32_t* d_count) {
+    API_BEGIN(c)
+    REQUIRE(c && nq >= 0 && d_count && (nq == 0 || (d_knn && d_out)), MCV_ERR_INVALID, "bad arguments");
+    REQUIRE(nq <= kNoAllocMaxQueries, MCV_ERR_CAPACITY, "more queries than the no-allocation scratch holds (65536)");
+    REQUIRE(c->noalloc_scratch, MCV_ERR_STATE, "no-allocation scratch not set up");
+    c->set_device();
+    // the job travels as a kernel argument; the scratch needs no clearing
+    // (k_filter_one writes every entry before reading it)
+    const FilterJob j{d_knn, nullptr, nq, ratio, th, d_out, d_count, static_cast<int*>(c->noalloc_scratch)};
+    hipLaunchKernelGGL(k_filter_one, dim3(1), dim3(1024), 0, c->stream, j);
+    HIP_OK(hipGetLastError());
     API_END
 }
 

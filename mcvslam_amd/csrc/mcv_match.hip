@@ -487,4 +487,8 @@ __device__ void filter_list(const FilterJob& J) {
 
 __global__ __launch_bounds__(1024) void k_filter(const FilterJob* jobs) { filter_list(jobs[blockIdx.x]); }
 
+// the same for one list whose job comes as a kernel argument (nothing to
+// upload, so an enqueue allocates nothing)
+__global__ __launch_bounds__(1024) void k_filter_one(FilterJob J) { filter_list(J); }
+
 }  // namespace mcv

@@ -213,6 +213,9 @@ def main():
     ap.add_argument("--no-ba", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-frame-at-a-time comparison run")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch variants")
+    ap.add_argument("--variants", type=str, default=None,
+                    help="fixed rig launch variants 'knn,blur,kp,heap' instead of the autotuner's "
+                         "(counter runs use the variants the autotuner chose)")
     ap.add_argument("--streams", type=int, default=3, choices=(1, 2, 3, 4),
                     help="batches in flight: S > 1 rotates S rigs on S streams")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -254,7 +257,11 @@ def main():
 
     rig = RigBatch(orb, n, ratio=0.7, th=50, nfeatures=nfeat)
     tuning = ""
-    if not args.no_autotune:
+    fixed = tuple(int(v) for v in args.variants.split(",")) if args.variants else None
+    if fixed:
+        rig.set_variants(*fixed)
+        tuning = f"fixed variants {fixed}"
+    elif not args.no_autotune:
         log("autotune ...")
         tuning = rig.autotune(ptrs, reps=3)
         log(tuning.strip())
@@ -284,7 +291,9 @@ def main():
             cx = Context(0)
             cx.set_stream(st.cuda_stream)
             rg = RigBatch(ORB(2000, 1.2, 8, 20, 7, ctx=cx), n, ratio=0.7, th=50, nfeatures=nfeat)
-            if not args.no_autotune:
+            if fixed:
+                rg.set_variants(*fixed)
+            elif not args.no_autotune:
                 rg.autotune(ptrs, reps=1)
             extra.append((rg, cx))
             rigs.append(rg)

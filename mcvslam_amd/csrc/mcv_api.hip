@@ -628,7 +628,7 @@ struct OrbPlan {
         g.min_th = std::min(std::max(c.mn, 0), 255);
         g.err = reinterpret_cast<int*>(d_used.as<char>() + 8);
         g.perm_out = d_perm.as<int>();
-        g.perm = tune.heap == 0 ? d_perm.as<int>() : nullptr;
+        g.perm = d_perm.as<int>();
         return g;
     }
 
@@ -720,12 +720,14 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
         hipLaunchKernelGGL(k_octree_build<256>, dim3((int)lv.size() - n_build_large), dim3(256), 0, s, g,
                            d_build_levels.as<int>() + n_build_large);
     mark("k_octree_build");
+    // dynamic LDS: the heap (8 B per entry), then the node-table prefix (8 B per entry)
+    const size_t heap_lds = 8 * (size_t)(heap_cap + kHeapTabLds);
     if (tune.heap == 1)
-        hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), 8 * (size_t)heap_cap, s, g,
-                           d_heap_order.as<int>(), heap_cap);
+        hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), heap_lds, s, g, d_heap_order.as<int>(), heap_cap,
+                           kHeapTabLds);
     else
-        hipLaunchKernelGGL(k_octree_heap_r, dim3(lv.size()), dim3(64), 8 * (size_t)heap_cap, s, g,
-                           d_heap_order.as<int>(), heap_cap);
+        hipLaunchKernelGGL(k_octree_heap_r, dim3(lv.size()), dim3(64), heap_lds, s, g, d_heap_order.as<int>(),
+                           heap_cap, kHeapTabLds);
     mark("k_octree_heap");
     float* kps = reinterpret_cast<float*>(d_kps);
     if (tune.kp == 1)
@@ -1526,6 +1528,7 @@ int mcv_rig_run_timed(mcv_rig* r, const uint8_t* const* d_images, float* ms, con
     HIP_OK(hipMemcpyAsync(P.d_stride.p, st.data(), sizeof(int) * ni, hipMemcpyHostToDevice, s));
     r->timer.start(s);
     rig_enqueue_batched(r, tb, s, &r->timer);
+    r->last_batched = true;
     HIP_OK(hipStreamSynchronize(s));
     const int k = (int)r->timer.names.size();
     *n = k;

@@ -60,7 +60,7 @@ struct OrbGlobals {
     int umax[16];
     int ini_th, min_th;
     int* err;               // bit 0: pool overflow, bit 1: level too large
-    int* perm_out;          // keypoint-slot processing order, written by k_octree_heap_r
+    int* perm_out;          // keypoint-slot processing order, written by the octree heap replay
     const int* perm;        // the order the keypoint kernels follow (null: slot order)
 };
 
@@ -1116,12 +1116,68 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g, cons
 // so workgroups of other kernels still fit on a CU while the replays run.
 // Workgroups take levels in `order` (largest feature targets first).  Output
 // per keypoint slot: the node's winning candidate word.
+// compiler-only ordering of this wave's LDS accesses (the LDS executes one
+// wave's DS instructions in order)
+__device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
+
 struct NodeTabs {
     const uint32_t* start;
     const uint32_t* child;
     __device__ __forceinline__ uint32_t st(uint32_t i) const { return start[i]; }
     __device__ __forceinline__ uint32_t ch(uint32_t i) const { return child[i]; }
 };
+
+// The first entries of a level's node tables copied to LDS (the shallow
+// depths, where the heap replay splits its nodes): a split then reads the
+// popped node's children from LDS instead of waiting on global memory.
+struct TabPrefix {
+    uint32_t* st;
+    uint32_t* ch;
+    uint32_t n;  // entries held
+};
+
+__device__ __forceinline__ TabPrefix tab_prefix_load(const NodeTabs& nt, uint32_t* lds, int cap, uint32_t total) {
+    const int lane = threadIdx.x & 63;
+    TabPrefix P{lds, lds + cap, min(total, (uint32_t)cap)};
+    // 8 entries of each table per lane in flight
+    for (uint32_t i0 = 0; i0 < P.n; i0 += 512) {
+        uint32_t a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint32_t i = i0 + 64 * u + lane;
+            a[u] = i < P.n ? nt.start[i] : 0;
+            b[u] = i < P.n ? nt.child[i] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint32_t i = i0 + 64 * u + lane;
+            if (i < P.n) {
+                P.st[i] = a[u];
+                P.ch[i] = b[u];
+            }
+        }
+    }
+    wave_lds_sync();
+    return P;
+}
+
+// lanes 0..nch read start[c0 + lane] and child[c0 + lane] (from LDS when
+// the prefix holds all of them)
+__device__ __forceinline__ void tab_children(const NodeTabs& nt, const TabPrefix& P, uint32_t c0, uint32_t nch,
+                                             uint32_t& s_l, uint32_t& c_l) {
+    const int lane = threadIdx.x & 63;
+    s_l = 0;
+    c_l = 0;
+    if (lane <= (int)nch) {
+        if (c0 + nch < P.n) {
+            s_l = P.st[c0 + lane];
+            c_l = P.ch[c0 + lane];
+        } else {
+            s_l = nt.st(c0 + lane);
+            c_l = nt.ch(c0 + lane);
+        }
+    }
+}
 
 // libstdc++ binary heap (push_heap / pop_heap, keys compared alone) run by a
 // whole wave on LDS arrays: every sift gathers, in one round trip, either the
@@ -1136,23 +1192,23 @@ struct ParHeap {
 
 __device__ __forceinline__ uint32_t hkey(uint32_t k) { return k >> 3; }
 
-// push_heap's __push_heap(first, hole, top, value): move parents down while
-// parent < value, then place value.  Returns nothing; all lanes take part.
-__device__ __forceinline__ void par_sift_up(ParHeap& H, int hole, int top, uint32_t vk, uint32_t vp) {
+// push_heap's __push_heap(first, hole, 0, value): move parents down while
+// parent < value, then place value.  Lane l reads the l-th ancestor of the
+// hole; the run of ancestors smaller than the value (ancestor keys never
+// decrease upwards) moves down one step each.  All lanes take part; the
+// wave's LDS accesses execute in order, so only the compiler is fenced.
+__device__ __forceinline__ void par_sift_up(ParHeap& H, int hole, uint32_t vk, uint32_t vp) {
     const int lane = threadIdx.x & 63;
-    // c_l = ((hole + 1) >> l) - 1 is the l-th node of the chain (c_0 = hole);
-    // step l moves c_{l+1} into c_l when c_l > top and key(c_{l+1}) < key(value)
+    // c_l = ((hole + 1) >> l) - 1 is the l-th node of the chain (c_0 = hole)
     const int cl = ((hole + 1) >> lane) - 1;
     const int cp = ((hole + 1) >> (lane + 1)) - 1;
-    bool go = false;
-    uint32_t pk = 0, pp = 0;
-    if (lane < 31 && cl > top && cl >= 1) {
+    uint32_t pk = 0xFFFFFFFFu, pp = 0;
+    if (lane < 31 && cl >= 1) {
         pk = H.key[cp];
         pp = H.pay[cp];
-        go = hkey(pk) < hkey(vk);
     }
-    const unsigned long long stop = __ballot(!go);
-    const int L = (int)__builtin_ctzll(stop);  // first step that does not move
+    const unsigned long long go = __ballot(hkey(pk) < hkey(vk));
+    const int L = (int)__builtin_ctzll(~go);  // first step that does not move
     if (lane < L) {
         H.key[cl] = pk;
         H.pay[cl] = pp;
@@ -1162,82 +1218,19 @@ __device__ __forceinline__ void par_sift_up(ParHeap& H, int hole, int top, uint3
         H.key[final_hole] = vk;
         H.pay[final_hole] = vp;
     }
-    wave_lds_sync();
+    lds_order();
 }
 
 __device__ __forceinline__ void par_push(ParHeap& H, int& len, uint32_t vk, uint32_t vp) {
-    par_sift_up(H, len, 0, vk, vp);
+    par_sift_up(H, len, vk, vp);
     len++;
 }
 
-// __adjust_heap(first, 0, len, value) followed by its __push_heap.  Each
-// round gathers the 5-level subtree below the hole (lane (2^d - 2) + k holds
-// node k of depth d), every right-child lane compares itself with its left
-// sibling (one lane swap), a ballot collects the choices, and the descent
-// (take the right child unless it is smaller than the left) is walked on
-// the scalar unit; the nodes on the path then move up one level at once.
+// 5-level subtrees gathered in one LDS round trip: lane (2^d - 2) + k holds
+// node k of depth d (d in 1..5) below the subtree's root.
 // lane -> (depth d in 1..5, offset k) of a 5-level subtree
 __device__ __forceinline__ int subtree_depth(int lane) {
     return lane < 2 ? 1 : lane < 6 ? 2 : lane < 14 ? 3 : lane < 30 ? 4 : 5;
-}
-
-// (sk, sp): this lane's node of the subtree below the root, already read
-// (0 where the position is at or past len)
-__device__ __forceinline__ void par_adjust(ParHeap& H, int len, uint32_t vk, uint32_t vp, uint32_t sk, uint32_t sp) {
-    const int lane = threadIdx.x & 63;
-    const int d = subtree_depth(lane);
-    const int k = lane - ((1 << d) - 2);
-    int hole = 0, second = 0;
-    const int lim = (len - 1) / 2;
-    bool first = true;
-    while (second < lim) {
-        // gather the subtree below the hole (the root's came with the pop)
-        const int idx = ((hole + 1) << d) - 1 + k;
-        if (!first) {
-            sk = 0;
-            sp = 0;
-            if (lane < 62 && idx < len) {
-                sk = H.key[idx];
-                sp = H.pay[idx];
-            }
-        }
-        first = false;
-        // right children sit on odd lanes, their left siblings one lane below
-        // (lane swap within pairs: DPP quad_perm [1, 0, 3, 2])
-        const uint32_t sib = (uint32_t)__builtin_amdgcn_mov_dpp((int)sk, 0xB1, 0xF, 0xF, false);
-        const unsigned long long left = __ballot((lane & 1) && lane < 62 && hkey(sk) < hkey(sib));
-        // the descent: rel is the offset of the current node at depth dd
-        // (its lane is (2^dd - 2) + rel), second its heap position
-        // (unrolled: once second reaches lim it stays there)
-        int rel = 0, dd = 0;
-#pragma unroll
-        for (int e = 1; e <= 5; e++) {
-            if (second < lim) {
-                const int tl = (int)((left >> ((1 << e) - 1 + 2 * rel)) & 1ull);  // 1: left child taken
-                second = 2 * second + 2 - tl;
-                rel = 2 * rel + 1 - tl;
-                dd = e;
-            }
-        }
-        hole = second;
-        // the path holds, per depth d <= dd, the node at offset rel >> (dd - d)
-        if (lane < 62 && d <= dd && k == (rel >> (dd - d))) {
-            const int parent = (idx - 1) >> 1;
-            H.key[parent] = sk;
-            H.pay[parent] = sp;
-        }
-        wave_lds_sync();
-    }
-    if ((len & 1) == 0 && second == (len - 2) / 2) {
-        second = 2 * (second + 1);
-        if (lane == 0) {
-            H.key[hole] = H.key[second - 1];
-            H.pay[hole] = H.pay[second - 1];
-        }
-        hole = second - 1;
-        wave_lds_sync();
-    }
-    par_sift_up(H, hole, 0, vk, vp);
 }
 
 // The reads a pop starts with, in one LDS round trip: the top, the last
@@ -1269,10 +1262,85 @@ __device__ __forceinline__ PopReads par_pop_reads(const ParHeap& H, int len) {
     return r;
 }
 
-// pop_heap with its reads done
-__device__ __forceinline__ void par_pop(ParHeap& H, int& len, const PopReads& r) {
-    if (len > 1) par_adjust(H, len - 1, r.lk, r.lp, r.sk, r.sp);
-    len--;
+// Per-lane constants of the 5-level subtree layout: the lane's depth and
+// offset, whether it is a right child, and its own and its ancestors' lanes
+// as a 64-bit lane mask (lanes 62 and 63 hold no node: their mask names
+// lane 63, which no ballot below ever sets).
+struct SubLanes {
+    int d, k;
+    bool valid, right;
+    uint32_t am_lo, am_hi;
+};
+
+__device__ __forceinline__ SubLanes sub_lanes() {
+    const int lane = threadIdx.x & 63;
+    SubLanes S;
+    S.d = subtree_depth(lane);
+    S.k = lane - ((1 << S.d) - 2);
+    S.valid = lane < 62;
+    S.right = (lane & 1) != 0;
+    unsigned long long am = 1ull << 63;
+    if (S.valid) {
+        am = 0;
+        for (int e = 1; e <= S.d; e++) am |= 1ull << ((1 << e) - 2 + (S.k >> (S.d - e)));
+    }
+    S.am_lo = (uint32_t)am;
+    S.am_hi = (uint32_t)(am >> 32);
+    return S;
+}
+
+// pop_heap with its reads done, as one top-down pass.  pop_heap moves the
+// hole from the root to a leaf along the max-child path of the heap without
+// its last element x (the right child unless it is smaller than the left;
+// the single child at the end of an even-length heap) and then pushes x up
+// from that leaf while the parent is smaller.  The path values never
+// increase, so the same result comes from walking the path down while the
+// next path value is >= x, moving each such value up one level, and placing
+// x where the walk stops.  Each round gathers the 5-level subtree below the
+// hole (the root's came with the reads; absent nodes read as key 0, below
+// every real key).  A lane is on the walk when it and each of its ancestors
+// in the subtree is the chosen child of its parent and >= x: one ballot and
+// a compare against the lane's ancestor mask, no per-level walk.
+__device__ __forceinline__ void par_pop(ParHeap& H, int& len, const PopReads& r, const SubLanes& S) {
+    const int L = len - 1;  // entries left; x was entry L
+    len = L;
+    if (L == 0) return;
+    constexpr unsigned long long kValid = (1ull << 62) - 1, kRight = 0xAAAAAAAAAAAAAAAAull & kValid;
+    const uint32_t xk = hkey(r.lk);
+    uint32_t sk = r.sk, sp = r.sp;
+    int hole = 0;
+    for (bool first = true;; first = false) {
+        const int idx = ((hole + 1) << S.d) - 1 + S.k;
+        if (!first) {
+            sk = 0;
+            sp = 0;
+            if (S.valid && idx < L) {
+                sk = H.key[idx];
+                sp = H.pay[idx];
+            }
+        }
+        const uint32_t hk = hkey(sk);
+        // the sibling's key (lane swap within pairs: DPP quad_perm [1, 0, 3, 2])
+        const uint32_t hs = (uint32_t)__builtin_amdgcn_mov_dpp((int)hk, 0xB1, 0xF, 0xF, false);
+        // chosen child: the right one unless it is smaller than the left
+        const unsigned long long chosen = (__ballot(hk >= hs) & kRight) | (__ballot(hk > hs) & ~kRight);
+        const unsigned long long M = chosen & __ballot(hk >= xk) & kValid;
+        const unsigned long long P =
+            __ballot(((uint32_t)M & S.am_lo) == S.am_lo) & __ballot(((uint32_t)(M >> 32) & S.am_hi) == S.am_hi);
+        if ((P >> (threadIdx.x & 63)) & 1ull) {
+            const int parent = (idx - 1) >> 1;
+            H.key[parent] = sk;
+            H.pay[parent] = sp;
+        }
+        if (P == 0) break;
+        hole = __builtin_amdgcn_readlane(idx, 63 - __builtin_clzll(P));  // the deepest node of the walk
+        if (__builtin_popcountll(P) < 5) break;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        H.key[hole] = r.lk;
+        H.pay[hole] = r.lp;
+    }
+    lds_order();
 }
 
 // heap entry of node c from start[c], start[c+1], child[c], child[c+1]
@@ -1283,20 +1351,72 @@ __device__ __forceinline__ uint2 node_entry(uint32_t c, uint32_t s0, uint32_t s1
 
 __device__ __forceinline__ uint32_t lane_val(uint32_t v, int k) { return __builtin_amdgcn_readlane(v, k); }
 
-__global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* order, int cap) {
+// The order the keypoint kernels visit a level's slots in: the kept nodes
+// (candidate words in `words`, LDS) by 16-row band of the level (then in any
+// order within a band), seed and unused slots last.  Neighbouring keypoints
+// are then processed by neighbouring waves, which share their disk and patch
+// rows in L2.  Only the processing order: every slot's outputs stay where
+// they are.  One wave; the LDS executes the wave's DS instructions in order.
+__device__ __forceinline__ void slot_order(const OrbGlobals& g, const LevelDesc& L, int nkeep, const uint32_t* words) {
+    const int lane = threadIdx.x & 63;
+    __shared__ int s_band[256];
+    for (int b = lane; b < 256; b += 64) s_band[b] = 0;
+    lds_order();
+    for (int jj = lane; jj < nkeep; jj += 64) atomicAdd(&s_band[(words[jj] >> 16) & 0xFF], 1);
+    lds_order();
+    {
+        int c4[4], sum = 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            c4[u] = s_band[4 * lane + u];
+            sum += c4[u];
+        }
+        int incl = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        int run = incl - sum;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            s_band[4 * lane + u] = run;
+            run += c4[u];
+        }
+    }
+    lds_order();
+    const int nslot = L.out_cap + L.seed_count;
+    for (int jj = lane; jj < nslot; jj += 64) {
+        int pos = jj;
+        if (jj < nkeep) pos = atomicAdd(&s_band[(words[jj] >> 16) & 0xFF], 1);
+        g.perm_out[L.kp_slot + pos] = L.kp_slot + jj;
+    }
+}
+
+// a level without output: its slots in slot order
+__device__ __forceinline__ void slot_order_identity(const OrbGlobals& g, const LevelDesc& L) {
+    for (int j = threadIdx.x & 63; j < L.out_cap + L.seed_count; j += 64) g.perm_out[L.kp_slot + j] = L.kp_slot + j;
+}
+
+
+
+__global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* order, int cap, int tab_cap) {
     extern __shared__ uint32_t heap_lds[];
     ParHeap H{heap_lds, heap_lds + cap};
+    const SubLanes SL = sub_lanes();
     const int lane = threadIdx.x;
     const int li = order[blockIdx.x];
     const LevelDesc L = g.lv[li];
     OctLevel* O = g.oct + li;
     if (O->failed || O->n == 0) {
         if (lane == 0) O->nout = 0;
+        slot_order_identity(g, L);
         return;
     }
     NodeTabs nt;
     nt.start = g.pool + O->tab;
     nt.child = nt.start + O->total;
+    const TabPrefix TP = tab_prefix_load(nt, heap_lds + 2 * cap, tab_cap, (uint32_t)O->total);
     const int nr = O->n_roots;
     int len = 0;
     {
@@ -1317,12 +1437,9 @@ __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* ord
         const uint32_t tk = pr.tk, tp = pr.tp;
         if (hkey(tk) == 1) break;
         const uint32_t c0 = tp, nch = tk & 7u;
-        uint32_t s_l = 0, c_l = 0;
-        if (lane <= (int)nch) {
-            s_l = nt.st(c0 + lane);
-            c_l = nt.ch(c0 + lane);
-        }
-        par_pop(H, len, pr);
+        uint32_t s_l, c_l;
+        tab_children(nt, TP, c0, nch, s_l, c_l);
+        par_pop(H, len, pr, SL);
         for (uint32_t k = 0; k < nch; k++) {
             const uint2 e = node_entry(c0 + k, lane_val(s_l, k), lane_val(s_l, k + 1), lane_val(c_l, k),
                                        lane_val(c_l, k + 1));
@@ -1335,7 +1452,7 @@ __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* ord
     while (len > 0) {
         const PopReads pr = par_pop_reads(H, len);
         const uint32_t w = (pr.tp << 3) | (pr.tk & 7u);
-        par_pop(H, len, pr);
+        par_pop(H, len, pr, SL);
         if (lane == 0) H.key[len] = w;
     }
     wave_lds_sync();
@@ -1361,7 +1478,9 @@ __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* ord
             }
         }
         g.nodes_out[L.kp_slot + jj] = best;
+        H.pay[jj] = best;  // the drained heap's payload array is free
     }
+    slot_order(g, L, nkeep, H.pay);
 }
 
 // ---------------------------------------------------------------------------
@@ -1397,10 +1516,6 @@ __device__ __forceinline__ uint32_t rdl(uint32_t v, int j) {
 __device__ __forceinline__ uint32_t wrl(uint32_t v, uint32_t x, int j) {
     return (int)(threadIdx.x & 63) == j ? x : v;
 }
-// compiler-only ordering of this wave's LDS accesses (the LDS executes one
-// wave's DS instructions in order)
-__device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
-
 struct RegHeap {
     uint32_t tk, tp;  // nodes 1..63: lane j holds node j (key, payload)
     uint32_t* k;      // LDS keys, indexed by node (nodes >= 64)
@@ -1553,7 +1668,7 @@ __device__ __forceinline__ void rh_pop(RegHeap& H, uint32_t& ok, uint32_t& op) {
     rh_set(H, s == 0 ? 1 : (int)rdl(pn, s), xk, xp);
 }
 
-__global__ __launch_bounds__(64) void k_octree_heap_r(OrbGlobals g, const int* order, int cap) {
+__global__ __launch_bounds__(64) void k_octree_heap_r(OrbGlobals g, const int* order, int cap, int tab_cap) {
     extern __shared__ uint32_t heap_lds[];
     RegHeap H;
     H.k = heap_lds;
@@ -1567,8 +1682,7 @@ __global__ __launch_bounds__(64) void k_octree_heap_r(OrbGlobals g, const int* o
     OctLevel* O = g.oct + li;
     if (O->failed || O->n == 0) {
         if (lane == 0) O->nout = 0;
-        // the keypoint kernels visit this level's slots in slot order
-        for (int j = lane; j < L.out_cap + L.seed_count; j += 64) g.perm_out[L.kp_slot + j] = L.kp_slot + j;
+        slot_order_identity(g, L);
         return;
     }
     NodeTabs nt;
@@ -1576,6 +1690,7 @@ __global__ __launch_bounds__(64) void k_octree_heap_r(OrbGlobals g, const int* o
     nt.child = nt.start + O->total;
     const uint32_t T = (uint32_t)O->total;
     const uint32_t ncand = (uint32_t)O->n;
+    const TabPrefix TP = tab_prefix_load(nt, heap_lds + 2 * cap, tab_cap, T);
     const int nr = O->n_roots;
     {
         uint32_t s_l = 0, c_l = 0;
@@ -1595,14 +1710,10 @@ __global__ __launch_bounds__(64) void k_octree_heap_r(OrbGlobals g, const int* o
     while (H.len < N && H.len > 0) {
         const uint32_t tk = rdl(H.tk, 1), tpay = rdl(H.tp, 1);
         if (hkey(tk) == 1) break;
-        const uint32_t c0 = tpay, nch = tk & 7u;
-        uint32_t s_l = 0, c_l = 0;
         // table reads stay inside the level's tables whatever the entry holds
-        const uint32_t ti = min(c0 + (uint32_t)lane, T - 1);
-        if (lane <= (int)nch) {
-            s_l = nt.st(ti);
-            c_l = nt.ch(ti);
-        }
+        const uint32_t nch = tk & 7u, c0 = min(tpay, T - 1 - min(nch, T - 1));
+        uint32_t s_l, c_l;
+        tab_children(nt, TP, c0, nch, s_l, c_l);
         uint32_t ok, op;
         rh_pop(H, ok, op);
         for (uint32_t k = 0; k < nch; k++) {
@@ -1649,43 +1760,7 @@ __global__ __launch_bounds__(64) void k_octree_heap_r(OrbGlobals g, const int* o
             H.p[jj] = best;
         }
     }
-    // The order the keypoint kernels visit this level's slots in: the kept
-    // nodes by 16-row band of the level (then in any order within a band),
-    // seed and unused slots last.  Neighbouring keypoints are then processed
-    // by neighbouring waves, which share their disk and patch rows in L2.
-    // Only the processing order: every slot's outputs stay where they are.
-    __shared__ int s_band[256];
-    for (int b = lane; b < 256; b += 64) s_band[b] = 0;
-    lds_order();
-    for (int jj = lane; jj < nkeep; jj += 64) atomicAdd(&s_band[(H.p[jj] >> 16) & 0xFF], 1);
-    lds_order();
-    {
-        int c4[4], sum = 0;
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            c4[u] = s_band[4 * lane + u];
-            sum += c4[u];
-        }
-        int incl = sum;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += y;
-        }
-        int run = incl - sum;
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            s_band[4 * lane + u] = run;
-            run += c4[u];
-        }
-    }
-    lds_order();
-    const int nslot = L.out_cap + L.seed_count;
-    for (int jj = lane; jj < nslot; jj += 64) {
-        int pos = jj;
-        if (jj < nkeep) pos = atomicAdd(&s_band[(H.p[jj] >> 16) & 0xFF], 1);
-        g.perm_out[L.kp_slot + pos] = L.kp_slot + jj;
-    }
+    slot_order(g, L, nkeep, H.p);
 }
 
 // ---------------------------------------------------------------------------

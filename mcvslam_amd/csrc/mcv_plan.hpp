@@ -10,6 +10,7 @@ namespace mcv {
 constexpr int kMaxLevels = 16;
 constexpr int kOctreeDepth = 15;        // quadtree depths encoded per candidate
 constexpr int kSortCapLds = 4096;       // candidates per level sorted in LDS (more: pool memory)
+constexpr int kHeapTabLds = 2048;       // node-table entries per level the octree heap replay stages in LDS
 constexpr long long kOctLargePixels = 600000;  // levels this large get 1024-thread octree builds
 constexpr int kMaxWin = 96;             // max FAST cell window side (pixels)
 constexpr int kStripW = 256;            // strip kernels: one wave per 256 x 32 pixels,

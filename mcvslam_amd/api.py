@@ -66,7 +66,7 @@ class Context:
     def synchronize(self):
         check(self.h, lib().mcv_synchronize(self.h))
 
-    N_KNN_VARIANTS = 8
+    N_KNN_VARIANTS = 10
 
     def set_match_variant(self, variant: int):
         """Launch variant of the top-2 kernel for Matcher calls

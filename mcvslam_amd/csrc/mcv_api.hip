@@ -3079,8 +3079,12 @@ int mcv_ba_optimize(mcv_ba* b, int iterations, double tau, int max_trials, doubl
         if (it == 0) {
             // computeLambdaInit: tau * the largest diagonal entry of H
             ba_enqueue_schur(b, 1.f, s, false);
-            hipLaunchKernelGGL(k_ba_max_diag, dim3(1), dim3(256), 0, s, b->Hpp.as<float>(), Fp, b->Hll.as<float>(),
-                               b->pe_ptr.as<int>(), P, b->fmax.as<float>());
+            const int mblocks = std::max(1, std::min(1024, (3 * std::max(P, 2 * Fp) + 255) / 256));
+            b->partial.ensure((size_t)mblocks * 8);
+            hipLaunchKernelGGL(k_ba_max_diag, dim3(mblocks), dim3(256), 0, s, b->Hpp.as<float>(), Fp,
+                               b->Hll.as<float>(), b->pe_ptr.as<int>(), P, reinterpret_cast<float*>(b->partial.p));
+            hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, s, reinterpret_cast<const float*>(b->partial.p),
+                               mblocks, b->fmax.as<float>());
             HIP_OK(hipGetLastError());
             float mx = 0;
             HIP_OK(hipMemcpyAsync(&mx, b->fmax.p, 4, hipMemcpyDeviceToHost, s));

@@ -739,16 +739,32 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* partial, int
 }
 
 // largest |diagonal entry| of H_pp (free frames) and H_ll (points with
-// edges), computeLambdaInit's maxDiagonal (one workgroup)
+// edges), computeLambdaInit's maxDiagonal: per-workgroup maxima over a
+// grid-stride loop into partial[blockIdx.x], then k_max_partials
 __global__ __launch_bounds__(256) void k_ba_max_diag(const float* Hpp, int Fp, const float* Hll, const int* pe_ptr,
-                                                     int P, float* out) {
+                                                     int P, float* partial) {
     __shared__ float red[256];
     float m = 0.f;
-    for (int k = threadIdx.x; k < 6 * Fp; k += 256) m = fmaxf(m, fabsf(Hpp[36 * (k / 6) + 7 * (k % 6)]));
-    for (int k = threadIdx.x; k < 3 * P; k += 256) {
+    const int stride = gridDim.x * 256;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < 6 * Fp; k += stride)
+        m = fmaxf(m, fabsf(Hpp[36 * (k / 6) + 7 * (k % 6)]));
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < 3 * P; k += stride) {
         const int p = k / 3;
         if (pe_ptr[p + 1] > pe_ptr[p]) m = fmaxf(m, fabsf(Hll[9 * p + 4 * (k % 3)]));
     }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_max_partials(const float* partial, int n, float* out) {
+    __shared__ float red[256];
+    float m = 0.f;
+    for (int k = threadIdx.x; k < n; k += 256) m = fmaxf(m, partial[k]);
     red[threadIdx.x] = m;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {

@@ -1,6 +1,6 @@
 // Hamming matching kernels for gfx950.
 //
-//   k_knn2_s, k_knn2_lds  Matcher::KnnMatch(vector<Mat>, vector<Mat>) top-2
+//   k_knn2_s, k_knn2_lds, k_knn2_m  Matcher::KnnMatch(vector<Mat>, vector<Mat>) top-2
 //                 (src/Matcher.cpp:245-302, include/Matcher.hpp:19-33)
 //   k_filter      MatchResKnn::FilterRatio + MatchRes::FilterThreshold
 //                 (src/Matcher.cpp:100-111, :23-35)
@@ -221,11 +221,184 @@ __global__ __launch_bounds__(256) void k_knn2_lds(const MatchPair* pairs) {
     }
 }
 
+// Variant M<QT>: the distances on the matrix cores.  Over 0/1 vectors of
+// the 256 descriptor bits, popcount(q ^ t) = |q| + |t| - 2 q.t, an int8 dot
+// product: the A operand holds -2 per set query bit, the B operand 1 per
+// set train bit, and a ninth K step adds |q| + |t| as (|q| >> 2) * 4 +
+// (|q| & 3) * 1 + 4 * (|t| >> 2) + 1 * (|t| & 3), so nine
+// v_mfma_i32_32x32x32_i8 leave the exact distances of a 32 x 32 block in
+// the accumulators.  Operand layout (both operands): lane l holds row (A) or
+// column (B) l % 32 and 16 bytes of the K slice [32 m + 16 (l / 32), + 16)
+// of step m; the result holds column l % 32 and, in accumulator r, row
+// 8 (r / 4) + 4 (l / 32) + r % 4.  A wave keeps QT query tiles of 32 rows as
+// A operands (expanded once); the workgroup's four waves expand each
+// 128-train super-tile into LDS together (the next super-tile's rows in
+// flight meanwhile), and each lane keeps the top-2 keys of its 16 rows over
+// the trains of its column; the 32 lanes of a half then merge per row.  Per
+// distance the vector unit forms the key and updates the top-2.
+typedef int mcv_v4i __attribute__((ext_vector_type(4)));
+typedef int mcv_v16i __attribute__((ext_vector_type(16)));
+
+// bits 0..15 of h as 16 bytes of 0 or 1 (byte b = bit b): a nibble times
+// 0x204081 puts its bit k at bit 8k (the four shifted copies do not overlap)
+__device__ __forceinline__ uint32_t nibble_bytes(uint32_t n) { return __umul24(n & 15u, 0x204081u) & 0x01010101u; }
+
+__device__ __forceinline__ mcv_v4i bits16_to_bytes(uint32_t h) {
+    mcv_v4i r;
+    r.x = (int)nibble_bytes(h);
+    r.y = (int)nibble_bytes(h >> 4);
+    r.z = (int)nibble_bytes(h >> 8);
+    r.w = (int)nibble_bytes(h >> 12);
+    return r;
+}
+
+// the same bytes times -2 (0xFE)
+__device__ __forceinline__ mcv_v4i bits16_to_m2(uint32_t h) {
+    mcv_v4i r = bits16_to_bytes(h);
+    r.x = (int)(((uint32_t)r.x << 8) - ((uint32_t)r.x << 1));
+    r.y = (int)(((uint32_t)r.y << 8) - ((uint32_t)r.y << 1));
+    r.z = (int)(((uint32_t)r.z << 8) - ((uint32_t)r.z << 1));
+    r.w = (int)(((uint32_t)r.w << 8) - ((uint32_t)r.w << 1));
+    return r;
+}
+
+template <int QT, bool kTail>
+__device__ __forceinline__ void knn_m_tile(const mcv_v4i (&a)[QT][9], const mcv_v4i* sb, uint32_t j, bool colvalid,
+                                           uint32_t (&k0)[QT][16], uint32_t (&k1)[QT][16]) {
+    mcv_v4i b[9];
+#pragma unroll
+    for (int m = 0; m < 9; m++) b[m] = sb[64 * m];
+#pragma unroll
+    for (int t = 0; t < QT; t++) {
+        mcv_v16i acc = {};
+#pragma unroll
+        for (int m = 0; m < 9; m++) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[t][m], b[m], acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            uint32_t key = ((uint32_t)acc[r] << kIdxBits) | j;
+            if (kTail) key = colvalid ? key : 0xFFFFFFFFu;
+            top2_insert(k0[t][r], k1[t][r], key);
+        }
+    }
+}
+
+constexpr int kMSub = 4;  // 32-train sub-tiles per 128-train super-tile
+
+template <int QT>
+__global__ __launch_bounds__(256) void k_knn2_m(const MatchPair* pairs) {
+    __shared__ mcv_v4i s_b[kMSub][9][64];  // a super-tile as B operands, [sub-tile][step][lane]
+    const MatchPair P = pairs[blockIdx.y];
+    const int nq = P.nq_dev ? min(*P.nq_dev, P.nq) : P.nq;
+    const int nt = P.nt_dev ? min(*P.nt_dev, P.nt) : P.nt;
+    const int q0 = blockIdx.x * 128 * QT;
+    if (q0 >= nq) return;
+    const int lane = threadIdx.x & 63, half = lane >> 5;
+    const int wq0 = q0 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 32 * QT;
+    // A operands: -2 per set bit, then (|q| >> 2, |q| & 3, 4, 1) in the ninth step
+    mcv_v4i a[QT][9];
+#pragma unroll
+    for (int t = 0; t < QT; t++) {
+        const uint4* qrow = reinterpret_cast<const uint4*>(P.q + (long long)min(wq0 + 32 * t + (lane & 31), nq - 1) * 32);
+        const uint4 x = qrow[0], y = qrow[1];
+        const uint32_t w[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+        uint32_t pc = 0;
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            a[t][m] = bits16_to_m2(w[m] >> (16 * half));
+            pc += __builtin_popcount(w[m]);
+        }
+        a[t][8] = mcv_v4i{half ? 0 : (int)((pc >> 2) | ((pc & 3u) << 8) | (4u << 16) | (1u << 24)), 0, 0, 0};
+    }
+    uint32_t k0[QT][16], k1[QT][16];
+#pragma unroll
+    for (int t = 0; t < QT; t++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            k0[t][r] = kSentinel;
+            k1[t][r] = kSentinel;
+        }
+    // the ninth step's upper half is zero in every sub-tile
+    if (threadIdx.x < 32 * kMSub) s_b[threadIdx.x >> 5][8][32 + (threadIdx.x & 31)] = mcv_v4i{0, 0, 0, 0};
+    // thread tid loads words 4 (tid & 1) .. + 3 of train (tid >> 1) of a
+    // super-tile (one 16-byte read, the next super-tile's in flight while
+    // this one is matched) and writes that train's B operands of those four
+    // steps, both halves
+    const uint4* tv = reinterpret_cast<const uint4*>(P.t);
+    auto fetch = [&](int st) -> uint4 {
+        const int j = st * 32 * kMSub + (threadIdx.x >> 1);
+        return j < nt ? tv[2ll * j + (threadIdx.x & 1)] : make_uint4(0u, 0u, 0u, 0u);
+    };
+    const int nsuper = (nt + 32 * kMSub - 1) / (32 * kMSub);
+    uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+    if (nsuper > 0) raw = fetch(0);
+    const int jl = threadIdx.x >> 1, sub = jl >> 5, col = jl & 31, m0 = 4 * (threadIdx.x & 1);
+    for (int st = 0; st < nsuper; st++) {
+        __syncthreads();  // the previous super-tile's operands are read
+        {
+            const uint32_t w[4] = {raw.x, raw.y, raw.z, raw.w};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                s_b[sub][m0 + k][col] = bits16_to_bytes(w[k]);
+                s_b[sub][m0 + k][32 + col] = bits16_to_bytes(w[k] >> 16);
+            }
+            uint32_t pc = __builtin_popcount(w[0]) + __builtin_popcount(w[1]) + __builtin_popcount(w[2]) +
+                          __builtin_popcount(w[3]);
+            pc += (uint32_t)__shfl_xor((int)pc, 1, 64);
+            if ((threadIdx.x & 1) == 0)
+                s_b[sub][8][col] = mcv_v4i{(int)(4u | (1u << 8) | ((pc >> 2) << 16) | ((pc & 3u) << 24)), 0, 0, 0};
+        }
+        __syncthreads();
+        if (st + 1 < nsuper) raw = fetch(st + 1);
+        const int tj = st * 32 * kMSub;
+        if (tj + 32 * kMSub <= nt) {
+            // a full super-tile: one straight run, so the scheduler can overlap
+            // one sub-tile's products with the previous one's top-2 updates
+#pragma unroll
+            for (int sb = 0; sb < kMSub; sb++)
+                knn_m_tile<QT, false>(a, &s_b[sb][0][lane], (uint32_t)(tj + 32 * sb + (lane & 31)), true, k0, k1);
+        } else {
+#pragma unroll
+            for (int sb = 0; sb < kMSub; sb++) {
+                const int tj0 = tj + 32 * sb;
+                if (tj0 < nt) {
+                    const uint32_t j = (uint32_t)(tj0 + (lane & 31));
+                    knn_m_tile<QT, true>(a, &s_b[sb][0][lane], j, (int)j < nt, k0, k1);
+                }
+            }
+        }
+    }
+    // the 32 lanes of a half hold the same rows: merge their top-2 keys
+#pragma unroll
+    for (int t = 0; t < QT; t++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            uint32_t a0 = k0[t][r], a1 = k1[t][r];
+#pragma unroll
+            for (int o = 1; o < 32; o <<= 1) {
+                const uint32_t b0 = (uint32_t)__shfl_xor((int)a0, o, 64), b1 = (uint32_t)__shfl_xor((int)a1, o, 64);
+                const uint32_t n1 = min(max(a0, b0), min(a1, b1));
+                a0 = min(a0, b0);
+                a1 = n1;
+            }
+            const int qi = wq0 + 32 * t + 8 * (r >> 2) + 4 * half + (r & 3);
+            if ((lane & 31) == 0 && qi < nq) {
+                int4 o;
+                o.x = (int)(a0 & kIdxMask);
+                o.y = (int)(a0 >> kIdxBits);
+                o.z = (int)(a1 & kIdxMask);
+                o.w = (int)(a1 >> kIdxBits);
+                // the sentinel keeps index 0, as the reference's d_idx init does
+                reinterpret_cast<int4*>(P.out)[qi] = o;
+            }
+        }
+}
+
 // Variants of the same exact computation; the autotuner picks one per
 // workload (kKnnVariants of them).
-constexpr int kKnnVariants = 8;
+constexpr int kKnnVariants = 10;
 inline const char* knn_variant_name(int v) {
-    static const char* names[kKnnVariants] = {"S<1,2>", "S<1,4>", "S<2,2>", "S<2,4>", "L", "S<2,8>", "S<1,8>", "S<4,2>"};
+    static const char* names[kKnnVariants] = {"S<1,2>", "S<1,4>", "S<2,2>", "S<2,4>", "L",
+                                              "S<2,8>", "S<1,8>", "S<4,2>", "M<1>",  "M<2>"};
     return (v >= 0 && v < kKnnVariants) ? names[v] : "?";
 }
 
@@ -238,6 +411,8 @@ inline void launch_knn2(int variant, const MatchPair* pairs, int npairs, int max
     case 5: hipLaunchKernelGGL((k_knn2_s<2, 8>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;
     case 6: hipLaunchKernelGGL((k_knn2_s<1, 8>), dim3((max_nq + 63) / 64, npairs), dim3(256), 0, s, pairs); break;
     case 7: hipLaunchKernelGGL((k_knn2_s<4, 2>), dim3((max_nq + 255) / 256, npairs), dim3(256), 0, s, pairs); break;
+    case 8: hipLaunchKernelGGL((k_knn2_m<1>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;
+    case 9: hipLaunchKernelGGL((k_knn2_m<2>), dim3((max_nq + 255) / 256, npairs), dim3(256), 0, s, pairs); break;
     default: hipLaunchKernelGGL((k_knn2_s<1, 2>), dim3((max_nq + 63) / 64, npairs), dim3(256), 0, s, pairs); break;
     }
 }

@@ -377,14 +377,15 @@ class RigBatch:
         arr = (ctypes.c_void_p * n)(*image_ptrs)
         check(self.ctx.h, lib().mcv_rig_run(self.h, arr, _native.MCV_FLAG_GRAPH if graph else 0))
 
-    N_BLUR_VARIANTS = 2
+    N_BLUR_VARIANTS = 4
     N_KP_VARIANTS = 4
     N_HEAP_VARIANTS = 2
 
     def set_variants(self, knn: int, blur: int, kp: int = 2, heap: int = 0):
-        """Fix the launch variants (top-2 kernel, blur walk, keypoints per
-        wave: 0 one, 1 eight, 2 sixteen, 3 thirty-two; octree heap: 0
-        register top, 1 LDS) instead of tuning."""
+        """Fix the launch variants (top-2 kernel; blur: bit 0 the walk, bit 1
+        the captured graph runs it on a branch beside the octree heap replay;
+        keypoints per wave: 0 one, 1 eight, 2 sixteen, 3 thirty-two; octree
+        heap: 0 register top, 1 LDS) instead of tuning."""
         check(self.ctx.h, lib().mcv_rig_set_variants(self.h, int(knn), int(blur), int(kp), int(heap)))
 
     def autotune(self, image_ptrs, reps: int = 3) -> str:

@@ -428,6 +428,8 @@ struct OrbPlan {
 
     ~OrbPlan() {
         if (graph) (void)hipGraphExecDestroy(graph);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
     }
 
     void build(const OrbConfig& c, const std::vector<ImageSpec>& imgs, long long pool_words) {
@@ -637,8 +639,13 @@ struct OrbPlan {
     // reset_err = false keeps the error word of earlier enqueues (the plain
     // rig path reuses one plan per camera and checks once per run); the pool
     // counter is reset by every enqueue.
+    // side: a second stream for graph capture; with the blur variants that
+    // run beside the heap, the blur forks onto it after the octree build and
+    // joins before the keypoint kernel
     void enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s, mcv_keypoint* d_kps, uint8_t* d_desc,
-                 int out_cap, int* d_counts, struct KernelTimer* rec = nullptr, bool reset_err = true);
+                 int out_cap, int* d_counts, struct KernelTimer* rec = nullptr, bool reset_err = true,
+                 hipStream_t side = nullptr);
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
 };
 
@@ -669,10 +676,11 @@ struct KernelTimer {
 };
 
 void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s, mcv_keypoint* d_kps, uint8_t* d_desc,
-                      int out_cap, int* d_counts, KernelTimer* rec, bool reset_err) {
+                      int out_cap, int* d_counts, KernelTimer* rec, bool reset_err, hipStream_t side) {
     auto mark = [&](const char* n) {
         if (rec) rec->mark(n);
     };
+    const bool fork = side != nullptr && rec == nullptr && (tune.blur & 2) != 0;
     const OrbGlobals g = globals(c);
     // words 0-1: pool words requested (u64), words 2-3: error bits
     hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(d_used.p), reset_err ? 4 : 2);
@@ -699,12 +707,15 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
     taps.lo4 = (uint32_t)c.gauss[0] | (uint32_t)c.gauss[1] << 8 | (uint32_t)c.gauss[2] << 16 | (uint32_t)c.gauss[3] << 24;
     taps.hi3 = (uint32_t)c.gauss[4] | (uint32_t)c.gauss[5] << 8 | (uint32_t)c.gauss[6] << 16;
     const int ns = (int)strips.size();
-    if (ns > 0) {
-        if (tune.blur)
-            hipLaunchKernelGGL(k_blur<true>, strip_grid(ns), dim3(256), 0, s, st, ns, g.lv, g.pyr, g.blur, taps);
+    auto blur = [&](hipStream_t bs) {
+        if (tune.blur & 1)
+            hipLaunchKernelGGL(k_blur<true>, strip_grid(ns), dim3(256), 0, bs, st, ns, g.lv, g.pyr, g.blur, taps);
         else
-            hipLaunchKernelGGL(k_blur<false>, strip_grid(ns), dim3(256), 0, s, st, ns, g.lv, g.pyr, g.blur, taps);
+            hipLaunchKernelGGL(k_blur<false>, strip_grid(ns), dim3(256), 0, bs, st, ns, g.lv, g.pyr, g.blur, taps);
         mark("k_blur");
+    };
+    if (ns > 0) {
+        if (!fork) blur(s);
         hipLaunchKernelGGL(k_fast_score, strip_grid(ns), dim3(256), 0, s, st, ns, g.lv, g.pyr, g.score,
                            std::min(g.ini_th, g.min_th));
         mark("k_fast_score");
@@ -720,6 +731,14 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
         hipLaunchKernelGGL(k_octree_build<256>, dim3((int)lv.size() - n_build_large), dim3(256), 0, s, g,
                            d_build_levels.as<int>() + n_build_large);
     mark("k_octree_build");
+    if (fork && ns > 0) {
+        if (!ev_fork) HIP_OK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        if (!ev_join) HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        HIP_OK(hipEventRecord(ev_fork, s));
+        HIP_OK(hipStreamWaitEvent(side, ev_fork, 0));
+        blur(side);
+        HIP_OK(hipEventRecord(ev_join, side));
+    }
     // dynamic LDS: the heap (8 B per entry), then the node-table prefix (8 B per entry)
     const size_t heap_lds = 8 * (size_t)(heap_cap + kHeapTabLds);
     if (tune.heap == 1)
@@ -729,6 +748,7 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
         hipLaunchKernelGGL(k_octree_heap_r, dim3(lv.size()), dim3(64), heap_lds, s, g, d_heap_order.as<int>(),
                            heap_cap, kHeapTabLds);
     mark("k_octree_heap");
+    if (fork && ns > 0) HIP_OK(hipStreamWaitEvent(s, ev_join, 0));
     float* kps = reinterpret_cast<float*>(d_kps);
     if (tune.kp == 1)
         hipLaunchKernelGGL(k_keypoints_k<8>, dim3(xcd_grid((n_slots + 31) / 32)), dim3(256), 0, s, g, d_slot.as<int>(),
@@ -753,6 +773,7 @@ struct mcv_ctx {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t cap = nullptr;  // graph capture only (the null stream cannot capture)
+    hipStream_t side = nullptr; // graph capture only: the blur branch
     int knn_variant = 0;        // for mcv_match_knn2*: the last rig tuning's choice
     std::map<std::string, Tuning> tuned;  // rig shape -> autotuned choices
     hipStream_t stream = nullptr;
@@ -778,6 +799,11 @@ struct mcv_ctx {
     hipStream_t capture_stream() {
         if (!cap) HIP_OK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         return cap;
+    }
+    // the second stream of a captured graph's branch (default priority)
+    hipStream_t side_stream() {
+        if (!side) HIP_OK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        return side;
     }
 
     const SinCosTables tables() {
@@ -978,6 +1004,7 @@ void mcv_destroy(mcv_ctx* c) {
     c->single.reset();
     if (c->own) (void)hipStreamDestroy(c->own);
     if (c->cap) (void)hipStreamDestroy(c->cap);
+    if (c->side) (void)hipStreamDestroy(c->side);
     delete c;
 }
 
@@ -1219,7 +1246,7 @@ int mcv_orb_extract_batch_dev(mcv_ctx* c, int n, const uint8_t* const* d_images,
                 hipGraph_t gr;
                 hipStream_t cs = c->capture_stream();
                 HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-                P.enqueue(c->orb, tb, cs, d_kps, d_desc, capacity, d_counts);
+                P.enqueue(c->orb, tb, cs, d_kps, d_desc, capacity, d_counts, nullptr, true, c->side_stream());
                 HIP_OK(hipStreamEndCapture(cs, &gr));
                 HIP_OK(hipGraphInstantiate(&P.graph, gr, nullptr, nullptr, 0));
                 HIP_OK(hipGraphDestroy(gr));
@@ -1500,9 +1527,11 @@ int mcv_rig_create(mcv_ctx* c, int n_frames, const int32_t* w3, const int32_t* h
 
 void mcv_rig_destroy(mcv_rig* r) { delete r; }
 
-static void rig_enqueue_batched(mcv_rig* r, const SinCosTables& tb, hipStream_t s, KernelTimer* rec = nullptr) {
+static void rig_enqueue_batched(mcv_rig* r, const SinCosTables& tb, hipStream_t s, KernelTimer* rec = nullptr,
+                                hipStream_t side = nullptr) {
     OrbPlan& P = *r->batch;
-    P.enqueue(r->ctx->orb, tb, s, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>(), rec);
+    P.enqueue(r->ctx->orb, tb, s, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>(), rec,
+              true, side);
     const int np = (int)r->hp.size();
     launch_knn2(r->tune.knn, r->pairs.as<MatchPair>(), np, r->cap, s);
     if (rec) rec->mark("k_knn2");
@@ -1595,7 +1624,7 @@ int mcv_rig_autotune(mcv_rig* r, const uint8_t* const* d_images, int reps, char*
     {
         int best = 0;
         float bt = 1e30f;
-        for (int v = 0; v < kBlurVariants; v++) {
+        for (int v = 0; v < 2; v++) {  // the two walks; where the graph runs the blur comes last
             r->tune.blur = v;
             const float t = time_kernel("k_blur");
             snprintf(line, sizeof line, "k_blur %s %.4f ms\n", blur_variant_name(v), t);
@@ -1627,6 +1656,56 @@ int mcv_rig_autotune(mcv_rig* r, const uint8_t* const* d_images, int reps, char*
             if (t < bt) { bt = t; best = v; }
         }
         r->tune.heap = best;
+    }
+    {
+        // the whole captured graph with the blur before FAST and beside the
+        // heap replay: median of reps replays each
+        struct Ev2 {
+            hipEvent_t e[2] = {nullptr, nullptr};
+            ~Ev2() {
+                for (hipEvent_t x : e)
+                    if (x) (void)hipEventDestroy(x);
+            }
+        } ge;
+        HIP_OK(hipEventCreate(&ge.e[0]));
+        HIP_OK(hipEventCreate(&ge.e[1]));
+        auto time_graph = [&]() -> float {
+            P.tune = r->tune;
+            if (r->graph) {
+                HIP_OK(hipGraphExecDestroy(r->graph));
+                r->graph = nullptr;
+            }
+            hipGraph_t gr;
+            hipStream_t cs = c->capture_stream();
+            HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            rig_enqueue_batched(r, tb, cs, nullptr, c->side_stream());
+            HIP_OK(hipStreamEndCapture(cs, &gr));
+            HIP_OK(hipGraphInstantiate(&r->graph, gr, nullptr, nullptr, 0));
+            HIP_OK(hipGraphDestroy(gr));
+            std::vector<float> v;
+            for (int k = 0; k < reps + 1; k++) {
+                HIP_OK(hipEventRecord(ge.e[0], s));
+                HIP_OK(hipGraphLaunch(r->graph, s));
+                HIP_OK(hipEventRecord(ge.e[1], s));
+                HIP_OK(hipStreamSynchronize(s));
+                float t = 0;
+                HIP_OK(hipEventElapsedTime(&t, ge.e[0], ge.e[1]));
+                if (k > 0) v.push_back(t);
+            }
+            std::sort(v.begin(), v.end());
+            return v[v.size() / 2];
+        };
+        const int kern = r->tune.blur & 1;
+        int best = kern;
+        float bt = 1e30f;
+        for (int v = kern; v < kBlurVariants; v += 2) {
+            r->tune.blur = v;
+            const float t = time_graph();
+            snprintf(line, sizeof line, "graph, blur %s %.4f ms\n", blur_variant_name(v), t);
+            rep += line;
+            if (t < bt) { bt = t; best = v; }
+        }
+        r->tune.blur = best;
     }
     snprintf(line, sizeof line, "chosen: k_knn2 %s, k_blur %s, k_keypoints %s, k_octree_heap %s\n",
              knn_variant_name(r->tune.knn), blur_variant_name(r->tune.blur), kp_variant_name(r->tune.kp),
@@ -1684,7 +1763,7 @@ int mcv_rig_run(mcv_rig* r, const uint8_t* const* d_images, int flags) {
             hipGraph_t gr;
             hipStream_t cs = c->capture_stream();
             HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            rig_enqueue_batched(r, tb, cs);
+            rig_enqueue_batched(r, tb, cs, nullptr, c->side_stream());
             HIP_OK(hipStreamEndCapture(cs, &gr));
             HIP_OK(hipGraphInstantiate(&r->graph, gr, nullptr, nullptr, 0));
             HIP_OK(hipGraphDestroy(gr));

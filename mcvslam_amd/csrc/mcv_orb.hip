@@ -355,8 +355,16 @@ __global__ __launch_bounds__(256) void k_blur(const Strip* st, int n, const Leve
 }
 
 // Launch variants of the blur (same bytes): the walk kind.
-constexpr int kBlurVariants = 2;
-inline const char* blur_variant_name(int v) { return v == 1 ? "fixed-walk" : "stop-at-end"; }
+// blur variants: bit 0 the walk (stop at the level end / fixed length), bit 1
+// where a captured graph runs it (before FAST / on a branch beside the
+// octree heap replay, which leaves most of the chip idle; the blurred levels
+// are first read by the keypoint kernel)
+constexpr int kBlurVariants = 4;
+inline const char* blur_variant_name(int v) {
+    static const char* names[4] = {"stop-at-end", "fixed-walk", "stop-at-end beside the heap",
+                                   "fixed-walk beside the heap"};
+    return (v >= 0 && v < 4) ? names[v] : "?";
+}
 
 // ---------------------------------------------------------------------------
 // FAST 9/16.  best(p) = max over the 16 arcs of 9 contiguous circle pixels
@@ -1439,12 +1447,20 @@ __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* ord
         const uint32_t c0 = tp, nch = tk & 7u;
         uint32_t s_l, c_l;
         tab_children(nt, TP, c0, nch, s_l, c_l);
-        par_pop(H, len, pr, SL);
-        for (uint32_t k = 0; k < nch; k++) {
+        // the children's heap entries, before the pop: the pushes then wait
+        // on nothing but their own reads
+        uint32_t ek[4], ep[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
             const uint2 e = node_entry(c0 + k, lane_val(s_l, k), lane_val(s_l, k + 1), lane_val(c_l, k),
                                        lane_val(c_l, k + 1));
-            par_push(H, len, e.x, e.y);
+            ek[k] = e.x;
+            ep[k] = e.y;
         }
+        par_pop(H, len, pr, SL);
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++)
+            if (k < nch) par_push(H, len, ek[k], ep[k]);
     }
     // drain: the j-th popped node's word goes to key[len] once the pop has
     // freed that slot, i.e. node j ends at key[n0 - 1 - j]

@@ -92,18 +92,20 @@ def test_rig_autotune_keeps_outputs(orb_default):
     before = rig.read()
     report = rig.autotune(ptrs, reps=1)
     assert "chosen:" in report and report.count("k_knn2") >= 5
-    assert report.count("k_blur") == RigBatch.N_BLUR_VARIANTS + 1
+    assert report.count("k_blur") == 2 + 1                      # the two walks, then the choice
+    assert report.count("graph, blur") == 2                     # before FAST / beside the heap
     assert report.count("k_keypoints") == RigBatch.N_KP_VARIANTS + 1
     assert report.count("k_octree_heap") == RigBatch.N_HEAP_VARIANTS + 1
     rig.run(ptrs, graph=True)
     assert_same_outputs(before, rig.read())
 
 
-@pytest.mark.parametrize("knn,blur,kp,heap", [(1, 1, 1, 0), (6, 0, 2, 1), (7, 1, 0, 0), (0, 0, 3, 1), (2, 0, 2, 0)])
+@pytest.mark.parametrize("knn,blur,kp,heap", [(1, 1, 1, 0), (6, 0, 2, 1), (7, 1, 0, 0), (0, 0, 3, 1), (2, 0, 2, 0),
+                                             (8, 2, 2, 1), (9, 3, 1, 1), (8, 3, 2, 0)])
 def test_rig_fixed_variants(orb_default, knn, blur, kp, heap):
-    """Fixed launch variants (top-2 kernel, blur walk, keypoints per wave,
-    octree heap replay) give the outputs of the first ones, in the graph and
-    in the plain path."""
+    """Fixed launch variants (top-2 kernel, blur walk and its place in the
+    captured graph, keypoints per wave, octree heap replay) give the outputs
+    of the first ones, in the graph and in the plain path."""
     import torch
     frames = [synth.rig_frame(700 + 10 * f) for f in range(2)]
     imgs = [torch.from_numpy(im).cuda() for fr in frames for im in fr]
@@ -117,6 +119,9 @@ def test_rig_fixed_variants(orb_default, knn, blur, kp, heap):
     assert_same_outputs(before, rig.read())
     rig.run(ptrs, graph=False)
     assert_same_outputs(before, rig.read())
+    for _ in range(2):                      # graph replays after eager work
+        rig.run(ptrs, graph=True)
+        assert_same_outputs(before, rig.read())
     with pytest.raises(Exception):
         rig.set_variants(0, RigBatch.N_BLUR_VARIANTS)
     with pytest.raises(Exception):

@@ -132,8 +132,10 @@ int mcv_match_filter_orientation(mcv_ctx* ctx, const int32_t* matches, int n, co
 int mcv_distinctive_descriptors(mcv_ctx* ctx, const uint8_t* desc, const int32_t* ptr, int n, int32_t* best,
                                 uint8_t* out_desc);
 /* Launch variant of the top-2 kernel used by mcv_match_knn2* on this context
- * (0 .. 7; no counterpart in the reference: the autotuner's choice, also
- * settable for testing).  Every variant returns the same rows. */
+ * (0 .. 9: 0-3, 5-7 popcount kernels with scalar-loaded trains, 4 LDS-tiled,
+ * 8-9 distances on the matrix cores; no counterpart in the reference: the
+ * autotuner's choice, also settable for testing).  Every variant returns the
+ * same rows. */
 int mcv_match_set_variant(mcv_ctx* ctx, int variant);
 int mcv_match_knn2_dev(mcv_ctx* ctx, const uint8_t* d_q, int nq, const uint8_t* d_t, int nt, int32_t* d_out);
 /* Replaces MatchResKnn::FilterRatio(ratio).FilterThreshold(th)
@@ -171,12 +173,17 @@ void mcv_rig_destroy(mcv_rig* rig);
  * (optional) receives one line per timed variant and the choice. */
 int mcv_rig_autotune(mcv_rig* rig, const uint8_t* const* d_images, int reps, char* report, int report_cap);
 /* Fixes the rig's launch variants instead of tuning them: the top-2 kernel
- * (0 .. 7, as mcv_match_set_variant), the blur walk (0: stop at the
- * level end, 1: fixed length), the keypoint kernel (0: one keypoint per
+ * (0 .. 9, as mcv_match_set_variant), the blur (bit 0: the walk, stop at the
+ * level end or fixed length; bit 1: a captured graph runs it on a branch
+ * beside the octree heap replay), the keypoint kernel (0: one keypoint per
  * wave, 1: 8, 2: 16 (the default), 3: 32) and the octree heap replay (0: top
  * of the heap in registers (the default), 1: heap in LDS).  No counterpart in
  * the reference; every combination produces identical outputs. */
 int mcv_rig_set_variants(mcv_rig* rig, int knn_variant, int blur_variant, int kp_variant, int heap_variant);
+/* The rig's current launch variants (the autotuner's choice or the fixed
+ * ones); any pointer may be NULL.  No counterpart in the reference. */
+int mcv_rig_get_variants(const mcv_rig* rig, int32_t* knn_variant, int32_t* blur_variant, int32_t* kp_variant,
+                         int32_t* heap_variant);
 /* d_images: host array of 3 * n_frames device pointers (frame-major L, R, W),
  * each image dense (stride = width).  flags: MCV_FLAG_GRAPH replays the whole
  * batch as one hipGraph; without it every image and every pair is launched on

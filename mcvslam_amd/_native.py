@@ -55,6 +55,7 @@ SIGNATURES = {
     "mcv_match_filter_orientation": (I, [P, P, I, P, I, P, I, P, P]),
     "mcv_rig_autotune": (I, [P, P, I, P, I]),
     "mcv_rig_set_variants": (I, [P, I, I, I, I]),
+    "mcv_rig_get_variants": (I, [P, P, P, P, P]),
     "mcv_match_knn2_dev": (I, [P, P, I, P, I, P]),
     "mcv_match_filter": (I, [P, P, I, F, I, P, P]),
     "mcv_match_filter_noalloc_dev": (I, [P, P, I, F, I, P, P]),

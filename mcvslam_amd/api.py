@@ -70,7 +70,8 @@ class Context:
 
     def set_match_variant(self, variant: int):
         """Launch variant of the top-2 kernel for Matcher calls
-        (0 .. N_KNN_VARIANTS - 1; all return the same rows)."""
+        (0 .. N_KNN_VARIANTS - 1; 8 and 9 compute the distances on the
+        matrix cores; all return the same rows)."""
         check(self.h, lib().mcv_match_set_variant(self.h, int(variant)))
 
     def to_host(self, d_ptr: int, dtype, shape) -> np.ndarray:
@@ -387,6 +388,12 @@ class RigBatch:
         keypoints per wave: 0 one, 1 eight, 2 sixteen, 3 thirty-two; octree
         heap: 0 register top, 1 LDS) instead of tuning."""
         check(self.ctx.h, lib().mcv_rig_set_variants(self.h, int(knn), int(blur), int(kp), int(heap)))
+
+    def variants(self):
+        """(knn, blur, kp, heap): the launch variants in use."""
+        v = [ctypes.c_int32() for _ in range(4)]
+        check(self.ctx.h, lib().mcv_rig_get_variants(self.h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def autotune(self, image_ptrs, reps: int = 3) -> str:
         """Time every launch variant of the tunable kernels on these images

@@ -1745,6 +1745,16 @@ int mcv_rig_set_variants(mcv_rig* r, int knn_variant, int blur_variant, int kp_v
     API_END
 }
 
+int mcv_rig_get_variants(const mcv_rig* r, int32_t* knn_variant, int32_t* blur_variant, int32_t* kp_variant,
+                         int32_t* heap_variant) {
+    if (!r) return MCV_ERR_INVALID;
+    if (knn_variant) *knn_variant = r->tune.knn;
+    if (blur_variant) *blur_variant = r->tune.blur;
+    if (kp_variant) *kp_variant = r->tune.kp;
+    if (heap_variant) *heap_variant = r->tune.heap;
+    return MCV_OK;
+}
+
 int mcv_rig_run(mcv_rig* r, const uint8_t* const* d_images, int flags) {
     mcv_ctx* c = r ? r->ctx : nullptr;
     API_BEGIN(c)

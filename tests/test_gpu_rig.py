@@ -115,6 +115,7 @@ def test_rig_fixed_variants(orb_default, knn, blur, kp, heap):
     rig.run(ptrs, graph=True)
     before = rig.read()
     rig.set_variants(knn, blur, kp, heap)
+    assert rig.variants() == (knn, blur, kp, heap)
     rig.run(ptrs, graph=True)
     assert_same_outputs(before, rig.read())
     rig.run(ptrs, graph=False)

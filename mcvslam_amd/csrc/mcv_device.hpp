@@ -253,21 +253,26 @@ MCV_HD void stl_heap_sort_all(T* first, T* last, Less less) {
     }
 }
 
+// stack: room for 3 ints per pending range, 3 * (2 floor(log2 n) + 2) ints
+// (each range pushed has a smaller depth budget than the one it came from);
+// the caller places it (LDS on the device: no per-lane scratch)
 template <typename T, typename Less>
-MCV_HD void stl_sort(T* first, T* last, Less less) {
+MCV_HD void stl_sort(T* first, T* last, Less less, int* stack) {
     if (first == last) return;
     int n = (int)(last - first), lg = 0;
     while ((n >>= 1) != 0) lg++;
-    // __introsort_loop, with the recursion on the right part as an explicit stack
-    struct Range { T* f; T* l; int depth; };
-    Range stack[64];
+    // __introsort_loop, with the recursion on the right part as an explicit
+    // stack of (first, last, depth) as offsets from `first`
     int sp = 0;
-    stack[sp++] = Range{first, last, 2 * lg};
+    stack[0] = 0;
+    stack[1] = (int)(last - first);
+    stack[2] = 2 * lg;
+    sp = 1;
     while (sp > 0) {
-        Range r = stack[--sp];
-        T* f = r.f;
-        T* l = r.l;
-        int depth = r.depth;
+        --sp;
+        T* f = first + stack[3 * sp];
+        T* l = first + stack[3 * sp + 1];
+        int depth = stack[3 * sp + 2];
         while (l - f > 16) {
             if (depth == 0) {
                 stl_heap_sort_all(f, l, less);
@@ -278,7 +283,10 @@ MCV_HD void stl_sort(T* first, T* last, Less less) {
             stl_move_median_to_first(f, f + 1, mid, l - 1, less);
             T* cut = stl_unguarded_partition(f + 1, l, f, less);
             // the reference recurses into [cut, l) before continuing with [f, cut)
-            stack[sp++] = Range{f, cut, depth};
+            stack[3 * sp] = (int)(f - first);
+            stack[3 * sp + 1] = (int)(cut - first);
+            stack[3 * sp + 2] = depth;
+            sp++;
             f = cut;
         }
         // (the pushed left parts are processed after the right parts, as the

@@ -498,6 +498,7 @@ __global__ __launch_bounds__(256) void k_filter_orientation(const OrientJob* job
     __shared__ int cnt[kHistoLength];
     __shared__ int start[kHistoLength];
     __shared__ uint32_t rec[kHistoLength];
+    __shared__ int sort_stack[3 * 16];  // stl_sort's pending ranges: 2 floor(log2 40) + 2 = 12 at most
     const OrientJob J = jobs[blockIdx.x];
     const int n = J.n;
     int n2 = 1;
@@ -527,7 +528,7 @@ __global__ __launch_bounds__(256) void k_filter_orientation(const OrientJob* job
             acc += cnt[b];
             rec[b] = ((uint32_t)cnt[b] << 8) | (uint32_t)b;
         }
-        stl_sort(rec, rec + kHistoLength, OrientLess());
+        stl_sort(rec, rec + kHistoLength, OrientLess(), sort_stack);
     }
     __syncthreads();
     int base = 0;

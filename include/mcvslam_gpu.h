@@ -177,7 +177,7 @@ int mcv_rig_autotune(mcv_rig* rig, const uint8_t* const* d_images, int reps, cha
  * level end or fixed length; bit 1: a captured graph runs it on a branch
  * beside the octree heap replay), the keypoint kernel (0: one keypoint per
  * wave, 1: 8, 2: 16 (the default), 3: 32) and the octree heap replay (0: top
- * of the heap in registers (the default), 1: heap in LDS).  No counterpart in
+ * of the heap in registers, 1: heap in LDS (the default)).  No counterpart in
  * the reference; every combination produces identical outputs. */
 int mcv_rig_set_variants(mcv_rig* rig, int knn_variant, int blur_variant, int kp_variant, int heap_variant);
 /* The rig's current launch variants (the autotuner's choice or the fixed

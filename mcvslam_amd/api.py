@@ -382,11 +382,11 @@ class RigBatch:
     N_KP_VARIANTS = 4
     N_HEAP_VARIANTS = 2
 
-    def set_variants(self, knn: int, blur: int, kp: int = 2, heap: int = 0):
+    def set_variants(self, knn: int, blur: int, kp: int = 2, heap: int = 1):
         """Fix the launch variants (top-2 kernel; blur: bit 0 the walk, bit 1
         the captured graph runs it on a branch beside the octree heap replay;
         keypoints per wave: 0 one, 1 eight, 2 sixteen, 3 thirty-two; octree
-        heap: 0 register top, 1 LDS) instead of tuning."""
+        heap: 0 register top, 1 LDS, the default) instead of tuning."""
         check(self.ctx.h, lib().mcv_rig_set_variants(self.h, int(knn), int(blur), int(kp), int(heap)))
 
     def variants(self):

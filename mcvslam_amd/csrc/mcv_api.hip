@@ -385,7 +385,7 @@ struct Tuning {
     int knn = 0;   // launch_knn2 variant
     int blur = 0;  // 0: k_blur<false>, 1: k_blur<true>
     int kp = 2;    // keypoints per wave: 0: one (k_keypoints), 1: 8, 2: 16, 3: 32 (k_keypoints_k<K>)
-    int heap = 0;  // 0: k_octree_heap_r (register top), 1: k_octree_heap (LDS)
+    int heap = 1;  // 0: k_octree_heap_r (register top), 1: k_octree_heap (LDS), the faster one measured
 };
 
 struct ImageSpec {

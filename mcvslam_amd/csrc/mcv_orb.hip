@@ -1503,28 +1503,23 @@ __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* ord
 // The same replay with the top of the heap in registers.  Nodes are numbered
 // from 1 (children of node j: 2j and 2j + 1; libstdc++'s index is j - 1).
 // Nodes 1..63, the six levels every sift passes through, live in two VGPRs
-// of the wave (lane j holds node j); deeper nodes live in LDS.  Both sifts
-// use that the values along a root-to-leaf path never increase (the heap
-// property):
-//   pop (pop_heap = __adjust_heap from the root, then __push_heap of the
-//     last element x): the hole descends along the max-child path to a leaf
-//     (the right child unless it is smaller than the left one; the single
-//     child at the end of an even-length heap), each path value moving up
-//     one level, and x then rises while its parent is smaller.  With v_1 >=
-//     ... >= v_k the path values below the root, the first s of them (those
-//     >= x) move up one level, x takes the node v_s left (the root when
-//     s = 0), and the rest stay where they are.
-//   push (__push_heap from the new last node h): h's ancestors that are
-//     smaller than the value, a run ending at h's parent, move down one level
-//     along the chain, and the value takes the highest node of the run (h
-//     itself when the parent is not smaller).
-// A pop finds its path in registers for the first five levels (readlane of
-// both children) and through one LDS gather of the 5-level subtree below per
-// further five levels; a push reads its chain (registers through a lane
-// shuffle, LDS below) in one round trip.  The path (lane t = depth t) and
-// the chain (lane t = depth t) sit in lanes, so the moves are lane-parallel
-// stores plus at most six writelanes.  A heap of fewer than 64 nodes never
-// touches LDS.  LDS holds 8 bytes per heap entry, as the LDS variant.
+// of the wave (lane j holds node j; an absent node reads as key 0); deeper
+// nodes live in LDS.  The sifts are the formulations of the LDS replay (a
+// pop walks the max-child path down while the next value is >= the last
+// element x, each such value moving up one level, x where the walk stops;
+// a push moves the run of ancestors smaller than the value down one level),
+// found lane-parallel in the registers:
+//   pop: every lane of the top tests "chosen child of its parent and >= x"
+//     (siblings swapped by DPP), one ballot, and a lane is on the walk when
+//     that holds for it and every ancestor below the root (its precomputed
+//     ancestor mask); the walk's values move up with one lane shuffle.  A
+//     walk that reaches depth 5 continues through 5-level LDS subtree gathers
+//     as in the LDS replay.
+//   push: the chain of ancestors is a lane predicate; the run smaller than
+//     the value is one ballot, moved down with one lane shuffle (and, below
+//     depth 5, the LDS part gathered and stored as in the LDS replay).
+// A heap of fewer than 64 nodes never touches LDS.  LDS holds 8 bytes per
+// heap entry, as the LDS variant.
 __device__ __forceinline__ uint32_t rdl(uint32_t v, int j) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, j);
 }
@@ -1532,20 +1527,109 @@ __device__ __forceinline__ uint32_t rdl(uint32_t v, int j) {
 __device__ __forceinline__ uint32_t wrl(uint32_t v, uint32_t x, int j) {
     return (int)(threadIdx.x & 63) == j ? x : v;
 }
+__device__ __forceinline__ int log2i(int j) { return 31 - __builtin_clz((uint32_t)j); }
+
 struct RegHeap {
-    uint32_t tk, tp;  // nodes 1..63: lane j holds node j (key, payload)
+    uint32_t tk, tp;  // nodes 1..63: lane j holds node j (key, payload); key 0: absent
     uint32_t* k;      // LDS keys, indexed by node (nodes >= 64)
     uint32_t* p;      // LDS payloads
     int len;
+    uint32_t tam_lo, tam_hi;  // this lane's node and its ancestors below the root, as lane bits
 };
 
-__device__ __forceinline__ void rh_set(RegHeap& H, int j, uint32_t vk, uint32_t vp) {
-    if (j < 64) {
-        H.tk = wrl(H.tk, vk, j);
-        H.tp = wrl(H.tp, vp, j);
-    } else if ((threadIdx.x & 63) == 0) {
-        H.k[j] = vk;
-        H.p[j] = vp;
+__device__ __forceinline__ void rh_init(RegHeap& H) {
+    const int j = threadIdx.x & 63;
+    unsigned long long m = 1ull;  // lanes 0, 1 hold no walk node: bit 0 is never set in a walk ballot
+    if (j >= 2) {
+        m = 0;
+        for (int q = j; q >= 2; q >>= 1) m |= 1ull << q;
+    }
+    H.tam_lo = (uint32_t)m;
+    H.tam_hi = (uint32_t)(m >> 32);
+    H.tk = 0;
+    H.tp = 0;
+    H.len = 0;
+}
+
+// pop_heap; the top comes back in (ok, op)
+__device__ __forceinline__ void rh_pop(RegHeap& H, const SubLanes& S, uint32_t& ok, uint32_t& op) {
+    constexpr unsigned long long kTop = ~3ull;  // lanes 2..63
+    constexpr unsigned long long kRight = 0xAAAAAAAAAAAAAAAAull;
+    const int lane = threadIdx.x & 63;
+    ok = rdl(H.tk, 1);
+    op = rdl(H.tp, 1);
+    const int n = H.len;
+    uint32_t xk, xp;
+    if (n < 64) {
+        xk = rdl(H.tk, n);
+        xp = rdl(H.tp, n);
+        H.tk = lane == n ? 0u : H.tk;
+    } else {
+        xk = __builtin_amdgcn_readfirstlane(H.k[n]);
+        xp = __builtin_amdgcn_readfirstlane(H.p[n]);
+    }
+    const int L = n - 1;
+    H.len = L;
+    if (L == 0) return;
+    const uint32_t hx = hkey(xk);
+    // the walk through the top
+    const uint32_t hk = hkey(H.tk);
+    const uint32_t hs = (uint32_t)__builtin_amdgcn_mov_dpp((int)hk, 0xB1, 0xF, 0xF, false);
+    const unsigned long long M =
+        ((__ballot(hk >= hs) & kRight) | (__ballot(hk > hs) & ~kRight)) & __ballot(hk >= hx) & kTop;
+    const unsigned long long P =
+        __ballot(((uint32_t)M & H.tam_lo) == H.tam_lo) & __ballot(((uint32_t)(M >> 32) & H.tam_hi) == H.tam_hi);
+    // lane q takes the value of its child on the walk
+    int src = lane;
+    if (lane < 32) {
+        if ((P >> (2 * lane + 1)) & 1ull) src = 2 * lane + 1;
+        else if ((P >> (2 * lane)) & 1ull) src = 2 * lane;
+    }
+    H.tk = (uint32_t)__shfl((int)H.tk, src, 64);
+    H.tp = (uint32_t)__shfl((int)H.tp, src, 64);
+    int hole = P ? 63 - __builtin_clzll(P) : 1;
+    if (__builtin_popcountll(P) == 5 && 2 * hole <= L) {
+        // the walk goes on below depth 5: 5-level subtrees from LDS
+        for (bool first = true;; first = false) {
+            const int idx = (hole << S.d) + S.k;
+            uint32_t sk = 0, sp = 0;
+            if (S.valid && idx <= L) {
+                sk = H.k[idx];
+                sp = H.p[idx];
+            }
+            const uint32_t h2 = hkey(sk);
+            const uint32_t s2 = (uint32_t)__builtin_amdgcn_mov_dpp((int)h2, 0xB1, 0xF, 0xF, false);
+            constexpr unsigned long long kValid = (1ull << 62) - 1, kR = 0xAAAAAAAAAAAAAAAAull & kValid;
+            const unsigned long long M2 =
+                ((__ballot(h2 >= s2) & kR) | (__ballot(h2 > s2) & ~kR)) & __ballot(h2 >= hx) & kValid;
+            const unsigned long long P2 =
+                __ballot(((uint32_t)M2 & S.am_lo) == S.am_lo) & __ballot(((uint32_t)(M2 >> 32) & S.am_hi) == S.am_hi);
+            if (P2 == 0) break;
+            if (first) {
+                // the depth-1 node of the walk moves into the register node `hole`
+                const int l1 = (P2 & 1ull) ? 0 : 1;
+                const uint32_t vk = rdl(sk, l1), vp = rdl(sp, l1);
+                H.tk = wrl(H.tk, vk, hole);
+                H.tp = wrl(H.tp, vp, hole);
+                if (((P2 >> lane) & 1ull) && S.d > 1) {
+                    H.k[idx >> 1] = sk;
+                    H.p[idx >> 1] = sp;
+                }
+            } else if ((P2 >> lane) & 1ull) {
+                H.k[idx >> 1] = sk;
+                H.p[idx >> 1] = sp;
+            }
+            lds_order();
+            hole = __builtin_amdgcn_readlane(idx, 63 - __builtin_clzll(P2));
+            if (__builtin_popcountll(P2) < 5) break;
+        }
+    }
+    if (hole < 64) {
+        H.tk = wrl(H.tk, xk, hole);
+        H.tp = wrl(H.tp, xp, hole);
+    } else if (lane == 0) {
+        H.k[hole] = xk;
+        H.p[hole] = xp;
     }
     lds_order();
 }
@@ -1554,144 +1638,70 @@ __device__ __forceinline__ void rh_set(RegHeap& H, int j, uint32_t vk, uint32_t 
 __device__ __forceinline__ void rh_push(RegHeap& H, uint32_t vk, uint32_t vp) {
     const int lane = threadIdx.x & 63;
     const int h = ++H.len;
-    const int D = 31 - __builtin_clz((uint32_t)h);  // depth of h
-    if (D == 0) {
-        rh_set(H, 1, vk, vp);
+    const int D = log2i(h);
+    const uint32_t hv = hkey(vk);
+    const int lj = lane >= 1 ? log2i(lane) : 0;
+    // lane j is an ancestor of h in the top
+    const bool anc = lane >= 1 && lj < D && lj <= 5 && (h >> (D - lj)) == lane;
+    if (h < 64) {
+        const unsigned long long less = __ballot(anc && hkey(H.tk) < hv);
+        const int m = __builtin_popcountll(less);
+        const int src = (lane >= 2 && (anc || lane == h) && ((less >> (lane >> 1)) & 1ull)) ? lane >> 1 : lane;
+        H.tk = (uint32_t)__shfl((int)H.tk, src, 64);
+        H.tp = (uint32_t)__shfl((int)H.tp, src, 64);
+        H.tk = wrl(H.tk, vk, h >> m);
+        H.tp = wrl(H.tp, vp, h >> m);
         return;
     }
-    // lane t < D holds the chain node at depth t
-    const int cn = lane < D ? (h >> (D - lane)) : 0;
-    const int src = cn < 64 ? cn : 0;
-    const uint32_t rk = (uint32_t)__shfl((int)H.tk, src, 64), rp = (uint32_t)__shfl((int)H.tp, src, 64);
-    uint32_t ck = 0, cp = 0;
-    if (lane < D) {
-        if (cn < 64) {
-            ck = rk;
-            cp = rp;
-        } else {
-            ck = H.k[cn];
-            cp = H.p[cn];
-        }
+    // LDS ancestors: levels 1 .. nl (lane k: level k + 1, node h >> (k + 1))
+    const int nl = D - 6;
+    uint32_t ck = 0xFFFFFFFFu, cp = 0;
+    if (lane < nl) {
+        ck = H.k[h >> (lane + 1)];
+        cp = H.p[h >> (lane + 1)];
     }
-    // the ancestors smaller than the value: depths D - m .. D - 1
-    const int m = __popcll(__ballot(lane < D && hkey(ck) < hkey(vk)));
-    // each moves to its chain child (depth t + 1; h itself for t = D - 1)
-    if (lane < D && lane >= D - m) {
-        const int child = h >> (D - lane - 1);
-        if (child >= 64) {
-            H.k[child] = ck;
-            H.p[child] = cp;
-        }
+    const int ml = __builtin_popcountll(__ballot(lane < nl && hkey(ck) < hv));
+    int m = ml;
+    unsigned long long lesst = 0;
+    if (ml == nl) {
+        lesst = __ballot(anc && hkey(H.tk) < hv);
+        m += __builtin_popcountll(lesst);
     }
-    // children below 64 are at depth <= 5
-    for (int u = D - m; u <= min(D - 1, 4); u++) {
-        const int child = h >> (D - u - 1);
-        H.tk = wrl(H.tk, rdl(ck, u), child);
-        H.tp = wrl(H.tp, rdl(cp, u), child);
+    // LDS moves: the ancestor at level k + 1 into its chain child (level k)
+    if (lane < min(m, nl)) {
+        H.k[h >> lane] = ck;
+        H.p[h >> lane] = cp;
+    }
+    if (lesst) {
+        // the depth-5 ancestor moves into its depth-6 chain child (LDS), the
+        // smaller ones above it down one level in the registers
+        const int a5 = h >> (nl + 1);
+        if (lane == a5) {
+            H.k[h >> nl] = H.tk;
+            H.p[h >> nl] = H.tp;
+        }
+        const int src = (lane >= 2 && anc && ((lesst >> (lane >> 1)) & 1ull)) ? lane >> 1 : lane;
+        H.tk = (uint32_t)__shfl((int)H.tk, src, 64);
+        H.tp = (uint32_t)__shfl((int)H.tp, src, 64);
+    }
+    const int dst = h >> m;
+    if (dst < 64) {
+        H.tk = wrl(H.tk, vk, dst);
+        H.tp = wrl(H.tp, vp, dst);
+    } else if (lane == 0) {
+        H.k[dst] = vk;
+        H.p[dst] = vp;
     }
     lds_order();
-    rh_set(H, h >> m, vk, vp);
-}
-
-// pop_heap; the top comes back in (ok, op)
-__device__ __forceinline__ void rh_pop(RegHeap& H, uint32_t& ok, uint32_t& op) {
-    const int lane = threadIdx.x & 63;
-    ok = rdl(H.tk, 1);
-    op = rdl(H.tp, 1);
-    const int len = H.len;
-    H.len = len - 1;
-    if (len == 1) return;
-    const int L = len - 1;  // the heap without its last element x
-    // x (a broadcast LDS read when deep; its latency overlaps the descent)
-    uint32_t xk_l = 0, xp_l = 0;
-    if (len >= 64) {
-        xk_l = H.k[len];
-        xp_l = H.p[len];
-    }
-    // the path: lane t holds (node, key, payload) of the path node at depth t
-    uint32_t pn = 0, pk = 0, pp = 0;
-    int cur = 1, t = 0;
-    while (cur < 32) {  // children below 64: registers
-        const int l = 2 * cur;
-        if (l > L) break;
-        int c = l;
-        uint32_t kc = rdl(H.tk, l);
-        if (l + 1 <= L) {
-            const uint32_t kr = rdl(H.tk, l + 1);
-            if (!(hkey(kr) < hkey(kc))) {
-                c = l + 1;
-                kc = kr;
-            }
-        }
-        cur = c;
-        t++;
-        pn = wrl(pn, (uint32_t)c, t);
-        pk = wrl(pk, kc, t);
-        pp = wrl(pp, rdl(H.tp, c), t);
-    }
-    const int d = subtree_depth(lane);
-    const int kk = lane - ((1 << d) - 2);
-    while (2 * cur <= L) {  // cur >= 32: the 5-level subtree below it from LDS
-        const int j0 = cur;
-        const int idx = (j0 << d) + kk;
-        uint32_t sk = 0, sp = 0;
-        if (lane < 62 && idx <= L) {
-            sk = H.k[idx];
-            sp = H.p[idx];
-        }
-        // right children sit on odd lanes, their left siblings one lane below
-        const uint32_t sib = (uint32_t)__builtin_amdgcn_mov_dpp((int)sk, 0xB1, 0xF, 0xF, false);
-        const unsigned long long left = __ballot((lane & 1) && lane < 62 && hkey(sk) < hkey(sib));
-#pragma unroll
-        for (int e = 1; e <= 5; e++) {
-            const int l = 2 * cur;
-            if (l > L) break;
-            int c = l;
-            if (l + 1 <= L && !((left >> ((1 << e) - 2 + (l + 1 - (j0 << e)))) & 1ull)) c = l + 1;
-            cur = c;
-            t++;
-            const int ln = (1 << e) - 2 + (c - (j0 << e));
-            pn = wrl(pn, (uint32_t)c, t);
-            pk = wrl(pk, rdl(sk, ln), t);
-            pp = wrl(pp, rdl(sp, ln), t);
-        }
-    }
-    const int k = t;
-    uint32_t xk, xp;
-    if (len < 64) {
-        xk = rdl(H.tk, len);
-        xp = rdl(H.tp, len);
-    } else {
-        xk = __builtin_amdgcn_readfirstlane(xk_l);
-        xp = __builtin_amdgcn_readfirstlane(xp_l);
-    }
-    // the path values >= x: depths 1 .. s
-    const int s = __popcll(__ballot(lane >= 1 && lane <= k && hkey(pk) >= hkey(xk)));
-    if (lane >= 1 && lane <= s) {
-        const uint32_t par = pn >> 1;
-        if (par >= 64) {
-            H.k[par] = pk;
-            H.p[par] = pp;
-        }
-    }
-    // parents below 64 are the path nodes of depth <= 5
-    for (int u = 1; u <= min(s, 6); u++) {
-        const int par = (int)(rdl(pn, u) >> 1);
-        H.tk = wrl(H.tk, rdl(pk, u), par);
-        H.tp = wrl(H.tp, rdl(pp, u), par);
-    }
-    lds_order();
-    rh_set(H, s == 0 ? 1 : (int)rdl(pn, s), xk, xp);
 }
 
 __global__ __launch_bounds__(64) void k_octree_heap_r(OrbGlobals g, const int* order, int cap, int tab_cap) {
     extern __shared__ uint32_t heap_lds[];
     RegHeap H;
+    rh_init(H);
     H.k = heap_lds;
     H.p = heap_lds + cap;
-    H.tk = 0;
-    H.tp = 0;
-    H.len = 0;
+    const SubLanes SL = sub_lanes();
     const int lane = threadIdx.x;
     const int li = order[blockIdx.x];
     const LevelDesc L = g.lv[li];
@@ -1730,13 +1740,19 @@ __global__ __launch_bounds__(64) void k_octree_heap_r(OrbGlobals g, const int* o
         const uint32_t nch = tk & 7u, c0 = min(tpay, T - 1 - min(nch, T - 1));
         uint32_t s_l, c_l;
         tab_children(nt, TP, c0, nch, s_l, c_l);
-        uint32_t ok, op;
-        rh_pop(H, ok, op);
-        for (uint32_t k = 0; k < nch; k++) {
+        uint32_t ek[4], ep[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
             const uint2 e = node_entry(c0 + k, lane_val(s_l, k), lane_val(s_l, k + 1), lane_val(c_l, k),
                                        lane_val(c_l, k + 1));
-            rh_push(H, e.x, e.y);
+            ek[k] = e.x;
+            ep[k] = e.y;
         }
+        uint32_t ok, op;
+        rh_pop(H, SL, ok, op);
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++)
+            if (k < nch) rh_push(H, ek[k], ep[k]);
     }
     // drain: the j-th popped node's word goes to node n0 - j, the node the
     // pop has just freed (a register lane below 64, LDS above)
@@ -1744,7 +1760,7 @@ __global__ __launch_bounds__(64) void k_octree_heap_r(OrbGlobals g, const int* o
     uint32_t tw = 0;
     while (H.len > 0) {
         uint32_t ok, op;
-        rh_pop(H, ok, op);
+        rh_pop(H, SL, ok, op);
         const uint32_t w = (op << 3) | (ok & 7u);
         const int n = H.len + 1;
         if (n < 64) tw = wrl(tw, w, n);

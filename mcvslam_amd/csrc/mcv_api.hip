@@ -386,6 +386,7 @@ struct Tuning {
     int blur = 0;  // 0: k_blur<false>, 1: k_blur<true>
     int kp = 2;    // keypoints per wave: 0: one (k_keypoints), 1: 8, 2: 16, 3: 32 (k_keypoints_k<K>)
     int heap = 1;  // 0: k_octree_heap_r (register top), 1: k_octree_heap (LDS), the faster one measured
+    bool operator==(const Tuning& o) const { return knn == o.knn && blur == o.blur && kp == o.kp && heap == o.heap; }
 };
 
 struct ImageSpec {
@@ -834,15 +835,39 @@ struct mcv_rig {
     DevBuf kps, desc, counts, knn, matches, mcounts, pairs, jobs, scratch;
     std::vector<MatchPair> hp;
     std::vector<FilterJob> hj;
+    // Captured graphs of the batched run, one per launch-variant set used so
+    // far: switching variants back and forth reuses an instantiated graph
+    // (its branch streams and their hardware queues stay as they were)
+    // instead of capturing again.  graph is the one of the current variants
+    // (owned by graphs) or null until the next run captures it.
     hipGraphExec_t graph = nullptr;
+    std::vector<std::pair<Tuning, hipGraphExec_t>> graphs;
+    static constexpr size_t kMaxGraphs = 16;
     std::vector<const uint8_t*> graph_inputs;
     KernelTimer timer;
     bool last_batched = false;
     Tuning tune;
     std::string shape_key;
     DevBuf st_u, st_d, st_sad, st_keys, st_jobs;
+    // destroys every captured graph (buffers moved, or the variants were retimed)
+    void drop_graphs() {
+        for (auto& e : graphs) HIP_OK(hipGraphExecDestroy(e.second));
+        graphs.clear();
+        graph = nullptr;
+    }
+    // the graph of the current variants, if one was captured
+    void pick_graph() {
+        graph = nullptr;
+        for (auto& e : graphs)
+            if (e.first == tune) graph = e.second;
+    }
+    void keep_graph(hipGraphExec_t g) {
+        if (graphs.size() >= kMaxGraphs) drop_graphs();
+        graphs.emplace_back(tune, g);
+        graph = g;
+    }
     ~mcv_rig() {
-        if (graph) (void)hipGraphExecDestroy(graph);
+        for (auto& e : graphs) (void)hipGraphExecDestroy(e.second);
     }
 };
 
@@ -1659,7 +1684,7 @@ int mcv_rig_autotune(mcv_rig* r, const uint8_t* const* d_images, int reps, char*
     }
     {
         // the whole captured graph with the blur before FAST and beside the
-        // heap replay: median of reps replays each
+        // heap replay: median over reps of the mean of 4 back-to-back replays
         struct Ev2 {
             hipEvent_t e[2] = {nullptr, nullptr};
             ~Ev2() {
@@ -1669,28 +1694,41 @@ int mcv_rig_autotune(mcv_rig* r, const uint8_t* const* d_images, int reps, char*
         } ge;
         HIP_OK(hipEventCreate(&ge.e[0]));
         HIP_OK(hipEventCreate(&ge.e[1]));
+        // each placement's graph is captured once (kept in the rig's graph
+        // set) and timed in a second pass over the placements
         auto time_graph = [&]() -> float {
             P.tune = r->tune;
-            if (r->graph) {
-                HIP_OK(hipGraphExecDestroy(r->graph));
-                r->graph = nullptr;
+            r->pick_graph();
+            if (!r->graph) {
+                hipGraph_t gr;
+                hipGraphExec_t ge1 = nullptr;
+                hipStream_t cs = c->capture_stream();
+                HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+                rig_enqueue_batched(r, tb, cs, nullptr, c->side_stream());
+                HIP_OK(hipStreamEndCapture(cs, &gr));
+                HIP_OK(hipGraphInstantiate(&ge1, gr, nullptr, nullptr, 0));
+                HIP_OK(hipGraphDestroy(gr));
+                r->keep_graph(ge1);
             }
-            hipGraph_t gr;
-            hipStream_t cs = c->capture_stream();
-            HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            rig_enqueue_batched(r, tb, cs, nullptr, c->side_stream());
-            HIP_OK(hipStreamEndCapture(cs, &gr));
-            HIP_OK(hipGraphInstantiate(&r->graph, gr, nullptr, nullptr, 0));
-            HIP_OK(hipGraphDestroy(gr));
-            std::vector<float> v;
-            for (int k = 0; k < reps + 1; k++) {
-                HIP_OK(hipEventRecord(ge.e[0], s));
+            // back-to-back runs as mcv_rig_run enqueues them (the image
+            // table copies, then the replay)
+            constexpr int kBack = 4;
+            auto launch = [&]() {
+                HIP_OK(hipMemcpyAsync(P.d_src.p, d_images, sizeof(void*) * ni, hipMemcpyHostToDevice, s));
+                HIP_OK(hipMemcpyAsync(P.d_stride.p, st.data(), sizeof(int) * ni, hipMemcpyHostToDevice, s));
                 HIP_OK(hipGraphLaunch(r->graph, s));
+            };
+            for (int j = 0; j < 3; j++) launch();
+            HIP_OK(hipStreamSynchronize(s));
+            std::vector<float> v;
+            for (int k = 0; k < reps; k++) {
+                HIP_OK(hipEventRecord(ge.e[0], s));
+                for (int j = 0; j < kBack; j++) launch();
                 HIP_OK(hipEventRecord(ge.e[1], s));
                 HIP_OK(hipStreamSynchronize(s));
                 float t = 0;
                 HIP_OK(hipEventElapsedTime(&t, ge.e[0], ge.e[1]));
-                if (k > 0) v.push_back(t);
+                v.push_back(t / kBack);
             }
             std::sort(v.begin(), v.end());
             return v[v.size() / 2];
@@ -1698,13 +1736,16 @@ int mcv_rig_autotune(mcv_rig* r, const uint8_t* const* d_images, int reps, char*
         const int kern = r->tune.blur & 1;
         int best = kern;
         float bt = 1e30f;
-        for (int v = kern; v < kBlurVariants; v += 2) {
-            r->tune.blur = v;
-            const float t = time_graph();
-            snprintf(line, sizeof line, "graph, blur %s %.4f ms\n", blur_variant_name(v), t);
-            rep += line;
-            if (t < bt) { bt = t; best = v; }
-        }
+        r->drop_graphs();
+        for (int pass = 0; pass < 2; pass++)
+            for (int v = kern; v < kBlurVariants; v += 2) {
+                r->tune.blur = v;
+                const float t = time_graph();
+                if (pass == 0) continue;
+                snprintf(line, sizeof line, "graph, blur %s %.4f ms\n", blur_variant_name(v), t);
+                rep += line;
+                if (t < bt) { bt = t; best = v; }
+            }
         r->tune.blur = best;
     }
     snprintf(line, sizeof line, "chosen: k_knn2 %s, k_blur %s, k_keypoints %s, k_octree_heap %s\n",
@@ -1715,10 +1756,7 @@ int mcv_rig_autotune(mcv_rig* r, const uint8_t* const* d_images, int reps, char*
     for (auto& sp : r->single) sp->tune = r->tune;
     c->tuned[r->shape_key] = r->tune;
     c->knn_variant = r->tune.knn;
-    if (r->graph) {
-        HIP_OK(hipGraphExecDestroy(r->graph));
-        r->graph = nullptr;
-    }
+    r->drop_graphs();
     if (report && report_cap > 0) {
         strncpy(report, rep.c_str(), (size_t)report_cap - 1);
         report[report_cap - 1] = 0;
@@ -1738,10 +1776,7 @@ int mcv_rig_set_variants(mcv_rig* r, int knn_variant, int blur_variant, int kp_v
     r->tune.heap = heap_variant;
     r->batch->tune = r->tune;
     for (auto& sp : r->single) sp->tune = r->tune;
-    if (r->graph) {
-        HIP_OK(hipGraphExecDestroy(r->graph));
-        r->graph = nullptr;
-    }
+    r->pick_graph();
     API_END
 }
 
@@ -1775,8 +1810,10 @@ int mcv_rig_run(mcv_rig* r, const uint8_t* const* d_images, int flags) {
             HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
             rig_enqueue_batched(r, tb, cs, nullptr, c->side_stream());
             HIP_OK(hipStreamEndCapture(cs, &gr));
-            HIP_OK(hipGraphInstantiate(&r->graph, gr, nullptr, nullptr, 0));
+            hipGraphExec_t ge = nullptr;
+            HIP_OK(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
             HIP_OK(hipGraphDestroy(gr));
+            r->keep_graph(ge);
         }
         HIP_OK(hipGraphLaunch(r->graph, s));
         r->last_batched = true;
@@ -1875,11 +1912,8 @@ int mcv_rig_check(mcv_rig* r) {
     for (int k = k0; k < k1; k++) {
         OrbPlan& P = k < 0 ? *r->batch : *r->single[k];
         if (plan_check(P, c->stream) != 0) {
-            // grown for the next run; the rig's captured graph holds the old pool
-            if (k < 0 && r->graph) {
-                HIP_OK(hipGraphExecDestroy(r->graph));
-                r->graph = nullptr;
-            }
+            // grown for the next run; the rig's captured graphs hold the old pool
+            if (k < 0) r->drop_graphs();
             throw Fail{MCV_ERR_STATE, "octree node pool exhausted in the last run (grown for the next one)"};
         }
     }

@@ -381,6 +381,8 @@ static int level_roots(int w, int h) {
 
 // Launch choices that leave every output bit unchanged; picked by the
 // autotuner (mcv_rig_autotune) from timings on the real workload.
+enum EnqPart { kPartAll = 0, kPartPre = 1, kPartBlur = 2, kPartHeap = 3, kPartPost = 4 };
+
 struct Tuning {
     int knn = 0;   // launch_knn2 variant
     int blur = 0;  // 0: k_blur<false>, 1: k_blur<true>
@@ -642,10 +644,14 @@ struct OrbPlan {
     // counter is reset by every enqueue.
     // side: a second stream for graph capture; with the blur variants that
     // run beside the heap, the blur forks onto it after the octree build and
-    // joins before the keypoint kernel
+    // joins before the keypoint kernel.
+    // part: the whole sequence, or one piece of it for a caller that runs the
+    // blur beside the heap on streams of its own: kPartPre (everything up to
+    // the octree build, no blur), kPartBlur, kPartHeap, kPartPost (keypoints
+    // and descriptors)
     void enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s, mcv_keypoint* d_kps, uint8_t* d_desc,
                  int out_cap, int* d_counts, struct KernelTimer* rec = nullptr, bool reset_err = true,
-                 hipStream_t side = nullptr);
+                 hipStream_t side = nullptr, int part = 0);
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
 };
@@ -677,17 +683,71 @@ struct KernelTimer {
 };
 
 void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s, mcv_keypoint* d_kps, uint8_t* d_desc,
-                      int out_cap, int* d_counts, KernelTimer* rec, bool reset_err, hipStream_t side) {
+                      int out_cap, int* d_counts, KernelTimer* rec, bool reset_err, hipStream_t side, int part) {
     auto mark = [&](const char* n) {
         if (rec) rec->mark(n);
     };
-    const bool fork = side != nullptr && rec == nullptr && (tune.blur & 2) != 0;
+    const bool fork = part == kPartAll && side != nullptr && rec == nullptr && (tune.blur & 2) != 0;
     const OrbGlobals g = globals(c);
+    const Strip* st = d_strips.as<Strip>();
+    auto strip_grid = [](int n) { return dim3(xcd_grid((n + 3) / 4)); };
+    BlurTaps taps{};
+    for (int k = 0; k < 7; k++) taps.t[k] = c.gauss[k];
+    taps.lo4 = (uint32_t)c.gauss[0] | (uint32_t)c.gauss[1] << 8 | (uint32_t)c.gauss[2] << 16 | (uint32_t)c.gauss[3] << 24;
+    taps.hi3 = (uint32_t)c.gauss[4] | (uint32_t)c.gauss[5] << 8 | (uint32_t)c.gauss[6] << 16;
+    const int ns = (int)strips.size();
+    auto blur = [&](hipStream_t bs) {
+        if (tune.blur & 1)
+            hipLaunchKernelGGL(k_blur<true>, strip_grid(ns), dim3(256), 0, bs, st, ns, g.lv, g.pyr, g.blur, taps);
+        else
+            hipLaunchKernelGGL(k_blur<false>, strip_grid(ns), dim3(256), 0, bs, st, ns, g.lv, g.pyr, g.blur, taps);
+        mark("k_blur");
+    };
+    // dynamic LDS: the heap (8 B per entry), then the node-table prefix (8 B per entry)
+    auto heap = [&]() {
+        const size_t heap_lds = 8 * (size_t)(heap_cap + kHeapTabLds);
+        if (tune.heap == 1)
+            hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), heap_lds, s, g, d_heap_order.as<int>(),
+                               heap_cap, kHeapTabLds);
+        else
+            hipLaunchKernelGGL(k_octree_heap_r, dim3(lv.size()), dim3(64), heap_lds, s, g, d_heap_order.as<int>(),
+                               heap_cap, kHeapTabLds);
+        mark("k_octree_heap");
+    };
+    auto keypoints = [&]() {
+        float* kps = reinterpret_cast<float*>(d_kps);
+        if (tune.kp == 1)
+            hipLaunchKernelGGL(k_keypoints_k<8>, dim3(xcd_grid((n_slots + 31) / 32)), dim3(256), 0, s, g,
+                               d_slot.as<int>(), n_slots, tb, kps, d_desc, out_cap, d_counts);
+        else if (tune.kp == 2)
+            hipLaunchKernelGGL(k_keypoints_k<16>, dim3(xcd_grid((n_slots + 63) / 64)), dim3(256), 0, s, g,
+                               d_slot.as<int>(), n_slots, tb, kps, d_desc, out_cap, d_counts);
+        else if (tune.kp == 3)
+            hipLaunchKernelGGL(k_keypoints_k<32>, dim3(xcd_grid((n_slots + 127) / 128)), dim3(256), 0, s, g,
+                               d_slot.as<int>(), n_slots, tb, kps, d_desc, out_cap, d_counts);
+        else
+            hipLaunchKernelGGL(k_keypoints, dim3(xcd_grid((n_slots + 3) / 4)), dim3(256), 0, s, g, d_slot.as<int>(),
+                               n_slots, tb, kps, d_desc, out_cap, d_counts);
+        mark("k_keypoints");
+    };
+    if (part == kPartBlur) {
+        if (ns > 0) blur(s);
+        HIP_OK(hipGetLastError());
+        return;
+    }
+    if (part == kPartHeap) {
+        heap();
+        HIP_OK(hipGetLastError());
+        return;
+    }
+    if (part == kPartPost) {
+        keypoints();
+        HIP_OK(hipGetLastError());
+        return;
+    }
     // words 0-1: pool words requested (u64), words 2-3: error bits
     hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(d_used.p), reset_err ? 4 : 2);
     mark("k_zero_words");
-    const Strip* st = d_strips.as<Strip>();
-    auto strip_grid = [](int n) { return dim3(xcd_grid((n + 3) / 4)); };
     const int n0 = strip_begin[1] - strip_begin[0];
     if (n0 > 0) {
         hipLaunchKernelGGL(k_copy_level0, strip_grid(n0), dim3(256), 0, s, st, n0, g.lv, d_src.as<const uint8_t*>(),
@@ -703,20 +763,8 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
             mark("k_resize");
         }
     }
-    BlurTaps taps{};
-    for (int k = 0; k < 7; k++) taps.t[k] = c.gauss[k];
-    taps.lo4 = (uint32_t)c.gauss[0] | (uint32_t)c.gauss[1] << 8 | (uint32_t)c.gauss[2] << 16 | (uint32_t)c.gauss[3] << 24;
-    taps.hi3 = (uint32_t)c.gauss[4] | (uint32_t)c.gauss[5] << 8 | (uint32_t)c.gauss[6] << 16;
-    const int ns = (int)strips.size();
-    auto blur = [&](hipStream_t bs) {
-        if (tune.blur & 1)
-            hipLaunchKernelGGL(k_blur<true>, strip_grid(ns), dim3(256), 0, bs, st, ns, g.lv, g.pyr, g.blur, taps);
-        else
-            hipLaunchKernelGGL(k_blur<false>, strip_grid(ns), dim3(256), 0, bs, st, ns, g.lv, g.pyr, g.blur, taps);
-        mark("k_blur");
-    };
     if (ns > 0) {
-        if (!fork) blur(s);
+        if (!fork && part == kPartAll) blur(s);
         hipLaunchKernelGGL(k_fast_score, strip_grid(ns), dim3(256), 0, s, st, ns, g.lv, g.pyr, g.score,
                            std::min(g.ini_th, g.min_th));
         mark("k_fast_score");
@@ -732,6 +780,10 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
         hipLaunchKernelGGL(k_octree_build<256>, dim3((int)lv.size() - n_build_large), dim3(256), 0, s, g,
                            d_build_levels.as<int>() + n_build_large);
     mark("k_octree_build");
+    if (part == kPartPre) {
+        HIP_OK(hipGetLastError());
+        return;
+    }
     if (fork && ns > 0) {
         if (!ev_fork) HIP_OK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
         if (!ev_join) HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
@@ -740,30 +792,9 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
         blur(side);
         HIP_OK(hipEventRecord(ev_join, side));
     }
-    // dynamic LDS: the heap (8 B per entry), then the node-table prefix (8 B per entry)
-    const size_t heap_lds = 8 * (size_t)(heap_cap + kHeapTabLds);
-    if (tune.heap == 1)
-        hipLaunchKernelGGL(k_octree_heap, dim3(lv.size()), dim3(64), heap_lds, s, g, d_heap_order.as<int>(), heap_cap,
-                           kHeapTabLds);
-    else
-        hipLaunchKernelGGL(k_octree_heap_r, dim3(lv.size()), dim3(64), heap_lds, s, g, d_heap_order.as<int>(),
-                           heap_cap, kHeapTabLds);
-    mark("k_octree_heap");
+    heap();
     if (fork && ns > 0) HIP_OK(hipStreamWaitEvent(s, ev_join, 0));
-    float* kps = reinterpret_cast<float*>(d_kps);
-    if (tune.kp == 1)
-        hipLaunchKernelGGL(k_keypoints_k<8>, dim3(xcd_grid((n_slots + 31) / 32)), dim3(256), 0, s, g, d_slot.as<int>(),
-                           n_slots, tb, kps, d_desc, out_cap, d_counts);
-    else if (tune.kp == 2)
-        hipLaunchKernelGGL(k_keypoints_k<16>, dim3(xcd_grid((n_slots + 63) / 64)), dim3(256), 0, s, g,
-                           d_slot.as<int>(), n_slots, tb, kps, d_desc, out_cap, d_counts);
-    else if (tune.kp == 3)
-        hipLaunchKernelGGL(k_keypoints_k<32>, dim3(xcd_grid((n_slots + 127) / 128)), dim3(256), 0, s, g,
-                           d_slot.as<int>(), n_slots, tb, kps, d_desc, out_cap, d_counts);
-    else
-        hipLaunchKernelGGL(k_keypoints, dim3(xcd_grid((n_slots + 3) / 4)), dim3(256), 0, s, g, d_slot.as<int>(),
-                           n_slots, tb, kps, d_desc, out_cap, d_counts);
-    mark("k_keypoints");
+    keypoints();
     HIP_OK(hipGetLastError());
 }
 
@@ -835,14 +866,22 @@ struct mcv_rig {
     DevBuf kps, desc, counts, knn, matches, mcounts, pairs, jobs, scratch;
     std::vector<MatchPair> hp;
     std::vector<FilterJob> hj;
-    // Captured graphs of the batched run, one per launch-variant set used so
-    // far: switching variants back and forth reuses an instantiated graph
-    // (its branch streams and their hardware queues stay as they were)
-    // instead of capturing again.  graph is the one of the current variants
-    // (owned by graphs) or null until the next run captures it.
-    hipGraphExec_t graph = nullptr;
-    std::vector<std::pair<Tuning, hipGraphExec_t>> graphs;
+    // Captured graphs of the batched run, one set per launch-variant set used
+    // so far: switching variants back and forth reuses instantiated graphs
+    // instead of capturing again.  With the blur beside the octree heap the
+    // step is two graphs, g up to the octree build and g2 from the keypoint
+    // kernel on, with the blur (on the context's side stream) and the heap
+    // replay (on the run's stream) launched between them; otherwise g alone
+    // is the step.  cur indexes the current variants' set, -1 until the next
+    // run captures it.
+    struct RigGraph {
+        Tuning t;
+        hipGraphExec_t g = nullptr, g2 = nullptr;
+    };
+    std::vector<RigGraph> graphs;
+    int cur = -1;
     static constexpr size_t kMaxGraphs = 16;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<const uint8_t*> graph_inputs;
     KernelTimer timer;
     bool last_batched = false;
@@ -851,23 +890,31 @@ struct mcv_rig {
     DevBuf st_u, st_d, st_sad, st_keys, st_jobs;
     // destroys every captured graph (buffers moved, or the variants were retimed)
     void drop_graphs() {
-        for (auto& e : graphs) HIP_OK(hipGraphExecDestroy(e.second));
+        for (auto& e : graphs) {
+            if (e.g) HIP_OK(hipGraphExecDestroy(e.g));
+            if (e.g2) HIP_OK(hipGraphExecDestroy(e.g2));
+        }
         graphs.clear();
-        graph = nullptr;
+        cur = -1;
     }
-    // the graph of the current variants, if one was captured
+    // the graphs of the current variants, if they were captured
     void pick_graph() {
-        graph = nullptr;
-        for (auto& e : graphs)
-            if (e.first == tune) graph = e.second;
+        cur = -1;
+        for (size_t i = 0; i < graphs.size(); i++)
+            if (graphs[i].t == tune) cur = (int)i;
     }
-    void keep_graph(hipGraphExec_t g) {
+    void keep_graph(const RigGraph& g) {
         if (graphs.size() >= kMaxGraphs) drop_graphs();
-        graphs.emplace_back(tune, g);
-        graph = g;
+        graphs.push_back(g);
+        cur = (int)graphs.size() - 1;
     }
     ~mcv_rig() {
-        for (auto& e : graphs) (void)hipGraphExecDestroy(e.second);
+        for (auto& e : graphs) {
+            if (e.g) (void)hipGraphExecDestroy(e.g);
+            if (e.g2) (void)hipGraphExecDestroy(e.g2);
+        }
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
     }
 };
 
@@ -1565,6 +1612,73 @@ static void rig_enqueue_batched(mcv_rig* r, const SinCosTables& tb, hipStream_t 
     HIP_OK(hipGetLastError());
 }
 
+// Captures the current variants' graphs of the batched step (the image
+// table copies stay outside: mcv_rig_run enqueues them before every launch).
+static void rig_capture(mcv_rig* r, const SinCosTables& tb) {
+    mcv_ctx* c = r->ctx;
+    OrbPlan& P = *r->batch;
+    P.tune = r->tune;
+    hipStream_t cs = c->capture_stream();
+    auto capture = [&](auto&& body) {
+        hipGraph_t gr;
+        hipGraphExec_t ge = nullptr;
+        HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        body();
+        HIP_OK(hipStreamEndCapture(cs, &gr));
+        HIP_OK(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
+        HIP_OK(hipGraphDestroy(gr));
+        return ge;
+    };
+    mcv_rig::RigGraph G;
+    G.t = r->tune;
+    if ((r->tune.blur & 2) == 0) {
+        G.g = capture([&] { rig_enqueue_batched(r, tb, cs); });
+    } else {
+        G.g = capture([&] {
+            P.enqueue(c->orb, tb, cs, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>(),
+                      nullptr, true, nullptr, kPartPre);
+        });
+        G.g2 = capture([&] {
+            P.enqueue(c->orb, tb, cs, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>(),
+                      nullptr, true, nullptr, kPartPost);
+            const int np = (int)r->hp.size();
+            launch_knn2(r->tune.knn, r->pairs.as<MatchPair>(), np, r->cap, cs);
+            hipLaunchKernelGGL(k_filter, dim3(np), dim3(1024), 0, cs, r->jobs.as<FilterJob>());
+            HIP_OK(hipGetLastError());
+        });
+    }
+    r->keep_graph(G);
+}
+
+// Launches the current variants' graphs on s.  A split step forks the blur
+// onto the context's side stream after the first graph and joins it before
+// the second; the heap replay runs on s in between.  A step's blur has
+// finished before the step's second graph starts, so the next step's
+// pyramid writes on s never overtake it.
+static void rig_launch(mcv_rig* r, const SinCosTables& tb, hipStream_t s) {
+    mcv_ctx* c = r->ctx;
+    const mcv_rig::RigGraph& G = r->graphs[(size_t)r->cur];
+    if (!G.g2) {
+        HIP_OK(hipGraphLaunch(G.g, s));
+        return;
+    }
+    OrbPlan& P = *r->batch;
+    P.tune = r->tune;
+    hipStream_t side = c->side_stream();
+    if (!r->ev_fork) HIP_OK(hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming));
+    if (!r->ev_join) HIP_OK(hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming));
+    HIP_OK(hipGraphLaunch(G.g, s));
+    HIP_OK(hipEventRecord(r->ev_fork, s));
+    HIP_OK(hipStreamWaitEvent(side, r->ev_fork, 0));
+    P.enqueue(c->orb, tb, side, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>(), nullptr,
+              true, nullptr, kPartBlur);
+    HIP_OK(hipEventRecord(r->ev_join, side));
+    P.enqueue(c->orb, tb, s, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>(), nullptr,
+              true, nullptr, kPartHeap);
+    HIP_OK(hipStreamWaitEvent(s, r->ev_join, 0));
+    HIP_OK(hipGraphLaunch(G.g2, s));
+}
+
 // The batched sequence launched eagerly with an event after every kernel;
 // ms[i] is the duration of the i-th launch, names[i] its kernel.
 int mcv_rig_run_timed(mcv_rig* r, const uint8_t* const* d_images, float* ms, const char** names, int cap, int* n) {
@@ -1699,24 +1813,14 @@ int mcv_rig_autotune(mcv_rig* r, const uint8_t* const* d_images, int reps, char*
         auto time_graph = [&]() -> float {
             P.tune = r->tune;
             r->pick_graph();
-            if (!r->graph) {
-                hipGraph_t gr;
-                hipGraphExec_t ge1 = nullptr;
-                hipStream_t cs = c->capture_stream();
-                HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-                rig_enqueue_batched(r, tb, cs, nullptr, c->side_stream());
-                HIP_OK(hipStreamEndCapture(cs, &gr));
-                HIP_OK(hipGraphInstantiate(&ge1, gr, nullptr, nullptr, 0));
-                HIP_OK(hipGraphDestroy(gr));
-                r->keep_graph(ge1);
-            }
+            if (r->cur < 0) rig_capture(r, tb);
             // back-to-back runs as mcv_rig_run enqueues them (the image
             // table copies, then the replay)
             constexpr int kBack = 4;
             auto launch = [&]() {
                 HIP_OK(hipMemcpyAsync(P.d_src.p, d_images, sizeof(void*) * ni, hipMemcpyHostToDevice, s));
                 HIP_OK(hipMemcpyAsync(P.d_stride.p, st.data(), sizeof(int) * ni, hipMemcpyHostToDevice, s));
-                HIP_OK(hipGraphLaunch(r->graph, s));
+                rig_launch(r, tb, s);
             };
             for (int j = 0; j < 3; j++) launch();
             HIP_OK(hipStreamSynchronize(s));
@@ -1804,18 +1908,8 @@ int mcv_rig_run(mcv_rig* r, const uint8_t* const* d_images, int flags) {
         for (int i = 0; i < n; i++) st[i] = r->w[i % 3];
         HIP_OK(hipMemcpyAsync(P.d_src.p, d_images, sizeof(void*) * n, hipMemcpyHostToDevice, s));
         HIP_OK(hipMemcpyAsync(P.d_stride.p, st.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-        if (!r->graph) {
-            hipGraph_t gr;
-            hipStream_t cs = c->capture_stream();
-            HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            rig_enqueue_batched(r, tb, cs, nullptr, c->side_stream());
-            HIP_OK(hipStreamEndCapture(cs, &gr));
-            hipGraphExec_t ge = nullptr;
-            HIP_OK(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
-            HIP_OK(hipGraphDestroy(gr));
-            r->keep_graph(ge);
-        }
-        HIP_OK(hipGraphLaunch(r->graph, s));
+        if (r->cur < 0) rig_capture(r, tb);
+        rig_launch(r, tb, s);
         r->last_batched = true;
     } else {
         r->last_batched = false;

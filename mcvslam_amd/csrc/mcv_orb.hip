@@ -415,15 +415,18 @@ __device__ __forceinline__ int byte_of(uint32_t w, int j) { return (int)((w >> (
 // fits the level, else 0 (values at or below both thresholds never change a
 // corner test or a 3x3 suppression).
 //
-// A wave walks its strip down row by row.  The rows it needs sit in an LDS
-// ring (16 rows x 264 bytes: the strip plus 4 bytes either side), each row
-// fetched with one coalesced u32 per lane one iteration ahead.  A compass
+// A wave walks its strip down in groups of 4 rows.  The rows it needs sit in
+// an LDS ring (16 rows x 264 bytes: the strip plus 4 bytes either side),
+// fetched with one coalesced u32 per lane per row: the 4 rows the next group
+// adds are loaded into registers when a group starts and stored to the ring
+// when it ends, so a group's compute covers the load latency (the prologue
+// issues its 10 rows at once).  A compass
 // bound rejects most pixels: every 9-arc holds two consecutive compass
 // points (0,4,8,12), so a corner at t needs such a pair both darker than
 // v - t or both brighter than v + t; the test runs on 2 pixels per packed
 // u16 operation.  Survivors queue up in LDS and are scored 64 at a time by
-// the whole wave from the ring; an 8-row block of scores is assembled in LDS
-// and stored once its queue is drained.
+// the whole wave from the ring; the group's 4-row block of scores is
+// assembled in LDS and stored once its queue is drained.
 
 typedef short mcv_s2 __attribute__((ext_vector_type(2)));
 
@@ -447,15 +450,20 @@ __device__ __forceinline__ uint32_t compass2(mcv_us2 v, mcv_us2 s, mcv_us2 e, mc
     return __builtin_bit_cast(uint32_t, r);
 }
 
-// exact best of the pixel at byte c of the ring rows r[0..6] (rows -3..+3).
+// exact best of the pixel at base[3 * kRowB]: base points into row -3 of 7
+// consecutive rows kRowB bytes apart (the circle's rows -3 .. +3), so every
+// circle byte is one LDS read at an immediate offset from one address.
 // Both polarities run in one packed-i16 min/max tree: lane half 0 carries
-// v - p_k (darker arcs), half 1 carries p_k - v (brighter arcs).
-
-__device__ __forceinline__ int fast_best_rows(const uint8_t* const* r, int c) {
-    const int v = r[3][c];
-    const int p[16] = {r[6][c],     r[6][c + 1], r[5][c + 2], r[4][c + 3], r[3][c + 3], r[2][c + 3],
-                       r[1][c + 2], r[0][c + 1], r[0][c],     r[0][c - 1], r[1][c - 2], r[2][c - 3],
-                       r[3][c - 3], r[4][c - 3], r[5][c - 2], r[6][c - 1]};
+// v - p_k (darker arcs), half 1 carries p_k - v (brighter arcs).  The 16
+// arcs of 9 are taken in pairs: for odd k, arcs k - 1 and k share
+// m8 = min(e[k .. k + 7]), so their maximum is min(m8, max(e[k - 1], e[k + 8])).
+template <int kRowB>
+__device__ __forceinline__ int fast_best_at(const uint8_t* base) {
+    const int v = base[3 * kRowB];
+    const int p[16] = {base[6 * kRowB],     base[6 * kRowB + 1], base[5 * kRowB + 2], base[4 * kRowB + 3],
+                       base[3 * kRowB + 3], base[2 * kRowB + 3], base[kRowB + 2],     base[1],
+                       base[0],             base[-1],            base[kRowB - 2],     base[2 * kRowB - 3],
+                       base[3 * kRowB - 3], base[4 * kRowB - 3], base[5 * kRowB - 2], base[6 * kRowB - 1]};
     mcv_s2 e[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) {
@@ -463,39 +471,41 @@ __device__ __forceinline__ int fast_best_rows(const uint8_t* const* r, int c) {
         const mcv_s2 pp = {(short)p[k], (short)p[k]};
         e[k] = pp * mcv_s2{-1, 1} + mcv_s2{(short)v, (short)-v};
     }
-    mcv_s2 m2[16], m4[16];
+    // i = 0..7 stands for k = 2 i + 1: m2[i] = min e[k .. k + 1], m4[i] = min e[k .. k + 3]
+    mcv_s2 m2[8], m4[8], a[8];
 #pragma unroll
-    for (int k = 0; k < 16; k++) m2[k] = __builtin_elementwise_min(e[k], e[(k + 1) & 15]);
+    for (int i = 0; i < 8; i++) m2[i] = __builtin_elementwise_min(e[2 * i + 1], e[(2 * i + 2) & 15]);
 #pragma unroll
-    for (int k = 0; k < 16; k++) m4[k] = __builtin_elementwise_min(m2[k], m2[(k + 2) & 15]);
-    // the 16 arc minima, then their maximum as a tree (independent packed
-    // operations back to back, no dependent pair)
-    mcv_s2 am[16];
+    for (int i = 0; i < 8; i++) m4[i] = __builtin_elementwise_min(m2[i], m2[(i + 1) & 7]);
 #pragma unroll
-    for (int k = 0; k < 16; k++)
-        am[k] = __builtin_elementwise_min(__builtin_elementwise_min(m4[k], m4[(k + 4) & 15]), e[(k + 8) & 15]);
+    for (int i = 0; i < 8; i++)
+        a[i] = __builtin_elementwise_min(__builtin_elementwise_min(m4[i], m4[(i + 2) & 7]),
+                                         __builtin_elementwise_max(e[2 * i], e[(2 * i + 9) & 15]));
 #pragma unroll
-    for (int h = 8; h >= 1; h >>= 1)
+    for (int h = 4; h >= 1; h >>= 1)
 #pragma unroll
-        for (int k = 0; k < h; k++) am[k] = __builtin_elementwise_max(am[k], am[k + h]);
-    const mcv_s2 best = am[0];
-    const int b = max((int)best.x, (int)best.y);
+        for (int i = 0; i < h; i++) a[i] = __builtin_elementwise_max(a[i], a[i + h]);
+    const int b = max((int)a[0].x, (int)a[0].y);
     return b < 0 ? 0 : b;
 }
 
 constexpr int kRing = 16;                 // ring rows per wave (power of two)
+constexpr int kRingG = 6;                 // guard rows: slots 0..5 repeated after slot 15
 constexpr int kRingW = kStripW / 4 + 2;   // u32 per ring row: strip + one word either side
-constexpr int kBlk = 8;                   // rows per score block
+constexpr int kBlk = 4;                   // rows per group (prefetch) and per score block
+// the ring holds a group's rows -3 .. +3 (10) and the next group's 4 new
+// rows (stored at the group's end) without overwriting a row the group reads
+static_assert(2 * kBlk + 6 <= kRing, "FAST ring too small for the prefetch group");
 
 __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, const LevelDesc* lv, const uint8_t* pyr,
                                                     uint8_t* score, int t_lo) {
-    __shared__ uint32_t s_ring[4][kRing][kRingW];
+    __shared__ uint32_t s_ring[4][kRing + kRingG][kRingW];
     __shared__ uint32_t s_tile[4][kBlk][kStripW / 4];
-    __shared__ uint32_t s_queue[4][64 + kStripW];
+    __shared__ uint16_t s_queue[4][64 + kStripW];
     STRIP_SETUP(st, n, false)
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     uint32_t (*ring)[kRingW] = s_ring[wv];
-    uint32_t* queue = s_queue[wv];
+    uint16_t* queue = s_queue[wv];
     const uint8_t* img = pyr + L.pix;
     const bool own = x < L.w;
     const mcv_s2 tt = {(short)t_lo, (short)t_lo}, ntt = {(short)-t_lo, (short)-t_lo};
@@ -514,80 +524,102 @@ __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, cons
         if (lane == 0 && S.x0 >= 4) e = *reinterpret_cast<const uint32_t*>(row + S.x0 - 4);
         if (lane == 63 && S.x0 + kStripW < L.pitch) e = *reinterpret_cast<const uint32_t*>(row + S.x0 + kStripW);
     };
+    // slot r mod 16, and slot r mod 16 + 16 too when that is a guard row, so
+    // any 7 rows in a row of slots are consecutive in LDS
     auto store_row = [&](int r, uint32_t a, uint32_t e) {
-        uint32_t* R = ring[r & (kRing - 1)];
+        const int sl = r & (kRing - 1);
+        uint32_t* R = ring[sl];
         R[lane + 1] = a;
         if (lane == 0) R[0] = e;
         if (lane == 63) R[kRingW - 1] = e;
+        if (sl < kRingG) {
+            uint32_t* G = ring[sl + kRing];
+            G[lane + 1] = a;
+            if (lane == 0) G[0] = e;
+            if (lane == 63) G[kRingW - 1] = e;
+        }
     };
-    uint32_t pa, pe;
-    for (int r = -3; r < 3; r++) {
-        load_row(S.y0 + r, pa, pe);
-        store_row(S.y0 + r, pa, pe);
+    {
+        // rows y0 - 3 .. y0 + kBlk + 2: all loads in flight, then the stores
+        uint32_t a0[kBlk + 6], e0[kBlk + 6];
+#pragma unroll
+        for (int k = 0; k < kBlk + 6; k++) load_row(S.y0 - 3 + k, a0[k], e0[k]);
+#pragma unroll
+        for (int k = 0; k < kBlk + 6; k++) store_row(S.y0 - 3 + k, a0[k], e0[k]);
     }
-    load_row(S.y0 + 3, pa, pe);
     int qlen = 0;
-    for (int y = S.y0; y < yend; y++) {
-        store_row(y + 3, pa, pe);
-        if (y + 1 < yend) load_row(y + 4, pa, pe);
-        const int tb = (y - S.y0) & (kBlk - 1);
-        s_tile[wv][tb][lane] = 0u;
-        wave_lds_sync();
-        int surv = 0;
-        if (y >= 3 && y < L.h - 3) {
-            const uint32_t* C = ring[y & (kRing - 1)];
-            const uint32_t c0 = C[lane], c1 = C[lane + 1], c2 = C[lane + 2];
-            const uint32_t up = ring[(y - 3) & (kRing - 1)][lane + 1];
-            const uint32_t dn = ring[(y + 3) & (kRing - 1)][lane + 1];
-            // pixels 0, 2 (even bytes) and 1, 3 (odd bytes) of the lane's
-            // word in u16 lanes; byte permutes pick the centre, north and
-            // south bytes and the east (x+3) and west (x-3) neighbours out of
-            // the adjacent words (selector 0x0c: zero byte)
-            const uint32_t m = 0x00FF00FFu, odd = 0x0c030c01u;
-            const uint32_t ev = compass2(as_us2(c1 & m), as_us2(dn & m), as_us2(__builtin_amdgcn_perm(c2, c1, 0x0c050c03u)),
-                                         as_us2(up & m), as_us2(__builtin_amdgcn_perm(c1, c0, 0x0c030c01u)), tt, ntt);
-            const uint32_t od = compass2(as_us2(__builtin_amdgcn_perm(0u, c1, odd)), as_us2(__builtin_amdgcn_perm(0u, dn, odd)),
-                                         as_us2(__builtin_amdgcn_perm(c2, c1, 0x0c060c04u)),
-                                         as_us2(__builtin_amdgcn_perm(0u, up, odd)),
-                                         as_us2(__builtin_amdgcn_perm(c1, c0, 0x0c040c02u)), tt, ntt);
-            // each 16-bit lane capped at 1, then bit 0 / 16 of ev and od
-            // moved to bits 0..3 (pixel order 0, 1, 2, 3)
-            const uint32_t e1 = as_u32(__builtin_elementwise_min(as_us2(ev), mcv_us2{1, 1}));
-            const uint32_t o1 = as_u32(__builtin_elementwise_min(as_us2(od), mcv_us2{1, 1}));
-            const uint32_t t2 = e1 | (o1 << 1);
-            surv = (int)(((t2 & 3u) | ((t2 >> 14) & 12u)) & (uint32_t)bmask);
-        }
+    for (int yg = S.y0; yg < yend; yg += kBlk) {
+        // the next group's new rows yg + kBlk + 3 .. yg + 2 kBlk + 2
+        const bool more = yg + kBlk < yend;
+        uint32_t na[kBlk], ne[kBlk];
+        if (more) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const unsigned long long b = __ballot((surv >> j) & 1);
-            if ((surv >> j) & 1) queue[qlen + lanes_below(b)] = ((uint32_t)tb << 8) | (uint32_t)(4 * lane + j);
-            qlen += __popcll(b);
+            for (int k = 0; k < kBlk; k++) load_row(yg + kBlk + 3 + k, na[k], ne[k]);
         }
-        const bool block_end = tb == kBlk - 1 || y == yend - 1;
-        while (qlen >= 64 || (block_end && qlen > 0)) {
+        const int ye = min(yg + kBlk, yend);
+        for (int y = yg; y < ye; y++) {
+            const int tb = y - yg;
+            s_tile[wv][tb][lane] = 0u;
             wave_lds_sync();
-            const int cnt = min(qlen, 64);
-            if (lane < cnt) {
-                const uint32_t e = queue[qlen - cnt + lane];
-                const int et = (int)(e >> 8), pos = (int)(e & 0xFF);
-                const int yy = y - tb + et;
-                const uint8_t* rows[7];
+            int surv = 0;
+            if (y >= 3 && y < L.h - 3) {
+                const uint32_t* C = ring[y & (kRing - 1)];
+                const uint32_t c0 = C[lane], c1 = C[lane + 1], c2 = C[lane + 2];
+                const uint32_t up = ring[(y - 3) & (kRing - 1)][lane + 1];
+                const uint32_t dn = ring[(y + 3) & (kRing - 1)][lane + 1];
+                // pixels 0, 2 (even bytes) and 1, 3 (odd bytes) of the lane's
+                // word in u16 lanes; byte permutes pick the centre, north and
+                // south bytes and the east (x+3) and west (x-3) neighbours out of
+                // the adjacent words (selector 0x0c: zero byte)
+                const uint32_t m = 0x00FF00FFu, odd = 0x0c030c01u;
+                const uint32_t ev = compass2(as_us2(c1 & m), as_us2(dn & m), as_us2(__builtin_amdgcn_perm(c2, c1, 0x0c050c03u)),
+                                             as_us2(up & m), as_us2(__builtin_amdgcn_perm(c1, c0, 0x0c030c01u)), tt, ntt);
+                const uint32_t od = compass2(as_us2(__builtin_amdgcn_perm(0u, c1, odd)), as_us2(__builtin_amdgcn_perm(0u, dn, odd)),
+                                             as_us2(__builtin_amdgcn_perm(c2, c1, 0x0c060c04u)),
+                                             as_us2(__builtin_amdgcn_perm(0u, up, odd)),
+                                             as_us2(__builtin_amdgcn_perm(c1, c0, 0x0c040c02u)), tt, ntt);
+                // each 16-bit lane capped at 1, then bit 0 / 16 of ev and od
+                // moved to bits 0..3 (pixel order 0, 1, 2, 3)
+                const uint32_t e1 = as_u32(__builtin_elementwise_min(as_us2(ev), mcv_us2{1, 1}));
+                const uint32_t o1 = as_u32(__builtin_elementwise_min(as_us2(od), mcv_us2{1, 1}));
+                const uint32_t t2 = e1 | (o1 << 1);
+                surv = (int)(((t2 & 3u) | ((t2 >> 14) & 12u)) & (uint32_t)bmask);
+            }
 #pragma unroll
-                for (int k = 0; k < 7; k++)
-                    rows[k] = reinterpret_cast<const uint8_t*>(ring[(yy - 3 + k) & (kRing - 1)]);
-                const int b = fast_best_rows(rows, 4 + pos);
-                if (b > t_lo) reinterpret_cast<uint8_t*>(s_tile[wv][et])[pos] = (uint8_t)b;
+            for (int j = 0; j < 4; j++) {
+                const unsigned long long b = __ballot((surv >> j) & 1);
+                if ((surv >> j) & 1) queue[qlen + lanes_below(b)] = (uint16_t)((tb << 8) | (4 * lane + j));
+                qlen += __popcll(b);
             }
-            qlen -= cnt;
+            const bool block_end = y == ye - 1;
+            while (qlen >= 64 || (block_end && qlen > 0)) {
+                wave_lds_sync();
+                const int cnt = min(qlen, 64);
+                if (lane < cnt) {
+                    const uint32_t e = queue[qlen - cnt + lane];
+                    const int et = (int)(e >> 8), pos = (int)(e & 0xFF);
+                    const int yy = y - tb + et;
+                    // rows yy - 3 .. yy + 3 from slot (yy - 3) mod 16 on (guard rows past 15)
+                    const uint8_t* base = reinterpret_cast<const uint8_t*>(ring[(yy - 3) & (kRing - 1)]) + 4 + pos;
+                    const int b = fast_best_at<kRingW * 4>(base);
+                    if (b > t_lo) reinterpret_cast<uint8_t*>(s_tile[wv][et])[pos] = (uint8_t)b;
+                }
+                qlen -= cnt;
+            }
+            if (block_end) {
+                wave_lds_sync();
+                if (own) {
+                    for (int k = 0; k <= tb; k++)
+                        *reinterpret_cast<uint32_t*>(score + L.pix + (uint32_t)((y - tb + k) * L.pitch + x)) =
+                            s_tile[wv][k][lane];
+                }
+                wave_lds_sync();
+            }
         }
-        if (block_end) {
-            wave_lds_sync();
-            if (own) {
-                for (int k = 0; k <= tb; k++)
-                    *reinterpret_cast<uint32_t*>(score + L.pix + (uint32_t)((y - tb + k) * L.pitch + x)) =
-                        s_tile[wv][k][lane];
-            }
-            wave_lds_sync();
+        if (more) {
+            // the ring slots of rows yg + kBlk + 3 + k last held rows yg - 13 + kBlk + k < yg - 3
+#pragma unroll
+            for (int k = 0; k < kBlk; k++) store_row(yg + kBlk + 3 + k, na[k], ne[k]);
         }
     }
 }

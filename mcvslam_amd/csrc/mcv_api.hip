@@ -381,7 +381,24 @@ static int level_roots(int w, int h) {
 
 // Launch choices that leave every output bit unchanged; picked by the
 // autotuner (mcv_rig_autotune) from timings on the real workload.
-enum EnqPart { kPartAll = 0, kPartPre = 1, kPartBlur = 2, kPartHeap = 3, kPartPost = 4 };
+// Pieces of OrbPlan::enqueue.  kPartPre: everything up to the octree build
+// except the blur; kPartPreA + kPartFastLo + kPartPreB + kPartPreC: the same
+// split so that FAST on the large levels (below kFastSplitLevel) can run
+// beside the resize of the small ones: the copy and the resizes below the
+// split level, FAST on the levels below it, the remaining resizes and FAST
+// on those levels, then the cell suppression and the octree build.
+enum EnqPart {
+    kPartAll = 0,
+    kPartPre = 1,
+    kPartBlur = 2,
+    kPartHeap = 3,
+    kPartPost = 4,
+    kPartPreA = 5,
+    kPartFastLo = 6,
+    kPartPreB = 7,
+    kPartPreC = 8
+};
+constexpr int kFastSplitLevel = 3;
 
 struct Tuning {
     int knn = 0;   // launch_knn2 variant
@@ -745,41 +762,73 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
         HIP_OK(hipGetLastError());
         return;
     }
-    // words 0-1: pool words requested (u64), words 2-3: error bits
-    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(d_used.p), reset_err ? 4 : 2);
-    mark("k_zero_words");
-    const int n0 = strip_begin[1] - strip_begin[0];
-    if (n0 > 0) {
-        hipLaunchKernelGGL(k_copy_level0, strip_grid(n0), dim3(256), 0, s, st, n0, g.lv, d_src.as<const uint8_t*>(),
-                           d_stride.as<int>(), g.pyr);
-        mark("k_copy_level0");
-    }
-    for (int l = 1; l < nlevels; l++) {
-        const int n = strip_begin[l + 1] - strip_begin[l];
-        if (n > 0) {
-            hipLaunchKernelGGL(k_resize, dim3(xcd_grid(n)), dim3(64), resize_rows[l] * resize_rw[l], s,
-                               st + strip_begin[l], n,
-                               g.lv, d_coef.as<int2>(), g.pyr, resize_rw[l]);
-            mark("k_resize");
+    // words 0-1: pool words requested (u64), words 2-3: error bits; level 0
+    auto head = [&]() {
+        hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(d_used.p),
+                           reset_err ? 4 : 2);
+        mark("k_zero_words");
+        const int n0 = strip_begin[1] - strip_begin[0];
+        if (n0 > 0) {
+            hipLaunchKernelGGL(k_copy_level0, strip_grid(n0), dim3(256), 0, s, st, n0, g.lv,
+                               d_src.as<const uint8_t*>(), d_stride.as<int>(), g.pyr);
+            mark("k_copy_level0");
         }
+    };
+    // levels l0 .. l1 - 1, each from the one before
+    auto resize_levels = [&](int l0, int l1) {
+        for (int l = std::max(l0, 1); l < l1; l++) {
+            const int n = strip_begin[l + 1] - strip_begin[l];
+            if (n > 0) {
+                hipLaunchKernelGGL(k_resize, dim3(xcd_grid(n)), dim3(64), resize_rows[l] * resize_rw[l], s,
+                                   st + strip_begin[l], n, g.lv, d_coef.as<int2>(), g.pyr, resize_rw[l]);
+                mark("k_resize");
+            }
+        }
+    };
+    // FAST scores of levels l0 .. l1 - 1 (their strips are contiguous)
+    auto fast = [&](int l0, int l1) {
+        const int b0 = strip_begin[l0], n = strip_begin[l1] - b0;
+        if (n > 0) {
+            hipLaunchKernelGGL(k_fast_score, strip_grid(n), dim3(256), 0, s, st + b0, n, g.lv, g.pyr, g.score,
+                               std::min(g.ini_th, g.min_th));
+            mark("k_fast_score");
+        }
+    };
+    auto cells_build = [&]() {
+        const int nc = (int)cells.size();
+        if (nc > 0) {
+            hipLaunchKernelGGL(k_fast_cells, dim3(xcd_grid((nc + 3) / 4)), dim3(256), 0, s, g, nc);
+            mark("k_fast_cells");
+        }
+        if (n_build_large > 0)
+            hipLaunchKernelGGL(k_octree_build<1024>, dim3(n_build_large), dim3(1024), 0, s, g,
+                               d_build_levels.as<int>());
+        if ((int)lv.size() > n_build_large)
+            hipLaunchKernelGGL(k_octree_build<256>, dim3((int)lv.size() - n_build_large), dim3(256), 0, s, g,
+                               d_build_levels.as<int>() + n_build_large);
+        mark("k_octree_build");
+    };
+    const int lsplit = std::min(kFastSplitLevel, nlevels);
+    if (part >= kPartPreA) {
+        if (part == kPartPreA) {
+            head();
+            resize_levels(1, lsplit);
+        } else if (part == kPartFastLo) {
+            fast(0, lsplit);
+        } else if (part == kPartPreB) {
+            resize_levels(lsplit, nlevels);
+            fast(lsplit, nlevels);
+        } else {
+            cells_build();
+        }
+        HIP_OK(hipGetLastError());
+        return;
     }
-    if (ns > 0) {
-        if (!fork && part == kPartAll) blur(s);
-        hipLaunchKernelGGL(k_fast_score, strip_grid(ns), dim3(256), 0, s, st, ns, g.lv, g.pyr, g.score,
-                           std::min(g.ini_th, g.min_th));
-        mark("k_fast_score");
-    }
-    const int nc = (int)cells.size();
-    if (nc > 0) {
-        hipLaunchKernelGGL(k_fast_cells, dim3(xcd_grid((nc + 3) / 4)), dim3(256), 0, s, g, nc);
-        mark("k_fast_cells");
-    }
-    if (n_build_large > 0)
-        hipLaunchKernelGGL(k_octree_build<1024>, dim3(n_build_large), dim3(1024), 0, s, g, d_build_levels.as<int>());
-    if ((int)lv.size() > n_build_large)
-        hipLaunchKernelGGL(k_octree_build<256>, dim3((int)lv.size() - n_build_large), dim3(256), 0, s, g,
-                           d_build_levels.as<int>() + n_build_large);
-    mark("k_octree_build");
+    head();
+    resize_levels(1, nlevels);
+    if (ns > 0 && !fork && part == kPartAll) blur(s);
+    fast(0, nlevels);
+    cells_build();
     if (part == kPartPre) {
         HIP_OK(hipGetLastError());
         return;
@@ -869,19 +918,18 @@ struct mcv_rig {
     // Captured graphs of the batched run, one set per launch-variant set used
     // so far: switching variants back and forth reuses instantiated graphs
     // instead of capturing again.  With the blur beside the octree heap the
-    // step is two graphs, g up to the octree build and g2 from the keypoint
-    // kernel on, with the blur (on the context's side stream) and the heap
-    // replay (on the run's stream) launched between them; otherwise g alone
-    // is the step.  cur indexes the current variants' set, -1 until the next
-    // run captures it.
+    // step is four graphs with launches on the context's side stream between
+    // them (rig_launch); otherwise g alone is the step.  cur indexes the
+    // current variants' set, -1 until the next run captures it.
     struct RigGraph {
         Tuning t;
         hipGraphExec_t g = nullptr, g2 = nullptr;
+        hipGraphExec_t pre_b = nullptr, pre_c = nullptr;  // split steps: see rig_launch
     };
     std::vector<RigGraph> graphs;
     int cur = -1;
     static constexpr size_t kMaxGraphs = 16;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_a = nullptr, ev_b = nullptr;
     std::vector<const uint8_t*> graph_inputs;
     KernelTimer timer;
     bool last_batched = false;
@@ -890,10 +938,9 @@ struct mcv_rig {
     DevBuf st_u, st_d, st_sad, st_keys, st_jobs;
     // destroys every captured graph (buffers moved, or the variants were retimed)
     void drop_graphs() {
-        for (auto& e : graphs) {
-            if (e.g) HIP_OK(hipGraphExecDestroy(e.g));
-            if (e.g2) HIP_OK(hipGraphExecDestroy(e.g2));
-        }
+        for (auto& e : graphs)
+            for (hipGraphExec_t x : {e.g, e.g2, e.pre_b, e.pre_c})
+                if (x) HIP_OK(hipGraphExecDestroy(x));
         graphs.clear();
         cur = -1;
     }
@@ -909,12 +956,11 @@ struct mcv_rig {
         cur = (int)graphs.size() - 1;
     }
     ~mcv_rig() {
-        for (auto& e : graphs) {
-            if (e.g) (void)hipGraphExecDestroy(e.g);
-            if (e.g2) (void)hipGraphExecDestroy(e.g2);
-        }
-        if (ev_fork) (void)hipEventDestroy(ev_fork);
-        if (ev_join) (void)hipEventDestroy(ev_join);
+        for (auto& e : graphs)
+            for (hipGraphExec_t x : {e.g, e.g2, e.pre_b, e.pre_c})
+                if (x) (void)hipGraphExecDestroy(x);
+        for (hipEvent_t x : {ev_fork, ev_join, ev_a, ev_b})
+            if (x) (void)hipEventDestroy(x);
     }
 };
 
@@ -1631,13 +1677,16 @@ static void rig_capture(mcv_rig* r, const SinCosTables& tb) {
     };
     mcv_rig::RigGraph G;
     G.t = r->tune;
+    auto piece = [&](int part) {
+        P.enqueue(c->orb, tb, cs, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>(),
+                  nullptr, true, nullptr, part);
+    };
     if ((r->tune.blur & 2) == 0) {
         G.g = capture([&] { rig_enqueue_batched(r, tb, cs); });
     } else {
-        G.g = capture([&] {
-            P.enqueue(c->orb, tb, cs, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>(),
-                      nullptr, true, nullptr, kPartPre);
-        });
+        G.g = capture([&] { piece(kPartPreA); });
+        G.pre_b = capture([&] { piece(kPartPreB); });
+        G.pre_c = capture([&] { piece(kPartPreC); });
         G.g2 = capture([&] {
             P.enqueue(c->orb, tb, cs, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>(),
                       nullptr, true, nullptr, kPartPost);
@@ -1650,11 +1699,14 @@ static void rig_capture(mcv_rig* r, const SinCosTables& tb) {
     r->keep_graph(G);
 }
 
-// Launches the current variants' graphs on s.  A split step forks the blur
-// onto the context's side stream after the first graph and joins it before
-// the second; the heap replay runs on s in between.  A step's blur has
-// finished before the step's second graph starts, so the next step's
-// pyramid writes on s never overtake it.
+// Launches the current variants' graphs on s.  A split step runs on s: the
+// copy and the resizes below the split level (g), the other resizes and
+// their FAST (pre_b), the cell suppression and the octree build (pre_c), the
+// heap replay, then the keypoints, top-2 and filter (g2); on the context's
+// side stream: FAST of the large levels once g is done (joined before pre_c)
+// and the blur once pre_c is done (joined before g2).  A step's side work
+// has finished before its g2 starts, so the next step's pyramid writes on s
+// never overtake it.
 static void rig_launch(mcv_rig* r, const SinCosTables& tb, hipStream_t s) {
     mcv_ctx* c = r->ctx;
     const mcv_rig::RigGraph& G = r->graphs[(size_t)r->cur];
@@ -1665,9 +1717,17 @@ static void rig_launch(mcv_rig* r, const SinCosTables& tb, hipStream_t s) {
     OrbPlan& P = *r->batch;
     P.tune = r->tune;
     hipStream_t side = c->side_stream();
-    if (!r->ev_fork) HIP_OK(hipEventCreateWithFlags(&r->ev_fork, hipEventDisableTiming));
-    if (!r->ev_join) HIP_OK(hipEventCreateWithFlags(&r->ev_join, hipEventDisableTiming));
+    for (hipEvent_t* e : {&r->ev_fork, &r->ev_join, &r->ev_a, &r->ev_b})
+        if (!*e) HIP_OK(hipEventCreateWithFlags(e, hipEventDisableTiming));
     HIP_OK(hipGraphLaunch(G.g, s));
+    HIP_OK(hipEventRecord(r->ev_a, s));
+    HIP_OK(hipStreamWaitEvent(side, r->ev_a, 0));
+    P.enqueue(c->orb, tb, side, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>(), nullptr,
+              true, nullptr, kPartFastLo);
+    HIP_OK(hipEventRecord(r->ev_b, side));
+    HIP_OK(hipGraphLaunch(G.pre_b, s));
+    HIP_OK(hipStreamWaitEvent(s, r->ev_b, 0));
+    HIP_OK(hipGraphLaunch(G.pre_c, s));
     HIP_OK(hipEventRecord(r->ev_fork, s));
     HIP_OK(hipStreamWaitEvent(side, r->ev_fork, 0));
     P.enqueue(c->orb, tb, side, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>(), nullptr,

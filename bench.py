@@ -98,7 +98,8 @@ def measured_traffic(path=ROOT / "profiles" / "hbm_traffic_rig64.txt"):
     calibration copy) from the committed rocprofv3 --pmc summary of this
     workload, keyed by the names the timed run reports; {} when it is absent.
     The summary lists bytes per dispatch and the dispatch count of the run;
-    k_fast_score runs once per step, so its count is the number of steps."""
+    k_zero_words runs once per step (graph replay or eager pass), so its
+    count is the number of steps."""
     rows = {}
     if not path.exists():
         return rows
@@ -106,9 +107,9 @@ def measured_traffic(path=ROOT / "profiles" / "hbm_traffic_rig64.txt"):
         parts = line.split()
         if len(parts) == 5 and parts[0].startswith("k_") and parts[1].isdigit():
             rows[parts[0]] = (int(parts[1]), int(parts[4]))
-    if "k_fast_score" not in rows:
+    if "k_zero_words" not in rows:
         return {}
-    steps = rows["k_fast_score"][0]
+    steps = rows["k_zero_words"][0]
     out = {}
     for name, (calls, per_call) in rows.items():
         # launch variants (k_knn2_s / k_knn2_lds, k_keypoints_k) under the

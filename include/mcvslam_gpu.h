@@ -174,8 +174,9 @@ void mcv_rig_destroy(mcv_rig* rig);
 int mcv_rig_autotune(mcv_rig* rig, const uint8_t* const* d_images, int reps, char* report, int report_cap);
 /* Fixes the rig's launch variants instead of tuning them: the top-2 kernel
  * (0 .. 9, as mcv_match_set_variant), the blur (bit 0: the walk, stop at the
- * level end or fixed length; bit 1: a captured graph runs it on a branch
- * beside the octree heap replay), the keypoint kernel (0: one keypoint per
+ * level end or fixed length; bit 1: the step runs as several graphs with
+ * FAST of the three largest levels and the blur on the context's side
+ * stream, the blur beside the octree heap replay), the keypoint kernel (0: one keypoint per
  * wave, 1: 8, 2: 16 (the default), 3: 32) and the octree heap replay (0: top
  * of the heap in registers, 1: heap in LDS (the default)).  No counterpart in
  * the reference; every combination produces identical outputs. */
@@ -186,8 +187,10 @@ int mcv_rig_get_variants(const mcv_rig* rig, int32_t* knn_variant, int32_t* blur
                          int32_t* heap_variant);
 /* d_images: host array of 3 * n_frames device pointers (frame-major L, R, W),
  * each image dense (stride = width).  flags: MCV_FLAG_GRAPH replays the whole
- * batch as one hipGraph; without it every image and every pair is launched on
- * its own ("plain" path). */
+ * batch as captured hipGraphs (one, or, with blur variant bit 1, several
+ * joined through the context's side stream; the run's stream waits for all
+ * of it); without it every image and every pair is launched on its own
+ * ("plain" path). */
 int mcv_rig_run(mcv_rig* rig, const uint8_t* const* d_images, int flags);
 /* Stereo matching of every frame's L-R pair on the pyramids of the last
  * batched run (Frame::ComputeStereoMatch, src/Frame.cpp:150-328): row band

@@ -5,7 +5,7 @@ step = one hipGraph launch over a batch of 64 rig frames (L and R 1280x720, W
 640x480; per-image ORB feature counts drawn from U{500..4000}, seed 40 + frame;
 8 levels, scale 1.2, FAST 20/7), then brute-force Hamming top-2 + ratio 0.7 +
 threshold 50 for L-R, W-L and W-R.  Inputs are resident in HBM before timing
-starts.  With --streams S (default 4) consecutive batches rotate over S rigs
+starts.  With --streams S (default 5) consecutive batches rotate over S rigs
 on S streams, so each batch's serial octree replay overlaps the other
 batches' pixel kernels; the one-stream rate is reported beside it.  Streams
 only run side by side on distinct hardware queues: the HIP runtime's
@@ -221,7 +221,7 @@ def main():
     ap.add_argument("--variants", type=str, default=None,
                     help="fixed rig launch variants 'knn,blur,kp,heap' instead of the autotuner's "
                          "(counter runs use the variants the autotuner chose)")
-    ap.add_argument("--streams", type=int, default=4, choices=(1, 2, 3, 4),
+    ap.add_argument("--streams", type=int, default=5, choices=range(1, 9), metavar="S",
                     help="batches in flight: S > 1 rotates S rigs on S streams")
     ap.add_argument("--hw-queues", type=int, default=16, choices=range(1, 33), metavar="Q",
                     help="hardware queues of this process (GPU_MAX_HW_QUEUES, 1..32)")

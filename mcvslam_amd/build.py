@@ -29,6 +29,10 @@ HIP_FLAGS = [
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     # packed FP32 instructions disabled
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # matrix-core accumulators in ordinary VGPRs: the top-2 kernel reads
+    # every result with a vector op, which would otherwise first copy it
+    # out of the accumulation registers
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]
 
 

@@ -129,3 +129,43 @@ def test_rig_fixed_variants(orb_default, knn, blur, kp, heap):
         rig.set_variants(0, 0, RigBatch.N_KP_VARIANTS)
     with pytest.raises(Exception):
         rig.set_variants(0, 0, 2, RigBatch.N_HEAP_VARIANTS)
+
+
+def test_rigs_in_flight(orb_default):
+    """Several rigs on contexts and streams of their own, steps enqueued back
+    to back without waiting (the split step forks onto each context's side
+    stream), variants switched back and forth between steps (the rig keeps
+    one set of graphs per variant set): every rig's outputs equal one rig's
+    run on the default context."""
+    import torch
+    from mcvslam_amd import ORB, Context
+    frames = [synth.rig_frame(900 + 10 * f) for f in range(2)]
+    imgs = [torch.from_numpy(im).cuda() for fr in frames for im in fr]
+    ptrs = [im.data_ptr() for im in imgs]
+    ref = RigBatch(orb_default, 2, ratio=0.7, th=50)
+    ref.run(ptrs, graph=True)
+    want = ref.read()
+    torch.cuda.synchronize()
+    streams, ctxs, rigs = [], [], []
+    for _ in range(3):
+        st = torch.cuda.Stream()
+        cx = Context(0)
+        cx.set_stream(st.cuda_stream)
+        streams.append(st)
+        ctxs.append(cx)
+        rigs.append(RigBatch(ORB(2000, 1.2, 8, 20, 7, ctx=cx), 2, ratio=0.7, th=50))
+    variant_sets = [(8, 2, 2, 1), (1, 0, 2, 1), (8, 3, 2, 0), (8, 2, 2, 1)]
+    try:
+        for vs in variant_sets:
+            for rg in rigs:
+                rg.set_variants(*vs)
+            for k in range(3 * len(rigs)):
+                rigs[k % len(rigs)].run(ptrs, graph=True)
+            torch.cuda.synchronize()
+            for rg in rigs:
+                assert_same_outputs(want, rg.read())
+    finally:
+        for rg in rigs:
+            rg.close()
+        for cx in ctxs:
+            cx.close()

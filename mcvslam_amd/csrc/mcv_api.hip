@@ -562,6 +562,11 @@ struct OrbPlan {
         n_build_large = (int)build_levels.size();
         for (size_t k = 0; k < lv.size(); k++)
             if ((long long)lv[k].w * lv[k].h < kOctLargePixels) build_levels.push_back((int)k);
+        // largest levels first in each launch: when a launch takes more than
+        // one round of workgroups, the later rounds get the small levels
+        auto by_area = [&](int a, int b) { return (long long)lv[a].w * lv[a].h > (long long)lv[b].w * lv[b].h; };
+        std::stable_sort(build_levels.begin(), build_levels.begin() + n_build_large, by_area);
+        std::stable_sort(build_levels.begin() + n_build_large, build_levels.end(), by_area);
         heap_order.resize(lv.size());
         for (size_t k = 0; k < lv.size(); k++) heap_order[k] = (int)k;
         std::stable_sort(heap_order.begin(), heap_order.end(),

@@ -935,8 +935,12 @@ __device__ __forceinline__ void block_excl_scan_depths(int (&v)[kOctreeDepth + 1
 template <int kOctThreads>
 __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g, const int* levels) {
     constexpr int kWaves = kOctThreads / 64;
-    __shared__ unsigned long long s_keys[kSortCapLds];
-    __shared__ uint8_t s_div[kSortCapLds];
+    // candidates sorted in LDS: the 256-thread workgroups (the small levels,
+    // at most a few hundred candidates) take half the capacity, so that more
+    // of them fit on a CU at once
+    constexpr int kCap = kOctThreads >= 1024 ? kSortCapLds : kSortCapLds / 2;
+    __shared__ unsigned long long s_keys[kCap];
+    __shared__ uint8_t s_div[kCap];
     __shared__ int ws[kWaves];
     __shared__ int s_ws[kWaves + 1][kOctreeDepth + 1];
     __shared__ long long s_off;
@@ -987,7 +991,7 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g, cons
     // pointers; the workgroup's waves share one L1, so barriers order it)
     unsigned long long* keys = s_keys;
     uint8_t* dv = s_div;
-    if (n > kSortCapLds) {
+    if (n > kCap) {
         if (threadIdx.x == 0) {
             const long long need = 2ll * n2 + (n + 3) / 4 + 2;
             const unsigned long long off = atomicAdd(g.pool_used, (unsigned long long)need);
@@ -1017,7 +1021,7 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g, cons
         keys[i] = k;
     }
     __syncthreads();
-    bitonic_sort<unsigned long long, (kSortCapLds / 2 + kOctThreads - 1) / kOctThreads>(keys, n2);
+    bitonic_sort<unsigned long long, (kCap / 2 + kOctThreads - 1) / kOctThreads>(keys, n2);
     // 3. divergence depth of each sorted position
     for (int i = threadIdx.x; i < n; i += kOctThreads)
         dv[i] = (uint8_t)(i == 0 ? 0 : code_div(keys[i] >> 24, keys[i - 1] >> 24));
@@ -1067,14 +1071,14 @@ __global__ __launch_bounds__(kOctThreads) void k_octree_build(OrbGlobals g, cons
     uint32_t* child = start + T;
     uint32_t* sorted = child + T;
     for (int i = threadIdx.x; i < n; i += kOctThreads) sorted[i] = (uint32_t)(keys[i] & 0xFFFFFF);
-    if (n <= kSortCapLds) {
+    if (n <= kCap) {
         // depth by depth: the entries of depth d (start) and of depth d - 1
         // (child) are assembled in LDS over the dead sort keys, then stored
         // as contiguous runs.  A thread's entries of one depth are the ranks
         // rank[d] .. of its positions that start a node there.
         __syncthreads();
         uint32_t* st_start = reinterpret_cast<uint32_t*>(s_keys);
-        uint32_t* st_child = st_start + kSortCapLds;
+        uint32_t* st_child = st_start + kCap;
 #pragma unroll
         for (int d = 0; d <= kOctreeDepth; d++) {
             if (d <= D) {

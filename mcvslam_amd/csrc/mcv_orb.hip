@@ -2193,7 +2193,12 @@ __global__ __launch_bounds__(256, 6) void k_keypoints_k(OrbGlobals g, const int*
     // The disk's 31 rows are read as u32 words from the aligned column below
     // cx - 15 (at most 9 words a row, 279 words over the wave); a lane's
     // (row, word) pairs do not depend on the keypoint.
-    int mv[5], mw[5], mdu[5];
+    // Per (row, word) pair the bytes inside the disk depend only on the
+    // alignment c = (cx - 15) & 3 of the word column: u of byte 0 is
+    // 4 w - 15 - c.  mpk[it] holds the 4-bit byte masks of the four
+    // alignments (nibble c), so a keypoint's masks are one shift away.
+    int mv[5], mw[5], mw4[5];
+    uint32_t mpk[5];
     {
         unsigned long long um = 0;
 #pragma unroll
@@ -2204,8 +2209,20 @@ __global__ __launch_bounds__(256, 6) void k_keypoints_k(OrbGlobals g, const int*
             const int row = t / 9;
             mv[it] = row - 15;
             mw[it] = t < 31 * 9 ? t - 9 * row : 99;  // 99: no word
+            mw4[it] = 4 * mw[it] - 15;
             const int av = mv[it] < 0 ? -mv[it] : mv[it];
-            mdu[it] = (int)((um >> (4 * min(av, 15))) & 15u);
+            const int du = (int)((um >> (4 * min(av, 15))) & 15u);
+            uint32_t pk = 0;
+            if (mw[it] != 99) {
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const int u0 = mw4[it] - c;
+                    const int blo = max(-du - u0, 0), bhi = min(du - u0, 3);
+                    const uint32_t nib = blo <= bhi ? ((0xFu << blo) & (0xFu >> (3 - bhi))) : 0u;
+                    pk |= nib << (4 * c);
+                }
+            }
+            mpk[it] = pk;
         }
     }
     int M10 = 0, M01 = 0;
@@ -2234,19 +2251,19 @@ __global__ __launch_bounds__(256, 6) void k_keypoints_k(OrbGlobals g, const int*
             for (int it = 0; it < 5; it++) cur[it] = dw[it];
             const int kn = todo ? __builtin_ctzll(todo) : -1;
             if (kn >= 0) mload(kn, dw);
-            const int base = (cx - 15) & ~3;
+            const int c = (cx - 15) & 3;  // wave-uniform
             int m10 = 0, m01 = 0;
 #pragma unroll
             for (int it = 0; it < 5; it++) {
-                const int u0 = base + 4 * mw[it] - cx;  // u of byte 0
-                const int blo = max(-mdu[it] - u0, 0), bhi = min(mdu[it] - u0, 3);
-                if (blo <= bhi) {
-                    const uint32_t m = cur[it] & (0xFFFFFFFFu << (8 * blo)) & (0xFFFFFFFFu >> (8 * (3 - bhi)));
-                    const int s1 = (int)__builtin_amdgcn_udot4(m, 0x01010101u, 0u, false);
-                    const int sb = (int)__builtin_amdgcn_udot4(m, 0x03020100u, 0u, false);
-                    m10 += __mul24(u0, s1) + sb;
-                    m01 += __mul24(mv[it], s1);
-                }
+                const int u0 = mw4[it] - c;  // u of byte 0
+                // the disk's bytes of the word as 0/1 weights, and as weights b
+                const uint32_t nib = (mpk[it] >> (4 * c)) & 15u;
+                const uint32_t w1 = __umul24(nib, 0x204081u) & 0x01010101u;
+                const uint32_t wb = (w1 * 255u) & 0x03020100u;
+                const int s1 = (int)__builtin_amdgcn_udot4(cur[it], w1, 0u, false);
+                const int sb = (int)__builtin_amdgcn_udot4(cur[it], wb, 0u, false);
+                m10 += __mul24(u0, s1) + sb;
+                m01 += __mul24(mv[it], s1);
             }
             wave_sum2_dpp(m10, m01);
             if (lane == k) {

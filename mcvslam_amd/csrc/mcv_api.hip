@@ -421,7 +421,6 @@ struct OrbPlan {
     std::vector<CellDesc> cells;
     std::vector<Strip> strips;
     std::vector<int> strip_begin;  // per level index l: [strip_begin[l], strip_begin[l+1])
-    std::vector<int> resize_rows, resize_rw;  // per level: staged source rows and bytes per row
     std::vector<int2> coef;
     std::vector<int> slot_lvl;
     std::vector<float> seeds;
@@ -582,29 +581,19 @@ struct OrbPlan {
             }
         }
         strip_begin[nlevels] = (int)strips.size();
-        // k_resize stages each pass's source rows and the strip's column window in LDS
-        resize_rows.assign(nlevels, 0);
-        resize_rw.assign(nlevels, 4);
-        for (int l = 1; l < nlevels; l++)
-            for (int k = strip_begin[l]; k < strip_begin[l + 1]; k++) {
-                const Strip& sp = strips[k];
-                const LevelDesc& d = lv[sp.lvl];
-                const LevelDesc& sl = lv[d.src];
-                const int yend = std::min(sp.y0 + kStripH, d.h);
-                const int xl = std::min(sp.x0 + kStripW - 1, d.w - 1);
-                const int cb = coef[d.coef_x + sp.x0].x & ~3;
-                const int c_hi = std::min(coef[d.coef_x + xl].x + 1, sl.w - 1);
-                for (int ys = sp.y0; ys < yend; ys += kResizePass) {  // the kernel's passes
-                    const int ye = std::min(ys + kResizePass, yend);
-                    const int r_lo = std::min(std::max(coef[d.coef_y + ys].x, 0), sl.h - 1);
-                    const int r_hi = std::min(std::max(coef[d.coef_y + ye - 1].x + 1, 0), sl.h - 1);
-                    resize_rows[l] = std::max(resize_rows[l], r_hi - r_lo + 1);
+        // k_resize: the source bytes of 4 adjacent outputs (both taps)
+        // lie in the 12 bytes from the first one's aligned word
+        for (const LevelDesc& d : lv) {
+            if (d.level == 0) continue;
+            const LevelDesc& sl = lv[d.src];
+            for (int x = 0; x < d.w; x += 4) {
+                const int wb = coef[d.coef_x + x].x & ~3;
+                for (int j = 0; j < 4; j++) {
+                    const int off = coef[d.coef_x + std::min(x + j, d.w - 1)].x - wb;
+                    REQUIRE(off >= 0 && off + 1 < 12 && coef[d.coef_x + std::min(x + j, d.w - 1)].x < sl.w,
+                            MCV_ERR_INVALID, "scale factor too large for the resize taps");
                 }
-                resize_rw[l] = std::max(resize_rw[l], ((c_hi - cb) / 4 + 1) * 4);
             }
-        for (int l = 1; l < nlevels; l++) {
-            REQUIRE(resize_rw[l] <= 4 * 3 * 64, MCV_ERR_INVALID, "scale factor too large for the resize window");
-            REQUIRE(resize_rows[l] * resize_rw[l] <= 65536, MCV_ERR_INVALID, "resize window exceeds LDS");
         }
         pool_cap = pool_words;
     }
@@ -784,8 +773,8 @@ void OrbPlan::enqueue(const OrbConfig& c, const SinCosTables& tb, hipStream_t s,
         for (int l = std::max(l0, 1); l < l1; l++) {
             const int n = strip_begin[l + 1] - strip_begin[l];
             if (n > 0) {
-                hipLaunchKernelGGL(k_resize, dim3(xcd_grid(n)), dim3(64), resize_rows[l] * resize_rw[l], s,
-                                   st + strip_begin[l], n, g.lv, d_coef.as<int2>(), g.pyr, resize_rw[l]);
+                hipLaunchKernelGGL(k_resize, dim3(xcd_grid(n)), dim3(64), 0, s, st + strip_begin[l], n, g.lv,
+                                   d_coef.as<int2>(), g.pyr);
                 mark("k_resize");
             }
         }

@@ -127,14 +127,15 @@ __global__ __launch_bounds__(256) void k_copy_level0(const Strip* st, int n, con
 // cv::resize INTER_LINEAR, CV_8U: horizontal int taps (coef 2048 scale) on
 // the two source rows, then the vertical pass
 // (((b0 * (r0 >> 4)) >> 16) + ((b1 * (r1 >> 4)) >> 16) + 2) >> 2.
-// One wave per strip (64-thread workgroups), in passes of kResizePass
-// output rows: every source row a pass reads is staged first, over the
-// strip's source column window, into dynamic LDS (`rw` bytes per row,
-// u32-aligned window start) with the loads in flight in batches of 8; the
-// taps then gather bytes from LDS.
+// One wave per strip (64-thread workgroups), no LDS: a lane's 4 outputs
+// read source bytes from at most 12 consecutive bytes of a row from its
+// aligned word `wb` on (the plan checks it per level), loaded as three u32.  The two horizontal taps of an
+// output column are one byte permute (a u16 pair) and one v_dot2, computed
+// once per source row.  The wave walks the strip's source rows in order, 4
+// rows per step with the next 4 rows' loads in flight, and stores an output
+// row once both of its source rows are done.
 __global__ __launch_bounds__(64) void k_resize(const Strip* st, int n, const LevelDesc* lv, const int2* coef,
-                                               uint8_t* pyr, int rw) {
-    extern __shared__ uint32_t s_rows[];
+                                               uint8_t* pyr) {
     const int sidx = xcd_block(n);
     if (sidx < 0) return;
     const int lane = threadIdx.x;
@@ -142,87 +143,82 @@ __global__ __launch_bounds__(64) void k_resize(const Strip* st, int n, const Lev
     const LevelDesc L = lv[S.lvl];
     const LevelDesc SL = lv[L.src];
     const int x = S.x0 + 4 * lane;
-    const int yend = min(S.y0 + kStripH, L.h);
+    const int ny = min(S.y0 + kStripH, L.h) - S.y0;
     const uint8_t* sp = pyr + SL.pix;
-    // source window: columns [cb, c_hi], rows [r_lo, r_hi]
-    const int xl = min(S.x0 + kStripW - 1, L.w - 1);
-    const int cb = coef[L.coef_x + S.x0].x & ~3;
-    const int c_hi = min(coef[L.coef_x + xl].x + 1, SL.w - 1);
-    const int nw = (c_hi - cb) / 4 + 1;
-    const int rwd = rw >> 2;
-    // the strip's row coefficients, one per lane, requested with the
-    // staging loads so that no output row waits on its own load
+    // the strip's row coefficients, lane k for output row S.y0 + k
     int2 cyl = make_int2(0, 0);
-    if (lane < yend - S.y0) cyl = coef[L.coef_y + S.y0 + lane];
-    // column taps of this lane's 4 outputs
-    int o0[4], o1[4], a0[4], a1[4];
+    if (lane < ny) cyl = coef[L.coef_y + S.y0 + lane];
+    // column taps: output j takes window bytes off_j and off_j + 1 (off_0 < 4)
+    const int wb = coef[L.coef_x + min(x, L.w - 1)].x & ~3;
+    uint32_t sel[4], ab[4];
+    bool hiw[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const int2 cx = coef[L.coef_x + min(x + j, L.w - 1)];
-        o0[j] = cx.x - cb;
-        o1[j] = min(cx.x + 1, SL.w - 1) - cb;
-        a0[j] = cx.y & 0xFFFF;
-        a1[j] = cx.y >> 16;
+        const int off = cx.x - wb;
+        hiw[j] = j > 0 && off >= 4;
+        const uint32_t o = (uint32_t)(hiw[j] ? off - 4 : off);
+        sel[j] = o | (0x0cu << 8) | ((o + 1) << 16) | (0x0cu << 24);
+        ab[j] = (uint32_t)cx.y;  // a0 | a1 << 16, both in [0, 2048]
     }
     const bool own = x < L.w;  // every lane stays: lane k holds row k's coefficients
-    const uint8_t* rows = reinterpret_cast<const uint8_t*>(s_rows);
     uint8_t* dp = pyr + L.pix;  // output row y, column x at byte y * pitch + x
-    // the strip in passes of kResizePass output rows, each staging only the
-    // source rows it reads (a smaller LDS window per wave)
-    for (int ys = S.y0; ys < yend; ys += kResizePass) {
-        const int ye = min(ys + kResizePass, yend);
-        const int r_lo = min(max(__builtin_amdgcn_readlane(cyl.x, ys - S.y0), 0), SL.h - 1);
-        const int r_hi = min(max(__builtin_amdgcn_readlane(cyl.x, ye - 1 - S.y0) + 1, 0), SL.h - 1);
-        const int nr = r_hi - r_lo + 1;
-        // source rows in batches of 8, the next batch requested before the
-        // current one is written to LDS
-        auto load8 = [&](int k0, uint32_t (&v)[8][3]) {
+    auto row_a = [&](int k) { return min(max(__builtin_amdgcn_readlane(cyl.x, k), 0), SL.h - 1); };
+    auto row_b = [&](int k) { return min(max(__builtin_amdgcn_readlane(cyl.x, k) + 1, 0), SL.h - 1); };
+    const int r_lo = row_a(0), r_hi = row_b(ny - 1);
+    auto load4 = [&](int r0, uint32_t (&w)[4][3]) {
 #pragma unroll
-            for (int kk = 0; kk < 8; kk++)
-#pragma unroll
-                for (int h = 0; h < 3; h++) {
-                    const int d = lane + 64 * h;
-                    v[kk][h] = (k0 + kk < nr && d < nw) ? *reinterpret_cast<const uint32_t*>(
-                                                              sp + (uint32_t)((r_lo + k0 + kk) * SL.pitch + cb + 4 * d))
-                                                        : 0u;
-                }
-        };
-        auto store8 = [&](int k0, const uint32_t (&v)[8][3]) {
-#pragma unroll
-            for (int kk = 0; kk < 8; kk++)
-#pragma unroll
-                for (int h = 0; h < 3; h++) {
-                    const int d = lane + 64 * h;
-                    if (k0 + kk < nr && d < nw) s_rows[(k0 + kk) * rwd + d] = v[kk][h];
-                }
-        };
-        if (ys != S.y0) wave_lds_sync();  // the previous pass's reads are done
-        uint32_t va[8][3], vb[8][3];
-        load8(0, va);
-        for (int k0 = 0; k0 < nr; k0 += 16) {
-            if (k0 + 8 < nr) load8(k0 + 8, vb);
-            store8(k0, va);
-            if (k0 + 16 < nr) load8(k0 + 16, va);
-            if (k0 + 8 < nr) store8(k0 + 8, vb);
+        for (int u = 0; u < 4; u++) {
+            const uint32_t* rp = reinterpret_cast<const uint32_t*>(
+                sp + (uint32_t)(min(r0 + u, SL.h - 1) * SL.pitch + wb));
+            w[u][0] = rp[0];
+            w[u][1] = rp[1];
+            w[u][2] = rp[2];
         }
-        wave_lds_sync();
-        for (int y = ys; y < ye; y++) {
-            const int2 cy = make_int2(__builtin_amdgcn_readlane(cyl.x, y - S.y0), __builtin_amdgcn_readlane(cyl.y, y - S.y0));
-            const int b0 = cy.y & 0xFFFF, b1 = cy.y >> 16;
-            const uint8_t* r0p = rows + (min(max(cy.x, 0), SL.h - 1) - r_lo) * rw;
-            const uint8_t* r1p = rows + (min(max(cy.x + 1, 0), SL.h - 1) - r_lo) * rw;
-            uint32_t v = 0;
+    };
+    // horizontal taps of one loaded source row
+    auto hrow = [&](const uint32_t (&w)[3], uint32_t (&h)[4]) {
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int r0 = r0p[o0[j]] * a0[j] + r0p[o1[j]] * a1[j];
-                const int r1 = r1p[o0[j]] * a0[j] + r1p[o1[j]] * a1[j];
-                // b < 2^12 and r >> 4 < 2^15: 24-bit multiplies, same products
-                const uint32_t o = (((__umul24((uint32_t)b0, (uint32_t)(r0 >> 4))) >> 16) +
-                                    ((__umul24((uint32_t)b1, (uint32_t)(r1 >> 4))) >> 16) + 2) >> 2;
-                v |= (o & 0xFF) << (8 * j);
+        for (int j = 0; j < 4; j++) {
+            const uint32_t pr = hiw[j] ? __builtin_amdgcn_perm(w[2], w[1], sel[j]) : __builtin_amdgcn_perm(w[1], w[0], sel[j]);
+            h[j] = __builtin_amdgcn_udot2(as_us2(pr), as_us2(ab[j]), 0u, false);
+        }
+    };
+    uint32_t ha[4] = {0, 0, 0, 0}, hb[4] = {0, 0, 0, 0};  // source rows r - 1 and r
+    uint32_t wc[4][3], wn[4][3];
+    load4(r_lo, wc);
+    int yo = 0;
+    for (int r0 = r_lo; r0 <= r_hi; r0 += 4) {
+        if (r0 + 4 <= r_hi) load4(r0 + 4, wn);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int r = r0 + u;
+            if (r <= r_hi) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) ha[j] = hb[j];
+                hrow(wc[u], hb);
+                // the output rows whose lower source row is r (their upper one is r - 1, or r where clamped)
+                while (yo < ny && row_b(yo) == r) {
+                    const bool same = row_a(yo) == r;
+                    const uint32_t cb = (uint32_t)__builtin_amdgcn_readlane(cyl.y, yo);
+                    const uint32_t b0 = cb & 0xFFFFu, b1 = cb >> 16;
+                    uint32_t v = 0;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const uint32_t r0v = same ? hb[j] : ha[j];
+                        // b < 2^12 and r >> 4 < 2^15: 24-bit multiplies, same products
+                        const uint32_t o = ((__umul24(b0, r0v >> 4) >> 16) + (__umul24(b1, hb[j] >> 4) >> 16) + 2) >> 2;
+                        v |= (o & 0xFF) << (8 * j);
+                    }
+                    if (own) *reinterpret_cast<uint32_t*>(dp + (uint32_t)((S.y0 + yo) * L.pitch + x)) = v;
+                    yo++;
+                }
             }
-            if (own) *reinterpret_cast<uint32_t*>(dp + (uint32_t)(y * L.pitch + x)) = v;
         }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) wc[u][q] = wn[u][q];
     }
 }
 

@@ -15,7 +15,6 @@ constexpr long long kOctLargePixels = 600000;  // levels this large get 1024-thr
 constexpr int kMaxWin = 96;             // max FAST cell window side (pixels)
 constexpr int kStripW = 256;            // strip kernels: one wave per 256 x 32 pixels,
 constexpr int kStripH = 32;             //   4 adjacent pixels per lane
-constexpr int kResizePass = 16;         // k_resize stages the source rows of 16 output rows at a time
 constexpr int kPitchAlign = 64;         // level rows start on 64-byte boundaries
 
 struct LevelDesc {

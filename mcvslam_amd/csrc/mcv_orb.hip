@@ -650,20 +650,36 @@ struct CellWin {
     int pitch, DW, DH;
 };
 
+// staged in batches of 6 words per lane: the batch's loads are all in
+// flight before the first store
+constexpr int kStageBatch = 6;
 __device__ __forceinline__ void cell_stage(const CellWin& C, uint32_t* __restrict__ band, int y0, int nb) {
     const int lane = threadIdx.x & 63;
     const int tot = (nb + 2) * C.nwr;
-    for (int k = lane; k < tot; k += 64) {
-        const int r = (int)(__umul24((uint32_t)k, C.wmagic) >> 16), w = k - __mul24(r, C.nwr);
-        const int ry = y0 - 1 + r;
-        uint32_t v = 0;
-        if (ry >= 0 && ry < C.DH) {
-            v = *reinterpret_cast<const uint32_t*>(C.wbase + (__umul24(ry, C.pitch) + 4 * w));
-            // keep the bytes of detection columns [0, DW)
-            const int lo = min(max(C.lead + 1 - 4 * w, 0), 4), hi = min(max(C.lead + 1 + C.DW - 4 * w, 0), 4);
-            v = hi > lo ? v & ((0xFFFFFFFFu >> (32 - 8 * (hi - lo))) << (8 * lo)) : 0u;
+    for (int k0 = 0; k0 < tot; k0 += 64 * kStageBatch) {
+        uint32_t v[kStageBatch], keep[kStageBatch];
+        int dst[kStageBatch];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; u++) {
+            const int k = k0 + 64 * u + lane;
+            v[u] = 0;
+            keep[u] = 0;
+            dst[u] = -1;
+            if (k < tot) {
+                const int r = (int)(__umul24((uint32_t)k, C.wmagic) >> 16), w = k - __mul24(r, C.nwr);
+                const int ry = y0 - 1 + r;
+                dst[u] = r * (kCellSW / 4) + w;
+                if (ry >= 0 && ry < C.DH) {
+                    v[u] = *reinterpret_cast<const uint32_t*>(C.wbase + (__umul24(ry, C.pitch) + 4 * w));
+                    // keep the bytes of detection columns [0, DW)
+                    const int lo = min(max(C.lead + 1 - 4 * w, 0), 4), hi = min(max(C.lead + 1 + C.DW - 4 * w, 0), 4);
+                    keep[u] = hi > lo ? (0xFFFFFFFFu >> (32 - 8 * (hi - lo))) << (8 * lo) : 0u;
+                }
+            }
         }
-        band[r * (kCellSW / 4) + w] = v;
+#pragma unroll
+        for (int u = 0; u < kStageBatch; u++)
+            if (dst[u] >= 0) band[dst[u]] = v[u] & keep[u];
     }
 }
 

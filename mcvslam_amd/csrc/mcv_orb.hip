@@ -421,8 +421,9 @@ __device__ __forceinline__ int byte_of(uint32_t w, int j) { return (int)((w >> (
 // points (0,4,8,12), so a corner at t needs such a pair both darker than
 // v - t or both brighter than v + t; the test runs on 2 pixels per packed
 // u16 operation.  Survivors queue up in LDS and are scored 64 at a time by
-// the whole wave from the ring; the group's 4-row block of scores is
-// assembled in LDS and stored once its queue is drained.
+// the whole wave from the ring; the group's score rows are zeroed when the
+// group starts and each score above t_lo is stored as a byte (no LDS tile,
+// so that six workgroups fit a CU's LDS).
 
 typedef short mcv_s2 __attribute__((ext_vector_type(2)));
 
@@ -496,7 +497,6 @@ static_assert(2 * kBlk + 6 <= kRing, "FAST ring too small for the prefetch group
 __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, const LevelDesc* lv, const uint8_t* pyr,
                                                     uint8_t* score, int t_lo) {
     __shared__ uint32_t s_ring[4][kRing + kRingG][kRingW];
-    __shared__ uint32_t s_tile[4][kBlk][kStripW / 4];
     __shared__ uint16_t s_queue[4][64 + kStripW];
     STRIP_SETUP(st, n, false)
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -553,9 +553,13 @@ __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, cons
             for (int k = 0; k < kBlk; k++) load_row(yg + kBlk + 3 + k, na[k], ne[k]);
         }
         const int ye = min(yg + kBlk, yend);
+        // the group's score rows start at 0; the drain below writes the
+        // scores above t_lo into them (this wave's stores, in program order)
+        if (own)
+            for (int k = 0; k < ye - yg; k++)
+                *reinterpret_cast<uint32_t*>(score + L.pix + (uint32_t)((yg + k) * L.pitch + x)) = 0u;
         for (int y = yg; y < ye; y++) {
             const int tb = y - yg;
-            s_tile[wv][tb][lane] = 0u;
             wave_lds_sync();
             int surv = 0;
             if (y >= 3 && y < L.h - 3) {
@@ -598,18 +602,9 @@ __global__ __launch_bounds__(256) void k_fast_score(const Strip* st, int n, cons
                     // rows yy - 3 .. yy + 3 from slot (yy - 3) mod 16 on (guard rows past 15)
                     const uint8_t* base = reinterpret_cast<const uint8_t*>(ring[(yy - 3) & (kRing - 1)]) + 4 + pos;
                     const int b = fast_best_at<kRingW * 4>(base);
-                    if (b > t_lo) reinterpret_cast<uint8_t*>(s_tile[wv][et])[pos] = (uint8_t)b;
+                    if (b > t_lo) score[L.pix + (uint32_t)(yy * L.pitch + S.x0 + pos)] = (uint8_t)b;
                 }
                 qlen -= cnt;
-            }
-            if (block_end) {
-                wave_lds_sync();
-                if (own) {
-                    for (int k = 0; k <= tb; k++)
-                        *reinterpret_cast<uint32_t*>(score + L.pix + (uint32_t)((y - tb + k) * L.pitch + x)) =
-                            s_tile[wv][k][lane];
-                }
-                wave_lds_sync();
             }
         }
         if (more) {

@@ -351,9 +351,10 @@ __global__ __launch_bounds__(256) void k_knn2_m(const MatchPair* pairs) {
         if (st + 1 < nsuper) raw = fetch(st + 1);
         const int tj = st * 32 * kMSub;
         if (tj + 32 * kMSub <= nt) {
-            // a full super-tile: one straight run, so the scheduler can overlap
-            // one sub-tile's products with the previous one's top-2 updates
-#pragma unroll
+            // a full super-tile, one sub-tile at a time (not unrolled: one set of
+            // accumulators keeps the kernel at 118 VGPRs, four waves per SIMD,
+            // whose products and top-2 updates overlap each other's)
+#pragma unroll 1
             for (int sb = 0; sb < kMSub; sb++)
                 knn_m_tile<QT, false>(a, &s_b[sb][0][lane], (uint32_t)(tj + 32 * sb + (lane & 31)), true, k0, k1);
         } else {

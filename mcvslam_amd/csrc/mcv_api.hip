@@ -1408,7 +1408,9 @@ int mcv_orb_batch_pyramid_dev(mcv_ctx* c, int image, int level, int blurred, con
 static void launch_knn(hipStream_t s, DevBuf& pbuf, const std::vector<MatchPair>& pairs, int max_nq, int variant) {
     if (pairs.empty() || max_nq <= 0) return;
     upload(pbuf, pairs, s);
-    launch_knn2(variant, pbuf.as<MatchPair>(), (int)pairs.size(), max_nq, s);
+    int max_nt = 0;
+    for (const MatchPair& p : pairs) max_nt = std::max(max_nt, p.nt);
+    launch_knn2(variant, pbuf.as<MatchPair>(), (int)pairs.size(), max_nq, s, max_nt);
     HIP_OK(hipGetLastError());
 }
 

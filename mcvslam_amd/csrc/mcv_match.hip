@@ -394,17 +394,204 @@ __global__ __launch_bounds__(256) void k_knn2_m(const MatchPair* pairs) {
         }
 }
 
+// Variant F4: the same block products on the fp4 (e2m1) matrix cores,
+// v_mfma_scale_f32_32x32x64_f8f6f4 with unit scales, K = 64 per step.
+// Operand layout (probed): lane l holds row (A) or column (B) l % 32 and
+// the 32 nibbles of K slice [32 (l / 32), + 32) of the step, nibble j at
+// bits 4 j of the 16 bytes; the result layout is that of the 32 x 32 i32
+// products.  Steps 0..3 take the descriptor bits (A: -2.0 = 0xC per set
+// query bit, B: 1.0 = 0x2 per set train bit); step 4 adds |q| (A: |q| as
+// 6.0 nibbles and a remainder, B: all 1.0), step 5 adds |t| (A: all 1.0, B:
+// |t| the same way), so the f32 accumulators hold the distances exactly
+// (integers <= 256).  Such a float's top 16 bits order it like the integer
+// and keep it exactly, so a key is (bits & 0xFFFF0000) | j: one and-or per
+// distance, then the med3/min top-2 update; j < 65536 (the launcher checks).
+typedef int mcv_v8i __attribute__((ext_vector_type(8)));
+typedef float mcv_v16f __attribute__((ext_vector_type(16)));
+
+// 8 bits of x as 8 nibbles of 0 or 1 (bit i at bit 4 i)
+__device__ __forceinline__ uint32_t spread8(uint32_t x) {
+    x &= 0xFFu;
+    x = (x | (x << 12)) & 0x000F000Fu;
+    x = (x | (x << 6)) & 0x03030303u;
+    x = (x | (x << 3)) & 0x11111111u;
+    return x;
+}
+
+// the 32 nibbles of a count c <= 256 over K positions [32 h, 32 h + 32):
+// c = 6 n + r as n nibbles 6.0 (0x7), then r (1: 0x2, 2: 0x4, 3: 0x5,
+// 4: 0x6, 5: 0x6 then 0x2); at most 44 positions
+__device__ __forceinline__ mcv_v4i count_nibbles(uint32_t c, int h) {
+    const int n6 = (int)(c / 6u), r = (int)(c - 6u * (uint32_t)n6);
+    // the remainder's nibbles as one pair (second << 4 | first)
+    const uint32_t rem = r == 0 ? 0u : r == 1 ? 0x2u : r == 2 ? 0x4u : r == 3 ? 0x5u : r == 4 ? 0x6u : 0x26u;
+    mcv_v4i v;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int p = n6 - (32 * h + 8 * q);  // first position past the 6.0 run, relative to this dword
+        const int f = min(max(p, 0), 8);
+        uint32_t w = f >= 8 ? 0x77777777u : 0x77777777u & ((1u << (4 * f)) - 1u);
+        if (p >= 0 && p < 8) w |= rem << (4 * p);              // a pair straddling the end keeps its first nibble
+        if (p == -1) w |= rem >> 4;                            // the second nibble of a pair from the dword before
+        v[q] = (int)w;
+    }
+    return v;
+}
+
+__device__ __forceinline__ mcv_v16f mfma_f4(mcv_v4i a, mcv_v4i b, mcv_v16f c) {
+    const mcv_v8i A = {a.x, a.y, a.z, a.w, 0, 0, 0, 0}, B = {b.x, b.y, b.z, b.w, 0, 0, 0, 0};
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, c, 4, 4, 0, 127, 0, 127);
+}
+
+constexpr uint32_t kF4Sentinel = 0x44790000u;  // top 16 bits of 999.0f, index 0
+
+template <bool kTail>
+__device__ __forceinline__ void knn_f4_tile(const mcv_v4i (&a)[5], mcv_v4i ones, const mcv_v4i* sb, uint32_t j,
+                                            bool colvalid, uint32_t (&k0)[16], uint32_t (&k1)[16]) {
+    mcv_v4i b[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) b[m] = sb[64 * m];
+    mcv_v16f acc = {};
+#pragma unroll
+    for (int m = 0; m < 4; m++) acc = mfma_f4(a[m], b[m], acc);
+    acc = mfma_f4(a[4], ones, acc);  // + |q|
+    acc = mfma_f4(ones, b[4], acc);  // + |t|
+    // the accumulators read as one integer vector: element-wise bit casts of
+    // the float vector were folded into acc[0] for every row by the compiler
+    const mcv_v16i bits = __builtin_bit_cast(mcv_v16i, acc);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        uint32_t key = ((uint32_t)bits[r] & 0xFFFF0000u) | j;
+        if (kTail) key = colvalid ? key : 0xFFFFFFFFu;
+        top2_insert(k0[r], k1[r], key);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_knn2_f4(const MatchPair* pairs) {
+    __shared__ mcv_v4i s_b[kMSub][5][64];  // a super-tile as B operands: 4 bit steps, then |t|
+    const MatchPair P = pairs[blockIdx.y];
+    const int nq = P.nq_dev ? min(*P.nq_dev, P.nq) : P.nq;
+    const int nt = P.nt_dev ? min(*P.nt_dev, P.nt) : P.nt;
+    const int q0 = blockIdx.x * 128;
+    if (q0 >= nq) return;
+    const int lane = threadIdx.x & 63, half = lane >> 5;
+    const int wq0 = q0 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 32;
+    const mcv_v4i ones = {0x22222222, 0x22222222, 0x22222222, 0x22222222};
+    // A operands: -2.0 per set bit of word 2 m + half in step m; |q| in step 4
+    mcv_v4i a[5];
+    {
+        const uint4* qrow = reinterpret_cast<const uint4*>(P.q + (long long)min(wq0 + (lane & 31), nq - 1) * 32);
+        const uint4 x = qrow[0], y = qrow[1];
+        const uint32_t w[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+        uint32_t pc = 0;
+#pragma unroll
+        for (int m = 0; m < 8; m++) pc += __builtin_popcount(w[m]);
+        const uint32_t hm = half ? 0xFFFFFFFFu : 0u;  // a blend, not an index (keeps w in registers)
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const uint32_t wm = (w[2 * m] & ~hm) | (w[2 * m + 1] & hm);
+#pragma unroll
+            for (int q = 0; q < 4; q++) a[m][q] = (int)(spread8(wm >> (8 * q)) * 0xCu);
+        }
+        a[4] = count_nibbles(pc, half);
+    }
+    uint32_t k0[16], k1[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        k0[r] = kF4Sentinel;
+        k1[r] = kF4Sentinel;
+    }
+    // thread tid expands words 4 (tid & 1) .. + 3 (steps 2 (tid & 1), + 1,
+    // both halves) of train (tid >> 1) of a super-tile, and half (tid & 1)
+    // of its |t| step
+    const uint4* tv = reinterpret_cast<const uint4*>(P.t);
+    auto fetch = [&](int st) -> uint4 {
+        const int j = st * 32 * kMSub + (threadIdx.x >> 1);
+        return j < nt ? tv[2ll * j + (threadIdx.x & 1)] : make_uint4(0u, 0u, 0u, 0u);
+    };
+    const int nsuper = (nt + 32 * kMSub - 1) / (32 * kMSub);
+    uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+    if (nsuper > 0) raw = fetch(0);
+    const int jl = threadIdx.x >> 1, sub = jl >> 5, col = jl & 31, th = threadIdx.x & 1;
+    for (int st = 0; st < nsuper; st++) {
+        __syncthreads();  // the previous super-tile's operands are read
+        {
+            const uint32_t w[4] = {raw.x, raw.y, raw.z, raw.w};
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    mcv_v4i v;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) v[q] = (int)(spread8(w[2 * k + h] >> (8 * q)) << 1);
+                    s_b[sub][2 * th + k][32 * h + col] = v;
+                }
+            uint32_t pc = __builtin_popcount(w[0]) + __builtin_popcount(w[1]) + __builtin_popcount(w[2]) +
+                          __builtin_popcount(w[3]);
+            pc += (uint32_t)__shfl_xor((int)pc, 1, 64);
+            s_b[sub][4][32 * th + col] = count_nibbles(pc, th);
+        }
+        __syncthreads();
+        if (st + 1 < nsuper) raw = fetch(st + 1);
+        const int tj = st * 32 * kMSub;
+        if (tj + 32 * kMSub <= nt) {
+#pragma unroll 1
+            for (int sb = 0; sb < kMSub; sb++)
+                knn_f4_tile<false>(a, ones, &s_b[sb][0][lane], (uint32_t)(tj + 32 * sb + (lane & 31)), true, k0, k1);
+        } else {
+#pragma unroll
+            for (int sb = 0; sb < kMSub; sb++) {
+                const int tj0 = tj + 32 * sb;
+                if (tj0 < nt) {
+                    const uint32_t j = (uint32_t)(tj0 + (lane & 31));
+                    knn_f4_tile<true>(a, ones, &s_b[sb][0][lane], j, (int)j < nt, k0, k1);
+                }
+            }
+        }
+    }
+    // the 32 lanes of a half hold the same rows: merge their top-2 keys
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        uint32_t a0 = k0[r], a1 = k1[r];
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const uint32_t b0 = (uint32_t)__shfl_xor((int)a0, o, 64), b1 = (uint32_t)__shfl_xor((int)a1, o, 64);
+            const uint32_t n1 = min(max(a0, b0), min(a1, b1));
+            a0 = min(a0, b0);
+            a1 = n1;
+        }
+        const int qi = wq0 + 8 * (r >> 2) + 4 * half + (r & 3);
+        if ((lane & 31) == 0 && qi < nq) {
+            // the sentinel keeps index 0 and distance 999, as the reference's d_idx init does
+            auto dist = [](uint32_t k) {
+                return k == kF4Sentinel ? 999 : (int)__builtin_bit_cast(float, k & 0xFFFF0000u);
+            };
+            int4 o;
+            o.x = (int)(a0 & 0xFFFFu);
+            o.y = dist(a0);
+            o.z = (int)(a1 & 0xFFFFu);
+            o.w = dist(a1);
+            reinterpret_cast<int4*>(P.out)[qi] = o;
+        }
+    }
+}
+
 // Variants of the same exact computation; the autotuner picks one per
 // workload (kKnnVariants of them).
-constexpr int kKnnVariants = 10;
+constexpr int kKnnVariants = 11;
 inline const char* knn_variant_name(int v) {
-    static const char* names[kKnnVariants] = {"S<1,2>", "S<1,4>", "S<2,2>", "S<2,4>", "L",
-                                              "S<2,8>", "S<1,8>", "S<4,2>", "M<1>",  "M<2>"};
+    static const char* names[kKnnVariants] = {"S<1,2>", "S<1,4>", "S<2,2>", "S<2,4>", "L",   "S<2,8>",
+                                              "S<1,8>", "S<4,2>", "M<1>",  "M<2>",  "F4"};
     return (v >= 0 && v < kKnnVariants) ? names[v] : "?";
 }
 
-inline void launch_knn2(int variant, const MatchPair* pairs, int npairs, int max_nq, hipStream_t s) {
+// max_nt: the largest train count of the pairs (the fp4 variant keys train
+// indices in 16 bits and hands larger sets to M<1>)
+inline void launch_knn2(int variant, const MatchPair* pairs, int npairs, int max_nq, hipStream_t s, int max_nt = -1) {
+    if (max_nt < 0) max_nt = max_nq;
+    if (variant == 10 && max_nt >= 65536) variant = 8;
     switch (variant) {
+    case 10: hipLaunchKernelGGL(k_knn2_f4, dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;
     case 1: hipLaunchKernelGGL((k_knn2_s<1, 4>), dim3((max_nq + 63) / 64, npairs), dim3(256), 0, s, pairs); break;
     case 2: hipLaunchKernelGGL((k_knn2_s<2, 2>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;
     case 3: hipLaunchKernelGGL((k_knn2_s<2, 4>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;

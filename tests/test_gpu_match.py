@@ -37,6 +37,24 @@ def test_knn2_every_variant(gpu_ctx, variant, nq, nt):
     np.testing.assert_array_equal(got, oracle.knn2(q, t))
 
 
+def test_knn2_fp4_variant_large_train_sets(gpu_ctx):
+    """The fp4 variant keys train indices in 16 bits: a train set of 65536 or
+    more goes to the int8 kernel, and the best match past index 65535 comes
+    back whole (planted exact copies there)."""
+    rng = np.random.default_rng(77)
+    nq, nt = 40, 70000
+    q, t = _desc(rng, nq), _desc(rng, nt)
+    t[69000:69000 + nq // 2] = q[: nq // 2]
+    try:
+        gpu_ctx.set_match_variant(10)
+        got = Matcher.KnnMatch(q, t, ctx=gpu_ctx).rows
+    finally:
+        gpu_ctx.set_match_variant(0)
+    want = oracle.knn2(q, t)
+    np.testing.assert_array_equal(got, want)
+    assert (want[: nq // 2, 0] == np.arange(69000, 69000 + nq // 2)).all()
+
+
 def test_knn2_ties_keep_scan_order(gpu_ctx):
     """Duplicated trains: the first occurrence wins (strict '<' in order)."""
     rng = np.random.default_rng(3)

@@ -400,7 +400,9 @@ __global__ __launch_bounds__(256) void k_knn2_m(const MatchPair* pairs) {
 // the 32 nibbles of K slice [32 (l / 32), + 32) of the step, nibble j at
 // bits 4 j of the 16 bytes; the result layout is that of the 32 x 32 i32
 // products.  Steps 0..3 take the descriptor bits (A: -2.0 = 0xC per set
-// query bit, B: 1.0 = 0x2 per set train bit); step 4 adds |q| (A: |q| as
+// query bit, B: 1.0 = 0x2 per set train bit), bit q + 4 i of a word as
+// nibble i of dword q (any order serves that A and B share: this one is a
+// shift and a mask per dword); step 4 adds |q| (A: |q| as
 // 6.0 nibbles and a remainder, B: all 1.0), step 5 adds |t| (A: all 1.0, B:
 // |t| the same way), so the f32 accumulators hold the distances exactly
 // (integers <= 256).  Such a float's top 16 bits order it like the integer
@@ -409,14 +411,8 @@ __global__ __launch_bounds__(256) void k_knn2_m(const MatchPair* pairs) {
 typedef int mcv_v8i __attribute__((ext_vector_type(8)));
 typedef float mcv_v16f __attribute__((ext_vector_type(16)));
 
-// 8 bits of x as 8 nibbles of 0 or 1 (bit i at bit 4 i)
-__device__ __forceinline__ uint32_t spread8(uint32_t x) {
-    x &= 0xFFu;
-    x = (x | (x << 12)) & 0x000F000Fu;
-    x = (x | (x << 6)) & 0x03030303u;
-    x = (x | (x << 3)) & 0x11111111u;
-    return x;
-}
+// bits q, q + 4, .., q + 28 of w as 8 nibbles of 0 or 1 (bit q + 4 i at bit 4 i)
+__device__ __forceinline__ uint32_t nibbles_of(uint32_t w, int q) { return (w >> q) & 0x11111111u; }
 
 // the 32 nibbles of a count c <= 256 over K positions [32 h, 32 h + 32):
 // c = 6 n + r as n nibbles 6.0 (0x7), then r (1: 0x2, 2: 0x4, 3: 0x5,
@@ -491,7 +487,7 @@ __global__ __launch_bounds__(256) void k_knn2_f4(const MatchPair* pairs) {
         for (int m = 0; m < 4; m++) {
             const uint32_t wm = (w[2 * m] & ~hm) | (w[2 * m + 1] & hm);
 #pragma unroll
-            for (int q = 0; q < 4; q++) a[m][q] = (int)(spread8(wm >> (8 * q)) * 0xCu);
+            for (int q = 0; q < 4; q++) a[m][q] = (int)(nibbles_of(wm, q) * 0xCu);
         }
         a[4] = count_nibbles(pc, half);
     }
@@ -523,7 +519,7 @@ __global__ __launch_bounds__(256) void k_knn2_f4(const MatchPair* pairs) {
                 for (int h = 0; h < 2; h++) {
                     mcv_v4i v;
 #pragma unroll
-                    for (int q = 0; q < 4; q++) v[q] = (int)(spread8(w[2 * k + h] >> (8 * q)) << 1);
+                    for (int q = 0; q < 4; q++) v[q] = (int)(nibbles_of(w[2 * k + h], q) << 1);
                     s_b[sub][2 * th + k][32 * h + col] = v;
                 }
             uint32_t pc = __builtin_popcount(w[0]) + __builtin_popcount(w[1]) + __builtin_popcount(w[2]) +

@@ -464,7 +464,10 @@ __device__ __forceinline__ void knn_f4_tile(const mcv_v4i (&a)[5], mcv_v4i ones,
 }
 
 __global__ __launch_bounds__(256) void k_knn2_f4(const MatchPair* pairs) {
-    __shared__ mcv_v4i s_b[kMSub][5][64];  // a super-tile as B operands: 4 bit steps, then |t|
+    // two super-tiles as B operands (4 bit steps, then |t|): the next one is
+    // expanded between the current one's first and second blocks, one
+    // barrier per super-tile
+    __shared__ mcv_v4i s_b[2][kMSub][5][64];
     const MatchPair P = pairs[blockIdx.y];
     const int nq = P.nq_dev ? min(*P.nq_dev, P.nq) : P.nq;
     const int nt = P.nt_dev ? min(*P.nt_dev, P.nt) : P.nt;
@@ -506,41 +509,52 @@ __global__ __launch_bounds__(256) void k_knn2_f4(const MatchPair* pairs) {
         return j < nt ? tv[2ll * j + (threadIdx.x & 1)] : make_uint4(0u, 0u, 0u, 0u);
     };
     const int nsuper = (nt + 32 * kMSub - 1) / (32 * kMSub);
-    uint4 raw = make_uint4(0u, 0u, 0u, 0u);
-    if (nsuper > 0) raw = fetch(0);
     const int jl = threadIdx.x >> 1, sub = jl >> 5, col = jl & 31, th = threadIdx.x & 1;
+    auto expand = [&](const uint4 raw, int buf) {
+        const uint32_t w[4] = {raw.x, raw.y, raw.z, raw.w};
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                mcv_v4i v;
+#pragma unroll
+                for (int q = 0; q < 4; q++) v[q] = (int)(nibbles_of(w[2 * k + h], q) << 1);
+                s_b[buf][sub][2 * th + k][32 * h + col] = v;
+            }
+        uint32_t pc = __builtin_popcount(w[0]) + __builtin_popcount(w[1]) + __builtin_popcount(w[2]) +
+                      __builtin_popcount(w[3]);
+        pc += (uint32_t)__shfl_xor((int)pc, 1, 64);
+        s_b[buf][sub][4][32 * th + col] = count_nibbles(pc, th);
+    };
+    uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+    if (nsuper > 0) {
+        expand(fetch(0), 0);
+        if (nsuper > 1) raw = fetch(1);
+    }
     for (int st = 0; st < nsuper; st++) {
-        __syncthreads();  // the previous super-tile's operands are read
-        {
-            const uint32_t w[4] = {raw.x, raw.y, raw.z, raw.w};
-#pragma unroll
-            for (int k = 0; k < 2; k++)
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    mcv_v4i v;
-#pragma unroll
-                    for (int q = 0; q < 4; q++) v[q] = (int)(nibbles_of(w[2 * k + h], q) << 1);
-                    s_b[sub][2 * th + k][32 * h + col] = v;
-                }
-            uint32_t pc = __builtin_popcount(w[0]) + __builtin_popcount(w[1]) + __builtin_popcount(w[2]) +
-                          __builtin_popcount(w[3]);
-            pc += (uint32_t)__shfl_xor((int)pc, 1, 64);
-            s_b[sub][4][32 * th + col] = count_nibbles(pc, th);
-        }
+        // super-tile st is in buffer st & 1; the other buffer was last read
+        // in iteration st - 1, which every wave has finished here
         __syncthreads();
-        if (st + 1 < nsuper) raw = fetch(st + 1);
+        const int cb = st & 1;
         const int tj = st * 32 * kMSub;
         if (tj + 32 * kMSub <= nt) {
+            knn_f4_tile<false>(a, ones, &s_b[cb][0][0][lane], (uint32_t)(tj + (lane & 31)), true, k0, k1);
+            if (st + 1 < nsuper) {
+                expand(raw, cb ^ 1);
+                if (st + 2 < nsuper) raw = fetch(st + 2);
+            }
 #pragma unroll 1
-            for (int sb = 0; sb < kMSub; sb++)
-                knn_f4_tile<false>(a, ones, &s_b[sb][0][lane], (uint32_t)(tj + 32 * sb + (lane & 31)), true, k0, k1);
+            for (int sb = 1; sb < kMSub; sb++)
+                knn_f4_tile<false>(a, ones, &s_b[cb][sb][0][lane], (uint32_t)(tj + 32 * sb + (lane & 31)), true, k0,
+                                   k1);
         } else {
+            // a partial super-tile is the last one: nothing left to expand
 #pragma unroll
             for (int sb = 0; sb < kMSub; sb++) {
                 const int tj0 = tj + 32 * sb;
                 if (tj0 < nt) {
                     const uint32_t j = (uint32_t)(tj0 + (lane & 31));
-                    knn_f4_tile<true>(a, ones, &s_b[sb][0][lane], j, (int)j < nt, k0, k1);
+                    knn_f4_tile<true>(a, ones, &s_b[cb][sb][0][lane], j, (int)j < nt, k0, k1);
                 }
             }
         }

@@ -906,7 +906,7 @@ struct mcv_rig {
     std::unique_ptr<OrbPlan> batch;                 // all 3n images
     std::vector<std::unique_ptr<OrbPlan>> single;   // per camera, one image
     int cap = 0;
-    DevBuf kps, desc, counts, knn, matches, mcounts, pairs, jobs, scratch;
+    DevBuf kps, desc, counts, knn, matches, mcounts, pairs, jobs, scratch, porder;
     std::vector<MatchPair> hp;
     std::vector<FilterJob> hj;
     // Captured graphs of the batched run, one set per launch-variant set used
@@ -1607,6 +1607,7 @@ int mcv_rig_create(mcv_ctx* c, int n_frames, const int32_t* w3, const int32_t* h
     r->matches.ensure((size_t)n * r->cap * 12);
     r->mcounts.ensure((size_t)n * 4);
     r->scratch.ensure((size_t)n * r->cap * 8);
+    r->porder.ensure((size_t)n * sizeof(int));
     // pairs per frame: L-R (query L), W-L (query W), W-R (query W)
     const int qa[3] = {0, 2, 2}, ta[3] = {1, 0, 1};
     for (int f = 0; f < n_frames; f++)
@@ -1641,13 +1642,26 @@ int mcv_rig_create(mcv_ctx* c, int n_frames, const int32_t* w3, const int32_t* h
 
 void mcv_rig_destroy(mcv_rig* r) { delete r; }
 
+// the rig's top-2 launch: the pairs ordered by train count first (the
+// counts are on the device), then the chosen variant
+static void rig_knn(mcv_rig* r, hipStream_t s) {
+    const int np = (int)r->hp.size();
+    const int* order = nullptr;
+    if (np <= kPairOrderMax) {
+        hipLaunchKernelGGL(k_pair_order, dim3(1), dim3(kPairOrderMax), 0, s, r->pairs.as<MatchPair>(), np,
+                           r->porder.as<int>());
+        order = r->porder.as<int>();
+    }
+    launch_knn2(r->tune.knn, r->pairs.as<MatchPair>(), np, r->cap, s, -1, order);
+}
+
 static void rig_enqueue_batched(mcv_rig* r, const SinCosTables& tb, hipStream_t s, KernelTimer* rec = nullptr,
                                 hipStream_t side = nullptr) {
     OrbPlan& P = *r->batch;
     P.enqueue(r->ctx->orb, tb, s, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>(), rec,
               true, side);
     const int np = (int)r->hp.size();
-    launch_knn2(r->tune.knn, r->pairs.as<MatchPair>(), np, r->cap, s);
+    rig_knn(r, s);
     if (rec) rec->mark("k_knn2");
     hipLaunchKernelGGL(k_filter, dim3(np), dim3(1024), 0, s, r->jobs.as<FilterJob>());
     if (rec) rec->mark("k_filter");
@@ -1687,7 +1701,7 @@ static void rig_capture(mcv_rig* r, const SinCosTables& tb) {
             P.enqueue(c->orb, tb, cs, r->kps.as<mcv_keypoint>(), r->desc.as<uint8_t>(), r->cap, r->counts.as<int>(),
                       nullptr, true, nullptr, kPartPost);
             const int np = (int)r->hp.size();
-            launch_knn2(r->tune.knn, r->pairs.as<MatchPair>(), np, r->cap, cs);
+            rig_knn(r, cs);
             hipLaunchKernelGGL(k_filter, dim3(np), dim3(1024), 0, cs, r->jobs.as<FilterJob>());
             HIP_OK(hipGetLastError());
         });

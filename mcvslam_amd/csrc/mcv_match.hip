@@ -463,12 +463,13 @@ __device__ __forceinline__ void knn_f4_tile(const mcv_v4i (&a)[5], mcv_v4i ones,
     }
 }
 
-__global__ __launch_bounds__(256) void k_knn2_f4(const MatchPair* pairs) {
+__global__ __launch_bounds__(256) void k_knn2_f4(const MatchPair* pairs, const int* order) {
     // two super-tiles as B operands (4 bit steps, then |t|): the next one is
     // expanded between the current one's first and second blocks, one
     // barrier per super-tile
     __shared__ mcv_v4i s_b[2][kMSub][5][64];
-    const MatchPair P = pairs[blockIdx.y];
+    // pairs in the order given (largest train sets first), else in index order
+    const MatchPair P = pairs[order ? order[blockIdx.y] : (int)blockIdx.y];
     const int nq = P.nq_dev ? min(*P.nq_dev, P.nq) : P.nq;
     const int nt = P.nt_dev ? min(*P.nt_dev, P.nt) : P.nt;
     const int q0 = blockIdx.x * 128;
@@ -586,6 +587,39 @@ __global__ __launch_bounds__(256) void k_knn2_f4(const MatchPair* pairs) {
     }
 }
 
+// Dispatch order of the pairs for the top-2 kernels: a workgroup's work is
+// proportional to its pair's train count, so the pairs with the most trains
+// go first (longest-processing-time first) and the launch does not end on a
+// few large late workgroups.  One workgroup sorts up to kPairOrderMax pairs
+// by (train count descending, index ascending) in LDS (bitonic network).
+constexpr int kPairOrderMax = 1024;
+__global__ __launch_bounds__(1024) void k_pair_order(const MatchPair* pairs, int np, int* order) {
+    __shared__ unsigned long long key[kPairOrderMax];
+    const int t = threadIdx.x;
+    unsigned long long k = ~0ull;
+    if (t < np) {
+        const MatchPair P = pairs[t];
+        const int nt = P.nt_dev ? min(*P.nt_dev, P.nt) : P.nt;
+        k = ((unsigned long long)(0xFFFFFFFFu - (uint32_t)max(nt, 0)) << 32) | (uint32_t)t;
+    }
+    key[t] = k;
+    __syncthreads();
+    for (int size = 2; size <= kPairOrderMax; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const int o = t ^ stride;
+            if (o > t) {
+                const unsigned long long a = key[t], b = key[o];
+                const bool up = (t & size) == 0;
+                if ((a > b) == up) {
+                    key[t] = b;
+                    key[o] = a;
+                }
+            }
+            __syncthreads();
+        }
+    if (t < np) order[t] = (int)(uint32_t)key[t];
+}
+
 // Variants of the same exact computation; the autotuner picks one per
 // workload (kKnnVariants of them).
 constexpr int kKnnVariants = 11;
@@ -596,12 +630,14 @@ inline const char* knn_variant_name(int v) {
 }
 
 // max_nt: the largest train count of the pairs (the fp4 variant keys train
-// indices in 16 bits and hands larger sets to M<1>)
-inline void launch_knn2(int variant, const MatchPair* pairs, int npairs, int max_nq, hipStream_t s, int max_nt = -1) {
+// indices in 16 bits and hands larger sets to M<1>); order: the dispatch
+// order of k_pair_order, or null (used by the fp4 variant)
+inline void launch_knn2(int variant, const MatchPair* pairs, int npairs, int max_nq, hipStream_t s, int max_nt = -1,
+                        const int* order = nullptr) {
     if (max_nt < 0) max_nt = max_nq;
     if (variant == 10 && max_nt >= 65536) variant = 8;
     switch (variant) {
-    case 10: hipLaunchKernelGGL(k_knn2_f4, dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;
+    case 10: hipLaunchKernelGGL(k_knn2_f4, dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs, order); break;
     case 1: hipLaunchKernelGGL((k_knn2_s<1, 4>), dim3((max_nq + 63) / 64, npairs), dim3(256), 0, s, pairs); break;
     case 2: hipLaunchKernelGGL((k_knn2_s<2, 2>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;
     case 3: hipLaunchKernelGGL((k_knn2_s<2, 4>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;

@@ -22,7 +22,7 @@ def test_knn2_random(gpu_ctx, nq, nt):
 
 
 @pytest.mark.parametrize("variant", range(Context.N_KNN_VARIANTS))
-@pytest.mark.parametrize("nq,nt", [(3, 1), (65, 5), (130, 257), (700, 1531)])
+@pytest.mark.parametrize("nq,nt", [(4, 0), (3, 1), (65, 5), (130, 257), (700, 1531)])
 def test_knn2_every_variant(gpu_ctx, variant, nq, nt):
     """All launch variants the autotuner chooses from return the oracle's rows
     (ragged query and train counts against the variants' block shapes)."""

@@ -1648,7 +1648,7 @@ static void rig_knn(mcv_rig* r, hipStream_t s) {
     const int np = (int)r->hp.size();
     const int* order = nullptr;
     if (np <= kPairOrderMax) {
-        hipLaunchKernelGGL(k_pair_order, dim3(1), dim3(kPairOrderMax), 0, s, r->pairs.as<MatchPair>(), np,
+        hipLaunchKernelGGL(k_pair_order, dim3(1), dim3(256), 0, s, r->pairs.as<MatchPair>(), np,
                            r->porder.as<int>());
         order = r->porder.as<int>();
     }

@@ -590,34 +590,25 @@ __global__ __launch_bounds__(256) void k_knn2_f4(const MatchPair* pairs, const i
 // Dispatch order of the pairs for the top-2 kernels: a workgroup's work is
 // proportional to its pair's train count, so the pairs with the most trains
 // go first (longest-processing-time first) and the launch does not end on a
-// few large late workgroups.  One workgroup sorts up to kPairOrderMax pairs
-// by (train count descending, index ascending) in LDS (bitonic network).
+// few large late workgroups.  One workgroup ranks up to kPairOrderMax pairs
+// by (train count descending, index ascending): the keys are distinct, so a
+// key's rank (the number of smaller keys, counted over LDS broadcasts) is
+// its place.
 constexpr int kPairOrderMax = 1024;
-__global__ __launch_bounds__(1024) void k_pair_order(const MatchPair* pairs, int np, int* order) {
+__global__ __launch_bounds__(256) void k_pair_order(const MatchPair* pairs, int np, int* order) {
     __shared__ unsigned long long key[kPairOrderMax];
-    const int t = threadIdx.x;
-    unsigned long long k = ~0ull;
-    if (t < np) {
+    for (int t = threadIdx.x; t < np; t += 256) {
         const MatchPair P = pairs[t];
         const int nt = P.nt_dev ? min(*P.nt_dev, P.nt) : P.nt;
-        k = ((unsigned long long)(0xFFFFFFFFu - (uint32_t)max(nt, 0)) << 32) | (uint32_t)t;
+        key[t] = ((unsigned long long)(0xFFFFFFFFu - (uint32_t)max(nt, 0)) << 32) | (uint32_t)t;
     }
-    key[t] = k;
     __syncthreads();
-    for (int size = 2; size <= kPairOrderMax; size <<= 1)
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            const int o = t ^ stride;
-            if (o > t) {
-                const unsigned long long a = key[t], b = key[o];
-                const bool up = (t & size) == 0;
-                if ((a > b) == up) {
-                    key[t] = b;
-                    key[o] = a;
-                }
-            }
-            __syncthreads();
-        }
-    if (t < np) order[t] = (int)(uint32_t)key[t];
+    for (int t = threadIdx.x; t < np; t += 256) {
+        const unsigned long long k = key[t];
+        int rank = 0;
+        for (int j = 0; j < np; j++) rank += key[j] < k ? 1 : 0;
+        order[rank] = t;
+    }
 }
 
 // Variants of the same exact computation; the autotuner picks one per

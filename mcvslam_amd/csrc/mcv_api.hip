@@ -1607,7 +1607,7 @@ int mcv_rig_create(mcv_ctx* c, int n_frames, const int32_t* w3, const int32_t* h
     r->matches.ensure((size_t)n * r->cap * 12);
     r->mcounts.ensure((size_t)n * 4);
     r->scratch.ensure((size_t)n * r->cap * 8);
-    r->porder.ensure((size_t)n * sizeof(int));
+    r->porder.ensure((size_t)(2 * n + 1) * sizeof(int));
     // pairs per frame: L-R (query L), W-L (query W), W-R (query W)
     const int qa[3] = {0, 2, 2}, ta[3] = {1, 0, 1};
     for (int f = 0; f < n_frames; f++)

@@ -463,16 +463,30 @@ __device__ __forceinline__ void knn_f4_tile(const mcv_v4i (&a)[5], mcv_v4i ones,
     }
 }
 
-__global__ __launch_bounds__(256) void k_knn2_f4(const MatchPair* pairs, const int* order) {
+__global__ __launch_bounds__(256) void k_knn2_f4(const MatchPair* pairs, const int* order, int np) {
     // two super-tiles as B operands (4 bit steps, then |t|): the next one is
     // expanded between the current one's first and second blocks, one
     // barrier per super-tile
     __shared__ mcv_v4i s_b[2][kMSub][5][64];
-    // pairs in the order given (largest train sets first), else in index order
-    const MatchPair P = pairs[order ? order[blockIdx.y] : (int)blockIdx.y];
+    // with an order: a 1-D grid over the real 128-query tiles of the pairs
+    // in that order (largest train sets first), the spare workgroups last;
+    // without: (tile, pair) in index order
+    int pi = (int)blockIdx.y, tile = (int)blockIdx.x;
+    if (order) {
+        const int b = (int)blockIdx.x;
+        if (b >= order[2 * np]) return;
+        int lo = 0, hi = np - 1;  // the last rank whose first tile is <= b
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (order[np + mid] <= b) lo = mid; else hi = mid - 1;
+        }
+        pi = order[lo];
+        tile = b - order[np + lo];
+    }
+    const MatchPair P = pairs[pi];
     const int nq = P.nq_dev ? min(*P.nq_dev, P.nq) : P.nq;
     const int nt = P.nt_dev ? min(*P.nt_dev, P.nt) : P.nt;
-    const int q0 = blockIdx.x * 128;
+    const int q0 = tile * 128;
     if (q0 >= nq) return;
     const int lane = threadIdx.x & 63, half = lane >> 5;
     const int wq0 = q0 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 32;
@@ -603,11 +617,26 @@ __global__ __launch_bounds__(256) void k_pair_order(const MatchPair* pairs, int 
         key[t] = ((unsigned long long)(0xFFFFFFFFu - (uint32_t)max(nt, 0)) << 32) | (uint32_t)t;
     }
     __syncthreads();
+    __shared__ int s_tiles[kPairOrderMax];
     for (int t = threadIdx.x; t < np; t += 256) {
         const unsigned long long k = key[t];
         int rank = 0;
         for (int j = 0; j < np; j++) rank += key[j] < k ? 1 : 0;
         order[rank] = t;
+        const MatchPair P = pairs[t];
+        const int nq = P.nq_dev ? min(*P.nq_dev, P.nq) : P.nq;
+        s_tiles[rank] = (max(nq, 0) + 127) / 128;
+    }
+    __syncthreads();
+    // order[np + r]: the first 128-query tile of the r-th pair in a 1-D grid
+    // of the real tiles (order[2 np]: their total)
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int r = 0; r < np; r++) {
+            order[np + r] = run;
+            run += s_tiles[r];
+        }
+        order[2 * np] = run;
     }
 }
 
@@ -621,14 +650,19 @@ inline const char* knn_variant_name(int v) {
 }
 
 // max_nt: the largest train count of the pairs (the fp4 variant keys train
-// indices in 16 bits and hands larger sets to M<1>); order: the dispatch
-// order of k_pair_order, or null (used by the fp4 variant)
+// indices in 16 bits and hands larger sets to M<1>); order: the output of
+// k_pair_order (2 npairs + 1 ints), or null (used by the fp4 variant)
 inline void launch_knn2(int variant, const MatchPair* pairs, int npairs, int max_nq, hipStream_t s, int max_nt = -1,
                         const int* order = nullptr) {
     if (max_nt < 0) max_nt = max_nq;
     if (variant == 10 && max_nt >= 65536) variant = 8;
     switch (variant) {
-    case 10: hipLaunchKernelGGL(k_knn2_f4, dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs, order); break;
+    case 10:
+        if (order)
+            hipLaunchKernelGGL(k_knn2_f4, dim3((max_nq + 127) / 128 * npairs), dim3(256), 0, s, pairs, order, npairs);
+        else
+            hipLaunchKernelGGL(k_knn2_f4, dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs, order, npairs);
+        break;
     case 1: hipLaunchKernelGGL((k_knn2_s<1, 4>), dim3((max_nq + 63) / 64, npairs), dim3(256), 0, s, pairs); break;
     case 2: hipLaunchKernelGGL((k_knn2_s<2, 2>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;
     case 3: hipLaunchKernelGGL((k_knn2_s<2, 4>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;

@@ -132,9 +132,9 @@ int mcv_match_filter_orientation(mcv_ctx* ctx, const int32_t* matches, int n, co
 int mcv_distinctive_descriptors(mcv_ctx* ctx, const uint8_t* desc, const int32_t* ptr, int n, int32_t* best,
                                 uint8_t* out_desc);
 /* Launch variant of the top-2 kernel used by mcv_match_knn2* on this context
- * (0 .. 10: 0-3, 5-7 popcount kernels with scalar-loaded trains, 4 LDS-tiled,
- * 8-9 distances on the int8 matrix cores, 10 on the fp4 ones (train sets of
- * 65536 or more rows run variant 8); no counterpart in the reference: the
+ * (0 .. 11: 0-3, 5-7 popcount kernels with scalar-loaded trains, 4 LDS-tiled,
+ * 8-9 distances on the int8 matrix cores, 10-11 on the fp4 ones (train sets
+ * of 65536 or more rows run variant 8); no counterpart in the reference: the
  * autotuner's choice, also settable for testing).  Every variant returns the
  * same rows. */
 int mcv_match_set_variant(mcv_ctx* ctx, int variant);
@@ -174,7 +174,7 @@ void mcv_rig_destroy(mcv_rig* rig);
  * (optional) receives one line per timed variant and the choice. */
 int mcv_rig_autotune(mcv_rig* rig, const uint8_t* const* d_images, int reps, char* report, int report_cap);
 /* Fixes the rig's launch variants instead of tuning them: the top-2 kernel
- * (0 .. 10, as mcv_match_set_variant), the blur (bit 0: the walk, stop at the
+ * (0 .. 11, as mcv_match_set_variant), the blur (bit 0: the walk, stop at the
  * level end or fixed length; bit 1: the step runs as several graphs with
  * FAST of the three largest levels and the blur on the context's side
  * stream, the blur beside the octree heap replay), the keypoint kernel (0: one keypoint per

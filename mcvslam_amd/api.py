@@ -66,12 +66,12 @@ class Context:
     def synchronize(self):
         check(self.h, lib().mcv_synchronize(self.h))
 
-    N_KNN_VARIANTS = 11
+    N_KNN_VARIANTS = 12
 
     def set_match_variant(self, variant: int):
         """Launch variant of the top-2 kernel for Matcher calls
         (0 .. N_KNN_VARIANTS - 1; 8 and 9 compute the distances on the int8
-        matrix cores, 10 on the fp4 ones; all return the same rows)."""
+        matrix cores, 10 and 11 on the fp4 ones; all return the same rows)."""
         check(self.h, lib().mcv_match_set_variant(self.h, int(variant)))
 
     def to_host(self, d_ptr: int, dtype, shape) -> np.ndarray:

@@ -463,11 +463,13 @@ __device__ __forceinline__ void knn_f4_tile(const mcv_v4i (&a)[5], mcv_v4i ones,
     }
 }
 
-__global__ __launch_bounds__(256) void k_knn2_f4(const MatchPair* pairs, const int* order, int np) {
-    // two super-tiles as B operands (4 bit steps, then |t|): the next one is
-    // expanded between the current one's first and second blocks, one
-    // barrier per super-tile
-    __shared__ mcv_v4i s_b[2][kMSub][5][64];
+// kDb: two super-tiles as B operands (4 bit steps, then |t|), the next one
+// expanded between the current one's first and second blocks, one barrier
+// per super-tile (40 KB of LDS); else one super-tile expanded between two
+// barriers (20 KB).  kWaves: waves per SIMD the registers are cut for.
+template <bool kDb, int kWaves>
+__global__ __launch_bounds__(256, kWaves) void k_knn2_f4(const MatchPair* pairs, const int* order, int np) {
+    __shared__ mcv_v4i s_b[kDb ? 2 : 1][kMSub][5][64];
     // with an order: a 1-D grid over the real 128-query tiles of the pairs
     // in that order (largest train sets first), the spare workgroups last;
     // without: (tile, pair) in index order
@@ -547,14 +549,21 @@ __global__ __launch_bounds__(256) void k_knn2_f4(const MatchPair* pairs, const i
         if (nsuper > 1) raw = fetch(1);
     }
     for (int st = 0; st < nsuper; st++) {
-        // super-tile st is in buffer st & 1; the other buffer was last read
-        // in iteration st - 1, which every wave has finished here
+        if constexpr (!kDb) {
+            if (st > 0) {
+                __syncthreads();  // the previous super-tile's operands are read
+                expand(raw, 0);
+                if (st + 1 < nsuper) raw = fetch(st + 1);
+            }
+        }
+        // (two buffers) super-tile st is in buffer st & 1; the other buffer
+        // was last read in iteration st - 1, which every wave has finished here
         __syncthreads();
-        const int cb = st & 1;
+        const int cb = kDb ? (st & 1) : 0;
         const int tj = st * 32 * kMSub;
         if (tj + 32 * kMSub <= nt) {
             knn_f4_tile<false>(a, ones, &s_b[cb][0][0][lane], (uint32_t)(tj + (lane & 31)), true, k0, k1);
-            if (st + 1 < nsuper) {
+            if (kDb && st + 1 < nsuper) {
                 expand(raw, cb ^ 1);
                 if (st + 2 < nsuper) raw = fetch(st + 2);
             }
@@ -642,10 +651,10 @@ __global__ __launch_bounds__(256) void k_pair_order(const MatchPair* pairs, int 
 
 // Variants of the same exact computation; the autotuner picks one per
 // workload (kKnnVariants of them).
-constexpr int kKnnVariants = 11;
+constexpr int kKnnVariants = 12;
 inline const char* knn_variant_name(int v) {
     static const char* names[kKnnVariants] = {"S<1,2>", "S<1,4>", "S<2,2>", "S<2,4>", "L",   "S<2,8>",
-                                              "S<1,8>", "S<4,2>", "M<1>",  "M<2>",  "F4"};
+                                              "S<1,8>", "S<4,2>", "M<1>",  "M<2>",  "F4",    "F4s5"};
     return (v >= 0 && v < kKnnVariants) ? names[v] : "?";
 }
 
@@ -655,14 +664,18 @@ inline const char* knn_variant_name(int v) {
 inline void launch_knn2(int variant, const MatchPair* pairs, int npairs, int max_nq, hipStream_t s, int max_nt = -1,
                         const int* order = nullptr) {
     if (max_nt < 0) max_nt = max_nq;
-    if (variant == 10 && max_nt >= 65536) variant = 8;
+    if ((variant == 10 || variant == 11) && max_nt >= 65536) variant = 8;
     switch (variant) {
     case 10:
-        if (order)
-            hipLaunchKernelGGL(k_knn2_f4, dim3((max_nq + 127) / 128 * npairs), dim3(256), 0, s, pairs, order, npairs);
+    case 11: {
+        // with an order, a 1-D grid of the real tiles first
+        const dim3 grid = order ? dim3((max_nq + 127) / 128 * npairs) : dim3((max_nq + 127) / 128, npairs);
+        if (variant == 10)
+            hipLaunchKernelGGL((k_knn2_f4<true, 4>), grid, dim3(256), 0, s, pairs, order, npairs);
         else
-            hipLaunchKernelGGL(k_knn2_f4, dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs, order, npairs);
+            hipLaunchKernelGGL((k_knn2_f4<false, 5>), grid, dim3(256), 0, s, pairs, order, npairs);
         break;
+    }
     case 1: hipLaunchKernelGGL((k_knn2_s<1, 4>), dim3((max_nq + 63) / 64, npairs), dim3(256), 0, s, pairs); break;
     case 2: hipLaunchKernelGGL((k_knn2_s<2, 2>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;
     case 3: hipLaunchKernelGGL((k_knn2_s<2, 4>), dim3((max_nq + 127) / 128, npairs), dim3(256), 0, s, pairs); break;

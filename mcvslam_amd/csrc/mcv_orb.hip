@@ -1489,21 +1489,19 @@ __global__ __launch_bounds__(64) void k_octree_heap(OrbGlobals g, const int* ord
         if (hkey(tk) == 1) break;
         const uint32_t c0 = tp, nch = tk & 7u;
         uint32_t s_l, c_l;
+        // the children's table entries are requested before the pop and used
+        // after it: their LDS latency passes during the pop's first round,
+        // which needs nothing new from LDS
         tab_children(nt, TP, c0, nch, s_l, c_l);
-        // the children's heap entries, before the pop: the pushes then wait
-        // on nothing but their own reads
-        uint32_t ek[4], ep[4];
-#pragma unroll
-        for (uint32_t k = 0; k < 4; k++) {
-            const uint2 e = node_entry(c0 + k, lane_val(s_l, k), lane_val(s_l, k + 1), lane_val(c_l, k),
-                                       lane_val(c_l, k + 1));
-            ek[k] = e.x;
-            ep[k] = e.y;
-        }
         par_pop(H, len, pr, SL);
 #pragma unroll
-        for (uint32_t k = 0; k < 4; k++)
-            if (k < nch) par_push(H, len, ek[k], ep[k]);
+        for (uint32_t k = 0; k < 4; k++) {
+            if (k < nch) {
+                const uint2 e = node_entry(c0 + k, lane_val(s_l, k), lane_val(s_l, k + 1), lane_val(c_l, k),
+                                           lane_val(c_l, k + 1));
+                par_push(H, len, e.x, e.y);
+            }
+        }
     }
     // drain: the j-th popped node's word goes to key[len] once the pop has
     // freed that slot, i.e. node j ends at key[n0 - 1 - j]

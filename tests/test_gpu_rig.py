@@ -80,6 +80,41 @@ def test_rig_uneven_feature_counts(orb_default):
         np.testing.assert_array_equal(desc[i, :counts[i]], rd)
 
 
+@pytest.mark.parametrize("knn", [0, 10, 11])
+def test_rig_images_without_keypoints(orb_default, knn):
+    """Blank images in a batch: a pair without queries (W blank) and pairs
+    without trains (R blank) run through the top-2 launch (the fp4 variants:
+    the grid of real query tiles in train-count order) and match the oracle
+    rows (index 0, distance 999 where a query has no train)."""
+    import torch
+    frames = [list(synth.rig_frame(500 + 10 * f)) for f in range(3)]
+    frames[1][2] = np.full_like(frames[1][2], 128)   # W of frame 1
+    frames[2][1] = np.full_like(frames[2][1], 77)    # R of frame 2
+    imgs = [torch.from_numpy(np.ascontiguousarray(im)).cuda() for fr in frames for im in fr]
+    rig = RigBatch(orb_default, 3, ratio=0.7, th=50)
+    rig.set_variants(knn, 2, 2, 1)
+    outs = []
+    for graph in (True, False):
+        rig.run([im.data_ptr() for im in imgs], graph=graph)
+        outs.append(rig.read())
+    assert_same_outputs(outs[0], outs[1])
+    kps, desc, counts, matches, mcounts, knn_rows = outs[0]
+    ref = [oracle.orb_extract(im) for fr in frames for im in fr]
+    assert counts[5] == 0 and counts[7] == 0
+    for i, (rk, rd) in enumerate(ref):
+        assert counts[i] == len(rk)
+        np.testing.assert_array_equal(desc[i, :counts[i]], rd)
+    for f in range(3):
+        for k, (qa, ta) in enumerate(PAIRS):
+            rq, rt = ref[3 * f + qa][1], ref[3 * f + ta][1]
+            want_knn = oracle.knn2(rq, rt)
+            p = 3 * f + k
+            np.testing.assert_array_equal(knn_rows[p, :len(rq)], want_knn)
+            want = oracle.filter_ratio_threshold(want_knn, 0.7, 50)
+            assert mcounts[p] == len(want)
+            np.testing.assert_array_equal(matches[p, :mcounts[p]], want)
+
+
 def test_rig_autotune_keeps_outputs(orb_default):
     """The autotuner only changes launch shapes: outputs after tuning (graph
     re-captured with the chosen variants) equal the untuned ones."""
@@ -101,7 +136,8 @@ def test_rig_autotune_keeps_outputs(orb_default):
 
 
 @pytest.mark.parametrize("knn,blur,kp,heap", [(1, 1, 1, 0), (6, 0, 2, 1), (7, 1, 0, 0), (0, 0, 3, 1), (2, 0, 2, 0),
-                                             (8, 2, 2, 1), (9, 3, 1, 1), (8, 3, 2, 0)])
+                                             (8, 2, 2, 1), (9, 3, 1, 1), (8, 3, 2, 0), (10, 2, 2, 1),
+                                             (11, 3, 2, 1)])
 def test_rig_fixed_variants(orb_default, knn, blur, kp, heap):
     """Fixed launch variants (top-2 kernel, blur walk and its place in the
     captured graph, keypoints per wave, octree heap replay) give the outputs
